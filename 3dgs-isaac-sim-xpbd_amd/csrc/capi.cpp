@@ -1,0 +1,432 @@
+// capi.cpp — the drop-in C ABI (include/xpbd_capi.h, part 1 and 2).
+//
+// Mirrors native/xpbd_capi.cu: one library-global instance, 0 / -1 return
+// codes with the message in xpbd_last_error and on stderr, the same call-order
+// rules, and settings that persist across create / destroy like the
+// reference's forward.cu globals.  The device work is done by an Engine with
+// one body; the graph upload is deferred to the first xpbd_step as in the
+// reference's ensureChainmailGPU (forward.cu:10519-10845).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "xpbd_capi.h"
+
+using namespace xpbd;
+
+namespace {
+
+struct Global {
+    Settings set;                       // persists across create / destroy
+    ColliderDev cols[kMaxColliders] = {};
+    int colCount = 0;
+    float colMargin = 0.0f, colFriction = 0.0f;
+    bool pressAutoLog = true;
+    bool volJStats = true;
+    float jvw = 1.0f, jvwStd = 0.0f;
+    std::unique_ptr<BodyInput> input;
+    std::unique_ptr<Engine> engine;
+    int n = 0;
+    bool stepped = false;
+    bool contactOn = false;
+    std::string err;
+};
+
+Global& G()
+{
+    static Global g;
+    return g;
+}
+
+int fail(const std::string& msg)
+{
+    G().err = msg;
+    std::fprintf(stderr, "[xpbd_dll] %s\n", msg.c_str());
+    return -1;
+}
+
+void free_all()
+{
+    Global& g = G();
+    g.engine.reset();
+    g.input.reset();
+    g.n = 0;
+    g.stepped = false;
+    g.contactOn = false;
+    g.set.contactColliders = false;
+    g.set.contactGround = false;
+}
+
+void push_colliders()
+{
+    Global& g = G();
+    if (g.engine) g.engine->setColliderArray(0, g.cols, g.colCount, g.colMargin, g.colFriction);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* xpbd_last_error(void) { return G().err.c_str(); }
+const char* xpbd_ext_backend(void) { return "hip-gfx950"; }
+
+int xpbd_create(int n, const float* pos, const float* scales, const float* rots, const float* opacity,
+                int e, const int* edges, const float* rest, const float* stiff)
+{
+    free_all();
+    if (n <= 0 || !pos || !scales || !rots || !opacity || e < 0 || (e > 0 && (!edges || !rest || !stiff)))
+        return fail("xpbd_create: invalid arguments");
+    Global& g = G();
+    g.input.reset(new BodyInput());
+    build_input(*g.input, n, pos, scales, rots, opacity, e, edges, rest, stiff);
+    g.n = n;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        free_all();
+        return fail("xpbd_create: no HIP device");
+    }
+    return 0;
+}
+
+void xpbd_destroy(void)
+{
+    xpbd_set_colliders(0, nullptr, nullptr, nullptr, 0.0f, 0.0f);
+    Global& g = G();
+    if (g.engine) g.engine->attach(0, 0, nullptr, nullptr, 1.0f);
+    free_all();
+}
+
+int xpbd_count(void) { return G().n; }
+
+void xpbd_set_solver(int iters, float dt, float underRelax, float velDamping, float stiffnessScale,
+                     float invMassScale, float distCompliance, float shapeCompliance, float shapeBlend,
+                     float angleCompliance, float angleBlend)
+{
+    Settings& s = G().set;
+    s.iters = iters < 1 ? 1 : iters;
+    s.dt = std::fmax(1e-5f, dt);
+    s.underRelax = std::fmax(0.0f, underRelax);
+    s.velDamping = std::fmax(0.0f, velDamping);
+    s.stiffnessScale = std::fmax(0.0f, stiffnessScale);
+    s.invMassScale = std::fmax(0.0f, invMassScale);
+    s.distCompliance = std::fmax(0.0f, distCompliance);
+    s.shapeCompliance = std::fmax(0.0f, shapeCompliance);
+    s.shapeBlend = std::fmin(std::fmax(0.0f, shapeBlend), 1.0f);
+    s.angleCompliance = std::fmax(0.0f, angleCompliance);
+    s.angleBlend = std::fmin(std::fmax(0.0f, angleBlend), 1.0f);
+}
+
+void xpbd_set_constraints(int distance, int shape, int angle, int volume)
+{
+    Settings& s = G().set;
+    s.useDistance = distance != 0;
+    s.useShape = shape != 0;
+    s.useAngle = angle != 0;
+    s.useVolume = volume != 0;
+}
+
+void xpbd_set_volume(float compliance, int ringK, int maxMembers, int leaderMinHop)
+{
+    Settings& s = G().set;
+    s.volCompliance = compliance;
+    s.volRingK = std::max(1, std::min(8, ringK));
+    s.volMaxMembers = std::max(8, std::min(8192, maxMembers));
+    s.volLeaderMinHop = std::max(1, std::min(8, leaderMinHop));
+    s.volMemberSelect = 1;
+}
+
+void xpbd_set_ground(int enabled, float ux, float uy, float uz, float height, float friction,
+                     float restitution, float contactRadius, float contactSlop, float gravity)
+{
+    Settings& s = G().set;
+    float nx = ux, ny = uy, nz = uz;
+    const float l = std::sqrt(nx * nx + ny * ny + nz * nz);
+    if (!(l > 1e-8f) || !std::isfinite(l)) { nx = 0; ny = 0; nz = 1; }
+    else { nx /= l; ny /= l; nz /= l; }
+    s.groundN[0] = nx; s.groundN[1] = ny; s.groundN[2] = nz;
+    s.groundEnabled = enabled != 0;
+    s.groundHeight = std::isfinite(height) ? height : 0.0f;
+    s.groundFriction = std::fmax(0.0f, friction);
+    s.groundRestitution = std::fmin(std::fmax(restitution, 0.0f), 1.0f);
+    s.groundRadius = std::fmax(0.0f, contactRadius);
+    s.groundSlop = std::fmax(0.0f, contactSlop);
+    s.groundGravity = std::fmax(0.0f, gravity);
+}
+
+void xpbd_set_object_shape(float stiffness)
+{
+    Settings& s = G().set;
+    float v = std::fmin(std::fmax(stiffness, 0.0f), 1.0f);
+    if (!std::isfinite(v)) v = 0.0f;
+    s.objShapeStiffness = v;
+    s.objMinComponent = 50;
+}
+
+void xpbd_set_object_shape_gpu(int enabled) { G().set.objShapeGPU = enabled != 0; }
+
+void xpbd_set_colliders(int count, const int* types, const float* poses, const float* dims, float margin,
+                        float friction)
+{
+    Global& g = G();
+    if (count < 0) count = 0;
+    if (count > kMaxColliders) {
+        std::printf("[Collider] %d colliders requested, using the first %d\n", count, kMaxColliders);
+        count = kMaxColliders;
+    }
+    const bool samePrev = (count == g.colCount);
+    for (int c = 0; c < count; ++c) {
+        ColliderDev& K = g.cols[c];
+        const bool keep = samePrev && K.type == types[c];
+        for (int a = 0; a < 9; ++a) K.Rp[a] = keep ? K.R[a] : poses[12 * c + a];
+        for (int a = 0; a < 3; ++a) K.tp[a] = keep ? K.t[a] : poses[12 * c + 9 + a];
+        K.type = types[c];
+        for (int a = 0; a < 9; ++a) K.R[a] = poses[12 * c + a];
+        for (int a = 0; a < 3; ++a) { K.t[a] = poses[12 * c + 9 + a]; K.h[a] = std::fmax(dims[3 * c + a], 0.0f); }
+    }
+    g.colCount = count;
+    g.colMargin = (std::isfinite(margin) && margin > 0.0f) ? margin : 0.0f;
+    g.colFriction = (std::isfinite(friction) && friction > 0.0f) ? friction : 0.0f;
+    push_colliders();
+}
+
+int xpbd_collider_stats(int* hits, float* push, float* centroid, int maxCount)
+{
+    Global& g = G();
+    if (!g.engine) return 0;
+    return g.engine->colliderStats(0, hits, push, centroid, maxCount);
+}
+
+int xpbd_add_rigid_velocity(const float* dv, const float* dw, const float* center)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_add_rigid_velocity: no step yet");
+    g.engine->addRigidVelocity(0, dv, dw, center);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(std::string("xpbd_add_rigid_velocity: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int xpbd_set_contact_shape(int enabled, float tau, float radiusCap)
+{
+    Global& g = G();
+    if (!g.input) return fail("xpbd_set_contact_shape: call xpbd_create first");
+    g.contactOn = enabled != 0;
+    g.set.contactTau = tau;
+    g.set.contactCap = radiusCap;
+    g.set.contactColliders = (enabled & 1) != 0;
+    g.set.contactGround = (enabled & 2) != 0;
+    return 0;
+}
+
+int xpbd_set_attached(int count, const int* idx, const float* pos, float weight)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_set_attached: no step yet");
+    if (!g.engine->attach(0, count, idx, pos, weight)) return fail("xpbd_set_attached: device error");
+    return 0;
+}
+
+void xpbd_set_self_collision(int enabled, float radiusScale, float excludeScale, int withinBody)
+{
+    Settings& s = G().set;
+    s.scEnabled = enabled != 0;
+    s.scWithinBody = withinBody != 0;
+    s.scRadiusScale = std::fmin(std::fmax(std::isfinite(radiusScale) ? radiusScale : 1.5f, 0.1f), 10.0f);
+    s.scExcludeScale = std::fmin(std::fmax(std::isfinite(excludeScale) ? excludeScale : 2.0f, 1.05f), 20.0f);
+}
+
+void xpbd_self_collision_stats(int* out)
+{
+    if (!out) return;
+    out[0] = 0; out[1] = 0; out[2] = 0;
+}
+
+void xpbd_set_shape_robust(int enabled) { G().set.shapeRobust = enabled != 0; }
+
+int xpbd_step(void)
+{
+    Global& g = G();
+    if (!g.input) return fail("xpbd_step: call xpbd_create first");
+    std::string err;
+    if (!g.engine) {
+        g.engine.reset(new Engine());
+        if (!g.engine->build({g.input.get()}, err)) {
+            g.engine.reset();
+            return fail("xpbd_step: " + err);
+        }
+        push_colliders();
+    }
+    Engine& E = *g.engine;
+    E.settings(0) = g.set;
+    E.press(0).autoLog = E.press(0).active ? E.press(0).autoLog : g.pressAutoLog;
+    if (!E.step(1, err) || !E.sync(err)) return fail("xpbd_step: " + err);
+    g.stepped = true;
+    return 0;
+}
+
+unsigned long long xpbd_positions_device(void)
+{
+    Global& g = G();
+    if (!g.stepped) return 0ull;
+    return reinterpret_cast<unsigned long long>(g.engine->positionsDevice());
+}
+
+int xpbd_positions_host(float* out, int n)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_positions_host: no step yet");
+    if (!out || n != g.n) return fail("xpbd_positions_host: size mismatch");
+    if (!g.engine->getState(0, out, nullptr)) return fail("xpbd_positions_host: device copy failed");
+    return 0;
+}
+
+int xpbd_compute_shapes(float deformEps, float* hostScales, float* hostQuats, int n)
+{
+    Global& g = G();
+    (void)deformEps; (void)hostScales; (void)hostQuats;
+    if (!g.stepped) return fail("xpbd_compute_shapes: no step yet");
+    if (n != g.n) return fail("xpbd_compute_shapes: size mismatch");
+    return fail("xpbd_compute_shapes: not available in this build");
+}
+
+int xpbd_reset(void)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_reset: no step yet");
+    g.engine->attach(0, 0, nullptr, nullptr, 1.0f);
+    g.engine->resetToRest(0);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(std::string("xpbd_reset: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int xpbd_launch(const float* lin, const float* ang, const float* tilt)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_launch: no step yet");
+    g.engine->launchFromRest(0, lin, ang, tilt);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(std::string("xpbd_launch: ") + hipGetErrorString(e));
+    return 0;
+}
+
+// squashStart in press mode (forward.cu:4310-4387; native/xpbd_capi.cu:281-288)
+int xpbd_press_start(int axis, float rampPerSec, float maxDispPct, int fromLo)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_press_start: step once first");
+    PressState& ps = g.engine->press(0);
+    ps.fromLo = fromLo != 0;
+    if (axis < 0 || axis > 2) axis = 1;
+    const float slabPct = 0.08f;
+    maxDispPct = std::fmax(0.01f, std::fmin(1.0f, maxDispPct));
+    ps.axis = axis;
+    ps.ramp = rampPerSec;
+    const BodyInput& in = *g.input;
+    float lo = 3.402823466e+38f, hi = -3.402823466e+38f;
+    for (int i = 0; i < in.n; ++i) {
+        const float v = in.pos[3 * i + axis];
+        lo = std::fmin(lo, v);
+        hi = std::fmax(hi, v);
+    }
+    const float extent = hi - lo;
+    ps.lo = lo;
+    ps.hi = hi;
+    ps.maxDisp = maxDispPct * extent;
+    ps.curDisp = 0.0f;
+    // the slabs must both be non-empty for the reference to start (4350-4355)
+    const float topThr = hi - slabPct * extent, botThr = lo + slabPct * extent;
+    int top = 0, bot = 0;
+    for (int i = 0; i < in.n; ++i) {
+        const float v = in.pos[3 * i + axis];
+        if (v >= topThr) ++top;
+        else if (v <= botThr) ++bot;
+    }
+    if (top == 0 || bot == 0) {
+        ps.active = false;
+        return fail("xpbd_press_start: squashStart failed");
+    }
+    ps.active = true;
+    ps.autoLog = g.pressAutoLog;
+    ps.ckptIdx = 0;
+    ps.dwell = 0;
+    return 0;
+}
+
+void xpbd_press_autolog(int enabled) { G().pressAutoLog = enabled != 0; }
+
+void xpbd_press_stop(void)
+{
+    Global& g = G();
+    if (!g.engine || !g.engine->press(0).active) return;
+    g.engine->press(0).active = false;
+    g.engine->releaseInvMass(0);
+}
+
+int xpbd_press_state(float* lo, float* hi, float* curDisp, float* maxDisp)
+{
+    Global& g = G();
+    PressState ps;
+    if (g.engine) ps = g.engine->press(0);
+    const bool moving = ps.active;
+    if (curDisp) *curDisp = ps.curDisp;
+    if (maxDisp) *maxDisp = ps.maxDisp;
+    if (lo) *lo = ps.lo + ((moving && ps.fromLo) ? ps.curDisp : 0.0f);
+    if (hi) *hi = ps.hi - ((moving && !ps.fromLo) ? ps.curDisp : 0.0f);
+    return ps.active ? 1 : 0;
+}
+
+void xpbd_volume_stats(int enable, float* jvw, float* jvwStd)
+{
+    Global& g = G();
+    g.volJStats = enable != 0;
+    if (jvw) *jvw = g.jvw;
+    if (jvwStd) *jvwStd = g.jvwStd;
+}
+
+int xpbd_ext_get_state(float* pos, float* vel, int n)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_ext_get_state: no step yet");
+    if (n != g.n) return fail("xpbd_ext_get_state: size mismatch");
+    if (!g.engine->getState(0, pos, vel)) return fail("xpbd_ext_get_state: device copy failed");
+    return 0;
+}
+
+int xpbd_ext_set_state(const float* pos, const float* vel, int n)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_ext_set_state: no step yet");
+    if (n != g.n) return fail("xpbd_ext_set_state: size mismatch");
+    if (!g.engine->setState(0, pos, vel)) return fail("xpbd_ext_set_state: device copy failed");
+    return 0;
+}
+
+int xpbd_ext_profile(int substeps, char* json, int len)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_ext_profile: no step yet");
+    if (!json || len <= 0 || substeps <= 0) return fail("xpbd_ext_profile: invalid arguments");
+    std::string out, err;
+    Engine& E = *g.engine;
+    E.settings(0) = g.set;
+    if (!E.profile(substeps, out, err)) return fail("xpbd_ext_profile: " + err);
+    if ((int)out.size() + 1 > len) return fail("xpbd_ext_profile: buffer too small");
+    std::memcpy(json, out.c_str(), out.size() + 1);
+    return 0;
+}
+
+int xpbd_ext_get_inv_mass(float* invMass, int n)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_ext_get_inv_mass: no step yet");
+    if (n != g.n || !invMass) return fail("xpbd_ext_get_inv_mass: size mismatch");
+    if (!g.engine->getInvMass(0, invMass)) return fail("xpbd_ext_get_inv_mass: device copy failed");
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,604 @@
+// engine.cpp — host orchestration of the MI355X XPBD solver (see engine.h).
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+namespace xpbd {
+
+#define XPBD_HIP(call)                                                                    \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            err = std::string(#call) + ": " + hipGetErrorString(e_);                      \
+            return false;                                                                 \
+        }                                                                                 \
+    } while (0)
+
+template <typename T>
+void DevBuf<T>::alloc(size_t count)
+{
+    free();
+    n = count;
+    if (count) {
+        if (hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count) != hipSuccess) {
+            p = nullptr;
+            n = 0;
+        }
+    }
+}
+
+template <typename T>
+void DevBuf<T>::free()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+}
+
+template struct DevBuf<float>;
+template struct DevBuf<float3>;
+template struct DevBuf<float4>;
+template struct DevBuf<int>;
+template struct DevBuf<BodyDev>;
+template struct DevBuf<ColliderDev>;
+
+void build_input(BodyInput& in, int n, const float* pos, const float* scales, const float* rots, const float* opacity,
+                 int e, const int* edges, const float* rest, const float* stiff)
+{
+    in.n = n;
+    in.pos.assign(pos, pos + 3 * (size_t)n);
+    in.scales.assign(scales, scales + 3 * (size_t)n);
+    in.rots.assign(rots, rots + 4 * (size_t)n);
+    in.opacity.assign(opacity, opacity + n);
+    // counting pass, then fill in input edge order (a->b then b->a per edge)
+    std::vector<int> deg(n + 1, 0);
+    for (int k = 0; k < e; ++k) {
+        const int a = edges[2 * k], b = edges[2 * k + 1];
+        if (a < 0 || b < 0 || a >= n || b >= n || a == b) continue;
+        deg[a]++;
+        deg[b]++;
+    }
+    in.rowOff.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) in.rowOff[i + 1] = in.rowOff[i] + deg[i];
+    const int m = in.rowOff[n];
+    in.nbr.resize(m);
+    in.nrest.resize(m);
+    in.nstiff.resize(m);
+    std::vector<int> cur(in.rowOff.begin(), in.rowOff.end() - 1);
+    for (int k = 0; k < e; ++k) {
+        const int a = edges[2 * k], b = edges[2 * k + 1];
+        if (a < 0 || b < 0 || a >= n || b >= n || a == b) continue;
+        int s = cur[a]++;
+        in.nbr[s] = b; in.nrest[s] = rest[k]; in.nstiff[s] = stiff[k];
+        s = cur[b]++;
+        in.nbr[s] = a; in.nrest[s] = rest[k]; in.nstiff[s] = stiff[k];
+    }
+}
+
+Engine::Engine()
+{
+    (void)hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
+}
+
+Engine::~Engine()
+{
+    if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (hostBodies_) (void)hipHostFree(hostBodies_);
+    if (hostCols_) (void)hipHostFree(hostCols_);
+    if (stream_) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
+}
+
+bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err)
+{
+    const int B = (int)inputs.size();
+    if (B <= 0) { err = "engine: no bodies"; return false; }
+    inputs_ = inputs;
+    bodyBegin_.resize(B);
+    bodyCount_.resize(B);
+    long long base = 0, m = 0;
+    for (int b = 0; b < B; ++b) {
+        bodyBegin_[b] = (int)base;
+        bodyCount_[b] = inputs[b]->n;
+        base += ((long long)inputs[b]->n + kTile - 1) / kTile * kTile;
+        m += inputs[b]->rowOff[inputs[b]->n];
+    }
+    if (base > 0x7fffffffLL - kTile || m > 0x7fffffffLL) { err = "engine: problem too large for int32 indexing"; return false; }
+    nPad_ = (int)base;
+    m_ = m;
+    set_.assign(B, Settings());
+    press_.assign(B, PressState());
+    attActive_.assign(B, false);
+    colHost_.assign((size_t)B * kMaxColliders, ColliderDev{});
+    colCount_.assign(B, 0);
+    colMargin_.assign(B, 0.0f);
+    colFriction_.assign(B, 0.0f);
+    colDirty_ = true;
+    colStatsValid_ = false;
+
+    // host staging of the padded node space
+    std::vector<float3> hx(nPad_, make_float3(0, 0, 0));
+    std::vector<int> hrow(nPad_ + 1, 0), htile(nPad_ / kTile, 0);
+    std::vector<int> hnbr(m);
+    std::vector<float> hrest(m), hstiff(m);
+    long long eo = 0;
+    for (int b = 0; b < B; ++b) {
+        const BodyInput& in = *inputs[b];
+        const int nb = bodyBegin_[b];
+        for (int i = 0; i < in.n; ++i) hx[nb + i] = make_float3(in.pos[3 * i], in.pos[3 * i + 1], in.pos[3 * i + 2]);
+        const int padEnd = (b + 1 < B) ? bodyBegin_[b + 1] : nPad_;
+        for (int t = nb / kTile; t < padEnd / kTile; ++t) htile[t] = b;
+        for (int i = 0; i < in.n; ++i) {
+            hrow[nb + i] = (int)(eo + in.rowOff[i]);
+            for (int k = in.rowOff[i]; k < in.rowOff[i + 1]; ++k) {
+                hnbr[eo + k] = nb + in.nbr[k];
+                hrest[eo + k] = in.nrest[k];
+                hstiff[eo + k] = in.nstiff[k];
+            }
+        }
+        eo += in.rowOff[in.n];
+        for (int i = nb + in.n; i <= padEnd && i <= nPad_; ++i) hrow[i] = (int)eo;
+    }
+    hrow[nPad_] = (int)eo;
+
+    x_.alloc(nPad_); v_.alloc(nPad_); rest_.alloc(nPad_);
+    p0_.alloc(nPad_); p1_.alloc(nPad_);
+    invMass_.alloc(nPad_); invMassSaved_.alloc(nPad_); objW_.alloc(nPad_);
+    rowOff_.alloc(nPad_ + 1); nbr_.alloc(std::max<long long>(m, 1));
+    nrest_.alloc(std::max<long long>(m, 1)); nstiff_.alloc(std::max<long long>(m, 1));
+    nalpha_.alloc(std::max<long long>(m, 1)); lam_.alloc(std::max<long long>(m, 1));
+    tileBody_.alloc(nPad_ / kTile);
+    bodies_.alloc(B);
+    cols_.alloc((size_t)B * kMaxColliders);
+    colHits_.alloc((size_t)B * kMaxColliders);
+    colPush_.alloc((size_t)B * kMaxColliders * kColAcc);
+    if (!x_.p || !v_.p || !rest_.p || !p0_.p || !p1_.p || !invMass_.p || !invMassSaved_.p || !objW_.p || !rowOff_.p ||
+        !nbr_.p || !nrest_.p || !nstiff_.p || !nalpha_.p || !lam_.p || !tileBody_.p || !bodies_.p || !cols_.p ||
+        !colHits_.p || !colPush_.p) {
+        err = "engine: hipMalloc failed (out of device memory?)";
+        return false;
+    }
+    XPBD_HIP(hipMemcpyAsync(x_.p, hx.data(), sizeof(float3) * nPad_, hipMemcpyHostToDevice, stream_));
+    XPBD_HIP(hipMemcpyAsync(rest_.p, hx.data(), sizeof(float3) * nPad_, hipMemcpyHostToDevice, stream_));
+    XPBD_HIP(hipMemsetAsync(v_.p, 0, sizeof(float3) * nPad_, stream_));
+    XPBD_HIP(hipMemsetAsync(p0_.p, 0, sizeof(float4) * nPad_, stream_));
+    XPBD_HIP(hipMemsetAsync(p1_.p, 0, sizeof(float4) * nPad_, stream_));
+    XPBD_HIP(hipMemcpyAsync(rowOff_.p, hrow.data(), sizeof(int) * (nPad_ + 1), hipMemcpyHostToDevice, stream_));
+    if (m > 0) {
+        XPBD_HIP(hipMemcpyAsync(nbr_.p, hnbr.data(), sizeof(int) * m, hipMemcpyHostToDevice, stream_));
+        XPBD_HIP(hipMemcpyAsync(nrest_.p, hrest.data(), sizeof(float) * m, hipMemcpyHostToDevice, stream_));
+        XPBD_HIP(hipMemcpyAsync(nstiff_.p, hstiff.data(), sizeof(float) * m, hipMemcpyHostToDevice, stream_));
+    }
+    XPBD_HIP(hipMemcpyAsync(tileBody_.p, htile.data(), sizeof(int) * htile.size(), hipMemcpyHostToDevice, stream_));
+    launch_fill_float(stream_, nPad_, invMass_.p, 1.0f);
+    launch_fill_float(stream_, nPad_, objW_.p, 1.0f);
+    launch_fill_int(stream_, B * kMaxColliders, colHits_.p, 0);
+    launch_fill_float(stream_, B * kMaxColliders * kColAcc, colPush_.p, 0.0f);
+    if (!hostBodies_) XPBD_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostBodies_), sizeof(BodyDev) * B, 0));
+    XPBD_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostCols_), sizeof(ColliderDev) * B * kMaxColliders, 0));
+    lastBodies_.clear();
+    alphaKey_.clear();
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    XPBD_HIP(hipGetLastError());
+    return true;
+}
+
+// forward.cu:4478-4500: same list length keeps the previous pose for friction.
+void Engine::setColliders(int b, int count, const int* types, const float* poses, const float* dims, float margin,
+                          float friction)
+{
+    if (count < 0) count = 0;
+    if (count > kMaxColliders) count = kMaxColliders;
+    const bool samePrev = (count == colCount_[b]);
+    for (int c = 0; c < count; ++c) {
+        ColliderDev& K = colHost_[(size_t)b * kMaxColliders + c];
+        const bool keep = samePrev && K.type == types[c];
+        for (int a = 0; a < 9; ++a) K.Rp[a] = keep ? K.R[a] : poses[12 * c + a];
+        for (int a = 0; a < 3; ++a) K.tp[a] = keep ? K.t[a] : poses[12 * c + 9 + a];
+        K.type = types[c];
+        for (int a = 0; a < 9; ++a) K.R[a] = poses[12 * c + a];
+        for (int a = 0; a < 3; ++a) { K.t[a] = poses[12 * c + 9 + a]; K.h[a] = std::fmax(dims[3 * c + a], 0.0f); }
+    }
+    colCount_[b] = count;
+    colMargin_[b] = (std::isfinite(margin) && margin > 0.0f) ? margin : 0.0f;
+    colFriction_[b] = (std::isfinite(friction) && friction > 0.0f) ? friction : 0.0f;
+    colDirty_ = true;
+}
+
+void Engine::setColliderArray(int b, const ColliderDev* cols, int count, float margin, float friction)
+{
+    count = std::max(0, std::min(count, kMaxColliders));
+    for (int c = 0; c < count; ++c) colHost_[(size_t)b * kMaxColliders + c] = cols[c];
+    colCount_[b] = count;
+    colMargin_[b] = margin;
+    colFriction_[b] = friction;
+    colDirty_ = true;
+}
+
+// forward.cu:4502-4519
+int Engine::colliderStats(int b, int* hits, float* push, float* centroid, int maxCount)
+{
+    const int n = std::min(maxCount, colCount_[b]);
+    if (n <= 0 || !colStatsValid_) return 0;
+    std::vector<int> h(n);
+    std::vector<float> acc((size_t)kColAcc * n);
+    const size_t c0 = (size_t)b * kMaxColliders;
+    if (hipMemcpyAsync(h.data(), colHits_.p + c0, sizeof(int) * n, hipMemcpyDeviceToHost, stream_) != hipSuccess) return 0;
+    if (hipMemcpyAsync(acc.data(), colPush_.p + c0 * kColAcc, sizeof(float) * kColAcc * n, hipMemcpyDeviceToHost,
+                       stream_) != hipSuccess)
+        return 0;
+    if (hipStreamSynchronize(stream_) != hipSuccess) return 0;
+    for (int c = 0; c < n; ++c) {
+        if (hits) hits[c] = h[c];
+        const float* a = &acc[(size_t)kColAcc * c];
+        for (int k = 0; k < 3; ++k) {
+            if (push) push[3 * c + k] = a[k];
+            if (centroid)
+                centroid[3 * c + k] = (a[6] > 0.0f) ? a[3 + k] / a[6] : colHost_[c0 + c].t[k];
+        }
+    }
+    return n;
+}
+
+// setAttachedParticles (forward.cu:6759-6788)
+bool Engine::attach(int b, int count, const int* idx, const float* pos, float weight)
+{
+    const int nb = bodyBegin_[b], nn = bodyCount_[b];
+    if (attActive_[b])
+        (void)hipMemcpyAsync(invMass_.p + nb, invMassSaved_.p + nb, sizeof(float) * nn, hipMemcpyDeviceToDevice, stream_);
+    if (count <= 0 || !idx || !pos) {
+        attActive_[b] = false;
+        return hipStreamSynchronize(stream_) == hipSuccess;
+    }
+    if (!attActive_[b])
+        (void)hipMemcpyAsync(invMassSaved_.p + nb, invMass_.p + nb, sizeof(float) * nn, hipMemcpyDeviceToDevice, stream_);
+    launch_fill_float(stream_, nn, objW_.p + nb, 1.0f);
+    if ((size_t)count > attIdx_.n) {
+        attIdx_.alloc(count);
+        attPos_.alloc((size_t)3 * count);
+    }
+    (void)hipMemcpyAsync(attIdx_.p, idx, sizeof(int) * count, hipMemcpyHostToDevice, stream_);
+    (void)hipMemcpyAsync(attPos_.p, pos, sizeof(float) * 3 * count, hipMemcpyHostToDevice, stream_);
+    launch_attach(stream_, count, attIdx_.p, attPos_.p, nb, nb + nn, x_.p, v_.p, invMass_.p, objW_.p,
+                  std::fmax(weight, 1.0f));
+    attActive_[b] = true;
+    return hipStreamSynchronize(stream_) == hipSuccess;
+}
+
+void Engine::addRigidVelocity(int b, const float* dv, const float* dw, const float* c)
+{
+    const int nb = bodyBegin_[b];
+    launch_rigid_velocity(stream_, nb, nb + bodyCount_[b], x_.p, v_.p, invMass_.p, make_float3(dv[0], dv[1], dv[2]),
+                          make_float3(dw[0], dw[1], dw[2]), make_float3(c[0], c[1], c[2]));
+    (void)hipStreamSynchronize(stream_);
+}
+
+// squashReset (forward.cu:4411-4427)
+void Engine::resetToRest(int b)
+{
+    const int nb = bodyBegin_[b], nn = bodyCount_[b];
+    (void)hipMemcpyAsync(x_.p + nb, rest_.p + nb, sizeof(float3) * nn, hipMemcpyDeviceToDevice, stream_);
+    (void)hipMemsetAsync(v_.p + nb, 0, sizeof(float3) * nn, stream_);
+    launch_fill_float(stream_, nn, invMass_.p + nb, 1.0f);
+    (void)hipStreamSynchronize(stream_);
+    PressState& ps = press_[b];
+    ps.active = false; ps.curDisp = 0.0f; ps.ckptIdx = 0; ps.dwell = 0;
+}
+
+void Engine::releaseInvMass(int b)
+{
+    launch_fill_float(stream_, bodyCount_[b], invMass_.p + bodyBegin_[b], 1.0f);
+    (void)hipStreamSynchronize(stream_);
+}
+
+// groundLaunch (forward.cu:4551-4610): rigid tilt about the rest centre of
+// mass plus a rigid velocity field, computed on the host like the reference.
+void Engine::launchFromRest(int b, const float* lin, const float* ang, const float* tilt)
+{
+    const BodyInput& in = *inputs_[b];
+    const int n = in.n;
+    double cx = 0, cy = 0, cz = 0;
+    for (int i = 0; i < n; ++i) { cx += in.pos[3 * i]; cy += in.pos[3 * i + 1]; cz += in.pos[3 * i + 2]; }
+    const float comX = float(cx / n), comY = float(cy / n), comZ = float(cz / n);
+    const float ax = tilt ? tilt[0] : 0.0f, ay = tilt ? tilt[1] : 0.0f, az = tilt ? tilt[2] : 0.0f;
+    const float th = std::sqrt(ax * ax + ay * ay + az * az);
+    float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (th > 1e-8f) {
+        const float kx = ax / th, ky = ay / th, kz = az / th;
+        const float s = std::sin(th), c = std::cos(th), t = 1.0f - c;
+        R[0] = c + t * kx * kx;      R[1] = t * kx * ky - s * kz; R[2] = t * kx * kz + s * ky;
+        R[3] = t * ky * kx + s * kz; R[4] = c + t * ky * ky;      R[5] = t * ky * kz - s * kx;
+        R[6] = t * kz * kx - s * ky; R[7] = t * kz * ky + s * kx; R[8] = c + t * kz * kz;
+    }
+    const float vx = lin ? lin[0] : 0.0f, vy = lin ? lin[1] : 0.0f, vz = lin ? lin[2] : 0.0f;
+    const float wx = ang ? ang[0] : 0.0f, wy = ang ? ang[1] : 0.0f, wz = ang ? ang[2] : 0.0f;
+    std::vector<float3> hp(n), hv(n);
+    for (int i = 0; i < n; ++i) {
+        const float px = in.pos[3 * i] - comX, py = in.pos[3 * i + 1] - comY, pz = in.pos[3 * i + 2] - comZ;
+        const float rx = R[0] * px + R[1] * py + R[2] * pz;
+        const float ry = R[3] * px + R[4] * py + R[5] * pz;
+        const float rz = R[6] * px + R[7] * py + R[8] * pz;
+        hp[i] = make_float3(comX + rx, comY + ry, comZ + rz);
+        hv[i] = make_float3(vx + (wy * rz - wz * ry), vy + (wz * rx - wx * rz), vz + (wx * ry - wy * rx));
+    }
+    const int nb = bodyBegin_[b];
+    (void)hipMemcpyAsync(x_.p + nb, hp.data(), sizeof(float3) * n, hipMemcpyHostToDevice, stream_);
+    (void)hipMemcpyAsync(v_.p + nb, hv.data(), sizeof(float3) * n, hipMemcpyHostToDevice, stream_);
+    launch_fill_float(stream_, n, invMass_.p + nb, 1.0f);
+    (void)hipStreamSynchronize(stream_);
+    PressState& ps = press_[b];
+    ps.active = false; ps.curDisp = 0.0f; ps.ckptIdx = 0; ps.dwell = 0;
+}
+
+// Press ramp / checkpoint dwell of one step (forward.cu:12095-12112).
+void Engine::pressAdvance(int b)
+{
+    PressState& ps = press_[b];
+    if (!ps.active) return;
+    if (ps.autoLog && ps.dwell > 0) {
+        if (--ps.dwell == 0) ++ps.ckptIdx;
+    } else {
+        ps.curDisp = std::fmin(ps.curDisp + ps.ramp * set_[b].dt, ps.maxDisp);
+        if (ps.autoLog && ps.ckptIdx < 4) {
+            static const float ck[4] = {0.25f, 0.50f, 0.75f, 1.00f};
+            const float target = ck[ps.ckptIdx] * ps.maxDisp;
+            if (ps.curDisp >= target) { ps.curDisp = target; ps.dwell = ps.dwellFrames; }
+        }
+    }
+}
+
+void Engine::fillBodyDev(int b, BodyDev& d) const
+{
+    const Settings& s = set_[b];
+    const PressState& ps = press_[b];
+    std::memset(&d, 0, sizeof(d));
+    d.nodeBegin = bodyBegin_[b];
+    d.nodeEnd = bodyBegin_[b] + bodyCount_[b];
+    d.distanceOn = s.useDistance ? 1 : 0;
+    d.iters = s.iters;
+    d.dt = s.dt;
+    d.underRelax = s.underRelax;
+    d.velDamping = s.velDamping;
+    d.invMassScale = s.invMassScale;
+    d.distCompliance = s.distCompliance;
+    d.stiffnessScale = s.stiffnessScale;
+    // stepPhysicsOnly (forward.cu:13258-13262): gravity along -up, off during press
+    const float g = (s.groundGravity > 0.0f && !ps.active) ? s.groundGravity : 0.0f;
+    d.grav[0] = g > 0.0f ? -g * s.groundN[0] : 0.0f;
+    d.grav[1] = g > 0.0f ? -g * s.groundN[1] : 0.0f;
+    d.grav[2] = g > 0.0f ? -g * s.groundN[2] : 0.0f;
+    d.groundOn = (s.groundEnabled && !ps.active) ? 1 : 0;
+    for (int k = 0; k < 3; ++k) d.gn[k] = s.groundN[k];
+    d.planeD = s.groundHeight + s.groundRadius;
+    d.slop = s.groundSlop;
+    d.friction = s.groundFriction;
+    d.restitution = s.groundRestitution;
+    d.restingSpeed = 2.0f * s.groundGravity * s.dt;
+    d.colBegin = b * kMaxColliders;
+    d.colCount = colCount_[b];
+    d.colMargin = colMargin_[b];
+    d.colFriction = colFriction_[b];
+    d.pressOn = ps.active ? 1 : 0;
+    d.pressAxis = ps.axis;
+    d.pressLo = ps.lo + (ps.fromLo ? ps.curDisp : 0.0f);
+    d.pressHi = ps.hi - (ps.fromLo ? 0.0f : ps.curDisp);
+    d.contactColliders = 0;
+    d.contactGround = 0;
+}
+
+bool Engine::uploadParams(std::string& err)
+{
+    const int B = numBodies();
+    for (int b = 0; b < B; ++b) fillBodyDev(b, hostBodies_[b]);
+    bool changed = (int)lastBodies_.size() != B ||
+                   std::memcmp(lastBodies_.data(), hostBodies_, sizeof(BodyDev) * B) != 0;
+    if (changed) {
+        XPBD_HIP(hipMemcpyAsync(bodies_.p, hostBodies_, sizeof(BodyDev) * B, hipMemcpyHostToDevice, stream_));
+        lastBodies_.assign(hostBodies_, hostBodies_ + B);
+    }
+    if (colDirty_) {
+        std::memcpy(hostCols_, colHost_.data(), sizeof(ColliderDev) * colHost_.size());
+        XPBD_HIP(hipMemcpyAsync(cols_.p, hostCols_, sizeof(ColliderDev) * colHost_.size(), hipMemcpyHostToDevice, stream_));
+        colDirty_ = false;
+    }
+    // alpha tilde depends on (dt, compliance, stiffness scale) of each body
+    std::vector<float> key(3 * (size_t)B);
+    for (int b = 0; b < B; ++b) {
+        key[3 * b] = set_[b].dt;
+        key[3 * b + 1] = set_[b].distCompliance;
+        key[3 * b + 2] = set_[b].stiffnessScale;
+    }
+    if (key != alphaKey_) {
+        NodeArgs na{nPad_, tileBody_.p, bodies_.p};
+        launch_distance_alpha(stream_, na, rowOff_.p, nstiff_.p, nalpha_.p);
+        alphaKey_ = key;
+    }
+    return true;
+}
+
+// Everything that changes the launch sequence (not the parameter values).
+std::vector<int> Engine::planSignature() const
+{
+    const int B = numBodies();
+    int iters = set_[0].iters;
+    int anyDist = 0, anyCol = 0, anyGround = 0, anyPress = 0;
+    for (int b = 0; b < B; ++b) {
+        anyDist |= set_[b].useDistance ? 1 : 0;
+        anyCol |= colCount_[b] > 0 ? 1 : 0;
+        anyGround |= set_[b].groundEnabled ? 1 : 0;
+        anyPress |= press_[b].active ? 1 : 0;
+    }
+    return {iters, anyDist, anyCol, anyGround, anyPress, nPad_};
+}
+
+// Event pairs around kernel launches (profile mode only).
+struct Engine::Timer {
+    hipStream_t s;
+    std::vector<std::string> label;
+    std::vector<hipEvent_t> ev;
+    void begin(const char* l)
+    {
+        hipEvent_t a;
+        (void)hipEventCreate(&a);
+        (void)hipEventRecord(a, s);
+        label.emplace_back(l);
+        ev.push_back(a);
+    }
+    void end()
+    {
+        hipEvent_t b;
+        (void)hipEventCreate(&b);
+        (void)hipEventRecord(b, s);
+        ev.push_back(b);
+    }
+    ~Timer()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+#define XPBD_T_BEGIN(l) do { if (t) t->begin(l); } while (0)
+#define XPBD_T_END() do { if (t) t->end(); } while (0)
+
+// Record the kernels of one substep on stream_ (under graph capture when t is
+// null, eagerly with timing events otherwise).
+void Engine::recordSubstep(Timer* t)
+{
+    const std::vector<int> sig = planSignature();
+    const int iters = sig[0];
+    const bool anyDist = sig[1], anyCol = sig[2], anyGround = sig[3], anyPress = sig[4];
+    const NodeArgs na{nPad_, tileBody_.p, bodies_.p};
+    EpiArgs ea{x_.p, v_.p, cols_.p, colHits_.p, colPush_.p, nullptr, nullptr, nullptr};
+
+    XPBD_T_BEGIN("predict");
+    launch_predict(stream_, na, x_.p, v_.p, invMass_.p, p0_.p);
+    XPBD_T_END();
+    if (anyCol) {
+        const int B = numBodies();
+        XPBD_T_BEGIN("collider_stats_clear");
+        launch_fill_int(stream_, B * kMaxColliders, colHits_.p, 0);
+        launch_fill_float(stream_, B * kMaxColliders * kColAcc, colPush_.p, 0.0f);
+        XPBD_T_END();
+    }
+    float4* pin = p0_.p;
+    float4* pout = p1_.p;
+    for (int it = 0; it < iters; ++it) {
+        const bool last = it == iters - 1;
+        unsigned epi = 0;
+        if (anyPress) epi |= kEpiPress;
+        if (anyCol) epi |= kEpiColliders | (last ? kEpiColliderHits : 0u);
+        if (anyGround) epi |= kEpiGround;
+        if (last) epi |= kEpiVelocity;
+        if (anyDist) {
+            XPBD_T_BEGIN(it == 0 ? (last ? "distance_jacobi_only" : "distance_jacobi_first")
+                                 : (last ? "distance_jacobi_last" : "distance_jacobi_mid"));
+            launch_distance_jacobi(stream_, na, it == 0, last, pin, pout, rowOff_.p, nbr_.p, nrest_.p, nalpha_.p,
+                                   lam_.p, epi, ea);
+            XPBD_T_END();
+            std::swap(pin, pout);
+        } else if (epi) {
+            XPBD_T_BEGIN("epilogue");
+            launch_epilogue(stream_, na, pin, epi, ea);
+            XPBD_T_END();
+        }
+    }
+}
+
+bool Engine::rebuildGraph(std::string& err)
+{
+    if (graphExec_) { (void)hipGraphExecDestroy(graphExec_); graphExec_ = nullptr; }
+    if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+    XPBD_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    recordSubstep(nullptr);
+    hipError_t ce = hipStreamEndCapture(stream_, &graph_);
+    if (ce != hipSuccess) { err = std::string("hipStreamEndCapture: ") + hipGetErrorString(ce); return false; }
+    XPBD_HIP(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0));
+    graphSig_ = planSignature();
+    return true;
+}
+
+bool Engine::step(int substeps, std::string& err)
+{
+    for (int s = 0; s < substeps; ++s) {
+        for (int b = 0; b < numBodies(); ++b) pressAdvance(b);
+        if (!uploadParams(err)) return false;
+        if (!graphExec_ || planSignature() != graphSig_) {
+            if (!rebuildGraph(err)) return false;
+        }
+        XPBD_HIP(hipGraphLaunch(graphExec_, stream_));
+    }
+    colStatsValid_ = true;
+    return true;
+}
+
+bool Engine::profile(int substeps, std::string& json, std::string& err)
+{
+    Timer t;
+    t.s = stream_;
+    for (int s = 0; s < substeps; ++s) {
+        for (int b = 0; b < numBodies(); ++b) pressAdvance(b);
+        if (!uploadParams(err)) return false;
+        recordSubstep(&t);
+    }
+    colStatsValid_ = true;
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    XPBD_HIP(hipGetLastError());
+    std::vector<std::string> keys;
+    std::vector<double> tot;
+    std::vector<int> cnt;
+    for (size_t k = 0; k < t.label.size(); ++k) {
+        float ms = 0.0f;
+        XPBD_HIP(hipEventElapsedTime(&ms, t.ev[2 * k], t.ev[2 * k + 1]));
+        size_t j = 0;
+        while (j < keys.size() && keys[j] != t.label[k]) ++j;
+        if (j == keys.size()) { keys.push_back(t.label[k]); tot.push_back(0.0); cnt.push_back(0); }
+        tot[j] += ms;
+        cnt[j] += 1;
+    }
+    json = "{";
+    for (size_t j = 0; j < keys.size(); ++j) {
+        char buf[256];
+        std::snprintf(buf, sizeof(buf), "%s\"%s\": [%.6f, %d]", j ? ", " : "", keys[j].c_str(), tot[j], cnt[j]);
+        json += buf;
+    }
+    json += "}";
+    return true;
+}
+
+bool Engine::sync(std::string& err)
+{
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    XPBD_HIP(hipGetLastError());
+    return true;
+}
+
+bool Engine::getState(int b, float* pos, float* vel)
+{
+    const int nb = bodyBegin_[b], n = bodyCount_[b];
+    if (pos && hipMemcpyAsync(pos, x_.p + nb, sizeof(float3) * n, hipMemcpyDeviceToHost, stream_) != hipSuccess) return false;
+    if (vel && hipMemcpyAsync(vel, v_.p + nb, sizeof(float3) * n, hipMemcpyDeviceToHost, stream_) != hipSuccess) return false;
+    return hipStreamSynchronize(stream_) == hipSuccess;
+}
+
+bool Engine::setState(int b, const float* pos, const float* vel)
+{
+    const int nb = bodyBegin_[b], n = bodyCount_[b];
+    if (pos && hipMemcpyAsync(x_.p + nb, pos, sizeof(float3) * n, hipMemcpyHostToDevice, stream_) != hipSuccess) return false;
+    if (vel && hipMemcpyAsync(v_.p + nb, vel, sizeof(float3) * n, hipMemcpyHostToDevice, stream_) != hipSuccess) return false;
+    return hipStreamSynchronize(stream_) == hipSuccess;
+}
+
+bool Engine::getInvMass(int b, float* w)
+{
+    const int nb = bodyBegin_[b], n = bodyCount_[b];
+    if (hipMemcpyAsync(w, invMass_.p + nb, sizeof(float) * n, hipMemcpyDeviceToHost, stream_) != hipSuccess) return false;
+    return hipStreamSynchronize(stream_) == hipSuccess;
+}
+
+}  // namespace xpbd

@@ -1,0 +1,165 @@
+// engine.h — host orchestration of the MI355X XPBD solver.
+//
+// An Engine owns the device state of one or more bodies (independent
+// simulated instances) laid out back to back in one padded node space, and
+// steps all of them with one launch sequence per substep.  The C ABI
+// singleton (capi.cpp) drives an Engine with one body; the batched API drives
+// one with many.  Every substep's launch sequence is captured once in a
+// hipGraph and replayed; per-body settings live in device memory so changing
+// them does not invalidate the graph.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "xpbd_device.h"
+
+namespace xpbd {
+
+// Solver settings of one body, with the reference defaults (forward.cu:41-612).
+struct Settings {
+    int iters = 20;
+    float dt = 1.0f / 60.0f, underRelax = 0.6f, velDamping = 0.59f;
+    float stiffnessScale = 1.0f, invMassScale = 1.0f, distCompliance = 1e-3f;
+    float shapeCompliance = 0.0f, shapeBlend = 0.1f, angleCompliance = 0.0f, angleBlend = 0.2f;
+    bool useDistance = true, useShape = false, useAngle = false, useVolume = false;
+    bool shapeRobust = true;
+    bool groundEnabled = false;
+    float groundN[3] = {0, 0, 1};
+    float groundHeight = 0, groundFriction = 0.5f, groundRestitution = 0.3f;
+    float groundRadius = 0, groundSlop = 0, groundGravity = 0;
+    bool contactColliders = false, contactGround = false;
+    float contactTau = 0.2f, contactCap = 0.0f;
+    float volCompliance = 1e-6f, volAniso = 0.05f;
+    int volRingK = 1, volMaxMembers = 64, volLeaderMinHop = 1, volMemberSelect = 1;
+    float objShapeStiffness = 0.0f;
+    int objMinComponent = 50;
+    bool objShapeGPU = true;
+    bool scEnabled = false, scWithinBody = false;
+    float scRadiusScale = 1.5f, scExcludeScale = 2.0f;
+};
+
+// One body's input graph in the reference's loadGraph form (local ids).
+struct BodyInput {
+    int n = 0;
+    std::vector<float> pos, scales, rots, opacity;
+    std::vector<int> rowOff, nbr;
+    std::vector<float> nrest, nstiff;
+};
+
+// Build the symmetric CSR of ChainMail::loadGraph (forward.cu:2702-2779).
+void build_input(BodyInput& in, int n, const float* pos, const float* scales, const float* rots, const float* opacity,
+                 int e, const int* edges, const float* rest, const float* stiff);
+
+// Press-plate state of one body (forward.cu:535-575 squash globals, press mode).
+struct PressState {
+    bool active = false, fromLo = false, autoLog = true;
+    int axis = 1, ckptIdx = 0, dwell = 0, dwellFrames = 90;
+    float ramp = 0.15f, maxDisp = 0.30f, curDisp = 0.0f, lo = 0.0f, hi = 0.0f;
+};
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count);
+    void free();
+    ~DevBuf() { free(); }
+};
+
+class Engine {
+public:
+    Engine();
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    // Allocate and upload: x = rest, v = 0, invMass = 1 for every body.
+    bool build(const std::vector<const BodyInput*>& inputs, std::string& err);
+
+    int numBodies() const { return (int)bodyBegin_.size(); }
+    int nodeBegin(int b) const { return bodyBegin_[b]; }
+    int nodeCount(int b) const { return bodyCount_[b]; }
+    int padNodes() const { return nPad_; }
+    long long directedEntries() const { return m_; }
+
+    Settings& settings(int b) { return set_[b]; }
+    PressState& press(int b) { return press_[b]; }
+
+    void setColliders(int b, int count, const int* types, const float* poses, const float* dims, float margin,
+                      float friction);
+    // Install a collider list whose previous poses were already tracked by the caller.
+    void setColliderArray(int b, const ColliderDev* cols, int count, float margin, float friction);
+    int colliderStats(int b, int* hits, float* push, float* centroid, int maxCount);
+    bool attach(int b, int count, const int* idx, const float* pos, float weight);
+    void addRigidVelocity(int b, const float* dv, const float* dw, const float* c);
+    void resetToRest(int b);
+    void launchFromRest(int b, const float* lin, const float* ang, const float* tilt);
+    void releaseInvMass(int b);   // invMass = 1 on the body (squashStop)
+
+    // Advance every body by `substeps` substeps (one graph replay each).
+    bool step(int substeps, std::string& err);
+    bool sync(std::string& err);
+    // Run `substeps` substeps eagerly with HIP events around every kernel on the
+    // engine stream; returns JSON {"label": [total_ms, launches], ...}.
+    bool profile(int substeps, std::string& json, std::string& err);
+
+    float3* positionsDevice() { return x_.p; }
+    hipStream_t stream() const { return stream_; }
+    bool getState(int b, float* pos, float* vel);
+    bool setState(int b, const float* pos, const float* vel);
+    bool getInvMass(int b, float* invMass);
+    const BodyInput* input(int b) const { return inputs_[b]; }
+
+private:
+    void pressAdvance(int b);
+    void fillBodyDev(int b, BodyDev& d) const;
+    bool uploadParams(std::string& err);
+    std::vector<int> planSignature() const;
+    struct Timer;
+    void recordSubstep(Timer* t);
+    bool rebuildGraph(std::string& err);
+
+    hipStream_t stream_ = nullptr;
+    int nPad_ = 0;
+    long long m_ = 0;
+    std::vector<int> bodyBegin_, bodyCount_;
+    std::vector<const BodyInput*> inputs_;
+    std::vector<Settings> set_;
+    std::vector<PressState> press_;
+    std::vector<bool> attActive_;
+
+    // colliders (host mirror keeps the previous pose for friction)
+    std::vector<ColliderDev> colHost_;
+    std::vector<int> colCount_;
+    std::vector<float> colMargin_, colFriction_;
+    bool colDirty_ = true;
+    bool colStatsValid_ = false;
+
+    // device state
+    DevBuf<float3> x_, v_, rest_;
+    DevBuf<float4> p0_, p1_;
+    DevBuf<float> invMass_, invMassSaved_, objW_;
+    DevBuf<int> rowOff_, nbr_;
+    DevBuf<float> nrest_, nstiff_, nalpha_, lam_;
+    DevBuf<int> tileBody_;
+    DevBuf<BodyDev> bodies_;
+    DevBuf<ColliderDev> cols_;
+    DevBuf<int> colHits_, attIdx_;
+    DevBuf<float> colPush_, attPos_;
+
+    // parameter staging
+    BodyDev* hostBodies_ = nullptr;   // pinned
+    ColliderDev* hostCols_ = nullptr; // pinned
+    std::vector<BodyDev> lastBodies_;
+    std::vector<float> alphaKey_;
+
+    // graph
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graphExec_ = nullptr;
+    std::vector<int> graphSig_;
+};
+
+}  // namespace xpbd

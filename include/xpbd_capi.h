@@ -1,0 +1,156 @@
+/*
+ * xpbd_capi.h — C ABI of libxpbd_isaac.so, the MI355X XPBD Gaussian-graph solver.
+ *
+ * Part 1 is the drop-in surface: the 29 symbols the reference caller binds with
+ * ctypes (src/xpbd_bridge.py:58-112), with the reference's argument meaning,
+ * return codes (0 ok / -1 error, message in xpbd_last_error) and call-order
+ * rules.  Each declaration cites the reference entry point it replaces in
+ * native/xpbd_capi.cu.  State is one library-global instance, as in the
+ * reference (native/xpbd_capi.cu:6, 17-29); solver settings persist across
+ * xpbd_create / xpbd_destroy like the reference's forward.cu globals.
+ *
+ * Part 2 (xpbd_ext_*) has no counterpart in the reference: state access for
+ * parity checks, the explicit-cluster / hash-override recipe API, and
+ * diagnostics.  Part 3 (xpbd_batch_*) is the batched many-object API.
+ *
+ * All arrays are C-contiguous, little-endian, float32 / int32, in the caller's
+ * units and ordering.  No function here takes a device pointer except where
+ * stated.
+ */
+#ifndef XPBD_CAPI_H
+#define XPBD_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * Part 1: drop-in symbols (native/xpbd_capi.cu:66-310)
+ * ------------------------------------------------------------------------- */
+
+/* native/xpbd_capi.cu:66 — last error string, owned by the library. */
+const char* xpbd_last_error(void);
+
+/* native/xpbd_capi.cu:71-102 — n Gaussians, e undirected edges (i,j).
+ * pos[n*3], scales[n*3] (activated), rots[n*4] (w,x,y,z), opacity[n],
+ * edges[e*2], rest[e], stiff[e].  Frees any previous instance.  Invalid or
+ * self edges are dropped (forward.cu:2729-2731). */
+int xpbd_create(int n, const float* pos, const float* scales, const float* rots, const float* opacity,
+                int e, const int* edges, const float* rest, const float* stiff);
+
+/* native/xpbd_capi.cu:104-109 — clears colliders and attachments, frees. */
+void xpbd_destroy(void);
+
+/* native/xpbd_capi.cu:111 */
+int xpbd_count(void);
+
+/* native/xpbd_capi.cu:113-119 -> forward.cu:4896-4921 (clamps as there). */
+void xpbd_set_solver(int iters, float dt, float underRelax, float velDamping, float stiffnessScale,
+                     float invMassScale, float distCompliance, float shapeCompliance, float shapeBlend,
+                     float angleCompliance, float angleBlend);
+
+/* native/xpbd_capi.cu:121-127 */
+void xpbd_set_constraints(int distance, int shape, int angle, int volume);
+
+/* native/xpbd_capi.cu:130-137 -> forward.cu:3849-3853, 3934-3944, 3953-3965 */
+void xpbd_set_volume(float compliance, int ringK, int maxMembers, int leaderMinHop);
+
+/* native/xpbd_capi.cu:140-145 -> forward.cu:4430-4446 */
+void xpbd_set_ground(int enabled, float ux, float uy, float uz, float height, float friction,
+                     float restitution, float contactRadius, float contactSlop, float gravity);
+
+/* native/xpbd_capi.cu:148 -> forward.cu:4464-4476 (bodyContact off, minComponent 50) */
+void xpbd_set_object_shape(float stiffness);
+
+/* native/xpbd_capi.cu:151 -> forward.cu:4521 */
+void xpbd_set_object_shape_gpu(int enabled);
+
+/* native/xpbd_capi.cu:154-158 -> forward.cu:4478-4500.  types: 0 sphere,
+ * 1 box, 2 capsule; poses[count*12] = R (row-major 3x3, local->input) + t;
+ * dims[count*3].  At most 32 colliders are used. */
+void xpbd_set_colliders(int count, const int* types, const float* poses, const float* dims, float margin,
+                        float friction);
+
+/* native/xpbd_capi.cu:161-164 -> forward.cu:4502-4519 */
+int xpbd_collider_stats(int* hits, float* push, float* centroid, int maxCount);
+
+/* native/xpbd_capi.cu:167-172 -> forward.cu:6790-6796 */
+int xpbd_add_rigid_velocity(const float* dv, const float* dw, const float* center);
+
+/* native/xpbd_capi.cu:179-191 -> forward.cu:6733-6757 */
+int xpbd_set_contact_shape(int enabled, float tau, float radiusCap);
+
+/* native/xpbd_capi.cu:195-200 -> forward.cu:6759-6788 */
+int xpbd_set_attached(int count, const int* idx, const float* pos, float weight);
+
+/* native/xpbd_capi.cu:203-206 -> forward.cu:4532-4540 */
+void xpbd_set_self_collision(int enabled, float radiusScale, float excludeScale, int withinBody);
+
+/* native/xpbd_capi.cu:209-212: out[0] candidates, out[1] active, out[2] overflow */
+void xpbd_self_collision_stats(int* out);
+
+/* native/xpbd_capi.cu:215 -> forward.cu:3738 */
+void xpbd_set_shape_robust(int enabled);
+
+/* native/xpbd_capi.cu:218-225 — one substep; returns after device completion. */
+int xpbd_step(void);
+
+/* native/xpbd_capi.cu:228-232 — device pointer to float3[n] committed positions;
+ * 0 before the first step. */
+unsigned long long xpbd_positions_device(void);
+
+/* native/xpbd_capi.cu:235-243 */
+int xpbd_positions_host(float* out, int n);
+
+/* native/xpbd_capi.cu:248-260 — returns the deformed count or -1. */
+int xpbd_compute_shapes(float deformEps, float* hostScales, float* hostQuats, int n);
+
+/* native/xpbd_capi.cu:263-269 */
+int xpbd_reset(void);
+
+/* native/xpbd_capi.cu:272-276 -> forward.cu:4551-4610 */
+int xpbd_launch(const float* lin, const float* ang, const float* tilt);
+
+/* native/xpbd_capi.cu:281-288 */
+int xpbd_press_start(int axis, float rampPerSec, float maxDispPct, int fromLo);
+
+/* native/xpbd_capi.cu:291 */
+void xpbd_press_autolog(int enabled);
+
+/* native/xpbd_capi.cu:294 */
+void xpbd_press_stop(void);
+
+/* native/xpbd_capi.cu:297-303 */
+int xpbd_press_state(float* lo, float* hi, float* curDisp, float* maxDisp);
+
+/* native/xpbd_capi.cu:306-310 */
+void xpbd_volume_stats(int enable, float* jvw, float* jvwStd);
+
+/* ---------------------------------------------------------------------------
+ * Part 2: extensions (no counterpart in the reference)
+ * ------------------------------------------------------------------------- */
+
+/* Copy committed positions x[n*3] and velocities v[n*3] of the instance
+ * (after the first step).  Either pointer may be null. */
+int xpbd_ext_get_state(float* pos, float* vel, int n);
+
+/* Overwrite committed positions and velocities (after the first step), so two
+ * solvers can be stepped from an identical state.  Either pointer may be null. */
+int xpbd_ext_set_state(const float* pos, const float* vel, int n);
+
+/* Inverse masses invMass[n] (after the first step). */
+int xpbd_ext_get_inv_mass(float* invMass, int n);
+
+/* Step `substeps` substeps eagerly (no graph) with HIP events around every
+ * kernel on the library's stream; writes JSON {"kernel": [total_ms, launches]}
+ * into json[len].  The state advances exactly as with xpbd_step. */
+int xpbd_ext_profile(int substeps, char* json, int len);
+
+/* Library identity: "hip-gfx950" for the GPU library, "oracle-cpu" for the oracle. */
+const char* xpbd_ext_backend(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XPBD_CAPI_H */

@@ -1,0 +1,40 @@
+"""Shared fixtures: library paths, the GPU marker, and scene caching."""
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "3dgs-isaac-sim-xpbd_amd")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+
+PRODUCT_LIB = os.path.join(PKG, "libxpbd_isaac.so")
+ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle_xpbd.so")
+HEADER = os.path.join(ROOT, "include", "xpbd_capi.h")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP library")
+
+
+def gpu_available():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def oracle_path():
+    if not os.path.isfile(ORACLE_LIB):
+        pytest.fail("oracle library missing: run 3dgs-isaac-sim-xpbd_amd/build_lib.py oracle")
+    return ORACLE_LIB
+
+
+@pytest.fixture(scope="session")
+def product_path():
+    if not os.path.isfile(PRODUCT_LIB):
+        pytest.fail("libxpbd_isaac.so missing: run 3dgs-isaac-sim-xpbd_amd/build_lib.py product")
+    return PRODUCT_LIB
