@@ -30,6 +30,10 @@ struct Global {
     float jvw = 1.0f, jvwStd = 0.0f;
     std::unique_ptr<BodyInput> input;
     std::unique_ptr<Engine> engine;
+    // explicit volume clusters, applied when the engine is built
+    bool volExplicit = false;
+    std::vector<int> volOff, volMembers;
+    int volMinMembers = 4;
     int n = 0;
     bool stepped = false;
     bool contactOn = false;
@@ -54,6 +58,9 @@ void free_all()
     Global& g = G();
     g.engine.reset();
     g.input.reset();
+    g.volExplicit = false;
+    g.volOff.clear();
+    g.volMembers.clear();
     g.n = 0;
     g.stepped = false;
     g.contactOn = false;
@@ -259,6 +266,11 @@ int xpbd_step(void)
             return fail("xpbd_step: " + err);
         }
         push_colliders();
+        if (g.volExplicit &&
+            !g.engine->setVolumeClusters(0, g.volOff.data(), g.volMembers.data(), g.volMinMembers, err)) {
+            g.engine.reset();
+            return fail("xpbd_step: " + err);
+        }
     }
     Engine& E = *g.engine;
     E.settings(0) = g.set;
@@ -380,12 +392,60 @@ int xpbd_press_state(float* lo, float* hi, float* curDisp, float* maxDisp)
     return ps.active ? 1 : 0;
 }
 
+// J statistics of the last committed positions, evaluated on request instead
+// of after every step (forward.cu:12658-12736 computes them per step).
 void xpbd_volume_stats(int enable, float* jvw, float* jvwStd)
 {
     Global& g = G();
     g.volJStats = enable != 0;
+    if (g.volJStats && g.stepped) {
+        std::string err;
+        g.engine->settings(0) = g.set;
+        if (!g.engine->volumeStats(0, g.jvw, g.jvwStd, err)) fail("xpbd_volume_stats: " + err);
+    }
     if (jvw) *jvw = g.jvw;
     if (jvwStd) *jvwStd = g.jvwStd;
+}
+
+int xpbd_ext_set_volume_clusters(const int* off, const int* members, int minMembers)
+{
+    Global& g = G();
+    if (!g.input) return fail("xpbd_ext_set_volume_clusters: call xpbd_create first");
+    const int n = g.n;
+    if (!off) {
+        g.volExplicit = false;
+        g.volOff.clear();
+        g.volMembers.clear();
+    } else {
+        if (off[0] != 0) return fail("xpbd_ext_set_volume_clusters: off[0] must be 0");
+        for (int i = 0; i < n; ++i)
+            if (off[i + 1] < off[i]) return fail("xpbd_ext_set_volume_clusters: offsets must be non-decreasing");
+        for (int k = 0; k < off[n]; ++k)
+            if (members[k] < 0 || members[k] >= n) return fail("xpbd_ext_set_volume_clusters: member out of range");
+        g.volExplicit = true;
+        g.volOff.assign(off, off + n + 1);
+        g.volMembers.assign(members, members + off[n]);
+        g.volMinMembers = minMembers;
+    }
+    if (g.engine) {
+        std::string err;
+        if (!g.engine->setVolumeClusters(0, g.volExplicit ? g.volOff.data() : nullptr, g.volMembers.data(),
+                                         g.volMinMembers, err))
+            return fail("xpbd_ext_set_volume_clusters: " + err);
+    }
+    return 0;
+}
+
+int xpbd_ext_volume_topology(int* leaders, int* memberCount, int* revCount, int* matType, float* detRest, int n)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_ext_volume_topology: no step yet");
+    if (n != g.n) return fail("xpbd_ext_volume_topology: size mismatch");
+    std::string err;
+    g.engine->settings(0) = g.set;
+    if (!g.engine->volumeTopology(0, leaders, memberCount, revCount, matType, detRest, err))
+        return fail("xpbd_ext_volume_topology: " + err);
+    return 0;
 }
 
 int xpbd_ext_get_state(float* pos, float* vel, int n)

@@ -44,6 +44,10 @@ template struct DevBuf<float4>;
 template struct DevBuf<int>;
 template struct DevBuf<BodyDev>;
 template struct DevBuf<ColliderDev>;
+template struct DevBuf<float2>;
+
+// wave64-per-cluster volume solve from this mean member count up
+static const double kVolWaveMembers = 24.0;
 
 void build_input(BodyInput& in, int n, const float* pos, const float* scales, const float* rots, const float* opacity,
                  int e, const int* edges, const float* rest, const float* stiff)
@@ -121,6 +125,11 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     colFriction_.assign(B, 0.0f);
     colDirty_ = true;
     colStatsValid_ = false;
+    volTopo_.assign(B, VolumeTopo());
+    volDeviceValid_ = false;
+    volActive_ = 0;
+    detRest_.free(); Vrest_.free(); restLen_.free(); matType_.free(); lamVol_.free();
+    bbC_.free(); bbS_.free(); bbS2_.free(); volJ_.free();
 
     // host staging of the padded node space
     std::vector<float3> hx(nPad_, make_float3(0, 0, 0));
@@ -390,6 +399,8 @@ void Engine::fillBodyDev(int b, BodyDev& d) const
     d.pressHi = ps.hi - (ps.fromLo ? 0.0f : ps.curDisp);
     d.contactColliders = 0;
     d.contactGround = 0;
+    d.volumeOn = s.useVolume ? 1 : 0;
+    d.volCompliance = s.volCompliance;
 }
 
 bool Engine::uploadParams(std::string& err)
@@ -427,14 +438,16 @@ std::vector<int> Engine::planSignature() const
 {
     const int B = numBodies();
     int iters = set_[0].iters;
-    int anyDist = 0, anyCol = 0, anyGround = 0, anyPress = 0;
+    int anyDist = 0, anyCol = 0, anyGround = 0, anyPress = 0, anyVol = 0;
     for (int b = 0; b < B; ++b) {
         anyDist |= set_[b].useDistance ? 1 : 0;
         anyCol |= colCount_[b] > 0 ? 1 : 0;
         anyGround |= set_[b].groundEnabled ? 1 : 0;
         anyPress |= press_[b].active ? 1 : 0;
+        anyVol |= set_[b].useVolume ? 1 : 0;
     }
-    return {iters, anyDist, anyCol, anyGround, anyPress, nPad_};
+    const int volWave = (anyVol && volAvgMembers_ >= kVolWaveMembers) ? 1 : 0;
+    return {iters, anyDist, anyCol, anyGround, anyPress, nPad_, anyVol, volWave, anyVol ? volActive_ : 0};
 }
 
 // Event pairs around kernel launches (profile mode only).
@@ -495,14 +508,26 @@ void Engine::recordSubstep(Timer* t)
         if (anyCol) epi |= kEpiColliders | (last ? kEpiColliderHits : 0u);
         if (anyGround) epi |= kEpiGround;
         if (last) epi |= kEpiVelocity;
+        const bool volPass = sig[6] != 0;
         if (anyDist) {
             XPBD_T_BEGIN(it == 0 ? (last ? "distance_jacobi_only" : "distance_jacobi_first")
                                  : (last ? "distance_jacobi_last" : "distance_jacobi_mid"));
             launch_distance_jacobi(stream_, na, it == 0, last, pin, pout, rowOff_.p, nbr_.p, nrest_.p, nalpha_.p,
-                                   lam_.p, epi, ea);
+                                   lam_.p, volPass ? 0u : epi, ea);
             XPBD_T_END();
             std::swap(pin, pout);
-        } else if (epi) {
+        }
+        if (volPass) {
+            const VolumeArgs va = volArgs();
+            XPBD_T_BEGIN(sig[7] ? "volume_solve_wave" : "volume_solve_thread");
+            launch_vol_solve(stream_, na, sig[7] != 0, it == 0, last, pin, va);
+            XPBD_T_END();
+            XPBD_T_BEGIN("volume_apply");
+            launch_vol_apply(stream_, na, pin, pout, va, epi, ea);
+            XPBD_T_END();
+            std::swap(pin, pout);
+        }
+        if (!anyDist && !volPass && epi) {
             XPBD_T_BEGIN("epilogue");
             launch_epilogue(stream_, na, pin, epi, ea);
             XPBD_T_END();
@@ -523,11 +548,20 @@ bool Engine::rebuildGraph(std::string& err)
     return true;
 }
 
+bool Engine::prepare(std::string& err)
+{
+    for (int b = 0; b < numBodies(); ++b) pressAdvance(b);
+    if (!uploadParams(err)) return false;
+    bool anyVol = false;
+    for (int b = 0; b < numBodies(); ++b) anyVol |= set_[b].useVolume;
+    if (anyVol && !ensureVolume(false, err)) return false;
+    return true;
+}
+
 bool Engine::step(int substeps, std::string& err)
 {
     for (int s = 0; s < substeps; ++s) {
-        for (int b = 0; b < numBodies(); ++b) pressAdvance(b);
-        if (!uploadParams(err)) return false;
+        if (!prepare(err)) return false;
         if (!graphExec_ || planSignature() != graphSig_) {
             if (!rebuildGraph(err)) return false;
         }
@@ -542,8 +576,7 @@ bool Engine::profile(int substeps, std::string& json, std::string& err)
     Timer t;
     t.s = stream_;
     for (int s = 0; s < substeps; ++s) {
-        for (int b = 0; b < numBodies(); ++b) pressAdvance(b);
-        if (!uploadParams(err)) return false;
+        if (!prepare(err)) return false;
         recordSubstep(&t);
     }
     colStatsValid_ = true;

@@ -53,6 +53,21 @@ struct BodyInput {
 void build_input(BodyInput& in, int n, const float* pos, const float* scales, const float* rots, const float* opacity,
                  int e, const int* edges, const float* rest, const float* stiff);
 
+// Volume clusters of one body (local ids): leader i owns {i} + members
+// memIdx[memOff[i] .. memOff[i+1]); revOff/revIdx list the clusters containing
+// each node (own cluster first).
+struct VolumeTopo {
+    std::vector<char> isLeader;
+    std::vector<int> memOff, memIdx, revOff, revIdx;
+    int minMembers = 4;       // rest-precompute gate (forward.cu:7745)
+    int k = -1, cap = -1, r = -1;
+    bool explicitClusters = false;
+    void finish(int n);       // builds the reverse CSR from isLeader / members
+};
+
+// k-ring clusters by the reference rule (forward.cu:9814-10240).
+void build_volume_reference(const BodyInput& in, int k, int cap, int leaderR, VolumeTopo& t);
+
 // Press-plate state of one body (forward.cu:535-575 squash globals, press mode).
 struct PressState {
     bool active = false, fromLo = false, autoLog = true;
@@ -113,7 +128,17 @@ public:
     bool getInvMass(int b, float* invMass);
     const BodyInput* input(int b) const { return inputs_[b]; }
 
+    // Volume clusters: explicit per body (local ids, off == nullptr returns to
+    // the k-ring rule).  Statistics evaluate J on the committed positions.
+    bool setVolumeClusters(int b, const int* off, const int* members, int minMembers, std::string& err);
+    bool volumeStats(int b, float& jvw, float& jvwStd, std::string& err);
+    bool volumeTopology(int b, int* leaders, int* memberCount, int* revCount, int* matType, float* detRest,
+                        std::string& err);
+    double volumeAvgMembers() const { return volAvgMembers_; }
+
 private:
+    bool ensureVolume(bool needAll, std::string& err);
+    bool prepare(std::string& err);   // per-substep host work before launching
     void pressAdvance(int b);
     void fillBodyDev(int b, BodyDev& d) const;
     bool uploadParams(std::string& err);
@@ -155,6 +180,20 @@ private:
     ColliderDev* hostCols_ = nullptr; // pinned
     std::vector<BodyDev> lastBodies_;
     std::vector<float> alphaKey_;
+
+    // volume constraint
+    std::vector<VolumeTopo> volTopo_;
+    bool volDeviceValid_ = false;
+    int volActive_ = 0;
+    double volAvgMembers_ = 0.0;
+    float volAniso_ = 0.05f;
+    DevBuf<int> volOff_, volIdx_, revOff_, revIdx_, matType_, volActiveList_;
+    DevBuf<float> detRest_, Vrest_, restLen_, lamVol_, volJ_;
+    DevBuf<float4> bbC_, bbS_;
+    DevBuf<float2> bbS2_;
+    std::vector<int> hMatType_;
+    std::vector<float> hVrest_;
+    VolumeArgs volArgs() const;
 
     // graph
     hipGraph_t graph_ = nullptr;

@@ -1,0 +1,216 @@
+// engine_volume.cpp — volume-constraint topology management of the Engine:
+// per-body cluster construction (k-ring rule or explicit), upload into the
+// padded node space, rest precompute on the device, the active-leader list,
+// and J statistics on request.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "engine.h"
+
+namespace xpbd {
+
+#define XPBD_HIP(call)                                                                    \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            err = std::string(#call) + ": " + hipGetErrorString(e_);                      \
+            return false;                                                                 \
+        }                                                                                 \
+    } while (0)
+
+VolumeArgs Engine::volArgs() const
+{
+    VolumeArgs a;
+    a.nActive = volActive_;
+    a.active = volActiveList_.p;
+    a.volOff = volOff_.p;
+    a.volIdx = volIdx_.p;
+    a.revOff = revOff_.p;
+    a.revIdx = revIdx_.p;
+    a.detRest = detRest_.p;
+    a.Vrest = Vrest_.p;
+    a.restLen = restLen_.p;
+    a.matType = matType_.p;
+    a.lam = lamVol_.p;
+    a.bbC = bbC_.p;
+    a.bbS = bbS_.p;
+    a.bbS2 = bbS2_.p;
+    return a;
+}
+
+bool Engine::setVolumeClusters(int b, const int* off, const int* members, int minMembers, std::string& err)
+{
+    const int n = bodyCount_[b];
+    VolumeTopo t;
+    if (off) {
+        t.isLeader.assign(n, 0);
+        t.memOff.assign(n + 1, 0);
+        for (int i = 0; i < n; ++i) {
+            if (off[i + 1] < off[i]) { err = "volume clusters: offsets must be non-decreasing"; return false; }
+            t.memOff[i + 1] = t.memOff[i] + (off[i + 1] - off[i]);
+            t.isLeader[i] = off[i + 1] > off[i];
+        }
+        t.memIdx.resize(t.memOff[n]);
+        for (int i = 0; i < n; ++i)
+            for (int k = off[i]; k < off[i + 1]; ++k) {
+                const int j = members[k];
+                if (j < 0 || j >= n || j == i) { err = "volume clusters: member index out of range"; return false; }
+                t.memIdx[t.memOff[i] + (k - off[i])] = j;
+            }
+        t.minMembers = std::max(3, minMembers);
+        t.explicitClusters = true;
+        t.finish(n);
+    }
+    volTopo_[b] = std::move(t);
+    volDeviceValid_ = false;
+    return true;
+}
+
+// Make the device topology current.  needAll: build clusters for every body
+// (statistics), otherwise only for bodies with the volume constraint on.
+bool Engine::ensureVolume(bool needAll, std::string& err)
+{
+    const int B = numBodies();
+    bool changed = !volDeviceValid_;
+    for (int b = 0; b < B; ++b) {
+        const Settings& s = set_[b];
+        if (!needAll && !s.useVolume) continue;
+        VolumeTopo& t = volTopo_[b];
+        if (t.explicitClusters) continue;
+        if (!t.memOff.empty() && t.k == s.volRingK && t.cap == s.volMaxMembers && t.r == s.volLeaderMinHop) continue;
+        build_volume_reference(*inputs_[b], s.volRingK, s.volMaxMembers, s.volLeaderMinHop, t);
+        t.k = s.volRingK; t.cap = s.volMaxMembers; t.r = s.volLeaderMinHop;
+        changed = true;
+    }
+    if (!changed) return true;
+
+    // concatenate into the padded node space (bodies without clusters get empty rows)
+    std::vector<int> hOff(nPad_ + 1, 0), hRevOff(nPad_ + 1, 0), hIdx, hRev;
+    int minMembers = 4;
+    for (int b = 0; b < B; ++b) {
+        const VolumeTopo& t = volTopo_[b];
+        if (!t.memOff.empty()) minMembers = std::min(minMembers, t.minMembers);
+    }
+    for (int b = 0; b < B; ++b) {
+        const VolumeTopo& t = volTopo_[b];
+        const int nb = bodyBegin_[b], n = bodyCount_[b];
+        const int padEnd = (b + 1 < B) ? bodyBegin_[b + 1] : nPad_;
+        for (int i = 0; i < n; ++i) {
+            hOff[nb + i] = (int)hIdx.size();
+            hRevOff[nb + i] = (int)hRev.size();
+            if (t.memOff.empty()) continue;
+            for (int k = t.memOff[i]; k < t.memOff[i + 1]; ++k) hIdx.push_back(nb + t.memIdx[k]);
+            for (int k = t.revOff[i]; k < t.revOff[i + 1]; ++k) hRev.push_back(nb + t.revIdx[k]);
+        }
+        for (int i = nb + n; i <= padEnd && i <= nPad_; ++i) {
+            hOff[i] = (int)hIdx.size();
+            hRevOff[i] = (int)hRev.size();
+        }
+        if (t.explicitClusters && t.minMembers != minMembers) {
+            err = "volume clusters: bodies in one batch must share the member gate";
+            return false;
+        }
+    }
+    hOff[nPad_] = (int)hIdx.size();
+    hRevOff[nPad_] = (int)hRev.size();
+    volOff_.alloc(nPad_ + 1);
+    revOff_.alloc(nPad_ + 1);
+    volIdx_.alloc(std::max<size_t>(hIdx.size(), 1));
+    revIdx_.alloc(std::max<size_t>(hRev.size(), 1));
+    if (!detRest_.p) {
+        detRest_.alloc(nPad_); Vrest_.alloc(nPad_); restLen_.alloc(nPad_); matType_.alloc(nPad_);
+        lamVol_.alloc(nPad_); bbC_.alloc(nPad_); bbS_.alloc(nPad_); bbS2_.alloc(nPad_); volJ_.alloc(nPad_);
+    }
+    if (!volOff_.p || !revOff_.p || !volIdx_.p || !revIdx_.p || !detRest_.p || !Vrest_.p || !restLen_.p ||
+        !matType_.p || !lamVol_.p || !bbC_.p || !bbS_.p || !bbS2_.p || !volJ_.p) {
+        err = "volume: hipMalloc failed";
+        return false;
+    }
+    XPBD_HIP(hipMemcpyAsync(volOff_.p, hOff.data(), sizeof(int) * (nPad_ + 1), hipMemcpyHostToDevice, stream_));
+    XPBD_HIP(hipMemcpyAsync(revOff_.p, hRevOff.data(), sizeof(int) * (nPad_ + 1), hipMemcpyHostToDevice, stream_));
+    if (!hIdx.empty())
+        XPBD_HIP(hipMemcpyAsync(volIdx_.p, hIdx.data(), sizeof(int) * hIdx.size(), hipMemcpyHostToDevice, stream_));
+    if (!hRev.empty())
+        XPBD_HIP(hipMemcpyAsync(revIdx_.p, hRev.data(), sizeof(int) * hRev.size(), hipMemcpyHostToDevice, stream_));
+    XPBD_HIP(hipMemsetAsync(lamVol_.p, 0, sizeof(float) * nPad_, stream_));
+    XPBD_HIP(hipMemsetAsync(bbC_.p, 0, sizeof(float4) * nPad_, stream_));
+    // parameters of every body must be on the device before the precompute
+    std::string perr;
+    if (!uploadParams(perr)) { err = perr; return false; }
+    const NodeArgs na{nPad_, tileBody_.p, bodies_.p};
+    VolumeArgs va = volArgs();
+    launch_vol_precompute(stream_, na, rest_.p, rowOff_.p, nbr_.p, va, minMembers, volAniso_);
+    hMatType_.resize(nPad_);
+    hVrest_.resize(nPad_);
+    XPBD_HIP(hipMemcpyAsync(hMatType_.data(), matType_.p, sizeof(int) * nPad_, hipMemcpyDeviceToHost, stream_));
+    XPBD_HIP(hipMemcpyAsync(hVrest_.data(), Vrest_.p, sizeof(float) * nPad_, hipMemcpyDeviceToHost, stream_));
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    XPBD_HIP(hipGetLastError());
+    std::vector<int> act;
+    long long mem = 0;
+    for (int i = 0; i < nPad_; ++i)
+        if (hMatType_[i] == 0) {
+            act.push_back(i);
+            mem += hOff[i + 1] - hOff[i];
+        }
+    volActive_ = (int)act.size();
+    volAvgMembers_ = act.empty() ? 0.0 : (double)mem / (double)act.size();
+    volActiveList_.alloc(std::max<size_t>(act.size(), 1));
+    if (!act.empty())
+        XPBD_HIP(hipMemcpyAsync(volActiveList_.p, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, stream_));
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    volDeviceValid_ = true;
+    return true;
+}
+
+// Volume-weighted J of the committed positions (forward.cu:12658-12700).
+bool Engine::volumeStats(int b, float& jvw, float& jvwStd, std::string& err)
+{
+    if (!ensureVolume(true, err)) return false;
+    const NodeArgs na{nPad_, tileBody_.p, bodies_.p};
+    VolumeArgs va = volArgs();
+    launch_vol_J(stream_, na, x_.p, va, volJ_.p);
+    const int nb = bodyBegin_[b], n = bodyCount_[b];
+    std::vector<float> J(n);
+    XPBD_HIP(hipMemcpyAsync(J.data(), volJ_.p + nb, sizeof(float) * n, hipMemcpyDeviceToHost, stream_));
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    double wSum = 0.0, wJ = 0.0, wJ2 = 0.0;
+    for (int i = 0; i < n; ++i) {
+        if (hMatType_[nb + i] != 0) continue;
+        const float j = J[i];
+        if (!std::isfinite(j)) continue;
+        const double v = (double)hVrest_[nb + i];
+        if (v > 0.0 && std::isfinite(v)) { wSum += v; wJ += v * j; wJ2 += v * (double)j * j; }
+    }
+    if (wSum > 0.0) {
+        const double m = wJ / wSum;
+        jvw = (float)m;
+        jvwStd = (float)std::sqrt(std::fmax(0.0, wJ2 / wSum - m * m));
+    }
+    return true;
+}
+
+bool Engine::volumeTopology(int b, int* leaders, int* memberCount, int* revCount, int* matType, float* detRest,
+                            std::string& err)
+{
+    if (!ensureVolume(true, err)) return false;
+    const VolumeTopo& t = volTopo_[b];
+    const int nb = bodyBegin_[b], n = bodyCount_[b];
+    std::vector<float> dr(n);
+    if (detRest) {
+        XPBD_HIP(hipMemcpyAsync(dr.data(), detRest_.p + nb, sizeof(float) * n, hipMemcpyDeviceToHost, stream_));
+        XPBD_HIP(hipStreamSynchronize(stream_));
+    }
+    for (int i = 0; i < n; ++i) {
+        const bool has = !t.memOff.empty();
+        if (leaders) leaders[i] = has ? t.isLeader[i] : 0;
+        if (memberCount) memberCount[i] = has ? t.memOff[i + 1] - t.memOff[i] : 0;
+        if (revCount) revCount[i] = has ? t.revOff[i + 1] - t.revOff[i] : 0;
+        if (matType) matType[i] = hMatType_[nb + i];
+        if (detRest) detRest[i] = dr[i];
+    }
+    return true;
+}
+
+}  // namespace xpbd

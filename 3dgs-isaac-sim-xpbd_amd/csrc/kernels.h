@@ -25,6 +25,32 @@ struct EpiArgs {
     const float* rg;          // [N] ground support radius or null
 };
 
+// Volume-constraint buffers (node space; leaders own clusters).
+struct VolumeArgs {
+    int nActive;              // leaders with matType 0
+    const int* active;        // [nActive]
+    const int* volOff;        // [nPad + 1] members CSR by leader
+    const int* volIdx;
+    const int* revOff;        // [nPad + 1] clusters containing each node
+    const int* revIdx;
+    float* detRest;
+    float* Vrest;
+    float* restLen;
+    int* matType;
+    float* lam;               // [nPad] per leader
+    float4* bbC;              // [nPad] centroid + coef
+    float4* bbS;              // [nPad] S^-1 00 11 22 10
+    float2* bbS2;             // [nPad] S^-1 20 21
+};
+
+void launch_vol_precompute(hipStream_t s, const NodeArgs& na, const float3* rest, const int* rowOff, const int* nbr,
+                           const VolumeArgs& va, int minMembers, float aniso);
+void launch_vol_solve(hipStream_t s, const NodeArgs& na, bool wave, bool first, bool last, const float4* P,
+                      const VolumeArgs& va);
+void launch_vol_apply(hipStream_t s, const NodeArgs& na, const float4* Pin, float4* Pout, const VolumeArgs& va,
+                      unsigned epi, const EpiArgs& ea);
+void launch_vol_J(hipStream_t s, const NodeArgs& na, const float3* X, const VolumeArgs& va, float* outJ);
+
 void launch_predict(hipStream_t s, const NodeArgs& na, const float3* X, float3* V, const float* invMass, float4* P);
 void launch_distance_jacobi(hipStream_t s, const NodeArgs& na, bool first, bool last, const float4* Pin, float4* Pout,
                             const int* rowOff, const int* nbr, const float* nrest, const float* nalpha, float* lam,

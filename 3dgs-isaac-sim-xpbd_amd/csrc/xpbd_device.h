@@ -45,6 +45,9 @@ struct BodyDev {
     float pressLo, pressHi;
     // contact ellipsoids (forward.cu:7358-7399)
     int contactColliders, contactGround;
+    // volume constraint (forward.cu:8531-9243)
+    int volumeOn;
+    float volCompliance;
 };
 
 // forward.cu:128-136 KinCollider, same field meaning.

@@ -114,6 +114,10 @@ def bind(lib):
     L.xpbd_ext_backend.restype = ctypes.c_char_p
     L.xpbd_ext_profile.argtypes = [c_int, ctypes.c_char_p, c_int]
     L.xpbd_ext_profile.restype = c_int
+    L.xpbd_ext_set_volume_clusters.argtypes = [ctypes.c_void_p, ctypes.c_void_p, c_int]
+    L.xpbd_ext_set_volume_clusters.restype = c_int
+    L.xpbd_ext_volume_topology.argtypes = [ctypes.c_void_p] * 5 + [c_int]
+    L.xpbd_ext_volume_topology.restype = c_int
     return L
 
 
@@ -296,6 +300,28 @@ class XPBD:
         buf = ctypes.create_string_buffer(1 << 16)
         self._check(self.lib.xpbd_ext_profile(int(substeps), buf, len(buf)), "xpbd_ext_profile")
         return {k: (float(v[0]), int(v[1])) for k, v in json.loads(buf.value.decode()).items()}
+
+    def ext_set_volume_clusters(self, off=None, members=None, min_members=4):
+        """Explicit volume clusters: node i leads {i} + members[off[i]:off[i+1]].
+        None returns to the k-ring rule."""
+        if off is None:
+            self._check(self.lib.xpbd_ext_set_volume_clusters(None, None, 4), "xpbd_ext_set_volume_clusters")
+            return
+        off = np.ascontiguousarray(off, np.int32)
+        members = np.ascontiguousarray(members, np.int32)
+        if members.size == 0:
+            members = np.zeros(1, np.int32)
+        self._check(self.lib.xpbd_ext_set_volume_clusters(_ptr(off), _ptr(members), int(min_members)),
+                    "xpbd_ext_set_volume_clusters")
+
+    def ext_volume_topology(self):
+        """(leader flag, member count, clusters containing node, material type, rest det) per node."""
+        n = self.n
+        lead, mc, rc, mt = (np.zeros(n, np.int32) for _ in range(4))
+        dr = np.zeros(n, np.float32)
+        self._check(self.lib.xpbd_ext_volume_topology(_ptr(lead), _ptr(mc), _ptr(rc), _ptr(mt), _ptr(dr), n),
+                    "xpbd_ext_volume_topology")
+        return lead, mc, rc, mt, dr
 
     def ext_inv_mass(self):
         out = np.empty(self.n, np.float32)

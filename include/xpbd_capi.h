@@ -146,6 +146,20 @@ int xpbd_ext_get_inv_mass(float* invMass, int n);
  * into json[len].  The state advances exactly as with xpbd_step. */
 int xpbd_ext_profile(int substeps, char* json, int len);
 
+/* Volume clusters given explicitly instead of by the k-ring rule: node i leads
+ * the cluster {i} + members[off[i] .. off[i+1]) (an empty range = no cluster).
+ * minMembers is the rest-precompute gate (the reference rule rejects clusters
+ * with fewer than 4 members; 3 admits node + 3 nearest neighbours, a
+ * tetrahedron, whose covariance J equals |V/V0|).  off == NULL returns to the
+ * k-ring rule.  Call after xpbd_create. */
+int xpbd_ext_set_volume_clusters(const int* off, const int* members, int minMembers);
+
+/* Volume topology diagnostics per node (after the first step): leader flag,
+ * member count, number of clusters containing the node, material type
+ * (0 volume, 1 surface, 2 fibre, 3 too few members, 4 degenerate) and rest
+ * determinant.  Any pointer may be null. */
+int xpbd_ext_volume_topology(int* leaders, int* memberCount, int* revCount, int* matType, float* detRest, int n);
+
 /* Library identity: "hip-gfx950" for the GPU library, "oracle-cpu" for the oracle. */
 const char* xpbd_ext_backend(void);
 

@@ -12,17 +12,37 @@
 
 #include "xpbd_capi.h"
 #include "xpbd_oracle_internal.h"
+#include "oracle_volume.h"
 
 namespace xo {
 
-void extensions_on_init() {}
-void extensions_free() {}
+// Build (or rebuild after a parameter change) the volume clusters the way
+// ensureChainmailGPU / g_volTopoDirty do (forward.cu:10524-10530, 10842).
+static void volume_ensure()
+{
+    if (vol.explicitTopo) { if (!vol.built) volume_precompute(); return; }
+    if (vol.built && vol.k == s.volRingK && vol.cap == s.volMaxMembers && vol.r == s.volLeaderMinHop) return;
+    volume_build_reference(s.volRingK, s.volMaxMembers, s.volLeaderMinHop);
+    vol.k = s.volRingK; vol.cap = s.volMaxMembers; vol.r = s.volLeaderMinHop;
+}
+
+void extensions_on_init() { if (!vol.explicitTopo) vol.built = false; }
+void extensions_free() { vol = VolumeState(); }
 void extensions_on_reset() {}
-void extensions_begin_step() {}
+void extensions_begin_step()
+{
+    if (s.useVolume) {
+        volume_ensure();
+        std::fill(vol.lam.begin(), vol.lam.end(), 0.0f);
+    }
+}
 void extensions_after_predict() {}
 void extensions_before_iterations() {}
 bool extensions_distance(std::vector<V3>&, std::vector<V3>&, int) { return false; }
-void extensions_iteration(std::vector<V3>*&, std::vector<V3>*&, int) {}
+void extensions_iteration(std::vector<V3>*& pin, std::vector<V3>*& pout, int)
+{
+    if (s.useVolume && vol.built) volume_iteration(pin, pout);
+}
 void extensions_self_collision(std::vector<V3>&) {}
 void extensions_after_iterations(std::vector<V3>&, bool) {}
 float contact_rmax(int) { return 0.0f; }
@@ -94,11 +114,58 @@ int xpbd_compute_shapes(float, float*, float*, int n)
     return fail("xpbd_compute_shapes: not restated in the oracle yet");
 }
 
+static float g_jvw = 1.0f, g_jvwStd = 0.0f;
+
+// xpbd_volume_stats (native/xpbd_capi.cu:306-310): J statistics of the last
+// committed positions, evaluated on request.
 void xpbd_volume_stats(int enable, float* jvw, float* jvwStd)
 {
     s.volJStats = enable != 0;
-    if (jvw) *jvw = 1.0f;
-    if (jvwStd) *jvwStd = 0.0f;
+    if (s.volJStats && g.stepped) {
+        volume_ensure();
+        volume_stats(g.x, g_jvw, g_jvwStd);
+    }
+    if (jvw) *jvw = g_jvw;
+    if (jvwStd) *jvwStd = g_jvwStd;
+}
+
+// Explicit clusters: node i leads {i} + members[off[i]:off[i+1]] (empty = no cluster).
+int xpbd_ext_set_volume_clusters(const int* off, const int* members, int minMembers)
+{
+    if (g.n <= 0) return fail("xpbd_ext_set_volume_clusters: call xpbd_create first");
+    if (!off) {
+        vol = VolumeState();
+        return 0;
+    }
+    std::vector<char> isLeader(g.n, 0);
+    std::vector<std::vector<int>> mem(g.n);
+    for (int i = 0; i < g.n; ++i) {
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            const int j = members[k];
+            if (j < 0 || j >= g.n || j == i) return fail("xpbd_ext_set_volume_clusters: member index out of range");
+            mem[i].push_back(j);
+        }
+        isLeader[i] = off[i + 1] > off[i];
+    }
+    vol = VolumeState();
+    vol.explicitTopo = true;
+    volume_set_topology(isLeader, mem, std::max(3, minMembers));
+    return 0;
+}
+
+int xpbd_ext_volume_topology(int* leaders, int* memberCount, int* revCount, int* matType, float* detRest, int n)
+{
+    if (!g.stepped) return fail("xpbd_ext_volume_topology: no step yet");
+    if (n != g.n) return fail("xpbd_ext_volume_topology: size mismatch");
+    volume_ensure();
+    for (int i = 0; i < n; ++i) {
+        if (leaders) leaders[i] = vol.isLeader[i];
+        if (memberCount) memberCount[i] = vol.memOff[i + 1] - vol.memOff[i];
+        if (revCount) revCount[i] = vol.revOff[i + 1] - vol.revOff[i];
+        if (matType) matType[i] = vol.matType[i];
+        if (detRest) detRest[i] = vol.detRest[i];
+    }
+    return 0;
 }
 
 }  // extern "C"
