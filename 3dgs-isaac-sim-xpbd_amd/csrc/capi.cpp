@@ -37,6 +37,7 @@ struct Global {
     int n = 0;
     bool stepped = false;
     bool contactOn = false;
+    int contactEnabled = 0;
     std::string err;
 };
 
@@ -64,6 +65,7 @@ void free_all()
     g.n = 0;
     g.stepped = false;
     g.contactOn = false;
+    g.contactEnabled = 0;
     g.set.contactColliders = false;
     g.set.contactGround = false;
 }
@@ -222,10 +224,16 @@ int xpbd_set_contact_shape(int enabled, float tau, float radiusCap)
     Global& g = G();
     if (!g.input) return fail("xpbd_set_contact_shape: call xpbd_create first");
     g.contactOn = enabled != 0;
+    g.contactEnabled = enabled;
     g.set.contactTau = tau;
     g.set.contactCap = radiusCap;
     g.set.contactColliders = (enabled & 1) != 0;
     g.set.contactGround = (enabled & 2) != 0;
+    if (g.engine) {
+        std::string err;
+        if (!g.engine->setContactShape(0, enabled, tau, radiusCap, err))
+            return fail("xpbd_set_contact_shape: " + err);
+    }
     return 0;
 }
 
@@ -271,6 +279,11 @@ int xpbd_step(void)
             g.engine.reset();
             return fail("xpbd_step: " + err);
         }
+        if (g.contactEnabled &&
+            !g.engine->setContactShape(0, g.contactEnabled, g.set.contactTau, g.set.contactCap, err)) {
+            g.engine.reset();
+            return fail("xpbd_step: " + err);
+        }
     }
     Engine& E = *g.engine;
     E.settings(0) = g.set;
@@ -299,10 +312,13 @@ int xpbd_positions_host(float* out, int n)
 int xpbd_compute_shapes(float deformEps, float* hostScales, float* hostQuats, int n)
 {
     Global& g = G();
-    (void)deformEps; (void)hostScales; (void)hostQuats;
     if (!g.stepped) return fail("xpbd_compute_shapes: no step yet");
     if (n != g.n) return fail("xpbd_compute_shapes: size mismatch");
-    return fail("xpbd_compute_shapes: not available in this build");
+    std::string err;
+    int deformed = 0;
+    if (!g.engine->computeShapes(0, deformEps, hostScales, hostQuats, deformed, err))
+        return fail("xpbd_compute_shapes: " + err);
+    return deformed;
 }
 
 int xpbd_reset(void)

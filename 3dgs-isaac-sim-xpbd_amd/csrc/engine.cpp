@@ -45,6 +45,7 @@ template struct DevBuf<int>;
 template struct DevBuf<BodyDev>;
 template struct DevBuf<ColliderDev>;
 template struct DevBuf<float2>;
+template struct DevBuf<unsigned int>;
 
 // wave64-per-cluster volume solve from this mean member count up
 static const double kVolWaveMembers = 24.0;
@@ -130,6 +131,10 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     volActive_ = 0;
     detRest_.free(); Vrest_.free(); restLen_.free(); matType_.free(); lamVol_.free();
     bbC_.free(); bbS_.free(); bbS2_.free(); volJ_.free();
+    gsScale_.free(); gsRot_.free(); gsOpacity_.free(); outScale_.free(); outQuat_.free(); shapeCtr_.free();
+    cM_.free(); cRmax_.free(); cRg_.free();
+    contactOn_.assign(B, 0); contactColliders_.assign(B, 0); contactGround_.assign(B, 0);
+    contactTau_.assign(B, 0.2f); contactCap_.assign(B, 0.0f);
 
     // host staging of the padded node space
     std::vector<float3> hx(nPad_, make_float3(0, 0, 0));
@@ -401,6 +406,13 @@ void Engine::fillBodyDev(int b, BodyDev& d) const
     d.contactGround = 0;
     d.volumeOn = s.useVolume ? 1 : 0;
     d.volCompliance = s.volCompliance;
+    d.shapeOn = s.useShape ? 1 : 0;
+    d.shapeRobust = s.shapeRobust ? 1 : 0;
+    d.shapeCompliance = s.shapeCompliance;
+    d.shapeBlend = s.shapeBlend;
+    // contact ellipsoids act once packed (forward.cu:12219-12226)
+    d.contactColliders = (cM_.p && contactColliders_[b]) ? 1 : 0;
+    d.contactGround = (cM_.p && contactGround_[b] && s.groundEnabled) ? 1 : 0;
 }
 
 bool Engine::uploadParams(std::string& err)
@@ -438,16 +450,19 @@ std::vector<int> Engine::planSignature() const
 {
     const int B = numBodies();
     int iters = set_[0].iters;
-    int anyDist = 0, anyCol = 0, anyGround = 0, anyPress = 0, anyVol = 0;
+    int anyDist = 0, anyCol = 0, anyGround = 0, anyPress = 0, anyVol = 0, anyShape = 0, anyCGround = 0;
     for (int b = 0; b < B; ++b) {
         anyDist |= set_[b].useDistance ? 1 : 0;
         anyCol |= colCount_[b] > 0 ? 1 : 0;
         anyGround |= set_[b].groundEnabled ? 1 : 0;
         anyPress |= press_[b].active ? 1 : 0;
         anyVol |= set_[b].useVolume ? 1 : 0;
+        anyShape |= set_[b].useShape ? 1 : 0;
+        anyCGround |= (cM_.p && contactGround_[b] && set_[b].groundEnabled) ? 1 : 0;
     }
     const int volWave = (anyVol && volAvgMembers_ >= kVolWaveMembers) ? 1 : 0;
-    return {iters, anyDist, anyCol, anyGround, anyPress, nPad_, anyVol, volWave, anyVol ? volActive_ : 0};
+    return {iters, anyDist, anyCol, anyGround, anyPress, nPad_, anyVol, volWave, anyVol ? volActive_ : 0, anyShape,
+            anyCGround};
 }
 
 // Event pairs around kernel launches (profile mode only).
@@ -487,7 +502,13 @@ void Engine::recordSubstep(Timer* t)
     const int iters = sig[0];
     const bool anyDist = sig[1], anyCol = sig[2], anyGround = sig[3], anyPress = sig[4];
     const NodeArgs na{nPad_, tileBody_.p, bodies_.p};
-    EpiArgs ea{x_.p, v_.p, cols_.p, colHits_.p, colPush_.p, nullptr, nullptr, nullptr};
+    EpiArgs ea{x_.p, v_.p, cols_.p, colHits_.p, colPush_.p, cM_.p, cRmax_.p, cRg_.p};
+    const bool shapePass = sig[9] != 0;
+    if (sig[10]) {
+        XPBD_T_BEGIN("contact_ground_radius");
+        launch_contact_ground(stream_, na, cM_.p, cRg_.p);
+        XPBD_T_END();
+    }
 
     XPBD_T_BEGIN("predict");
     launch_predict(stream_, na, x_.p, v_.p, invMass_.p, p0_.p);
@@ -513,7 +534,7 @@ void Engine::recordSubstep(Timer* t)
             XPBD_T_BEGIN(it == 0 ? (last ? "distance_jacobi_only" : "distance_jacobi_first")
                                  : (last ? "distance_jacobi_last" : "distance_jacobi_mid"));
             launch_distance_jacobi(stream_, na, it == 0, last, pin, pout, rowOff_.p, nbr_.p, nrest_.p, nalpha_.p,
-                                   lam_.p, volPass ? 0u : epi, ea);
+                                   lam_.p, (volPass || shapePass) ? 0u : epi, ea);
             XPBD_T_END();
             std::swap(pin, pout);
         }
@@ -523,11 +544,17 @@ void Engine::recordSubstep(Timer* t)
             launch_vol_solve(stream_, na, sig[7] != 0, it == 0, last, pin, va);
             XPBD_T_END();
             XPBD_T_BEGIN("volume_apply");
-            launch_vol_apply(stream_, na, pin, pout, va, epi, ea);
+            launch_vol_apply(stream_, na, pin, pout, va, shapePass ? 0u : epi, ea);
             XPBD_T_END();
             std::swap(pin, pout);
         }
-        if (!anyDist && !volPass && epi) {
+        if (shapePass) {
+            XPBD_T_BEGIN("shape_match");
+            launch_shape_match(stream_, na, pin, pout, rest_.p, rowOff_.p, nbr_.p, epi, ea);
+            XPBD_T_END();
+            std::swap(pin, pout);
+        }
+        if (!anyDist && !volPass && !shapePass && epi) {
             XPBD_T_BEGIN("epilogue");
             launch_epilogue(stream_, na, pin, epi, ea);
             XPBD_T_END();

@@ -136,7 +136,15 @@ public:
                         std::string& err);
     double volumeAvgMembers() const { return volAvgMembers_; }
 
+    // Deformed Gaussian shapes of body b from the committed positions
+    // (forward.cu:13230-13247); host outputs may be null.  Returns the
+    // deformed count in `deformed`.  Repacks the contact ellipsoids when on.
+    bool computeShapes(int b, float deformEps, float* hostScales, float* hostQuats, int& deformed, std::string& err);
+    // Contact ellipsoids (native/xpbd_capi.cu:179-191): bit 1 colliders, bit 2 ground.
+    bool setContactShape(int b, int enabled, float tau, float radiusCap, std::string& err);
+
 private:
+    bool ensureShapeBuffers(std::string& err);
     bool ensureVolume(bool needAll, std::string& err);
     bool prepare(std::string& err);   // per-substep host work before launching
     void pressAdvance(int b);
@@ -194,6 +202,13 @@ private:
     std::vector<int> hMatType_;
     std::vector<float> hVrest_;
     VolumeArgs volArgs() const;
+
+    // Gaussian attributes, deformed shapes, contact ellipsoids (node space)
+    DevBuf<float> gsScale_, gsRot_, gsOpacity_, outScale_, outQuat_;
+    DevBuf<float> cM_, cRmax_, cRg_;
+    DevBuf<unsigned int> shapeCtr_;
+    std::vector<char> contactOn_, contactColliders_, contactGround_;
+    std::vector<float> contactTau_, contactCap_;
 
     // graph
     hipGraph_t graph_ = nullptr;

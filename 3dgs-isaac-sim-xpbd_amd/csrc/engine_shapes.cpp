@@ -1,0 +1,97 @@
+// engine_shapes.cpp — deformed Gaussian shapes and contact ellipsoids of the
+// Engine (forward.cu:13230-13247 computeDeformedShapes; 6733-6757
+// updateContactShapes; native/xpbd_capi.cu:179-191, 248-260).
+#include <algorithm>
+#include <cmath>
+
+#include "engine.h"
+
+namespace xpbd {
+
+#define XPBD_HIP(call)                                                                    \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            err = std::string(#call) + ": " + hipGetErrorString(e_);                      \
+            return false;                                                                 \
+        }                                                                                 \
+    } while (0)
+
+// Scales, rotations and opacity of every body in node space (uploaded once).
+bool Engine::ensureShapeBuffers(std::string& err)
+{
+    if (gsScale_.p) return true;
+    gsScale_.alloc((size_t)3 * nPad_);
+    gsRot_.alloc((size_t)4 * nPad_);
+    gsOpacity_.alloc(nPad_);
+    outScale_.alloc((size_t)3 * nPad_);
+    outQuat_.alloc((size_t)4 * nPad_);
+    shapeCtr_.alloc(1);
+    if (!gsScale_.p || !gsRot_.p || !gsOpacity_.p || !outScale_.p || !outQuat_.p || !shapeCtr_.p) {
+        err = "shapes: hipMalloc failed";
+        return false;
+    }
+    for (int b = 0; b < numBodies(); ++b) {
+        const BodyInput& in = *inputs_[b];
+        const size_t nb = bodyBegin_[b], n = in.n;
+        XPBD_HIP(hipMemcpyAsync(gsScale_.p + 3 * nb, in.scales.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, stream_));
+        XPBD_HIP(hipMemcpyAsync(gsRot_.p + 4 * nb, in.rots.data(), sizeof(float) * 4 * n, hipMemcpyHostToDevice, stream_));
+        XPBD_HIP(hipMemcpyAsync(gsOpacity_.p + nb, in.opacity.data(), sizeof(float) * n, hipMemcpyHostToDevice, stream_));
+    }
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    return true;
+}
+
+static float cap_or_zero(float c) { return (std::isfinite(c) && c > 0.0f) ? c : 0.0f; }
+
+bool Engine::setContactShape(int b, int enabled, float tau, float radiusCap, std::string& err)
+{
+    if (!ensureShapeBuffers(err)) return false;
+    contactOn_[b] = enabled != 0;
+    contactTau_[b] = tau;
+    contactCap_[b] = radiusCap;
+    if (contactOn_[b]) {
+        if (!cM_.p) {
+            cM_.alloc((size_t)6 * nPad_);
+            cRmax_.alloc(nPad_);
+            cRg_.alloc(nPad_);
+            if (!cM_.p || !cRmax_.p || !cRg_.p) { err = "contact: hipMalloc failed"; return false; }
+            XPBD_HIP(hipMemsetAsync(cM_.p, 0, sizeof(float) * 6 * nPad_, stream_));
+            XPBD_HIP(hipMemsetAsync(cRmax_.p, 0, sizeof(float) * nPad_, stream_));
+            XPBD_HIP(hipMemsetAsync(cRg_.p, 0, sizeof(float) * nPad_, stream_));
+        }
+        const size_t nb = bodyBegin_[b];
+        launch_contact_pack(stream_, (int)nb, bodyCount_[b], gsScale_.p + 3 * nb, gsRot_.p + 4 * nb, gsOpacity_.p + nb,
+                            tau, cap_or_zero(radiusCap), cM_.p, cRmax_.p);
+    }
+    contactColliders_[b] = (enabled & 1) != 0;
+    contactGround_[b] = (enabled & 2) != 0;
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    XPBD_HIP(hipGetLastError());
+    return true;
+}
+
+bool Engine::computeShapes(int b, float deformEps, float* hostScales, float* hostQuats, int& deformed, std::string& err)
+{
+    if (!ensureShapeBuffers(err)) return false;
+    const size_t nb = bodyBegin_[b];
+    const int n = bodyCount_[b];
+    XPBD_HIP(hipMemsetAsync(shapeCtr_.p, 0, sizeof(unsigned int), stream_));
+    launch_deformed_shapes(stream_, (int)nb, n, rest_.p, x_.p, rowOff_.p, nbr_.p, gsScale_.p + 3 * nb, gsRot_.p + 4 * nb,
+                           deformEps, outScale_.p + 3 * nb, outQuat_.p + 4 * nb, shapeCtr_.p);
+    if (contactOn_[b] && cM_.p)
+        launch_contact_pack(stream_, (int)nb, n, outScale_.p + 3 * nb, outQuat_.p + 4 * nb, gsOpacity_.p + nb,
+                            contactTau_[b], cap_or_zero(contactCap_[b]), cM_.p, cRmax_.p);
+    unsigned int c = 0;
+    XPBD_HIP(hipMemcpyAsync(&c, shapeCtr_.p, sizeof(unsigned int), hipMemcpyDeviceToHost, stream_));
+    if (hostScales)
+        XPBD_HIP(hipMemcpyAsync(hostScales, outScale_.p + 3 * nb, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, stream_));
+    if (hostQuats)
+        XPBD_HIP(hipMemcpyAsync(hostQuats, outQuat_.p + 4 * nb, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, stream_));
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    XPBD_HIP(hipGetLastError());
+    deformed = (int)c;
+    return true;
+}
+
+}  // namespace xpbd

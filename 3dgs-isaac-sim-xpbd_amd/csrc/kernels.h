@@ -51,6 +51,17 @@ void launch_vol_apply(hipStream_t s, const NodeArgs& na, const float4* Pin, floa
                       unsigned epi, const EpiArgs& ea);
 void launch_vol_J(hipStream_t s, const NodeArgs& na, const float3* X, const VolumeArgs& va, float* outJ);
 
+void launch_shape_match(hipStream_t s, const NodeArgs& na, const float4* Pin, float4* Pout, const float3* rest,
+                        const int* rowOff, const int* nbr, unsigned epi, const EpiArgs& ea);
+// scales / rots / outScale / outQuat are the body's own arrays (local index);
+// node arrays are indexed from nodeBegin.
+void launch_deformed_shapes(hipStream_t s, int nodeBegin, int n, const float3* rest, const float3* cur,
+                            const int* rowOff, const int* nbr, const float* scales, const float* rots, float deformEps,
+                            float* outScale, float* outQuat, unsigned int* ctr);
+void launch_contact_pack(hipStream_t s, int nodeBegin, int n, const float* sc, const float* qt, const float* op,
+                         float tau, float rcap, float* M, float* rmax);
+void launch_contact_ground(hipStream_t s, const NodeArgs& na, const float* M, float* rg);
+
 void launch_predict(hipStream_t s, const NodeArgs& na, const float3* X, float3* V, const float* invMass, float4* P);
 void launch_distance_jacobi(hipStream_t s, const NodeArgs& na, bool first, bool last, const float4* Pin, float4* Pout,
                             const int* rowOff, const int* nbr, const float* nrest, const float* nalpha, float* lam,

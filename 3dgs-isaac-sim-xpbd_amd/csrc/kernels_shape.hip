@@ -1,0 +1,456 @@
+// kernels_shape.hip — per-node 1-ring shape matching, the per-Gaussian
+// scale / orientation update from the local deformation gradient, and the
+// contact-ellipsoid packing.
+//
+// Shape matching (forward.cu:6088-6383): centroids of {node} + neighbours at
+// rest and now, a boundary-bias reject (mean unit direction > 0.4), the
+// Frobenius-normalised A = sum p q^T, its rotation by the fp64 polar
+// decomposition (forward.cu:5991-6086; relative Jacobi stop 1e-12, flat fix
+// S0 < 0.08 Smax, reflection fix) or the float path, goal = c + R q_i and an
+// XPBD pull toward the goal (4972-4988).  One thread per node; Jacobi.
+//
+// Deformed shapes (forward.cu:13063-13225): A = (Q P^T + aI)(P P^T + aI)^-1
+// from the 1-ring, singular values clamped to [1e-3, 1.5], M = S0 R0 S_f R^T,
+// Sigma' = M^T M, then a cyclic fp64 Jacobi eigen-decomposition (13022-13061)
+// of Sigma' / (tr / 3) gives the scales and the quaternion.
+#include "kernels.h"
+#include "mat3.h"
+#include "node_ops.h"
+
+namespace xpbd {
+
+// ---------------------------------------------------------------------------
+__device__ bool polar_rotation_f64(const M3& A, M3& Rout)
+{
+    double a[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) a[r][c] = (double)A.m[c][r];
+    double m[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) m[r][c] = a[0][r] * a[0][c] + a[1][r] * a[1][c] + a[2][r] * a[2][c];
+    double v[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}};
+    for (int iter = 0; iter < 50; ++iter) {
+        bool done = true;
+        int p = 0, q = 1;
+        double maxOff = -1.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int P = (k == 2) ? 1 : 0, Q = (k == 0) ? 1 : 2;
+            const double off = fabs(m[P][Q]);
+            if (off > 1e-12 * sqrt(fabs(m[P][P] * m[Q][Q])) + 1e-300) done = false;
+            if (off > maxOff) { maxOff = off; p = P; q = Q; }
+        }
+        if (done) break;
+        const double app = m[p][p], aqq = m[q][q], apq = m[p][q];
+        const double phi = 0.5 * (aqq - app) / (apq + 1e-300);
+        const double t = (phi >= 0.0) ? 1.0 / (phi + sqrt(1.0 + phi * phi)) : 1.0 / (phi - sqrt(1.0 + phi * phi));
+        const double c = 1.0 / sqrt(1.0 + t * t);
+        const double s = t * c;
+        m[p][p] = app - t * apq;
+        m[q][q] = aqq + t * apq;
+        m[p][q] = m[q][p] = 0.0;
+        const int r3 = 3 - p - q;
+        const double arp = m[r3][p], arq = m[r3][q];
+        m[r3][p] = m[p][r3] = c * arp - s * arq;
+        m[r3][q] = m[q][r3] = c * arq + s * arp;
+        for (int r = 0; r < 3; ++r) {
+            const double vrp = v[r][p], vrq = v[r][q];
+            v[r][p] = c * vrp - s * vrq;
+            v[r][q] = s * vrp + c * vrq;
+        }
+    }
+    double ev[3] = {m[0][0], m[1][1], m[2][2]};
+    for (int i = 0; i < 2; ++i)
+        for (int j = i + 1; j < 3; ++j)
+            if (ev[i] > ev[j]) {
+                const double te = ev[i]; ev[i] = ev[j]; ev[j] = te;
+                for (int r = 0; r < 3; ++r) { const double tv = v[r][i]; v[r][i] = v[r][j]; v[r][j] = tv; }
+            }
+    double S[3], Sinv[3];
+    for (int i = 0; i < 3; ++i) S[i] = sqrt(ev[i] > 0.0 ? ev[i] : 0.0);
+    const double Smax = (S[2] > 1e-300) ? S[2] : 1e-300;
+    for (int i = 0; i < 3; ++i) Sinv[i] = (S[i] > 1e-12 * Smax) ? 1.0 / S[i] : 0.0;
+    double u[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) u[r][c] = (a[r][0] * v[0][c] + a[r][1] * v[1][c] + a[r][2] * v[2][c]) * Sinv[c];
+    if (S[0] < 0.08 * Smax) {
+        const double cx = u[1][1] * u[2][2] - u[2][1] * u[1][2];
+        const double cy = u[2][1] * u[0][2] - u[0][1] * u[2][2];
+        const double cz = u[0][1] * u[1][2] - u[1][1] * u[0][2];
+        const double l = sqrt(cx * cx + cy * cy + cz * cz);
+        if (!(l > 1e-7)) return false;
+        u[0][0] = cx / l; u[1][0] = cy / l; u[2][0] = cz / l;
+    }
+    double R[3][3];
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) R[r][c] = u[r][0] * v[c][0] + u[r][1] * v[c][1] + u[r][2] * v[c][2];
+        const double det = R[0][0] * (R[1][1] * R[2][2] - R[1][2] * R[2][1]) -
+                           R[0][1] * (R[1][0] * R[2][2] - R[1][2] * R[2][0]) +
+                           R[0][2] * (R[1][0] * R[2][1] - R[1][1] * R[2][0]);
+        if (det >= 0.0 || pass == 1) break;
+        for (int r = 0; r < 3; ++r) u[r][0] = -u[r][0];
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            if (!isfinite(R[r][c])) return false;
+            Rout.m[c][r] = (float)R[r][c];
+        }
+    return true;
+}
+
+__device__ bool polar_rotation_f32(const M3& A, M3& R)
+{
+    const M3 AtA = m3_mul(m3_transpose(A), A);
+    float S2[3];
+    M3 V;
+    m3_eig_sym(AtA, S2, V);
+    const float s0 = sqrtf(fmaxf(0.0f, S2[0])), s1 = sqrtf(fmaxf(0.0f, S2[1])), s2 = sqrtf(fmaxf(0.0f, S2[2]));
+    if (!isfinite(s0) || !isfinite(s1) || !isfinite(s2)) return false;
+    const float e = 1e-7f;
+    const M3 Si = m3_diag(s0 > e ? 1.0f / s0 : 0.0f, s1 > e ? 1.0f / s1 : 0.0f, s2 > e ? 1.0f / s2 : 0.0f);
+    M3 U = m3_mul(m3_mul(A, V), Si);
+    if (!m3_finite(U)) return false;
+    if (s0 < 0.08f * fmaxf(s2, 1e-7f)) {
+        const float c0 = U.m[1][1] * U.m[2][2] - U.m[2][1] * U.m[1][2];
+        const float c1 = U.m[1][2] * U.m[2][0] - U.m[2][2] * U.m[1][0];
+        const float c2 = U.m[1][0] * U.m[2][1] - U.m[2][0] * U.m[1][1];
+        const float l = sqrtf(c0 * c0 + c1 * c1 + c2 * c2);
+        if (!(l > 1e-7f)) return false;
+        U.m[0][0] = c0 / l; U.m[0][1] = c1 / l; U.m[0][2] = c2 / l;
+    }
+    R = m3_mul(U, m3_transpose(V));
+    if (!m3_finite(R)) return false;
+    if (m3_det(R) < 0.0f) {
+        U.m[0][0] *= -1.0f; U.m[0][1] *= -1.0f; U.m[0][2] *= -1.0f;
+        R = m3_mul(U, m3_transpose(V));
+        if (!m3_finite(R)) return false;
+    }
+    return true;
+}
+
+// Per-node shape matching over the graph 1-ring, plus the iteration epilogue.
+__global__ void __launch_bounds__(256) shape_match_kernel(
+    int nPad, const int* __restrict__ tileBody, const BodyDev* __restrict__ bodies, const float4* __restrict__ Pin,
+    float4* __restrict__ Pout, const float3* __restrict__ rest, const int* __restrict__ rowOff,
+    const int* __restrict__ nbr, unsigned epi, EpiArgs ea)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nPad) return;
+    const BodyDev& B = bodies[__builtin_amdgcn_readfirstlane(tileBody[idx / kTile])];
+    if (idx >= B.nodeEnd) return;
+    const float4 p4 = Pin[idx];
+    float3 out = make_float3(p4.x, p4.y, p4.z);
+    const float wi = p4.w * B.invMassScale;
+    const int off = rowOff[idx], cnt = rowOff[idx + 1] - off;
+    while (B.shapeOn && wi > 0.0f && cnt >= 4) {
+        const float3 r0 = rest[idx];
+        float crx = r0.x, cry = r0.y, crz = r0.z;
+        float ccx = p4.x, ccy = p4.y, ccz = p4.z;
+        float cc = 1.0f;
+        for (int k = 0; k < cnt; ++k) {
+            const int j = nbr[off + k];
+            const float3 rj = rest[j];
+            const float4 pj = Pin[j];
+            crx = crx + rj.x; cry = cry + rj.y; crz = crz + rj.z;
+            ccx = ccx + pj.x; ccy = ccy + pj.y; ccz = ccz + pj.z;
+            cc += 1.0f;
+        }
+        const float invC = 1.0f / cc;
+        crx = crx * invC; cry = cry * invC; crz = crz * invC;
+        ccx = ccx * invC; ccy = ccy * invC; ccz = ccz * invC;
+        float dsx = 0.0f, dsy = 0.0f, dsz = 0.0f;
+        for (int k = 0; k < cnt; ++k) {
+            const float3 rj = rest[nbr[off + k]];
+            const float qx = rj.x - crx, qy = rj.y - cry, qz = rj.z - crz;
+            const float l = sqrtf(qx * qx + qy * qy + qz * qz);
+            if (l > 1e-7f) {
+                const float il = 1.0f / l;
+                dsx = dsx + qx * il; dsy = dsy + qy * il; dsz = dsz + qz * il;
+            }
+        }
+        const float bias = sqrtf(dsx * dsx + dsy * dsy + dsz * dsz) / fmaxf((float)cnt, 1.0f);
+        if (bias > 0.4f) break;
+        M3 A = m3_zero();
+        auto acc = [&](float px, float py, float pz, float qx, float qy, float qz) {
+            const float pv[3] = {px, py, pz}, qv[3] = {qx, qy, qz};
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) A.m[c][r] += pv[r] * qv[c];
+        };
+        acc(p4.x - ccx, p4.y - ccy, p4.z - ccz, r0.x - crx, r0.y - cry, r0.z - crz);
+        for (int k = 0; k < cnt; ++k) {
+            const int j = nbr[off + k];
+            const float3 rj = rest[j];
+            const float4 pj = Pin[j];
+            acc(pj.x - ccx, pj.y - ccy, pj.z - ccz, rj.x - crx, rj.y - cry, rj.z - crz);
+        }
+        float nsq = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) nsq += A.m[c][r] * A.m[c][r];
+        const float an = sqrtf(nsq);
+        if (!isfinite(an) || an < 1e-8f) break;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) A.m[c][r] /= an;
+        M3 R;
+        const bool ok = B.shapeRobust ? polar_rotation_f64(A, R) : polar_rotation_f32(A, R);
+        if (!ok) break;
+        const float3 rq = m3_mul_vec(R, r0.x - crx, r0.y - cry, r0.z - crz);
+        const float gx = ccx + rq.x, gy = ccy + rq.y, gz = ccz + rq.z;
+        const float ddx = gx - p4.x, ddy = gy - p4.y, ddz = gz - p4.z;
+        const float gd2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        if (!isfinite(gd2) || gd2 < 1e-18f) break;
+        const float dt2 = fmaxf(B.dt * B.dt, 1e-8f);
+        const float at = B.shapeCompliance / dt2;
+        const float denom = wi + at;
+        if (denom < 1e-8f) break;
+        const float sc = B.shapeBlend * (wi / denom);
+        out = make_float3(p4.x + ddx * sc, p4.y + ddy * sc, p4.z + ddz * sc);
+        break;
+    }
+    if (epi && epilogue_node(epi, idx, out, p4.w, B, ea)) return;
+    Pout[idx] = make_float4(out.x, out.y, out.z, p4.w);
+}
+
+// ---------------------------------------------------------------------------
+__device__ void sym_eig3_f64(double a[3][3], double ev[3], double V[3][3])
+{
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) V[r][c] = (r == c) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 32; ++sweep) {
+        const double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
+        const double diag = fabs(a[0][0]) + fabs(a[1][1]) + fabs(a[2][2]);
+        if (off <= 1e-15 * diag + 1e-300) break;
+        for (int k = 0; k < 3; ++k) {
+            const int p = (k == 2) ? 1 : 0, q = (k == 0) ? 1 : 2;
+            const double apq = a[p][q];
+            if (fabs(apq) <= 1e-300) continue;
+            const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+            const double t = ((theta >= 0.0) ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            const double c = 1.0 / sqrt(t * t + 1.0);
+            const double s = t * c;
+            for (int m = 0; m < 3; ++m) {
+                const double amp = a[m][p], amq = a[m][q];
+                a[m][p] = c * amp - s * amq;
+                a[m][q] = s * amp + c * amq;
+            }
+            for (int m = 0; m < 3; ++m) {
+                const double apm = a[p][m], aqm = a[q][m];
+                a[p][m] = c * apm - s * aqm;
+                a[q][m] = s * apm + c * aqm;
+            }
+            for (int m = 0; m < 3; ++m) {
+                const double vmp = V[m][p], vmq = V[m][q];
+                V[m][p] = c * vmp - s * vmq;
+                V[m][q] = s * vmp + c * vmq;
+            }
+        }
+    }
+    ev[0] = a[0][0];
+    ev[1] = a[1][1];
+    ev[2] = a[2][2];
+}
+
+__global__ void __launch_bounds__(256) deformed_shape_kernel(
+    int nodeBegin, int n, const float3* __restrict__ rest, const float3* __restrict__ cur,
+    const int* __restrict__ rowOff, const int* __restrict__ nbr, const float* __restrict__ scales,
+    const float* __restrict__ rots, float deformEps, float* __restrict__ outScale, float* __restrict__ outQuat,
+    unsigned int* __restrict__ ctr)
+{
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    bool counted = false;
+    if (li < n) {
+        const int i = nodeBegin + li;
+        const float s0x = scales[3 * li], s0y = scales[3 * li + 1], s0z = scales[3 * li + 2];
+        const float q0w = rots[4 * li], q0x = rots[4 * li + 1], q0y = rots[4 * li + 2], q0z = rots[4 * li + 3];
+        const float3 x = cur[i], x0 = rest[i];
+        const float dxv = x.x - x0.x, dyv = x.y - x0.y, dzv = x.z - x0.z;
+        const float diff = sqrtf(dxv * dxv + dyv * dyv + dzv * dzv);
+        const int off = rowOff[i], kn = rowOff[i + 1] - off;
+        bool deformed = false;
+        M3 Sigma = m3_zero();
+        if (diff > deformEps && kn >= 3) {
+            M3 PPt = m3_zero(), QPt = m3_zero();
+            for (int k = 0; k < kn; ++k) {
+                const int j = nbr[off + k];
+                const float3 rj = rest[j], cj = cur[j];
+                const float p[3] = {rj.x - x0.x, rj.y - x0.y, rj.z - x0.z};
+                const float q[3] = {cj.x - x.x, cj.y - x.y, cj.z - x.z};
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        PPt.m[c][r] += p[r] * p[c];
+                        QPt.m[c][r] += q[r] * p[c];
+                    }
+            }
+            const float tr = PPt.m[0][0] + PPt.m[1][1] + PPt.m[2][2];
+            const float alpha = (tr > 0.0f) ? (tr * 1e-4f) : 1e-6f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const float d = (c == r) ? alpha : 0.0f;
+                    PPt.m[c][r] += d;
+                    QPt.m[c][r] += d;
+                }
+            const float sP = fmaxf((PPt.m[0][0] + PPt.m[1][1] + PPt.m[2][2]) * (1.0f / 3.0f), 1e-30f);
+            const float invSP = 1.0f / sP;
+            M3 Ph = PPt;
+            m3_scale(Ph, invSP);
+            M3 invHat;
+            if (m3_inverse_safe(Ph, invHat, 1e-9f)) {
+                M3 invPPt = invHat;
+                m3_scale(invPPt, invSP);
+                const M3 A = m3_mul(QPt, invPPt);
+                const M3 AtA = m3_mul(m3_transpose(A), A);
+                float S2[3];
+                M3 V;
+                m3_eig_sym(AtA, S2, V);
+                float sv[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) sv[k] = fminf(fmaxf(sqrtf(fmaxf(0.0f, S2[k])), 1e-3f), 1.5f);
+                const float e = 1.0e-7f;
+                const M3 Sinv = m3_diag(sv[0] > e ? 1.0f / sv[0] : 0.0f, sv[1] > e ? 1.0f / sv[1] : 0.0f,
+                                        sv[2] > e ? 1.0f / sv[2] : 0.0f);
+                M3 U = m3_mul(m3_mul(A, V), Sinv);
+                M3 R = m3_mul(U, m3_transpose(V));
+                if (m3_det(R) < 0.0f) {
+                    U.m[0][0] *= -1.0f; U.m[0][1] *= -1.0f; U.m[0][2] *= -1.0f;
+                    R = m3_mul(U, m3_transpose(V));
+                }
+                const M3 Sf = m3_mul(m3_mul(V, m3_diag(sv[0], sv[1], sv[2])), m3_transpose(V));
+                const M3 S = m3_diag(s0x, s0y, s0z);
+                const float r = q0w, qx = q0x, qy = q0y, qz = q0z;
+                M3 R0;
+                R0.m[0][0] = 1.f - 2.f * (qy * qy + qz * qz); R0.m[0][1] = 2.f * (qx * qy - r * qz); R0.m[0][2] = 2.f * (qx * qz + r * qy);
+                R0.m[1][0] = 2.f * (qx * qy + r * qz); R0.m[1][1] = 1.f - 2.f * (qx * qx + qz * qz); R0.m[1][2] = 2.f * (qy * qz - r * qx);
+                R0.m[2][0] = 2.f * (qx * qz - r * qy); R0.m[2][1] = 2.f * (qy * qz + r * qx); R0.m[2][2] = 1.f - 2.f * (qx * qx + qy * qy);
+                const M3 M = m3_mul(m3_mul(m3_mul(S, R0), Sf), m3_transpose(R));
+                Sigma = m3_mul(m3_transpose(M), M);
+                deformed = m3_finite(Sigma);
+            }
+        }
+        const double tr = deformed ? ((double)Sigma.m[0][0] + Sigma.m[1][1] + Sigma.m[2][2]) : 0.0;
+        if (!deformed || !(tr > 0.0)) {
+            outScale[3 * li] = s0x; outScale[3 * li + 1] = s0y; outScale[3 * li + 2] = s0z;
+            outQuat[4 * li] = q0w; outQuat[4 * li + 1] = q0x; outQuat[4 * li + 2] = q0y; outQuat[4 * li + 3] = q0z;
+        } else {
+            counted = true;
+            const double inv = 3.0 / tr;
+            double a[3][3];
+            for (int rr = 0; rr < 3; ++rr)
+                for (int c = 0; c < 3; ++c) a[rr][c] = 0.5 * ((double)Sigma.m[c][rr] + (double)Sigma.m[rr][c]) * inv;
+            double ev[3], E[3][3];
+            sym_eig3_f64(a, ev, E);
+            const double det = E[0][0] * (E[1][1] * E[2][2] - E[1][2] * E[2][1]) -
+                               E[0][1] * (E[1][0] * E[2][2] - E[1][2] * E[2][0]) +
+                               E[0][2] * (E[1][0] * E[2][1] - E[1][1] * E[2][0]);
+            if (det < 0.0)
+                for (int rr = 0; rr < 3; ++rr) E[rr][2] = -E[rr][2];
+            const double sFloor = 1e-3 * (double)fmaxf(s0x, fmaxf(s0y, s0z));
+            for (int k = 0; k < 3; ++k) outScale[3 * li + k] = (float)fmax(sqrt(fmax(ev[k], 0.0) / inv), sFloor);
+            double w, qx, qy, qz;
+            const double t = E[0][0] + E[1][1] + E[2][2];
+            if (t > 0.0) {
+                const double sq = 2.0 * sqrt(t + 1.0);
+                w = 0.25 * sq; qx = (E[2][1] - E[1][2]) / sq; qy = (E[0][2] - E[2][0]) / sq; qz = (E[1][0] - E[0][1]) / sq;
+            } else if (E[0][0] > E[1][1] && E[0][0] > E[2][2]) {
+                const double sq = 2.0 * sqrt(1.0 + E[0][0] - E[1][1] - E[2][2]);
+                w = (E[2][1] - E[1][2]) / sq; qx = 0.25 * sq; qy = (E[0][1] + E[1][0]) / sq; qz = (E[0][2] + E[2][0]) / sq;
+            } else if (E[1][1] > E[2][2]) {
+                const double sq = 2.0 * sqrt(1.0 + E[1][1] - E[0][0] - E[2][2]);
+                w = (E[0][2] - E[2][0]) / sq; qx = (E[0][1] + E[1][0]) / sq; qy = 0.25 * sq; qz = (E[1][2] + E[2][1]) / sq;
+            } else {
+                const double sq = 2.0 * sqrt(1.0 + E[2][2] - E[0][0] - E[1][1]);
+                w = (E[1][0] - E[0][1]) / sq; qx = (E[0][2] + E[2][0]) / sq; qy = (E[1][2] + E[2][1]) / sq; qz = 0.25 * sq;
+            }
+            const double qn = sqrt(w * w + qx * qx + qy * qy + qz * qz);
+            outQuat[4 * li] = (float)(w / qn);
+            outQuat[4 * li + 1] = (float)(qx / qn);
+            outQuat[4 * li + 2] = (float)(qy / qn);
+            outQuat[4 * li + 3] = (float)(qz / qn);
+        }
+    }
+    // one atomic per wave for the deformed count
+    const unsigned long long ballot = __ballot(counted);
+    if ((threadIdx.x & 63) == 0 && ballot) atomicAdd(ctr, (unsigned int)__popcll(ballot));
+}
+
+// contactShapePackKernel (forward.cu:7358-7384)
+__global__ void contact_pack_kernel(int nodeBegin, int n, const float* __restrict__ sc, const float* __restrict__ qt,
+                                    const float* __restrict__ op, float tau, float rcap, float* __restrict__ M,
+                                    float* __restrict__ rmax)
+{
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= n) return;
+    const int i = nodeBegin + li;
+    const float a = op[li];
+    const float k = (a > tau && tau > 0.0f) ? sqrtf(2.0f * logf(a / tau)) : 0.0f;
+    float e0 = k * sc[3 * li], e1 = k * sc[3 * li + 1], e2 = k * sc[3 * li + 2];
+    if (rcap > 0.0f) { e0 = fminf(e0, rcap); e1 = fminf(e1, rcap); e2 = fminf(e2, rcap); }
+    if (!(isfinite(e0) && isfinite(e1) && isfinite(e2))) { e0 = e1 = e2 = 0.0f; }
+    float w = qt[4 * li], x = qt[4 * li + 1], y = qt[4 * li + 2], z = qt[4 * li + 3];
+    const float qn = sqrtf(w * w + x * x + y * y + z * z);
+    if (qn > 1e-20f) { w /= qn; x /= qn; y /= qn; z /= qn; } else { w = 1.0f; x = y = z = 0.0f; }
+    const float r00 = 1 - 2 * (y * y + z * z), r01 = 2 * (x * y - w * z), r02 = 2 * (x * z + w * y);
+    const float r10 = 2 * (x * y + w * z), r11 = 1 - 2 * (x * x + z * z), r12 = 2 * (y * z - w * x);
+    const float r20 = 2 * (x * z - w * y), r21 = 2 * (y * z + w * x), r22 = 1 - 2 * (x * x + y * y);
+    const float l0 = e0 * e0, l1 = e1 * e1, l2 = e2 * e2;
+    float* m = &M[6 * (size_t)i];
+    m[0] = r00 * r00 * l0 + r01 * r01 * l1 + r02 * r02 * l2;
+    m[1] = r00 * r10 * l0 + r01 * r11 * l1 + r02 * r12 * l2;
+    m[2] = r00 * r20 * l0 + r01 * r21 * l1 + r02 * r22 * l2;
+    m[3] = r10 * r10 * l0 + r11 * r11 * l1 + r12 * r12 * l2;
+    m[4] = r10 * r20 * l0 + r11 * r21 * l1 + r12 * r22 * l2;
+    m[5] = r20 * r20 * l0 + r21 * r21 * l1 + r22 * r22 * l2;
+    rmax[i] = fmaxf(e0, fmaxf(e1, e2));
+}
+
+// contactGroundRadiusKernel (forward.cu:7394-7399) for bodies with ground contact
+__global__ void contact_ground_kernel(int nPad, const int* __restrict__ tileBody, const BodyDev* __restrict__ bodies,
+                                      const float* __restrict__ M, float* __restrict__ rg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;
+    const BodyDev& B = bodies[__builtin_amdgcn_readfirstlane(tileBody[i / kTile])];
+    if (i >= B.nodeEnd || !B.contactGround) return;
+    rg[i] = contact_support(&M[6 * (size_t)i], B.gn[0], B.gn[1], B.gn[2]);
+}
+
+// ---------------------------------------------------------------------------
+static inline int nbk(int n, int bs) { return (n + bs - 1) / bs; }
+
+void launch_shape_match(hipStream_t s, const NodeArgs& na, const float4* Pin, float4* Pout, const float3* rest,
+                        const int* rowOff, const int* nbr, unsigned epi, const EpiArgs& ea)
+{
+    shape_match_kernel<<<nbk(na.nPad, 256), 256, 0, s>>>(na.nPad, na.tileBody, na.bodies, Pin, Pout, rest, rowOff,
+                                                        nbr, epi, ea);
+}
+
+void launch_deformed_shapes(hipStream_t s, int nodeBegin, int n, const float3* rest, const float3* cur,
+                            const int* rowOff, const int* nbr, const float* scales, const float* rots, float deformEps,
+                            float* outScale, float* outQuat, unsigned int* ctr)
+{
+    if (n > 0)
+        deformed_shape_kernel<<<nbk(n, 256), 256, 0, s>>>(nodeBegin, n, rest, cur, rowOff, nbr, scales, rots, deformEps,
+                                                         outScale, outQuat, ctr);
+}
+
+void launch_contact_pack(hipStream_t s, int nodeBegin, int n, const float* sc, const float* qt, const float* op,
+                         float tau, float rcap, float* M, float* rmax)
+{
+    if (n > 0) contact_pack_kernel<<<nbk(n, 256), 256, 0, s>>>(nodeBegin, n, sc, qt, op, tau, rcap, M, rmax);
+}
+
+void launch_contact_ground(hipStream_t s, const NodeArgs& na, const float* M, float* rg)
+{
+    contact_ground_kernel<<<nbk(na.nPad, 256), 256, 0, s>>>(na.nPad, na.tileBody, na.bodies, M, rg);
+}
+
+}  // namespace xpbd

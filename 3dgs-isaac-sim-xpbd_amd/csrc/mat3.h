@@ -131,6 +131,97 @@ __host__ __device__ __forceinline__ void m3_eig_sym_values(const M3& S, float ev
     if (ev[1] > ev[2]) { const float t = ev[1]; ev[1] = ev[2]; ev[2] = t; }
 }
 
+__host__ __device__ __forceinline__ M3 m3_diag(float a, float b, float c)
+{
+    M3 m = m3_zero();
+    m.m[0][0] = a; m.m[1][1] = b; m.m[2][2] = c;
+    return m;
+}
+
+// A * B with the operand order of GLM's mat3 product (type_mat3x3.inl:486-518)
+__host__ __device__ __forceinline__ M3 m3_mul(const M3& A, const M3& B)
+{
+    M3 R;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) R.m[c][r] = A.m[0][r] * B.m[c][0] + A.m[1][r] * B.m[c][1] + A.m[2][r] * B.m[c][2];
+    return R;
+}
+
+__host__ __device__ __forceinline__ M3 m3_transpose(const M3& A)
+{
+    M3 R;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) R.m[c][r] = A.m[r][c];
+    return R;
+}
+
+__host__ __device__ __forceinline__ bool m3_finite(const M3& A)
+{
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) ok = ok && isfinite(A.m[c][r]);
+    return ok;
+}
+
+// Same Jacobi scheme as m3_eig_sym_values, also returning the eigenvectors as
+// columns of V, sorted with the values (ascending).
+__host__ __device__ __forceinline__ void m3_eig_sym(const M3& S, float ev[3], M3& Vout)
+{
+    float A[3][3], V[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            A[r][c] = S.m[c][r];
+            V[r][c] = (r == c) ? 1.0f : 0.0f;
+        }
+    for (int iter = 0; iter < 50; ++iter) {
+        int p = 0, q = 1;
+        float maxOff = fabsf(A[0][1]);
+        if (fabsf(A[0][2]) > maxOff) { p = 0; q = 2; maxOff = fabsf(A[0][2]); }
+        if (fabsf(A[1][2]) > maxOff) { p = 1; q = 2; maxOff = fabsf(A[1][2]); }
+        if (maxOff < 1e-6f) break;
+        const float app = A[p][p], aqq = A[q][q], apq = A[p][q];
+        const float phi = 0.5f * (aqq - app) / (apq + 1e-20f);
+        const float t = (phi >= 0.0f) ? (1.0f / (phi + sqrtf(1.0f + phi * phi))) : (1.0f / (phi - sqrtf(1.0f + phi * phi)));
+        const float c = 1.0f / sqrtf(1.0f + t * t);
+        const float s = t * c;
+        A[p][p] = app - t * apq;
+        A[q][q] = aqq + t * apq;
+        A[p][q] = A[q][p] = 0.0f;
+        const int r3 = 3 - p - q;
+        const float arp = A[r3][p], arq = A[r3][q];
+        A[r3][p] = A[p][r3] = c * arp - s * arq;
+        A[r3][q] = A[q][r3] = c * arq + s * arp;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const float vrp = V[r][p], vrq = V[r][q];
+            V[r][p] = c * vrp - s * vrq;
+            V[r][q] = s * vrp + c * vrq;
+        }
+    }
+    ev[0] = A[0][0];
+    ev[1] = A[1][1];
+    ev[2] = A[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = i + 1; j < 3; ++j)
+            if (ev[i] > ev[j]) {
+                const float te = ev[i]; ev[i] = ev[j]; ev[j] = te;
+#pragma unroll
+                for (int r = 0; r < 3; ++r) { const float tv = V[r][i]; V[r][i] = V[r][j]; V[r][j] = tv; }
+            }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) Vout.m[c][r] = V[r][c];
+}
+
 // symmetric storage [00, 11, 22, 10, 20, 21] (forward.cu:7632-7652)
 __host__ __device__ __forceinline__ M3 m3_load_sym(float s0, float s1, float s2, float s3, float s4, float s5)
 {

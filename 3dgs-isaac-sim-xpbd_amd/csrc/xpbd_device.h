@@ -48,6 +48,9 @@ struct BodyDev {
     // volume constraint (forward.cu:8531-9243)
     int volumeOn;
     float volCompliance;
+    // per-node shape matching (forward.cu:6088-6383)
+    int shapeOn, shapeRobust;
+    float shapeCompliance, shapeBlend;
 };
 
 // forward.cu:128-136 KinCollider, same field meaning.

@@ -13,8 +13,12 @@
 #include "xpbd_capi.h"
 #include "xpbd_oracle_internal.h"
 #include "oracle_volume.h"
+#include "oracle_shapes.h"
 
 namespace xo {
+
+static std::vector<float> g_cM, g_cRmax, g_cRg;   // contact ellipsoids k^2 Sigma, max semi-axis, ground radius
+void contact_ground_radius_update();
 
 // Build (or rebuild after a parameter change) the volume clusters the way
 // ensureChainmailGPU / g_volTopoDirty do (forward.cu:10524-10530, 10842).
@@ -27,7 +31,11 @@ static void volume_ensure()
 }
 
 void extensions_on_init() { if (!vol.explicitTopo) vol.built = false; }
-void extensions_free() { vol = VolumeState(); }
+void extensions_free()
+{
+    vol = VolumeState();
+    g_cM.clear(); g_cRmax.clear(); g_cRg.clear();
+}
 void extensions_on_reset() {}
 void extensions_begin_step()
 {
@@ -37,18 +45,38 @@ void extensions_begin_step()
     }
 }
 void extensions_after_predict() {}
-void extensions_before_iterations() {}
+void extensions_before_iterations() { contact_ground_radius_update(); }
 bool extensions_distance(std::vector<V3>&, std::vector<V3>&, int) { return false; }
 void extensions_iteration(std::vector<V3>*& pin, std::vector<V3>*& pout, int)
 {
     if (s.useVolume && vol.built) volume_iteration(pin, pout);
+    if (s.useShape) {
+        shape_matching(*pin, *pout);
+        std::swap(pin, pout);
+    }
 }
 void extensions_self_collision(std::vector<V3>&) {}
 void extensions_after_iterations(std::vector<V3>&, bool) {}
-float contact_rmax(int) { return 0.0f; }
-bool contact_colliders_on() { return false; }
-float contact_support(int, float, float, float) { return 0.0f; }
-float ground_contact_radius(int) { return 0.0f; }
+// Contact ellipsoids (forward.cu:6733-6757, 7358-7399, 12219-12226)
+static bool contact_ready() { return (s.contactColliders || s.contactGround) && (int)g_cRmax.size() == g.n && g.n > 0; }
+bool contact_colliders_on() { return contact_ready() && s.contactColliders; }
+float contact_rmax(int i) { return contact_colliders_on() ? g_cRmax[i] : 0.0f; }
+float contact_support(int i, float nx, float ny, float nz)
+{
+    const float* m = &g_cM[(size_t)6 * i];
+    const float q = m[0] * nx * nx + m[3] * ny * ny + m[5] * nz * nz + 2.0f * (m[1] * nx * ny + m[2] * nx * nz + m[4] * ny * nz);
+    return std::sqrt(std::fmax(q, 0.0f));
+}
+float ground_contact_radius(int i)
+{
+    return (contact_ready() && s.contactGround && s.groundEnabled && (int)g_cRg.size() == g.n) ? g_cRg[i] : 0.0f;
+}
+void contact_ground_radius_update()
+{
+    if (!(contact_ready() && s.contactGround && s.groundEnabled)) return;
+    g_cRg.assign(g.n, 0.0f);
+    for (int i = 0; i < g.n; ++i) g_cRg[i] = contact_support(i, s.groundN[0], s.groundN[1], s.groundN[2]);
+}
 
 }  // namespace xo
 
@@ -96,22 +124,32 @@ void xpbd_self_collision_stats(int* out)
     out[0] = 0; out[1] = 0; out[2] = 0;
 }
 
+static float cap_or_zero(float c) { return (std::isfinite(c) && c > 0.0f) ? c : 0.0f; }
+
+// native/xpbd_capi.cu:179-191
 int xpbd_set_contact_shape(int enabled, float tau, float radiusCap)
 {
     if (g.n <= 0) return fail("xpbd_set_contact_shape: call xpbd_create first");
     g.contactOn = enabled != 0;
     s.contactTau = tau;
     s.contactCap = radiusCap;
+    if (g.contactOn) contact_pack(g.scales.data(), g.rots.data(), g.opacity.data(), tau, cap_or_zero(radiusCap), g_cM, g_cRmax);
     s.contactColliders = (enabled & 1) != 0;
     s.contactGround = (enabled & 2) != 0;
     return 0;
 }
 
-int xpbd_compute_shapes(float, float*, float*, int n)
+// native/xpbd_capi.cu:248-260
+int xpbd_compute_shapes(float deformEps, float* hostScales, float* hostQuats, int n)
 {
     if (!g.stepped) return fail("xpbd_compute_shapes: no step yet");
     if (n != g.n) return fail("xpbd_compute_shapes: size mismatch");
-    return fail("xpbd_compute_shapes: not restated in the oracle yet");
+    std::vector<float> sc, q;
+    const int k = deformed_shapes(deformEps, sc, q);
+    if (g.contactOn) contact_pack(sc.data(), q.data(), g.opacity.data(), s.contactTau, cap_or_zero(s.contactCap), g_cM, g_cRmax);
+    if (hostScales) std::copy(sc.begin(), sc.end(), hostScales);
+    if (hostQuats) std::copy(q.begin(), q.end(), hostQuats);
+    return k;
 }
 
 static float g_jvw = 1.0f, g_jvwStd = 0.0f;
