@@ -34,6 +34,9 @@ struct Global {
     bool volExplicit = false;
     std::vector<int> volOff, volMembers;
     int volMinMembers = 4;
+    // fixed self-collision grid (0 = the reference's displacement rule)
+    float scCell = 0.0f;
+    int scTable = 0;
     int n = 0;
     bool stepped = false;
     bool contactOn = false;
@@ -258,6 +261,10 @@ void xpbd_self_collision_stats(int* out)
 {
     if (!out) return;
     out[0] = 0; out[1] = 0; out[2] = 0;
+    Global& g = G();
+    if (!g.engine) return;
+    std::string err;
+    if (!g.engine->selfCollisionStats(0, out, err)) fail("xpbd_self_collision_stats: " + err);
 }
 
 void xpbd_set_shape_robust(int enabled) { G().set.shapeRobust = enabled != 0; }
@@ -274,6 +281,7 @@ int xpbd_step(void)
             return fail("xpbd_step: " + err);
         }
         push_colliders();
+        g.engine->setSelfCollisionGrid(0, g.scCell, g.scTable);
         if (g.volExplicit &&
             !g.engine->setVolumeClusters(0, g.volOff.data(), g.volMembers.data(), g.volMinMembers, err)) {
             g.engine.reset();
@@ -502,6 +510,28 @@ int xpbd_ext_get_inv_mass(float* invMass, int n)
     if (!g.stepped) return fail("xpbd_ext_get_inv_mass: no step yet");
     if (n != g.n || !invMass) return fail("xpbd_ext_get_inv_mass: size mismatch");
     if (!g.engine->getInvMass(0, invMass)) return fail("xpbd_ext_get_inv_mass: device copy failed");
+    return 0;
+}
+
+int xpbd_ext_set_self_collision_grid(float cellSize, int tableSize)
+{
+    Global& g = G();
+    if (tableSize != 0 && (tableSize < 1024 || (tableSize & (tableSize - 1))))
+        return fail("xpbd_ext_set_self_collision_grid: table size must be a power of two >= 1024");
+    g.scCell = (std::isfinite(cellSize) && cellSize > 0.0f) ? cellSize : 0.0f;
+    g.scTable = tableSize;
+    if (g.engine) g.engine->setSelfCollisionGrid(0, g.scCell, g.scTable);
+    return 0;
+}
+
+int xpbd_ext_self_collision_lists(unsigned int* keys, int* sorted, int* counts, int* contacts, int n)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_ext_self_collision_lists: no step yet");
+    if (n != g.n) return fail("xpbd_ext_self_collision_lists: size mismatch");
+    std::string err;
+    if (!g.engine->selfCollisionLists(0, keys, sorted, counts, contacts, err))
+        return fail("xpbd_ext_self_collision_lists: " + err);
     return 0;
 }
 

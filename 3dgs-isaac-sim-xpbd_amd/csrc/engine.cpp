@@ -2,6 +2,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -17,10 +18,16 @@ namespace xpbd {
         }                                                                                 \
     } while (0)
 
+// Bumped by every device allocation: a captured graph holds raw pointers, so
+// any reallocation must trigger a recapture.
+static std::atomic<int> g_allocEpoch{0};
+static int device_alloc_epoch() { return g_allocEpoch.load(); }
+
 template <typename T>
 void DevBuf<T>::alloc(size_t count)
 {
     free();
+    ++g_allocEpoch;
     n = count;
     if (count) {
         if (hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count) != hipSuccess) {
@@ -46,9 +53,15 @@ template struct DevBuf<BodyDev>;
 template struct DevBuf<ColliderDev>;
 template struct DevBuf<float2>;
 template struct DevBuf<unsigned int>;
+template struct DevBuf<unsigned char>;
+template struct DevBuf<double>;
 
 // wave64-per-cluster volume solve from this mean member count up
 static const double kVolWaveMembers = 24.0;
+// self-collision search radius and fast-pair threshold in units of d_c
+// (forward.cu:230, 262)
+static const float kScSearchScale = 1.5f;
+static const float kScFastRelScale = 0.0f;
 
 void build_input(BodyInput& in, int n, const float* pos, const float* scales, const float* rots, const float* opacity,
                  int e, const int* edges, const float* rest, const float* stiff)
@@ -135,6 +148,17 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     cM_.free(); cRmax_.free(); cRg_.free();
     contactOn_.assign(B, 0); contactColliders_.assign(B, 0); contactGround_.assign(B, 0);
     contactTau_.assign(B, 0.2f); contactCap_.assign(B, 0.0f);
+    objTopo_.assign(B, ObjTopo());
+    objBase_.assign(B, 0);
+    objShiftValid_.assign(B, 0);
+    objTotal_ = 0;
+    objDeviceValid_ = false;
+    scSpacing_.assign(B, 0.0f); scCellOverride_.assign(B, 0.0f);
+    scTable_.assign(B, 0); scTableBase_.assign(B, 0); scTableOverride_.assign(B, 0);
+    scTableTotal_ = 0; scSortBits_ = 0;
+    scDeviceValid_ = false;
+    scSortedIdx_ = nullptr;
+    scSortedKeys_ = nullptr;
 
     // host staging of the padded node space
     std::vector<float3> hx(nPad_, make_float3(0, 0, 0));
@@ -194,8 +218,11 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     launch_fill_float(stream_, nPad_, objW_.p, 1.0f);
     launch_fill_int(stream_, B * kMaxColliders, colHits_.p, 0);
     launch_fill_float(stream_, B * kMaxColliders * kColAcc, colPush_.p, 0.0f);
-    if (!hostBodies_) XPBD_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostBodies_), sizeof(BodyDev) * B, 0));
+    if (hostBodies_) { (void)hipHostFree(hostBodies_); hostBodies_ = nullptr; }
+    if (hostCols_) { (void)hipHostFree(hostCols_); hostCols_ = nullptr; }
+    XPBD_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostBodies_), sizeof(BodyDev) * B, 0));
     XPBD_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostCols_), sizeof(ColliderDev) * B * kMaxColliders, 0));
+    std::memset(hostBodies_, 0, sizeof(BodyDev) * B);
     lastBodies_.clear();
     alphaKey_.clear();
     XPBD_HIP(hipStreamSynchronize(stream_));
@@ -413,6 +440,28 @@ void Engine::fillBodyDev(int b, BodyDev& d) const
     // contact ellipsoids act once packed (forward.cu:12219-12226)
     d.contactColliders = (cM_.p && contactColliders_[b]) ? 1 : 0;
     d.contactGround = (cM_.p && contactGround_[b] && s.groundEnabled) ? 1 : 0;
+    // object shape after predict (forward.cu:12124-12166), off while pressing
+    const int K = objDeviceValid_ ? objTopo_[b].K : 0;
+    d.objOn = (s.objShapeStiffness > 0.0f && !ps.active && K > 0) ? 1 : 0;
+    d.objBase = objDeviceValid_ ? objBase_[b] : 0;
+    d.objK = K;
+    d.objWeighted = attActive_[b] ? 1 : 0;
+    d.objStiffness = s.objShapeStiffness;
+    d.objBoxes = (d.objOn && s.scEnabled) ? 1 : 0;
+    // active set only when the previous shift is valid (forward.cu:6959-7030)
+    d.objActiveAllowed = (d.objBoxes && !s.scWithinBody && objShiftValid_[b] && K >= 1 && K <= 64) ? 1 : 0;
+    // self-collision build / iterations (forward.cu:11796-11849)
+    const float dc = scDeviceValid_ ? s.scRadiusScale * scSpacing_[b] : 0.0f;
+    d.scOn = (s.scEnabled && !ps.active && dc > 0.0f) ? 1 : 0;
+    d.scWithin = s.scWithinBody ? 1 : 0;
+    d.scTableBase = scDeviceValid_ ? scTableBase_[b] : 0;
+    d.scTableMask = scDeviceValid_ ? scTable_[b] - 1 : 0;
+    d.scDc = dc;
+    d.scSearch = kScSearchScale * dc;
+    const float excl = s.scExcludeScale * dc;
+    d.scExcl2 = excl * excl;
+    d.scFastRel = kScFastRelScale * dc;
+    d.scCellOverride = scCellOverride_[b];
 }
 
 bool Engine::uploadParams(std::string& err)
@@ -451,7 +500,12 @@ std::vector<int> Engine::planSignature() const
     const int B = numBodies();
     int iters = set_[0].iters;
     int anyDist = 0, anyCol = 0, anyGround = 0, anyPress = 0, anyVol = 0, anyShape = 0, anyCGround = 0;
+    int anyObj = 0, anyBoxes = 0, anySc = 0;
     for (int b = 0; b < B; ++b) {
+        const BodyDev& d = hostBodies_[b];
+        anyObj |= d.objOn;
+        anyBoxes |= d.objBoxes;
+        anySc |= d.scOn;
         anyDist |= set_[b].useDistance ? 1 : 0;
         anyCol |= colCount_[b] > 0 ? 1 : 0;
         anyGround |= set_[b].groundEnabled ? 1 : 0;
@@ -462,7 +516,8 @@ std::vector<int> Engine::planSignature() const
     }
     const int volWave = (anyVol && volAvgMembers_ >= kVolWaveMembers) ? 1 : 0;
     return {iters, anyDist, anyCol, anyGround, anyPress, nPad_, anyVol, volWave, anyVol ? volActive_ : 0, anyShape,
-            anyCGround};
+            anyCGround, anyObj, anyBoxes, anySc, anyObj ? objTotal_ : 0, anySc ? scTableTotal_ : 0,
+            anySc ? scSortBits_ : 0, device_alloc_epoch()};
 }
 
 // Event pairs around kernel launches (profile mode only).
@@ -513,6 +568,22 @@ void Engine::recordSubstep(Timer* t)
     XPBD_T_BEGIN("predict");
     launch_predict(stream_, na, x_.p, v_.p, invMass_.p, p0_.p);
     XPBD_T_END();
+    const bool objPass = sig[11] != 0, scPass = sig[13] != 0;
+    const SelfColArgs sa = scArgs();
+    if (objPass) {
+        XPBD_T_BEGIN("object_shape");
+        launch_object_shape(stream_, na, sa, objTotal_, p0_.p, rest_.p, sig[12] != 0);
+        XPBD_T_END();
+    }
+    if (scPass) {
+        XPBD_T_BEGIN("selfcol_build");
+        const int* sorted = nullptr;
+        const bool inAlt = launch_selfcol_build(stream_, na, numBodies(), sa, x_.p, p0_.p, rest_.p, scSortBits_,
+                                                scKeysAlt_.p, scIdxAlt_.p, scHist_.p, &sorted);
+        XPBD_T_END();
+        scSortedIdx_ = sorted;
+        scSortedKeys_ = inAlt ? scKeysAlt_.p : scKeys_.p;
+    }
     if (anyCol) {
         const int B = numBodies();
         XPBD_T_BEGIN("collider_stats_clear");
@@ -528,13 +599,19 @@ void Engine::recordSubstep(Timer* t)
         if (anyPress) epi |= kEpiPress;
         if (anyCol) epi |= kEpiColliders | (last ? kEpiColliderHits : 0u);
         if (anyGround) epi |= kEpiGround;
-        if (last) epi |= kEpiVelocity;
+        // with self-collision the velocity commit moves behind the contact push
+        if (last && !scPass) epi |= kEpiVelocity;
         const bool volPass = sig[6] != 0;
+        const bool constraintPass = anyDist || volPass || shapePass;
+        // the press slab precedes self-collision; colliders and ground follow it
+        // (forward.cu:12474-12560)
+        const unsigned preEpi = scPass ? (constraintPass ? (epi & kEpiPress) : 0u) : epi;
+        const unsigned scEpi = constraintPass ? (epi & ~kEpiPress) : epi;
         if (anyDist) {
             XPBD_T_BEGIN(it == 0 ? (last ? "distance_jacobi_only" : "distance_jacobi_first")
                                  : (last ? "distance_jacobi_last" : "distance_jacobi_mid"));
             launch_distance_jacobi(stream_, na, it == 0, last, pin, pout, rowOff_.p, nbr_.p, nrest_.p, nalpha_.p,
-                                   lam_.p, (volPass || shapePass) ? 0u : epi, ea);
+                                   lam_.p, (volPass || shapePass) ? 0u : preEpi, ea);
             XPBD_T_END();
             std::swap(pin, pout);
         }
@@ -544,21 +621,33 @@ void Engine::recordSubstep(Timer* t)
             launch_vol_solve(stream_, na, sig[7] != 0, it == 0, last, pin, va);
             XPBD_T_END();
             XPBD_T_BEGIN("volume_apply");
-            launch_vol_apply(stream_, na, pin, pout, va, shapePass ? 0u : epi, ea);
+            launch_vol_apply(stream_, na, pin, pout, va, shapePass ? 0u : preEpi, ea);
             XPBD_T_END();
             std::swap(pin, pout);
         }
         if (shapePass) {
             XPBD_T_BEGIN("shape_match");
-            launch_shape_match(stream_, na, pin, pout, rest_.p, rowOff_.p, nbr_.p, epi, ea);
+            launch_shape_match(stream_, na, pin, pout, rest_.p, rowOff_.p, nbr_.p, preEpi, ea);
             XPBD_T_END();
             std::swap(pin, pout);
         }
-        if (!anyDist && !volPass && !shapePass && epi) {
+        if (scPass) {
+            XPBD_T_BEGIN("selfcol_iteration");
+            launch_selfcol_iteration(stream_, na, sa, pin, pout, x_.p, rest_.p, scEpi, ea);
+            XPBD_T_END();
+            std::swap(pin, pout);
+        } else if (!constraintPass && epi) {
             XPBD_T_BEGIN("epilogue");
             launch_epilogue(stream_, na, pin, epi, ea);
             XPBD_T_END();
         }
+    }
+    if (scPass) {
+        // contact-average push, ground re-projection and velocity commit
+        // (forward.cu:11858-11890, 12588-12596)
+        XPBD_T_BEGIN("contact_push_commit");
+        launch_contact_push(stream_, na, sa, objPass ? objTotal_ : 0, pin, ea);
+        XPBD_T_END();
     }
 }
 
@@ -578,11 +667,26 @@ bool Engine::rebuildGraph(std::string& err)
 bool Engine::prepare(std::string& err)
 {
     for (int b = 0; b < numBodies(); ++b) pressAdvance(b);
+    bool anyObj = false, anySc = false;
+    for (int b = 0; b < numBodies(); ++b) {
+        anyObj |= set_[b].objShapeStiffness > 0.0f || set_[b].scEnabled;
+        anySc |= set_[b].scEnabled;
+    }
+    if (anyObj && !ensureObjects(err)) return false;
+    if (anySc && !ensureSelfCol(err)) return false;
     if (!uploadParams(err)) return false;
     bool anyVol = false;
     for (int b = 0; b < numBodies(); ++b) anyVol |= set_[b].useVolume;
     if (anyVol && !ensureVolume(false, err)) return false;
     return true;
+}
+
+// The object shift of a body that ran object shape matching is a valid
+// reference for the next substep's active set.
+void Engine::afterSubstep()
+{
+    for (int b = 0; b < numBodies(); ++b)
+        if (hostBodies_[b].objOn) objShiftValid_[b] = 1;
 }
 
 bool Engine::step(int substeps, std::string& err)
@@ -593,6 +697,7 @@ bool Engine::step(int substeps, std::string& err)
             if (!rebuildGraph(err)) return false;
         }
         XPBD_HIP(hipGraphLaunch(graphExec_, stream_));
+        afterSubstep();
     }
     colStatsValid_ = true;
     return true;
@@ -605,6 +710,7 @@ bool Engine::profile(int substeps, std::string& json, std::string& err)
     for (int s = 0; s < substeps; ++s) {
         if (!prepare(err)) return false;
         recordSubstep(&t);
+        afterSubstep();
     }
     colStatsValid_ = true;
     XPBD_HIP(hipStreamSynchronize(stream_));

@@ -143,10 +143,22 @@ public:
     // Contact ellipsoids (native/xpbd_capi.cu:179-191): bit 1 colliders, bit 2 ground.
     bool setContactShape(int b, int enabled, float tau, float radiusCap, std::string& err);
 
+    // Self-collision (native/xpbd_capi.cu:203-212): statistics of the last build
+    // (candidates, active particles, overflow), an optional fixed grid
+    // (cell > 0 replaces the displacement rule; table a power of two >= 1024),
+    // and the build's artefacts in body-local ids.
+    bool selfCollisionStats(int b, int out[3], std::string& err);
+    void setSelfCollisionGrid(int b, float cellSize, int tableSize);
+    bool selfCollisionLists(int b, unsigned int* keys, int* sorted, int* counts, int* contacts, std::string& err);
+
 private:
     bool ensureShapeBuffers(std::string& err);
+    bool ensureObjects(std::string& err);
+    bool ensureSelfCol(std::string& err);
+    SelfColArgs scArgs() const;
     bool ensureVolume(bool needAll, std::string& err);
     bool prepare(std::string& err);   // per-substep host work before launching
+    void afterSubstep();              // host bookkeeping after a substep is queued
     void pressAdvance(int b);
     void fillBodyDev(int b, BodyDev& d) const;
     bool uploadParams(std::string& err);
@@ -209,6 +221,35 @@ private:
     DevBuf<unsigned int> shapeCtr_;
     std::vector<char> contactOn_, contactColliders_, contactGround_;
     std::vector<float> contactTau_, contactCap_;
+
+    // object components (forward.cu:11303-11439) and object shape matching
+    struct ObjTopo {
+        int K = 0, minComp = -1;
+        std::vector<int> id, count;
+        std::vector<float> c0;
+        std::vector<double> restRel, sumQ;
+    };
+    std::vector<ObjTopo> objTopo_;
+    std::vector<int> objBase_;
+    std::vector<char> objShiftValid_;
+    int objTotal_ = 0;
+    bool objDeviceValid_ = false;
+    DevBuf<int> objId_, objBody_, objCount_;
+    DevBuf<float> objC0_, objParams_, objDisp_, objBoxes_, pushSums_, pushT_;
+    DevBuf<double> objSumQ_, objShift_, objRestRel_, objSums_;
+
+    // self-collision: one hash table per body inside shared cell arrays
+    std::vector<float> scSpacing_, scCellOverride_;
+    std::vector<int> scTable_, scTableBase_, scTableOverride_;
+    int scTableTotal_ = 0, scSortBits_ = 0;
+    bool scDeviceValid_ = false;
+    const int* scSortedIdx_ = nullptr;
+    const unsigned int* scSortedKeys_ = nullptr;
+    DevBuf<unsigned int> scMax2_, scKeys_, scKeysAlt_, scHist_;
+    DevBuf<int> scCounters_, scUseActive_, scIdx_, scIdxAlt_, cellStart_, cellEnd_, contacts_, counts_;
+    DevBuf<float> scBodyDisp_, scInvCell_;
+    DevBuf<unsigned char> scActive_;
+    DevBuf<float3> scDp_, scDpCross_, scAccum_;
 
     // graph
     hipGraph_t graph_ = nullptr;

@@ -51,6 +51,62 @@ void launch_vol_apply(hipStream_t s, const NodeArgs& na, const float4* Pin, floa
                       unsigned epi, const EpiArgs& ea);
 void launch_vol_J(hipStream_t s, const NodeArgs& na, const float3* X, const VolumeArgs& va, float* outJ);
 
+constexpr int kSelfColCap = 32;   // forward.cu:223 SELFCOL_MAX_CONTACTS
+
+// Object-shape and self-collision buffers.  Objects are numbered globally
+// (objBase of each body + local component id).
+struct SelfColArgs {
+    const int* tileBody;
+    const BodyDev* bodies;
+    // objects
+    const int* objId;          // [nPad] global object id or -1
+    const int* objBody;        // [K]
+    const int* objCount;       // [K]
+    const float* objC0;        // [3K]
+    const double* objSumQ;     // [3K]
+    double* objShift;          // [3K]
+    const double* objRestRel;  // [3 nPad]
+    const float* objW;         // [nPad] fit weights while attached
+    double* objSums;           // [16K]
+    float* objParams;          // [16K]
+    float* objDisp;            // [K]
+    float* objBoxesArr;        // [6K]
+    // self-collision
+    unsigned int* scMax2;      // [B] max |p - x|^2 bits
+    int* scCounters;           // [3B] candidates, overflow, active
+    int* scUseActive;          // [B]
+    float* scBodyDisp;         // [B]
+    float* scInvCell;          // [B]
+    unsigned char* scActive;   // [nPad]
+    unsigned int* keys;        // [nPad]
+    int* idx;                  // [nPad]
+    unsigned int sentinel;     // = tableTotal
+    int tableTotal;
+    int* cellStart;            // [tableTotal]
+    int* cellEnd;
+    int* contacts;             // [nPad * kSelfColCap]
+    int* counts;               // [nPad]
+    float3* dp;
+    float3* dpCross;
+    float3* accum;
+    float* pushSums;           // [4K]
+    float* pushT;              // [4K]
+};
+
+void launch_object_shape(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, int K, float4* P, const float3* rest,
+                         bool boxes);
+// Returns true when the sorted (keys, idx) ended in the alternate buffers.
+bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, const SelfColArgs& a, const float3* X,
+                          const float4* P, const float3* rest, int sortBits, unsigned int* keysAlt, int* idxAlt,
+                          unsigned int* hist, const int** sortedIdxOut);
+void launch_selfcol_iteration(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, const float4* Pin,
+                              float4* Pout, const float3* X, const float3* rest, unsigned epi, const EpiArgs& ea);
+void launch_contact_push(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, int K, float4* P, const EpiArgs& ea);
+int radix_sort_tiles(int n);
+bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0, unsigned int* k1, int* v1,
+                       unsigned int* hist);
+void launch_fill_double(hipStream_t s, int n, double* a);
+
 void launch_shape_match(hipStream_t s, const NodeArgs& na, const float4* Pin, float4* Pout, const float3* rest,
                         const int* rowOff, const int* nbr, unsigned epi, const EpiArgs& ea);
 // scales / rots / outScale / outQuat are the body's own arrays (local index);

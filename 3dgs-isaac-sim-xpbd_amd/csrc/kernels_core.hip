@@ -238,4 +238,15 @@ void launch_fill_int(hipStream_t s, int n, int* a, int v)
     if (n > 0) fill_int_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, a, v);
 }
 
+__global__ void fill_double_zero_kernel(int n, double* a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+
+void launch_fill_double(hipStream_t s, int n, double* a)
+{
+    if (n > 0) fill_double_zero_kernel<<<blocks_for(n, 256), 256, 0, s>>>(n, a);
+}
+
 }  // namespace xpbd

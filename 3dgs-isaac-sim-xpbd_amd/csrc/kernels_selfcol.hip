@@ -1,0 +1,639 @@
+// kernels_selfcol.hip — object (connected-component) shape matching and the
+// spatial-hash self-collision of the MI355X XPBD solver.
+//
+// Object shape (forward.cu:6801-6937, 11671-11702): per object, fp64 sums of
+// (p - s) and (p - s) q^T (q = x_rest - c0, s = last centroid), wave-reduced
+// when a wave is one object, then fp64 polar decomposition and
+// p += stiffness * (c + R q - p).  Self-collision (forward.cu:6986-7322,
+// 11796-11849): max predicted displacement -> cell size, cell keys of the
+// frame-start positions, stable radix sort, cell ranges, per-node candidate
+// lists (27 cells, 32 slots, deterministic worst replacement), and per
+// iteration a read pass (separation to d_c / averaged-normal push for fast
+// pairs) and a write pass that also hosts the iteration epilogue.
+#include "kernels.h"
+#include "mat3.h"
+#include "node_ops.h"
+#include "polar.h"
+
+namespace xpbd {
+
+
+__device__ __forceinline__ const BodyDev& body_at(const SelfColArgs& a, int i)
+{
+    return a.bodies[__builtin_amdgcn_readfirstlane(a.tileBody[i / kTile])];
+}
+
+// ---------------------------------------------------------------------------
+// Object shape matching
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) obj_accum_kernel(int nPad, SelfColArgs a, const float4* __restrict__ P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int id = -1;
+    double v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = 0.0;
+    int stride = 12;
+    if (i < nPad) {
+        const BodyDev& B = body_at(a, i);
+        stride = B.objWeighted ? 16 : 12;
+        if (B.objOn && i < B.nodeEnd) id = a.objId[i];
+        if (id >= 0) {
+            const float4 p = P[i];
+            const double dx = (double)p.x - a.objShift[3 * id], dy = (double)p.y - a.objShift[3 * id + 1],
+                         dz = (double)p.z - a.objShift[3 * id + 2];
+            const double qx = a.objRestRel[3 * (size_t)i], qy = a.objRestRel[3 * (size_t)i + 1],
+                         qz = a.objRestRel[3 * (size_t)i + 2];
+            v[0] = dx; v[1] = dy; v[2] = dz;
+            v[3] = dx * qx; v[4] = dx * qy; v[5] = dx * qz;
+            v[6] = dy * qx; v[7] = dy * qy; v[8] = dy * qz;
+            v[9] = dz * qx; v[10] = dz * qy; v[11] = dz * qz;
+            if (B.objWeighted) {
+                const double w = (double)a.objW[i];
+#pragma unroll
+                for (int k = 0; k < 12; ++k) v[k] *= w;
+                v[12] = w; v[13] = w * qx; v[14] = w * qy; v[15] = w * qz;
+            }
+        }
+    }
+    const int id0 = __shfl(id, 0, 64);
+    const bool uniform = __all(id == id0);
+    if (uniform) {
+        if (id0 < 0) return;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            if (k >= stride) break;
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) v[k] += __shfl_xor(v[k], m, 64);
+        }
+        if (lane < stride) {
+            double mine = 0.0;
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+                if (k == lane) mine = v[k];
+            atomicAdd(&a.objSums[16 * (size_t)id0 + lane], mine);
+        }
+    } else if (id >= 0) {
+        for (int k = 0; k < stride; ++k) atomicAdd(&a.objSums[16 * (size_t)id + k], v[k]);
+    }
+}
+
+// One thread per object (forward.cu:6887-6937).
+__global__ void obj_solve_kernel(int K, SelfColArgs a)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int b = a.objBody[k];
+    const BodyDev& B = a.bodies[b];
+    if (!B.objOn) return;
+    float* Pm = &a.objParams[16 * (size_t)k];
+    const int n = a.objCount[k];
+    if (n < 4) { Pm[15] = 0.0f; a.objDisp[k] = 0.0f; return; }
+    const double* S = &a.objSums[16 * (size_t)k];
+    const bool wOn = B.objWeighted != 0;
+    const double W = wOn ? S[12] : (double)n;
+    if (!(W > 0.0)) { Pm[15] = 0.0f; a.objDisp[k] = 0.0f; return; }
+    const double inv = 1.0 / W;
+    const double dc[3] = {S[0] * inv, S[1] * inv, S[2] * inv};
+    const double* sq = wOn ? &S[13] : &a.objSumQ[3 * (size_t)k];
+    double m[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) m[3 * r + c] = S[3 + 3 * r + c] - dc[r] * sq[c];
+    double cNow[3];
+    for (int t = 0; t < 3; ++t) {
+        cNow[t] = a.objShift[3 * k + t] + dc[t];
+        a.objShift[3 * k + t] = cNow[t];
+    }
+    a.objDisp[k] = (float)sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]);
+    double fn = 0.0;
+    for (int t = 0; t < 9; ++t) fn += m[t] * m[t];
+    fn = sqrt(fn);
+    if (!(fn > 1e-300) || !isfinite(fn)) { Pm[15] = 0.0f; return; }
+    M3 A, R;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) A.m[c][r] = (float)(m[3 * r + c] / fn);
+    if (!polar_rotation_f64(A, R)) { Pm[15] = 0.0f; return; }
+    for (int t = 0; t < 3; ++t) {
+        Pm[t] = wOn ? (float)((double)a.objC0[3 * k + t] + S[13 + t] * inv) : a.objC0[3 * k + t];
+        Pm[3 + t] = (float)cNow[t];
+        for (int cc = 0; cc < 3; ++cc) Pm[6 + 3 * t + cc] = R.m[cc][t];
+    }
+    Pm[15] = 1.0f;
+}
+
+__device__ __forceinline__ void atomic_min_f(float* p, float v)
+{
+    v += 0.0f;
+    if (v >= 0.0f) atomicMin((int*)p, __float_as_int(v));
+    else atomicMax((unsigned int*)p, __float_as_uint(v));
+}
+
+__device__ __forceinline__ void atomic_max_f(float* p, float v)
+{
+    v += 0.0f;
+    if (v >= 0.0f) atomicMax((int*)p, __float_as_int(v));
+    else atomicMin((unsigned int*)p, __float_as_uint(v));
+}
+
+__global__ void obj_boxes_init_kernel(int K, float* boxes)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 6 * K) boxes[t] = ((t % 6) < 3) ? 3.402823466e+38f : -3.402823466e+38f;
+}
+
+// Object boxes of the predicted positions (forward.cu:6959-6981), then the
+// shape-matching move (6801-6823).  Boxes come first: they see the positions
+// before the move, as in the reference.
+__global__ void __launch_bounds__(256) obj_boxes_kernel(int nPad, SelfColArgs a, const float4* __restrict__ P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;
+    const BodyDev& B = body_at(a, i);
+    if (!B.objOn || !B.objBoxes || i >= B.nodeEnd) return;
+    const int id = a.objId[i];
+    const float4 p = P[i];
+    if (id < 0 || !isfinite(p.x) || !isfinite(p.y) || !isfinite(p.z)) return;
+    float* b = &a.objBoxesArr[6 * (size_t)id];
+    atomic_min_f(&b[0], p.x); atomic_min_f(&b[1], p.y); atomic_min_f(&b[2], p.z);
+    atomic_max_f(&b[3], p.x); atomic_max_f(&b[4], p.y); atomic_max_f(&b[5], p.z);
+}
+
+__global__ void __launch_bounds__(256) obj_match_kernel(int nPad, SelfColArgs a, float4* __restrict__ P,
+                                                       const float3* __restrict__ rest)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;
+    const BodyDev& B = body_at(a, i);
+    if (!B.objOn || i >= B.nodeEnd) return;
+    const int id = a.objId[i];
+    const float4 p = P[i];
+    if (id < 0 || p.w <= 0.0f) return;
+    const float* Pm = &a.objParams[16 * (size_t)id];
+    if (Pm[15] == 0.0f) return;
+    const float3 r = rest[i];
+    const float qx = r.x - Pm[0], qy = r.y - Pm[1], qz = r.z - Pm[2];
+    const float gx = Pm[3] + Pm[6] * qx + Pm[7] * qy + Pm[8] * qz;
+    const float gy = Pm[4] + Pm[9] * qx + Pm[10] * qy + Pm[11] * qz;
+    const float gz = Pm[5] + Pm[12] * qx + Pm[13] * qy + Pm[14] * qz;
+    const float s = B.objStiffness;
+    P[i] = make_float4(p.x + s * (gx - p.x), p.y + s * (gy - p.y), p.z + s * (gz - p.z), p.w);
+}
+
+// ---------------------------------------------------------------------------
+// Self-collision build
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned int sc_hash(int cx, int cy, int cz, unsigned int mask)
+{
+    return (((unsigned int)cx * 73856093u) ^ ((unsigned int)cy * 19349663u) ^ ((unsigned int)cz * 83492791u)) & mask;
+}
+
+__device__ __forceinline__ int sc_cell(float v, float invCell)
+{
+    return (int)floorf(fminf(fmaxf(v * invCell, -1.0e9f), 1.0e9f));
+}
+
+// Per body: clear the build counters and the max displacement, and decide
+// whether the active restriction applies (boxes valid, finite body motion).
+__global__ void sc_prepare_kernel(int B, SelfColArgs a)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const BodyDev& D = a.bodies[b];
+    a.scMax2[b] = 0u;
+    a.scCounters[3 * b] = 0; a.scCounters[3 * b + 1] = 0; a.scCounters[3 * b + 2] = 0;
+    float maxDisp = 0.0f;
+    for (int k = 0; k < D.objK; ++k) maxDisp = fmaxf(maxDisp, a.objDisp[D.objBase + k]);
+    a.scUseActive[b] = (D.scOn && D.objActiveAllowed && isfinite(maxDisp)) ? 1 : 0;
+    a.scBodyDisp[b] = maxDisp;
+}
+
+// Active mask (forward.cu:6998-7030) and max |p - x|^2 over active nodes (7058-7068).
+__global__ void __launch_bounds__(256) sc_active_disp_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
+                                                            const float4* __restrict__ P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;
+    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
+    const BodyDev& B = a.bodies[b];
+    if (!B.scOn || i >= B.nodeEnd) return;
+    const float3 x = X[i];
+    const float4 p = P[i];
+    unsigned char act = 1;
+    if (a.scUseActive[b]) {
+        const int id = a.objId[i];
+        act = 0;
+        if (id < 0) {
+            act = 1;
+        } else {
+            const float inflate = B.scSearch + 2.0f * a.scBodyDisp[b] + 2.0f * B.scDc;
+            for (int k = 0; k < B.objK; ++k) {
+                const int gk = B.objBase + k;
+                if (gk == id) continue;
+                const float* bx = &a.objBoxesArr[6 * (size_t)gk];
+                const float lx = bx[0] - inflate, ly = bx[1] - inflate, lz = bx[2] - inflate;
+                const float hx = bx[3] + inflate, hy = bx[4] + inflate, hz = bx[5] + inflate;
+                const bool inP = p.x >= lx && p.x <= hx && p.y >= ly && p.y <= hy && p.z >= lz && p.z <= hz;
+                const bool inX = x.x >= lx && x.x <= hx && x.y >= ly && x.y <= hy && x.z >= lz && x.z <= hz;
+                if (inP || inX) { act = 1; break; }
+            }
+        }
+        if (act) atomicAdd(&a.scCounters[3 * b + 2], 1);
+    }
+    a.scActive[i] = act;
+    if (!act) return;
+    const float dx = p.x - x.x, dy = p.y - x.y, dz = p.z - x.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    if (isfinite(d2) && d2 > 0.0f) atomicMax(&a.scMax2[b], __float_as_uint(d2));
+}
+
+// Cell size per body: search + 2 max displacement, capped at search + 16 d_c
+// (forward.cu:11820-11831), or the fixed override.
+__global__ void sc_reach_kernel(int B, SelfColArgs a)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const BodyDev& D = a.bodies[b];
+    const float maxDisp2 = __uint_as_float(a.scMax2[b]);
+    const float maxDisp = (isfinite(maxDisp2) && maxDisp2 > 0.0f) ? sqrtf(maxDisp2) : 0.0f;
+    float reach = D.scSearch + 2.0f * maxDisp;
+    const float cap = D.scSearch + 16.0f * D.scDc;
+    if (D.scCellOverride > 0.0f) reach = D.scCellOverride;
+    else if (reach > cap) reach = cap;
+    a.scInvCell[b] = 1.0f / reach;
+}
+
+// Keys of the frame-start positions (forward.cu:7034-7044) in the body's table.
+__global__ void __launch_bounds__(256) sc_keys_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;
+    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
+    const BodyDev& B = a.bodies[b];
+    unsigned int key = a.sentinel;
+    if (B.scOn && i < B.nodeEnd && a.scActive[i]) {
+        const float3 p = X[i];
+        if (isfinite(p.x) && isfinite(p.y) && isfinite(p.z)) {
+            const float inv = a.scInvCell[b];
+            key = (unsigned int)B.scTableBase +
+                  sc_hash(sc_cell(p.x, inv), sc_cell(p.y, inv), sc_cell(p.z, inv), (unsigned int)B.scTableMask);
+        }
+    }
+    a.keys[i] = key;
+}
+
+__global__ void __launch_bounds__(256) sc_range_kernel(int nPad, SelfColArgs a, const unsigned int* __restrict__ keysSorted)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;
+    const unsigned int k = keysSorted[i];
+    if (k >= a.sentinel) return;
+    if (i == 0 || keysSorted[i - 1] != k) a.cellStart[k] = i;
+    if (i == nPad - 1 || keysSorted[i + 1] != k) a.cellEnd[k] = i + 1;
+}
+
+// Candidate lists (forward.cu:7079-7180).
+__global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
+                                                      const float4* __restrict__ P, const float3* __restrict__ rest,
+                                                      const int* __restrict__ sortedIdx)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;
+    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
+    const BodyDev& B = a.bodies[b];
+    if (!B.scOn || i >= B.nodeEnd) return;
+    constexpr int CAP = kSelfColCap;
+    const float3 x = X[i];
+    const float4 p4 = P[i];
+    int cnt = 0, overflow = 0;
+    if (a.scActive[i] && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p4.x) && isfinite(p4.y) &&
+        isfinite(p4.z)) {
+        const float3 q = rest[i];
+        const float dix = p4.x - x.x, diy = p4.y - x.y, diz = p4.z - x.z;
+        const int oi = a.objId[i];
+        const float search = B.scSearch, search2 = search * search;
+        const float inv = a.scInvCell[b];
+        const unsigned int mask = (unsigned int)B.scTableMask;
+        const int cx = sc_cell(x.x, inv), cy = sc_cell(x.y, inv), cz = sc_cell(x.z, inv);
+        int* out = &a.contacts[(size_t)i * CAP];
+        unsigned int hs[27];
+        float prio[CAP];
+        int tier[CAP];
+        int nh = 0;
+        for (int oz = -1; oz <= 1; ++oz)
+            for (int oy = -1; oy <= 1; ++oy)
+                for (int ox = -1; ox <= 1; ++ox) {
+                    const unsigned int h = sc_hash(cx + ox, cy + oy, cz + oz, mask);
+                    bool dup = false;
+                    for (int m = 0; m < nh; ++m)
+                        if (hs[m] == h) { dup = true; break; }
+                    if (dup) continue;
+                    hs[nh++] = h;
+                    const int s = a.cellStart[B.scTableBase + h];
+                    if (s < 0) continue;
+                    const int e = a.cellEnd[B.scTableBase + h];
+                    for (int t = s; t < e; ++t) {
+                        const int j = sortedIdx[t];
+                        if (j == i) continue;
+                        const bool cross = a.objId[j] != oi;
+                        if (!cross && !B.scWithin) continue;
+                        const float4 pj = P[j];
+                        const float3 qj = rest[j];
+                        const float qx = q.x - qj.x, qy = q.y - qj.y, qz = q.z - qj.z;
+                        const float q2 = qx * qx + qy * qy + qz * qz;
+                        float pr;
+                        int code = j;
+                        if (cross) {
+                            const float3 xj = X[j];
+                            const float ex = x.x - xj.x, ey = x.y - xj.y, ez = x.z - xj.z;
+                            const float distX = sqrtf(ex * ex + ey * ey + ez * ez);
+                            const float rx = dix - (pj.x - xj.x), ry = diy - (pj.y - xj.y), rz = diz - (pj.z - xj.z);
+                            const float rel = sqrtf(rx * rx + ry * ry + rz * rz);
+                            if (!(distX < search + rel)) continue;
+                            if (!(q2 >= B.scDc * B.scDc)) continue;
+                            pr = distX - rel;
+                            if (rel > B.scFastRel) code = -(j + 1);
+                        } else {
+                            const float ex = p4.x - pj.x, ey = p4.y - pj.y, ez = p4.z - pj.z;
+                            const float d2 = ex * ex + ey * ey + ez * ez;
+                            if (!(d2 < search2)) continue;
+                            if (!(q2 >= B.scExcl2)) continue;
+                            pr = sqrtf(d2);
+                        }
+                        const int tr = cross ? 0 : 1;
+                        if (cnt < CAP) {
+                            out[cnt] = code;
+                            prio[cnt] = pr;
+                            tier[cnt] = tr;
+                            ++cnt;
+                        } else {
+                            int worst = 0;
+                            for (int m = 1; m < CAP; ++m)
+                                if (tier[m] > tier[worst] || (tier[m] == tier[worst] && prio[m] > prio[worst])) worst = m;
+                            if (tr < tier[worst] || (tr == tier[worst] && pr < prio[worst])) {
+                                out[worst] = code;
+                                prio[worst] = pr;
+                                tier[worst] = tr;
+                            }
+                            ++overflow;
+                        }
+                    }
+                }
+    }
+    a.counts[i] = cnt;
+    if (cnt > 0) atomicAdd(&a.scCounters[3 * b], cnt);
+    if (overflow > 0) atomicAdd(&a.scCounters[3 * b + 1], overflow);
+}
+
+__global__ void sc_fill_accum_kernel(int nPad, float3* accum)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nPad) accum[i] = make_float3(0.0f, 0.0f, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Self-collision per iteration
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool sc_start_normal(float3 xi, float3 xj, float3 qi, float3 qj, float& nx, float& ny, float& nz)
+{
+    nx = xi.x - xj.x; ny = xi.y - xj.y; nz = xi.z - xj.z;
+    float l2 = nx * nx + ny * ny + nz * nz;
+    if (!(l2 > 1e-24f)) {
+        nx = qi.x - qj.x; ny = qi.y - qj.y; nz = qi.z - qj.z;
+        l2 = nx * nx + ny * ny + nz * nz;
+        if (!(l2 > 1e-24f)) return false;
+    }
+    const float inv = 1.0f / sqrtf(l2);
+    nx *= inv; ny *= inv; nz *= inv;
+    return true;
+}
+
+// Read pass (forward.cu:7205-7307).
+__global__ void __launch_bounds__(256) sc_solve_kernel(int nPad, SelfColArgs a, const float4* __restrict__ P,
+                                                      const float3* __restrict__ X, const float3* __restrict__ rest)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;
+    const BodyDev& B = body_at(a, i);
+    if (!B.scOn || i >= B.nodeEnd) return;
+    const int cnt = a.counts[i];
+    if (cnt <= 0) return;
+    constexpr int CAP = kSelfColCap;
+    const float4 pi4 = P[i];
+    const float wi = pi4.w * B.invMassScale;
+    const float dc = B.scDc;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f, cpx = 0.0f, cpy = 0.0f, cpz = 0.0f;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f, wFast = 0.0f;
+    int act = 0, nFast = 0;
+    const int* list = &a.contacts[(size_t)i * CAP];
+    const float3 xi = X[i], qi = rest[i];
+    if (wi > 0.0f) {
+        const float dc2 = dc * dc;
+        for (int k = 0; k < cnt; ++k) {
+            const int code = list[k];
+            if (code < 0) {
+                const int j = -code - 1;
+                float nx, ny, nz;
+                if (!sc_start_normal(xi, X[j], qi, rest[j], nx, ny, nz)) continue;
+                const float4 pj = P[j];
+                const float proj = (pi4.x - pj.x) * nx + (pi4.y - pj.y) * ny + (pi4.z - pj.z) * nz;
+                if (!(proj < dc)) continue;
+                const float pen = dc - proj;
+                sx += nx * pen; sy += ny * pen; sz += nz * pen;
+                const float wj = pj.w * B.invMassScale;
+                wFast += wi / (wi + wj);
+                ++nFast;
+                continue;
+            }
+            const int j = code;
+            const float4 pj = P[j];
+            const float dx = pi4.x - pj.x, dy = pi4.y - pj.y, dz = pi4.z - pj.z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (!(d2 < dc2)) continue;
+            const float wj = pj.w * B.invMassScale;
+            const float wr = wi / (wi + wj);
+            if (d2 < 1e-24f) {
+                const float3 qj = rest[j];
+                const float rx = qi.x - qj.x, ry = qi.y - qj.y, rz = qi.z - qj.z;
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                if (!(r2 > 1e-24f)) continue;
+                const float s = dc * wr / sqrtf(r2);
+                ax += rx * s; ay += ry * s; az += rz * s;
+                ++act;
+                continue;
+            }
+            const float d = sqrtf(d2);
+            const float s = (dc - d) / d * wr;
+            ax += dx * s; ay += dy * s; az += dz * s;
+            ++act;
+        }
+    }
+    float3 out = (act > 0) ? make_float3(ax / act, ay / act, az / act) : make_float3(0.0f, 0.0f, 0.0f);
+    if (nFast > 0) {
+        const float sl = sqrtf(sx * sx + sy * sy + sz * sz);
+        if (sl > 1e-20f) {
+            const float bx = sx / sl, by = sy / sl, bz = sz / sl;
+            float delta = 0.0f;
+            for (int k = 0; k < cnt; ++k) {
+                const int code = list[k];
+                if (code >= 0) continue;
+                const int j = -code - 1;
+                float nx, ny, nz;
+                if (!sc_start_normal(xi, X[j], qi, rest[j], nx, ny, nz)) continue;
+                const float4 pj = P[j];
+                const float ddx = pi4.x - pj.x, ddy = pi4.y - pj.y, ddz = pi4.z - pj.z;
+                if (!(ddx * nx + ddy * ny + ddz * nz < dc)) continue;
+                const float pen = dc - (ddx * bx + ddy * by + ddz * bz);
+                if (pen > delta) delta = pen;
+            }
+            if (delta > 0.0f) {
+                const float sc = delta * (wFast / (float)nFast);
+                cpx = bx * sc; cpy = by * sc; cpz = bz * sc;
+                out = make_float3(out.x + cpx, out.y + cpy, out.z + cpz);
+            }
+        }
+    }
+    a.dp[i] = out;
+    a.dpCross[i] = make_float3(cpx, cpy, cpz);
+}
+
+// Write pass (forward.cu:7311-7322) plus the iteration epilogue.
+__global__ void __launch_bounds__(256) sc_apply_kernel(int nPad, SelfColArgs a, const float4* __restrict__ Pin,
+                                                      float4* __restrict__ Pout, unsigned epi, EpiArgs ea)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;
+    const BodyDev& B = body_at(a, i);
+    if (i >= B.nodeEnd) return;
+    const float4 p4 = Pin[i];
+    float3 p = make_float3(p4.x, p4.y, p4.z);
+    if (B.scOn && a.counts[i] > 0) {
+        const float3 d = a.dp[i];
+        p = make_float3(p.x + d.x, p.y + d.y, p.z + d.z);
+        const float3 c = a.dpCross[i];
+        const float3 acc = a.accum[i];
+        a.accum[i] = make_float3(acc.x + c.x, acc.y + c.y, acc.z + c.z);
+    }
+    if (epi && epilogue_node(epi, i, p, p4.w, B, ea)) return;
+    Pout[i] = make_float4(p.x, p.y, p.z, p4.w);
+}
+
+// ---------------------------------------------------------------------------
+// Contact-average push after the iterations (forward.cu:11858-11890, 7326-7353)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) push_sum_kernel(int nPad, SelfColArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;
+    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
+    const BodyDev& B = a.bodies[b];
+    if (!B.scOn || !B.objOn || i >= B.nodeEnd || a.scCounters[3 * b] <= 0) return;
+    const int id = a.objId[i];
+    if (id < 0) return;
+    const float3 c = a.accum[i];
+    if (!(c.x * c.x + c.y * c.y + c.z * c.z > 1e-30f)) return;
+    atomicAdd(&a.pushSums[4 * (size_t)id], c.x);
+    atomicAdd(&a.pushSums[4 * (size_t)id + 1], c.y);
+    atomicAdd(&a.pushSums[4 * (size_t)id + 2], c.z);
+    atomicAdd(&a.pushSums[4 * (size_t)id + 3], 1.0f);
+}
+
+__global__ void push_t_kernel(int K, SelfColArgs a)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const BodyDev& B = a.bodies[a.objBody[k]];
+    float* T = &a.pushT[4 * (size_t)k];
+    T[0] = T[1] = T[2] = T[3] = 0.0f;
+    const float* S = &a.pushSums[4 * (size_t)k];
+    const float n = S[3];
+    if (!(n > 0.0f)) return;
+    const float sc = B.objStiffness / n;
+    const float tx = S[0] * sc, ty = S[1] * sc, tz = S[2] * sc;
+    if (!isfinite(tx) || !isfinite(ty) || !isfinite(tz)) return;
+    T[0] = tx; T[1] = ty; T[2] = tz; T[3] = 1.0f;
+}
+
+// Push untouched nodes of pushed objects, re-project on the ground (12588-12596),
+// then the velocity commit.
+__global__ void __launch_bounds__(256) push_apply_kernel(int nPad, SelfColArgs a, float4* __restrict__ P, EpiArgs ea)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;
+    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
+    const BodyDev& B = a.bodies[b];
+    if (i >= B.nodeEnd) return;
+    const float4 p4 = P[i];
+    float3 p = make_float3(p4.x, p4.y, p4.z);
+    const bool pushStep = B.scOn && B.objOn && a.scCounters[3 * b] > 0;
+    if (pushStep) {
+        const int id = a.objId[i];
+        if (id >= 0 && p4.w > 0.0f) {
+            const float* T = &a.pushT[4 * (size_t)id];
+            const float3 c = a.accum[i];
+            if (T[3] != 0.0f && !(c.x * c.x + c.y * c.y + c.z * c.z > 1e-30f))
+                p = make_float3(p.x + T[0], p.y + T[1], p.z + T[2]);
+        }
+        if (B.groundOn && p4.w > 0.0f) {
+            const float* rg = B.contactGround ? ea.rg : nullptr;
+            const float c = B.gn[0] * p.x + B.gn[1] * p.y + B.gn[2] * p.z - B.planeD - (rg ? rg[i] : 0.0f);
+            if (c < 0.0f) p = make_float3(p.x - c * B.gn[0], p.y - c * B.gn[1], p.z - c * B.gn[2]);
+        }
+    }
+    commit_node(i, p, p4.w, B, ea.x, ea.v, B.contactGround ? ea.rg : nullptr);
+}
+
+// ---------------------------------------------------------------------------
+static inline int nbs(int n, int bs) { return (n + bs - 1) / bs; }
+
+void launch_object_shape(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, int K, float4* P, const float3* rest,
+                         bool boxes)
+{
+    if (K <= 0) return;
+    launch_fill_double(s, 16 * K, a.objSums);
+    obj_accum_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, P);
+    obj_solve_kernel<<<nbs(K, 64), 64, 0, s>>>(K, a);
+    if (boxes) {
+        obj_boxes_init_kernel<<<nbs(6 * K, 256), 256, 0, s>>>(K, a.objBoxesArr);
+        obj_boxes_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, P);
+    }
+    obj_match_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, P, rest);
+}
+
+bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, const SelfColArgs& a, const float3* X,
+                          const float4* P, const float3* rest, int sortBits, unsigned int* keysAlt, int* idxAlt,
+                          unsigned int* hist, const int** sortedIdxOut)
+{
+    sc_prepare_kernel<<<nbs(numBodies, 64), 64, 0, s>>>(numBodies, a);
+    sc_active_disp_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P);
+    sc_reach_kernel<<<nbs(numBodies, 64), 64, 0, s>>>(numBodies, a);
+    sc_keys_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X);
+    const bool inAlt = launch_radix_sort(s, na.nPad, sortBits, a.keys, a.idx, keysAlt, idxAlt, hist);
+    const unsigned int* ks = inAlt ? keysAlt : a.keys;
+    const int* vs = inAlt ? idxAlt : a.idx;
+    launch_fill_int(s, a.tableTotal, a.cellStart, -1);
+    sc_range_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, ks);
+    sc_build_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
+    sc_fill_accum_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a.accum);
+    *sortedIdxOut = vs;
+    return inAlt;
+}
+
+void launch_selfcol_iteration(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, const float4* Pin,
+                              float4* Pout, const float3* X, const float3* rest, unsigned epi, const EpiArgs& ea)
+{
+    sc_solve_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, Pin, X, rest);
+    sc_apply_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, Pin, Pout, epi, ea);
+}
+
+void launch_contact_push(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, int K, float4* P, const EpiArgs& ea)
+{
+    if (K > 0) {
+        launch_fill_float(s, 4 * K, a.pushSums, 0.0f);
+        push_sum_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a);
+        push_t_kernel<<<nbs(K, 64), 64, 0, s>>>(K, a);
+    }
+    push_apply_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, P, ea);
+}
+
+}  // namespace xpbd

@@ -173,15 +173,15 @@ __global__ void __launch_bounds__(256) vol_solve_thread_kernel(
     if (kFirst && !kLast && coef == 0.0f) lamVol[idx] = 0.0f;
 }
 
-__device__ __forceinline__ float wave_sum(float v)
+__device__ __forceinline__ float lane_val(float v, int lane)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// One wave64 per active leader.  Lane l owns members l, l+64, ...; the first
-// kCache of them stay in registers between the three passes.
+// One wave64 per active leader.  Lanes gather members l, l+64, ... in
+// parallel (the first 64 * kCache stay in registers between the passes); every
+// sum is then accumulated in member order with values read from the owning
+// lane, so the result equals the thread-per-cluster kernel bit for bit.
 template <bool kFirst, bool kLast, int kCache>
 __global__ void __launch_bounds__(256) vol_solve_wave_kernel(
     int nActive, const int* __restrict__ active, const int* __restrict__ tileBody, const BodyDev* __restrict__ bodies,
@@ -198,58 +198,78 @@ __global__ void __launch_bounds__(256) vol_solve_wave_kernel(
     const float4 pi = P[idx];
     const int off = volOff[idx], cnt = volOff[idx + 1] - off;
     float4 cache[kCache];
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-#pragma unroll
-    for (int r = 0; r < kCache; ++r) {
-        const int k = lane + 64 * r;
-        cache[r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (k < cnt) cache[r] = P[volIdx[off + k]];
-        sx += cache[r].x; sy += cache[r].y; sz += cache[r].z;
-    }
-    for (int k = lane + 64 * kCache; k < cnt; k += 64) {
-        const float4 q = P[volIdx[off + k]];
-        sx += q.x; sy += q.y; sz += q.z;
-    }
-    sx = wave_sum(sx); sy = wave_sum(sy); sz = wave_sum(sz);
-    const float invN = 1.0f / (float)(cnt + 1);
-    const float cx = (pi.x + sx) * invN, cy = (pi.y + sy) * invN, cz = (pi.z + sz) * invN;
-    float s00 = 0, s11 = 0, s22 = 0, s10 = 0, s20 = 0, s21 = 0;
-    auto acc = [&](float4 q) {
-        const float dx = q.x - cx, dy = q.y - cy, dz = q.z - cz;
-        s00 += dx * dx; s11 += dy * dy; s22 += dz * dz;
-        s10 += dx * dy; s20 += dx * dz; s21 += dy * dz;
+    auto member = [&](int chunk) -> float4 {
+        if (chunk < kCache) return cache[chunk];
+        const int k = chunk * 64 + lane;
+        return (k < cnt) ? P[volIdx[off + k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     };
+    const int chunks = (cnt + 63) / 64;
+    // pass 1: centroid, summed in member order
+    float cx = pi.x, cy = pi.y, cz = pi.z;
+    for (int ch = 0; ch < chunks; ++ch) {
+        float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const int k = ch * 64 + lane;
+        if (k < cnt) q = P[volIdx[off + k]];
 #pragma unroll
-    for (int r = 0; r < kCache; ++r)
-        if (lane + 64 * r < cnt) acc(cache[r]);
-    for (int k = lane + 64 * kCache; k < cnt; k += 64) acc(P[volIdx[off + k]]);
-    if (lane == 0) acc(pi);
-    s00 = wave_sum(s00); s11 = wave_sum(s11); s22 = wave_sum(s22);
-    s10 = wave_sum(s10); s20 = wave_sum(s20); s21 = wave_sum(s21);
-    M3 S = m3_load_sym(s00 * invN, s11 * invN, s22 * invN, s10 * invN, s20 * invN, s21 * invN);
+        for (int r = 0; r < kCache; ++r)
+            if (r == ch) cache[r] = q;
+        const int m = min(64, cnt - ch * 64);
+        for (int t = 0; t < m; ++t) {
+            cx = cx + lane_val(q.x, t);
+            cy = cy + lane_val(q.y, t);
+            cz = cz + lane_val(q.z, t);
+        }
+    }
+    const float invN = 1.0f / (float)(cnt + 1);
+    cx = cx * invN; cy = cy * invN; cz = cz * invN;
+    // pass 2: covariance (six distinct entries of the symmetric sum)
+    float s00, s11, s22, s10, s20, s21;
+    {
+        const float dx = pi.x - cx, dy = pi.y - cy, dz = pi.z - cz;
+        s00 = 0.0f + dx * dx; s11 = 0.0f + dy * dy; s22 = 0.0f + dz * dz;
+        s10 = 0.0f + dy * dx; s20 = 0.0f + dz * dx; s21 = 0.0f + dz * dy;
+    }
+    for (int ch = 0; ch < chunks; ++ch) {
+        const float4 q = member(ch);
+        const float dx = q.x - cx, dy = q.y - cy, dz = q.z - cz;
+        const float p00 = dx * dx, p11 = dy * dy, p22 = dz * dz, p10 = dy * dx, p20 = dz * dx, p21 = dz * dy;
+        const int m = min(64, cnt - ch * 64);
+        for (int t = 0; t < m; ++t) {
+            s00 = s00 + lane_val(p00, t);
+            s11 = s11 + lane_val(p11, t);
+            s22 = s22 + lane_val(p22, t);
+            s10 = s10 + lane_val(p10, t);
+            s20 = s20 + lane_val(p20, t);
+            s21 = s21 + lane_val(p21, t);
+        }
+    }
+    M3 S = m3_load_sym(s00, s11, s22, s10, s20, s21);
+    m3_scale(S, invN);
     float J, C;
     M3 Si;
     float coef = 0.0f;
     if (vol_step(S, invN, detRest[idx], J, C, Si)) {
         const float gs = J * invN;
-        float den = 0.0f;
-        auto accd = [&](float4 q) {
+        float denom = 0.0f;
+        {
+            const float3 gv = m3_mul_vec(Si, pi.x - cx, pi.y - cy, pi.z - cz);
+            const float gx = gs * gv.x, gy = gs * gv.y, gz = gs * gv.z;
+            denom += (pi.w * B.invMassScale) * (gx * gx + gy * gy + gz * gz);
+        }
+        for (int ch = 0; ch < chunks; ++ch) {
+            const float4 q = member(ch);
             const float3 gv = m3_mul_vec(Si, q.x - cx, q.y - cy, q.z - cz);
             const float gx = gs * gv.x, gy = gs * gv.y, gz = gs * gv.z;
-            den += (q.w * B.invMassScale) * (gx * gx + gy * gy + gz * gz);
-        };
-#pragma unroll
-        for (int r = 0; r < kCache; ++r)
-            if (lane + 64 * r < cnt) accd(cache[r]);
-        for (int k = lane + 64 * kCache; k < cnt; k += 64) accd(P[volIdx[off + k]]);
-        if (lane == 0) accd(pi);
-        den = wave_sum(den);
+            const float term = (q.w * B.invMassScale) * (gx * gx + gy * gy + gz * gz);
+            const int m = min(64, cnt - ch * 64);
+            for (int t = 0; t < m; ++t) denom += lane_val(term, t);
+        }
         const float dt2 = fmaxf(B.dt * B.dt, 1e-8f);
         const float alphaT = B.volCompliance / dt2;
-        const float dd = den + alphaT;
-        if (dd > 1e-12f) {
+        const float den = denom + alphaT;
+        if (den > 1e-12f) {
             const float lam = kFirst ? 0.0f : lamVol[idx];
-            const float dLam = (-C - alphaT * lam) / dd;
+            const float dLam = (-C - alphaT * lam) / den;
             if (isfinite(dLam)) {
                 if (!kLast && lane == 0) lamVol[idx] = lam + dLam;
                 coef = dLam * gs;

@@ -1,0 +1,136 @@
+// radix_sort.hip — stable LSD radix sort of (uint32 key, int32 value) pairs.
+//
+// 8-bit digits, one three-kernel pass per digit:
+//   hist    one block per tile of kTileKeys keys: digit histogram in LDS,
+//           written digit-major hist[digit * tiles + tile];
+//   scan    one block: exclusive scan of hist (digit-major order makes the
+//           global order digit, then tile, then position: stable);
+//   scatter one block per tile: keys are ranked in rounds of 256 (one per
+//           thread, in index order); within a wave64 the rank among equal
+//           digits comes from 8 ballots (match mask) and a popcount, across
+//           the four waves from LDS counts, across rounds from running
+//           per-digit counters.  Every key keeps its input order within its
+//           digit, so the sort is stable like an LSD sort must be.
+// Values start as the identity when `iota` is set.  Only the low `bits` bits
+// of the keys are sorted (keys must not have higher bits set).
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace xpbd {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortRounds = 8;
+constexpr int kTileKeys = kSortThreads * kSortRounds;   // 2048 keys per tile
+
+__global__ void __launch_bounds__(kSortThreads) radix_hist_kernel(int n, const unsigned int* __restrict__ keys, int shift,
+                                                                 int tiles, unsigned int* __restrict__ hist)
+{
+    __shared__ unsigned int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kTileKeys;
+    for (int r = 0; r < kSortRounds; ++r) {
+        const int i = base + r * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of `count` entries by one block of 1024 threads.
+__global__ void __launch_bounds__(1024) radix_scan_kernel(int count, unsigned int* __restrict__ data)
+{
+    __shared__ unsigned int part[1024];
+    const int t = threadIdx.x;
+    const int per = (count + 1023) / 1024;
+    const int b0 = t * per, b1 = min(count, b0 + per);
+    unsigned int s = 0;
+    for (int i = b0; i < b1; ++i) s += data[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const unsigned int v = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    unsigned int run = (t == 0) ? 0u : part[t - 1];
+    for (int i = b0; i < b1; ++i) {
+        const unsigned int v = data[i];
+        data[i] = run;
+        run += v;
+    }
+}
+
+__global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(
+    int n, const unsigned int* __restrict__ keysIn, const int* __restrict__ valsIn, unsigned int* __restrict__ keysOut,
+    int* __restrict__ valsOut, int shift, int tiles, const unsigned int* __restrict__ offsets, int iota)
+{
+    __shared__ unsigned int run[256];          // keys of each digit already placed by this tile
+    __shared__ unsigned int wcnt[4][256];      // per-wave digit counts of the current round
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    run[t] = offsets[t * tiles + blockIdx.x];
+    const int base = blockIdx.x * kTileKeys;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < kSortRounds; ++r) {
+        const int i = base + r * kSortThreads + t;
+        const bool valid = i < n;
+        unsigned int key = 0, digit = 0;
+        int val = 0;
+        if (valid) {
+            key = keysIn[i];
+            digit = (key >> shift) & 255u;
+            val = iota ? i : valsIn[i];
+        }
+        wcnt[0][t] = 0; wcnt[1][t] = 0; wcnt[2][t] = 0; wcnt[3][t] = 0;
+        __syncthreads();
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const unsigned long long bal = __ballot(valid && ((digit >> b) & 1u));
+            peers &= ((digit >> b) & 1u) ? bal : ~bal;
+        }
+        const unsigned int rankInWave = (unsigned int)__popcll(peers & lt);
+        if (valid && rankInWave == 0) wcnt[wave][digit] = (unsigned int)__popcll(peers);
+        __syncthreads();
+        unsigned int before = 0;
+        if (valid) {
+            for (int w = 0; w < wave; ++w) before += wcnt[w][digit];
+            const unsigned int pos = run[digit] + before + rankInWave;
+            keysOut[pos] = key;
+            valsOut[pos] = val;
+        }
+        __syncthreads();
+        run[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+        __syncthreads();
+    }
+}
+
+int radix_sort_tiles(int n) { return (n + kTileKeys - 1) / kTileKeys; }
+
+// keys/vals ping-pong between (k0, v0) and (k1, v1); the sorted result ends
+// in (k0, v0) when the number of passes is even, else in (k1, v1).  Returns
+// true when the result is in (k1, v1).
+bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0, unsigned int* k1, int* v1,
+                       unsigned int* hist)
+{
+    if (n <= 0) return false;
+    const int tiles = radix_sort_tiles(n);
+    const int passes = (bits + 7) / 8;
+    unsigned int* kin = k0;
+    int* vin = v0;
+    unsigned int* kout = k1;
+    int* vout = v1;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        radix_hist_kernel<<<tiles, kSortThreads, 0, s>>>(n, kin, shift, tiles, hist);
+        radix_scan_kernel<<<1, 1024, 0, s>>>(256 * tiles, hist);
+        radix_scatter_kernel<<<tiles, kSortThreads, 0, s>>>(n, kin, vin, kout, vout, shift, tiles, hist, p == 0 ? 1 : 0);
+        unsigned int* tk = kin; kin = kout; kout = tk;
+        int* tv = vin; vin = vout; vout = tv;
+    }
+    return kin == k1;
+}
+
+}  // namespace xpbd

@@ -51,6 +51,13 @@ struct BodyDev {
     // per-node shape matching (forward.cu:6088-6383)
     int shapeOn, shapeRobust;
     float shapeCompliance, shapeBlend;
+    // object (connected component) shape matching (forward.cu:6801-6937)
+    int objOn, objBase, objK, objWeighted, objBoxes, objActiveAllowed;
+    float objStiffness;
+    // self-collision (forward.cu:6986-7322); the hash table of the body is
+    // [scTableBase, scTableBase + scTableMask + 1) of the shared cell arrays
+    int scOn, scWithin, scTableBase, scTableMask;
+    float scDc, scSearch, scExcl2, scFastRel, scCellOverride;
 };
 
 // forward.cu:128-136 KinCollider, same field meaning.

@@ -72,6 +72,35 @@ def make_ellipsoid_scene(n, k, semi=(0.25, 0.15, 0.10), seed=1):
     }
 
 
+def merge_scenes(parts, offsets):
+    """Several scenes as one graph (disjoint components), part p moved by
+    offsets[p].  The components become the objects of object shape matching
+    and cross-object self-collision."""
+    keys = ("pos", "scales", "rots", "opacity", "rest", "stiff")
+    out = {k: [] for k in keys}
+    edges, base = [], 0
+    for sc, off in zip(parts, offsets):
+        out["pos"].append(sc["pos"] + np.asarray(off, np.float32))
+        for k in keys[1:]:
+            out[k].append(sc[k])
+        edges.append(sc["edges"] + base)
+        base += sc["pos"].shape[0]
+    merged = {k: np.ascontiguousarray(np.concatenate(v).astype(np.float32)) for k, v in out.items()}
+    merged["edges"] = np.ascontiguousarray(np.concatenate(edges).astype(np.int32))
+    merged["parts"] = [sc["pos"].shape[0] for sc in parts]
+    return merged
+
+
+def stacked_bodies(n_each, k, semi=(0.1, 0.1, 0.1), gap=0.0, count=2, seed=5):
+    """`count` ellipsoids stacked along z with `gap` between their surfaces
+    (negative = overlap), the lowest resting on z = 0."""
+    parts, offs = [], []
+    for c in range(count):
+        parts.append(make_ellipsoid_scene(n_each, k, semi=semi, seed=seed + c))
+        offs.append((0.0, 0.0, c * (2.0 * semi[2] + gap)))
+    return merge_scenes(parts, offs)
+
+
 def scene_extent(scene):
     p = scene["pos"]
     return float(np.linalg.norm(p.max(0) - p.min(0)))

@@ -118,6 +118,10 @@ def bind(lib):
     L.xpbd_ext_set_volume_clusters.restype = c_int
     L.xpbd_ext_volume_topology.argtypes = [ctypes.c_void_p] * 5 + [c_int]
     L.xpbd_ext_volume_topology.restype = c_int
+    L.xpbd_ext_set_self_collision_grid.argtypes = [c_float, c_int]
+    L.xpbd_ext_set_self_collision_grid.restype = c_int
+    L.xpbd_ext_self_collision_lists.argtypes = [ctypes.c_void_p] * 4 + [c_int]
+    L.xpbd_ext_self_collision_lists.restype = c_int
     return L
 
 
@@ -327,3 +331,19 @@ class XPBD:
         out = np.empty(self.n, np.float32)
         self._check(self.lib.xpbd_ext_get_inv_mass(out, self.n), "xpbd_ext_get_inv_mass")
         return out
+
+    def ext_set_self_collision_grid(self, cell_size=0.0, table_size=0):
+        """Fixed self-collision hash cell (0 = displacement rule) and table size (0 = default)."""
+        self._check(self.lib.xpbd_ext_set_self_collision_grid(float(cell_size), int(table_size)),
+                    "xpbd_ext_set_self_collision_grid")
+
+    def ext_self_collision_lists(self):
+        """(keys[n], stable key order[n], counts[n], contacts[n, 32]) of the last build."""
+        n = self.n
+        keys = np.zeros(n, np.uint32)
+        order = np.zeros(n, np.int32)
+        counts = np.zeros(n, np.int32)
+        contacts = np.zeros((n, 32), np.int32)
+        self._check(self.lib.xpbd_ext_self_collision_lists(_ptr(keys), _ptr(order), _ptr(counts), _ptr(contacts), n),
+                    "xpbd_ext_self_collision_lists")
+        return keys, order, counts, contacts

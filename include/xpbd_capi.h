@@ -160,6 +160,18 @@ int xpbd_ext_set_volume_clusters(const int* off, const int* members, int minMemb
  * determinant.  Any pointer may be null. */
 int xpbd_ext_volume_topology(int* leaders, int* memberCount, int* revCount, int* matType, float* detRest, int n);
 
+/* Self-collision grid: cellSize > 0 fixes the hash-cell edge (instead of
+ * search radius + 2 x max displacement, capped at search + 16 d_c); tableSize
+ * 0 keeps nextpow2(max(1024, 2n)) cells, else a power of two >= 1024.  Applies
+ * to later builds; persists like the solver settings. */
+int xpbd_ext_set_self_collision_grid(float cellSize, int tableSize);
+
+/* Artefacts of the last self-collision build (after the first step): cell key
+ * of every node (table size = inactive / non-finite), nodes in stable key
+ * order, candidate counts counts[n] and candidate lists contacts[n*32]
+ * (j, or -(j+1) for a fast cross-object pair).  Any pointer may be null. */
+int xpbd_ext_self_collision_lists(unsigned int* keys, int* sorted, int* counts, int* contacts, int n);
+
 /* Library identity: "hip-gfx950" for the GPU library, "oracle-cpu" for the oracle. */
 const char* xpbd_ext_backend(void);
 

@@ -433,7 +433,8 @@ static void run_step()
         if (colOn) kinematic_colliders(*pin, it == s.iters - 1);
         if (groundOn) ground_project(*pin);
     }
-    extensions_after_iterations(*pin, groundOn);
+    // 8: contact-average push, then one more ground projection (12588-12596)
+    if (extensions_after_iterations(*pin, groundOn) && s.groundEnabled) ground_project(*pin);
 
     // 9: velocity update and commit (12600-12628)
     if (groundOn) velocity_ground(*pin, dt, s.velDamping);

@@ -14,6 +14,7 @@
 #include "xpbd_oracle_internal.h"
 #include "oracle_volume.h"
 #include "oracle_shapes.h"
+#include "oracle_selfcol.h"
 
 namespace xo {
 
@@ -35,6 +36,8 @@ void extensions_free()
 {
     vol = VolumeState();
     g_cM.clear(); g_cRmax.clear(); g_cRg.clear();
+    obj = ObjectState();
+    sc = SelfColState();
 }
 void extensions_on_reset() {}
 void extensions_begin_step()
@@ -44,7 +47,8 @@ void extensions_begin_step()
         std::fill(vol.lam.begin(), vol.lam.end(), 0.0f);
     }
 }
-void extensions_after_predict() {}
+void extensions_after_predict_impl();
+void extensions_after_predict() { extensions_after_predict_impl(); }
 void extensions_before_iterations() { contact_ground_radius_update(); }
 bool extensions_distance(std::vector<V3>&, std::vector<V3>&, int) { return false; }
 void extensions_iteration(std::vector<V3>*& pin, std::vector<V3>*& pout, int)
@@ -55,8 +59,32 @@ void extensions_iteration(std::vector<V3>*& pin, std::vector<V3>*& pout, int)
         std::swap(pin, pout);
     }
 }
-void extensions_self_collision(std::vector<V3>&) {}
-void extensions_after_iterations(std::vector<V3>&, bool) {}
+void extensions_self_collision(std::vector<V3>& pin)
+{
+    if (s.scEnabled && sc.contactsTotal > 0) selfcol_iteration(pin);
+}
+// Object shape after predict (12124-12166) and the self-collision build (12179-12200)
+static bool g_objShapeStep = false;
+
+void extensions_after_predict_impl()
+{
+    g_objShapeStep = (s.objShapeStiffness > 0.0f) && !s.sqActive;
+    if (g_objShapeStep) {
+        if (object_ensure_components()) object_shape_match(g.p);
+        else g_objShapeStep = false;
+    }
+    if (s.scEnabled && !s.sqActive) selfcol_build(g.p);
+    else sc.contactsTotal = 0;
+}
+
+bool extensions_after_iterations(std::vector<V3>& pin, bool)
+{
+    if (s.scEnabled && sc.contactsTotal > 0 && g_objShapeStep) {
+        object_contact_push(pin);
+        return true;
+    }
+    return false;
+}
 // Contact ellipsoids (forward.cu:6733-6757, 7358-7399, 12219-12226)
 static bool contact_ready() { return (s.contactColliders || s.contactGround) && (int)g_cRmax.size() == g.n && g.n > 0; }
 bool contact_colliders_on() { return contact_ready() && s.contactColliders; }
@@ -116,12 +144,44 @@ void xpbd_set_self_collision(int enabled, float radiusScale, float excludeScale,
     s.scWithinBody = withinBody != 0;
     s.scRadiusScale = std::fmin(std::fmax(std::isfinite(radiusScale) ? radiusScale : 1.5f, 0.1f), 10.0f);
     s.scExcludeScale = std::fmin(std::fmax(std::isfinite(excludeScale) ? excludeScale : 2.0f, 1.05f), 20.0f);
+    if (!s.scEnabled) { sc.contactsTotal = 0; sc.overflow = 0; }
 }
 
+// native/xpbd_capi.cu:209-212: candidates, active particles, overflow replacements
 void xpbd_self_collision_stats(int* out)
 {
     if (!out) return;
-    out[0] = 0; out[1] = 0; out[2] = 0;
+    out[0] = (int)sc.contactsTotal;
+    out[1] = sc.activeCount;
+    out[2] = sc.overflow;
+}
+
+// Extension: fixed hash grid (cell size > 0 replaces the displacement rule;
+// table size, a power of two >= 1024, replaces nextpow2(2N)); 0 keeps the rule.
+int xpbd_ext_set_self_collision_grid(float cellSize, int tableSize)
+{
+    if (tableSize != 0 && (tableSize < 1024 || (tableSize & (tableSize - 1))))
+        return fail("xpbd_ext_set_self_collision_grid: table size must be a power of two >= 1024");
+    s.scCellOverride = (std::isfinite(cellSize) && cellSize > 0.0f) ? cellSize : 0.0f;
+    s.scTableOverride = tableSize;
+    sc.table = 0;
+    return 0;
+}
+
+// Extension: the last build's artefacts: cell keys (by node), node order
+// after the stable key sort, candidate counts and candidate lists [n x 32].
+int xpbd_ext_self_collision_lists(unsigned int* keys, int* sorted, int* counts, int* contacts, int n)
+{
+    if (!g.stepped) return fail("xpbd_ext_self_collision_lists: no step yet");
+    if (n != g.n) return fail("xpbd_ext_self_collision_lists: size mismatch");
+    if ((int)sc.keys.size() != n) return fail("xpbd_ext_self_collision_lists: no self-collision build yet");
+    for (int i = 0; i < n; ++i) {
+        if (keys) keys[i] = sc.keys[i];
+        if (sorted) sorted[i] = sc.order[i];
+        if (counts) counts[i] = sc.counts[i];
+    }
+    if (contacts) std::copy(sc.contacts.begin(), sc.contacts.end(), contacts);
+    return 0;
 }
 
 static float cap_or_zero(float c) { return (std::isfinite(c) && c > 0.0f) ? c : 0.0f; }

@@ -59,6 +59,8 @@ struct Settings {
     // self-collision (forward.cu:224-256)
     bool scEnabled = false, scWithinBody = false;
     float scRadiusScale = 1.5f, scExcludeScale = 2.0f, scSearchScale = 1.5f, scFastRel = 0.0f;
+    float scCellOverride = 0.0f;   // extension: fixed cell size instead of the displacement rule
+    int scTableOverride = 0;       // extension: fixed hash table size (power of two)
     // squash / press (forward.cu:535-575)
     bool sqActive = false, sqPress = false, sqPressNext = false, sqPressFromLo = false, sqSlip = true;
     bool sqAutoLog = true;
@@ -104,7 +106,7 @@ void extensions_before_iterations();
 bool extensions_distance(std::vector<V3>& pin, std::vector<V3>& pout, int it);
 void extensions_iteration(std::vector<V3>*& pin, std::vector<V3>*& pout, int it);
 void extensions_self_collision(std::vector<V3>& pin);
-void extensions_after_iterations(std::vector<V3>& pin, bool groundOn);
+bool extensions_after_iterations(std::vector<V3>& pin, bool groundOn);   // true: re-project on the ground
 float contact_rmax(int i);
 bool contact_colliders_on();
 float contact_support(int i, float nx, float ny, float nz);
