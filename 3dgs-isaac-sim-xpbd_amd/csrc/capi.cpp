@@ -37,6 +37,7 @@ struct Global {
     // fixed self-collision grid (0 = the reference's displacement rule)
     float scCell = 0.0f;
     int scTable = 0;
+    int nodeOrder = 1;   // device node order of the next engine (1 = Morton)
     int n = 0;
     bool stepped = false;
     bool contactOn = false;
@@ -276,6 +277,7 @@ int xpbd_step(void)
     std::string err;
     if (!g.engine) {
         g.engine.reset(new Engine());
+        g.engine->setNodeOrder(g.nodeOrder);
         if (!g.engine->build({g.input.get()}, err)) {
             g.engine.reset();
             return fail("xpbd_step: " + err);
@@ -510,6 +512,13 @@ int xpbd_ext_get_inv_mass(float* invMass, int n)
     if (!g.stepped) return fail("xpbd_ext_get_inv_mass: no step yet");
     if (n != g.n || !invMass) return fail("xpbd_ext_get_inv_mass: size mismatch");
     if (!g.engine->getInvMass(0, invMass)) return fail("xpbd_ext_get_inv_mass: device copy failed");
+    return 0;
+}
+
+int xpbd_ext_set_node_order(int mode)
+{
+    if (mode != 0 && mode != 1) return fail("xpbd_ext_set_node_order: mode must be 0 (caller order) or 1 (Morton)");
+    G().nodeOrder = mode;
     return 0;
 }
 

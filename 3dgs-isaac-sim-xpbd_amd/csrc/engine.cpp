@@ -5,6 +5,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 
 namespace xpbd {
@@ -96,6 +97,56 @@ void build_input(BodyInput& in, int n, const float* pos, const float* scales, co
     }
 }
 
+// Z-order of a body's rest positions: coordinates quantised to 21 bits on
+// the body's bounding cube, bits interleaved x, y, z; ties keep caller order.
+static uint64_t spread_bits21(uint64_t v)
+{
+    v &= 0x1fffffULL;
+    v = (v | (v << 32)) & 0x1f00000000ffffULL;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+    v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+    v = (v | (v << 2)) & 0x1249249249249249ULL;
+    return v;
+}
+
+static void morton_order(const BodyInput& in, std::vector<int>& ord)
+{
+    const int n = in.n;
+    ord.resize(n);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            const double v = in.pos[3 * i + a];
+            if (!std::isfinite(v)) continue;
+            lo[a] = std::min(lo[a], v);
+            hi[a] = std::max(hi[a], v);
+        }
+    double ext = 0.0;
+    for (int a = 0; a < 3; ++a) ext = std::max(ext, hi[a] - lo[a]);
+    const double scale = ext > 0.0 ? 2097151.0 / ext : 0.0;
+    std::vector<std::pair<uint64_t, int>> key(n);
+    for (int i = 0; i < n; ++i) {
+        uint64_t c = 0;
+        for (int a = 0; a < 3; ++a) {
+            double q = (in.pos[3 * i + a] - lo[a]) * scale;
+            if (!(q >= 0.0)) q = 0.0;
+            if (q > 2097151.0) q = 2097151.0;
+            c |= spread_bits21((uint64_t)q) << a;
+        }
+        key[i] = {c, i};
+    }
+    std::sort(key.begin(), key.end());
+    for (int i = 0; i < n; ++i) ord[i] = key[i].second;
+}
+
+// Caller-order copy of body b's committed positions after a host-side change.
+void Engine::refreshCallerPositions(int b)
+{
+    if (!permuted_) return;
+    launch_scatter_caller(stream_, bodyBegin_[b], bodyBegin_[b] + bodyCount_[b], callerPos_.p, x_.p, xOut_.p);
+}
+
 Engine::Engine()
 {
     (void)hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
@@ -160,27 +211,51 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     scSortedIdx_ = nullptr;
     scSortedKeys_ = nullptr;
 
-    // host staging of the padded node space
+    // device node order of every body (caller order or Morton order)
+    ord_.assign(B, std::vector<int>());
+    inv_.assign(B, std::vector<int>());
+    permuted_ = false;
+    for (int b = 0; b < B; ++b) {
+        const BodyInput& in = *inputs[b];
+        std::vector<int>& o = ord_[b];
+        if (nodeOrderMode_ == 1) morton_order(in, o);
+        else {
+            o.resize(in.n);
+            for (int i = 0; i < in.n; ++i) o[i] = i;
+        }
+        inv_[b].assign(in.n, 0);
+        for (int d = 0; d < in.n; ++d) {
+            inv_[b][o[d]] = d;
+            permuted_ |= o[d] != d;
+        }
+    }
+
+    // host staging of the padded node space (device order); every row keeps
+    // the caller's entry order, neighbours are renumbered
     std::vector<float3> hx(nPad_, make_float3(0, 0, 0));
-    std::vector<int> hrow(nPad_ + 1, 0), htile(nPad_ / kTile, 0);
+    std::vector<int> hrow(nPad_ + 1, 0), htile(nPad_ / kTile, 0), hcaller(nPad_);
     std::vector<int> hnbr(m);
     std::vector<float> hrest(m), hstiff(m);
+    for (int i = 0; i < nPad_; ++i) hcaller[i] = i;
     long long eo = 0;
     for (int b = 0; b < B; ++b) {
         const BodyInput& in = *inputs[b];
         const int nb = bodyBegin_[b];
-        for (int i = 0; i < in.n; ++i) hx[nb + i] = make_float3(in.pos[3 * i], in.pos[3 * i + 1], in.pos[3 * i + 2]);
+        const std::vector<int>& o = ord_[b];
+        const std::vector<int>& iv = inv_[b];
         const int padEnd = (b + 1 < B) ? bodyBegin_[b + 1] : nPad_;
         for (int t = nb / kTile; t < padEnd / kTile; ++t) htile[t] = b;
-        for (int i = 0; i < in.n; ++i) {
-            hrow[nb + i] = (int)(eo + in.rowOff[i]);
-            for (int k = in.rowOff[i]; k < in.rowOff[i + 1]; ++k) {
-                hnbr[eo + k] = nb + in.nbr[k];
-                hrest[eo + k] = in.nrest[k];
-                hstiff[eo + k] = in.nstiff[k];
+        for (int d = 0; d < in.n; ++d) {
+            const int c = o[d];
+            hx[nb + d] = make_float3(in.pos[3 * c], in.pos[3 * c + 1], in.pos[3 * c + 2]);
+            hcaller[nb + d] = nb + c;
+            hrow[nb + d] = (int)eo;
+            for (int k = in.rowOff[c]; k < in.rowOff[c + 1]; ++k, ++eo) {
+                hnbr[eo] = nb + iv[in.nbr[k]];
+                hrest[eo] = in.nrest[k];
+                hstiff[eo] = in.nstiff[k];
             }
         }
-        eo += in.rowOff[in.n];
         for (int i = nb + in.n; i <= padEnd && i <= nPad_; ++i) hrow[i] = (int)eo;
     }
     hrow[nPad_] = (int)eo;
@@ -189,15 +264,17 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     p0_.alloc(nPad_); p1_.alloc(nPad_);
     invMass_.alloc(nPad_); invMassSaved_.alloc(nPad_); objW_.alloc(nPad_);
     rowOff_.alloc(nPad_ + 1); nbr_.alloc(std::max<long long>(m, 1));
-    nrest_.alloc(std::max<long long>(m, 1)); nstiff_.alloc(std::max<long long>(m, 1));
-    nalpha_.alloc(std::max<long long>(m, 1)); lam_.alloc(std::max<long long>(m, 1));
+    callerPos_.alloc(nPad_);
+    if (permuted_) xOut_.alloc(nPad_);
+    else xOut_.free();
+    nstiff_.alloc(std::max<long long>(m, 1));
     tileBody_.alloc(nPad_ / kTile);
     bodies_.alloc(B);
     cols_.alloc((size_t)B * kMaxColliders);
     colHits_.alloc((size_t)B * kMaxColliders);
     colPush_.alloc((size_t)B * kMaxColliders * kColAcc);
     if (!x_.p || !v_.p || !rest_.p || !p0_.p || !p1_.p || !invMass_.p || !invMassSaved_.p || !objW_.p || !rowOff_.p ||
-        !nbr_.p || !nrest_.p || !nstiff_.p || !nalpha_.p || !lam_.p || !tileBody_.p || !bodies_.p || !cols_.p ||
+        !nbr_.p || !nstiff_.p || !tileBody_.p || !bodies_.p || !cols_.p || !callerPos_.p || (permuted_ && !xOut_.p) ||
         !colHits_.p || !colPush_.p) {
         err = "engine: hipMalloc failed (out of device memory?)";
         return false;
@@ -208,10 +285,51 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     XPBD_HIP(hipMemsetAsync(p0_.p, 0, sizeof(float4) * nPad_, stream_));
     XPBD_HIP(hipMemsetAsync(p1_.p, 0, sizeof(float4) * nPad_, stream_));
     XPBD_HIP(hipMemcpyAsync(rowOff_.p, hrow.data(), sizeof(int) * (nPad_ + 1), hipMemcpyHostToDevice, stream_));
+    XPBD_HIP(hipMemcpyAsync(callerPos_.p, hcaller.data(), sizeof(int) * nPad_, hipMemcpyHostToDevice, stream_));
+    if (permuted_) XPBD_HIP(hipMemsetAsync(xOut_.p, 0, sizeof(float3) * nPad_, stream_));
     if (m > 0) {
         XPBD_HIP(hipMemcpyAsync(nbr_.p, hnbr.data(), sizeof(int) * m, hipMemcpyHostToDevice, stream_));
-        XPBD_HIP(hipMemcpyAsync(nrest_.p, hrest.data(), sizeof(float) * m, hipMemcpyHostToDevice, stream_));
         XPBD_HIP(hipMemcpyAsync(nstiff_.p, hstiff.data(), sizeof(float) * m, hipMemcpyHostToDevice, stream_));
+    }
+    // sliced-ELL copy of the directed entries for the distance pass: slice t
+    // = node tile t, width = its longest row, entry k of lane l at
+    // sliceOff[t] + 64 k + l; rows keep their order, padding has nbr -1
+    {
+        const int tiles = nPad_ / kTile;
+        std::vector<int> hOff(tiles), hW(tiles);
+        long long total = 0;
+        for (int t = 0; t < tiles; ++t) {
+            int w = 0;
+            for (int l = 0; l < kTile; ++l) w = std::max(w, hrow[t * kTile + l + 1] - hrow[t * kTile + l]);
+            hOff[t] = (int)total;
+            hW[t] = w;
+            total += (long long)w * kTile;
+            if (total > 0x7fffffffLL) { err = "engine: sliced distance entries exceed int32 indexing"; return false; }
+        }
+        std::vector<int> hSellNbr(std::max<long long>(total, 1), -1);
+        std::vector<float2> hSellRA(std::max<long long>(total, 1), make_float2(0.0f, 0.0f));
+        for (int i = 0; i < nPad_; ++i) {
+            const int t = i / kTile, l = i % kTile;
+            for (int e = hrow[i]; e < hrow[i + 1]; ++e) {
+                const size_t slot = (size_t)hOff[t] + (size_t)(e - hrow[i]) * kTile + l;
+                hSellNbr[slot] = hnbr[e];
+                hSellRA[slot] = make_float2(hrest[e], 0.0f);
+            }
+        }
+        sellEntries_ = total;
+        sliceOff_.alloc(std::max(tiles, 1)); sliceW_.alloc(std::max(tiles, 1));
+        sellNbr_.alloc(hSellNbr.size()); sellRA_.alloc(hSellRA.size()); sellLam_.alloc(hSellNbr.size());
+        if (!sliceOff_.p || !sliceW_.p || !sellNbr_.p || !sellRA_.p || !sellLam_.p) {
+            err = "engine: hipMalloc failed (out of device memory?)";
+            return false;
+        }
+        XPBD_HIP(hipMemcpyAsync(sliceOff_.p, hOff.data(), sizeof(int) * tiles, hipMemcpyHostToDevice, stream_));
+        XPBD_HIP(hipMemcpyAsync(sliceW_.p, hW.data(), sizeof(int) * tiles, hipMemcpyHostToDevice, stream_));
+        XPBD_HIP(hipMemcpyAsync(sellNbr_.p, hSellNbr.data(), sizeof(int) * hSellNbr.size(), hipMemcpyHostToDevice,
+                                stream_));
+        XPBD_HIP(hipMemcpyAsync(sellRA_.p, hSellRA.data(), sizeof(float2) * hSellRA.size(), hipMemcpyHostToDevice,
+                                stream_));
+        XPBD_HIP(hipStreamSynchronize(stream_));   // host staging goes out of scope
     }
     XPBD_HIP(hipMemcpyAsync(tileBody_.p, htile.data(), sizeof(int) * htile.size(), hipMemcpyHostToDevice, stream_));
     launch_fill_float(stream_, nPad_, invMass_.p, 1.0f);
@@ -223,6 +341,7 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     XPBD_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostBodies_), sizeof(BodyDev) * B, 0));
     XPBD_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostCols_), sizeof(ColliderDev) * B * kMaxColliders, 0));
     std::memset(hostBodies_, 0, sizeof(BodyDev) * B);
+    for (int b = 0; b < B; ++b) refreshCallerPositions(b);
     lastBodies_.clear();
     alphaKey_.clear();
     XPBD_HIP(hipStreamSynchronize(stream_));
@@ -304,10 +423,15 @@ bool Engine::attach(int b, int count, const int* idx, const float* pos, float we
         attIdx_.alloc(count);
         attPos_.alloc((size_t)3 * count);
     }
-    (void)hipMemcpyAsync(attIdx_.p, idx, sizeof(int) * count, hipMemcpyHostToDevice, stream_);
+    // caller ids -> device ids (out-of-range ids stay out of range and are skipped)
+    std::vector<int> dIdx(idx, idx + count);
+    for (int& k : dIdx)
+        if (k >= 0 && k < nn) k = inv_[b][k];
+    (void)hipMemcpyAsync(attIdx_.p, dIdx.data(), sizeof(int) * count, hipMemcpyHostToDevice, stream_);
     (void)hipMemcpyAsync(attPos_.p, pos, sizeof(float) * 3 * count, hipMemcpyHostToDevice, stream_);
     launch_attach(stream_, count, attIdx_.p, attPos_.p, nb, nb + nn, x_.p, v_.p, invMass_.p, objW_.p,
                   std::fmax(weight, 1.0f));
+    refreshCallerPositions(b);
     attActive_[b] = true;
     return hipStreamSynchronize(stream_) == hipSuccess;
 }
@@ -327,6 +451,7 @@ void Engine::resetToRest(int b)
     (void)hipMemcpyAsync(x_.p + nb, rest_.p + nb, sizeof(float3) * nn, hipMemcpyDeviceToDevice, stream_);
     (void)hipMemsetAsync(v_.p + nb, 0, sizeof(float3) * nn, stream_);
     launch_fill_float(stream_, nn, invMass_.p + nb, 1.0f);
+    refreshCallerPositions(b);
     (void)hipStreamSynchronize(stream_);
     PressState& ps = press_[b];
     ps.active = false; ps.curDisp = 0.0f; ps.ckptIdx = 0; ps.dwell = 0;
@@ -369,9 +494,13 @@ void Engine::launchFromRest(int b, const float* lin, const float* ang, const flo
         hv[i] = make_float3(vx + (wy * rz - wz * ry), vy + (wz * rx - wx * rz), vz + (wx * ry - wy * rx));
     }
     const int nb = bodyBegin_[b];
-    (void)hipMemcpyAsync(x_.p + nb, hp.data(), sizeof(float3) * n, hipMemcpyHostToDevice, stream_);
-    (void)hipMemcpyAsync(v_.p + nb, hv.data(), sizeof(float3) * n, hipMemcpyHostToDevice, stream_);
+    std::vector<float3> dp(n), dv(n);
+    toDevice(b, hp.data(), dp.data(), 1);
+    toDevice(b, hv.data(), dv.data(), 1);
+    (void)hipMemcpyAsync(x_.p + nb, dp.data(), sizeof(float3) * n, hipMemcpyHostToDevice, stream_);
+    (void)hipMemcpyAsync(v_.p + nb, dv.data(), sizeof(float3) * n, hipMemcpyHostToDevice, stream_);
     launch_fill_float(stream_, n, invMass_.p + nb, 1.0f);
+    refreshCallerPositions(b);
     (void)hipStreamSynchronize(stream_);
     PressState& ps = press_[b];
     ps.active = false; ps.curDisp = 0.0f; ps.ckptIdx = 0; ps.dwell = 0;
@@ -488,10 +617,15 @@ bool Engine::uploadParams(std::string& err)
     }
     if (key != alphaKey_) {
         NodeArgs na{nPad_, tileBody_.p, bodies_.p};
-        launch_distance_alpha(stream_, na, rowOff_.p, nstiff_.p, nalpha_.p);
+        launch_distance_alpha(stream_, na, rowOff_.p, nstiff_.p, sellArgs());
         alphaKey_ = key;
     }
     return true;
+}
+
+SellArgs Engine::sellArgs() const
+{
+    return SellArgs{sliceOff_.p, sliceW_.p, sellNbr_.p, sellRA_.p, sellLam_.p};
 }
 
 // Everything that changes the launch sequence (not the parameter values).
@@ -557,7 +691,8 @@ void Engine::recordSubstep(Timer* t)
     const int iters = sig[0];
     const bool anyDist = sig[1], anyCol = sig[2], anyGround = sig[3], anyPress = sig[4];
     const NodeArgs na{nPad_, tileBody_.p, bodies_.p};
-    EpiArgs ea{x_.p, v_.p, cols_.p, colHits_.p, colPush_.p, cM_.p, cRmax_.p, cRg_.p};
+    EpiArgs ea{x_.p, v_.p, cols_.p, colHits_.p, colPush_.p, cM_.p, cRmax_.p, cRg_.p, callerPos_.p,
+               permuted_ ? xOut_.p : nullptr};
     const bool shapePass = sig[9] != 0;
     if (sig[10]) {
         XPBD_T_BEGIN("contact_ground_radius");
@@ -610,8 +745,8 @@ void Engine::recordSubstep(Timer* t)
         if (anyDist) {
             XPBD_T_BEGIN(it == 0 ? (last ? "distance_jacobi_only" : "distance_jacobi_first")
                                  : (last ? "distance_jacobi_last" : "distance_jacobi_mid"));
-            launch_distance_jacobi(stream_, na, it == 0, last, pin, pout, rowOff_.p, nbr_.p, nrest_.p, nalpha_.p,
-                                   lam_.p, (volPass || shapePass) ? 0u : preEpi, ea);
+            launch_distance_jacobi(stream_, na, it == 0, last, pin, pout, sellArgs(),
+                                   (volPass || shapePass) ? 0u : preEpi, ea);
             XPBD_T_END();
             std::swap(pin, pout);
         }
@@ -747,24 +882,40 @@ bool Engine::sync(std::string& err)
 bool Engine::getState(int b, float* pos, float* vel)
 {
     const int nb = bodyBegin_[b], n = bodyCount_[b];
-    if (pos && hipMemcpyAsync(pos, x_.p + nb, sizeof(float3) * n, hipMemcpyDeviceToHost, stream_) != hipSuccess) return false;
-    if (vel && hipMemcpyAsync(vel, v_.p + nb, sizeof(float3) * n, hipMemcpyDeviceToHost, stream_) != hipSuccess) return false;
-    return hipStreamSynchronize(stream_) == hipSuccess;
+    std::vector<float> hp(pos ? 3 * (size_t)n : 0), hv(vel ? 3 * (size_t)n : 0);
+    if (pos && hipMemcpyAsync(hp.data(), x_.p + nb, sizeof(float3) * n, hipMemcpyDeviceToHost, stream_) != hipSuccess)
+        return false;
+    if (vel && hipMemcpyAsync(hv.data(), v_.p + nb, sizeof(float3) * n, hipMemcpyDeviceToHost, stream_) != hipSuccess)
+        return false;
+    if (hipStreamSynchronize(stream_) != hipSuccess) return false;
+    if (pos) toCaller(b, hp.data(), pos, 3);
+    if (vel) toCaller(b, hv.data(), vel, 3);
+    return true;
 }
 
 bool Engine::setState(int b, const float* pos, const float* vel)
 {
     const int nb = bodyBegin_[b], n = bodyCount_[b];
-    if (pos && hipMemcpyAsync(x_.p + nb, pos, sizeof(float3) * n, hipMemcpyHostToDevice, stream_) != hipSuccess) return false;
-    if (vel && hipMemcpyAsync(v_.p + nb, vel, sizeof(float3) * n, hipMemcpyHostToDevice, stream_) != hipSuccess) return false;
+    std::vector<float> hp(pos ? 3 * (size_t)n : 0), hv(vel ? 3 * (size_t)n : 0);
+    if (pos) toDevice(b, pos, hp.data(), 3);
+    if (vel) toDevice(b, vel, hv.data(), 3);
+    if (pos && hipMemcpyAsync(x_.p + nb, hp.data(), sizeof(float3) * n, hipMemcpyHostToDevice, stream_) != hipSuccess)
+        return false;
+    if (vel && hipMemcpyAsync(v_.p + nb, hv.data(), sizeof(float3) * n, hipMemcpyHostToDevice, stream_) != hipSuccess)
+        return false;
+    if (pos) refreshCallerPositions(b);
     return hipStreamSynchronize(stream_) == hipSuccess;
 }
 
 bool Engine::getInvMass(int b, float* w)
 {
     const int nb = bodyBegin_[b], n = bodyCount_[b];
-    if (hipMemcpyAsync(w, invMass_.p + nb, sizeof(float) * n, hipMemcpyDeviceToHost, stream_) != hipSuccess) return false;
-    return hipStreamSynchronize(stream_) == hipSuccess;
+    std::vector<float> h(n);
+    if (hipMemcpyAsync(h.data(), invMass_.p + nb, sizeof(float) * n, hipMemcpyDeviceToHost, stream_) != hipSuccess)
+        return false;
+    if (hipStreamSynchronize(stream_) != hipSuccess) return false;
+    toCaller(b, h.data(), w, 1);
+    return true;
 }
 
 }  // namespace xpbd

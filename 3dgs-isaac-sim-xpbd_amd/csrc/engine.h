@@ -91,6 +91,13 @@ public:
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
 
+    // Node order in device memory, applied by the next build: 0 keeps the
+    // caller's order, 1 (default) sorts each body's nodes along a Morton
+    // (Z-order) curve of the rest positions so that graph neighbours share
+    // cache lines.  Every host-facing array stays in the caller's order.
+    void setNodeOrder(int mode) { nodeOrderMode_ = mode; }
+    bool permuted() const { return permuted_; }
+
     // Allocate and upload: x = rest, v = 0, invMass = 1 for every body.
     bool build(const std::vector<const BodyInput*>& inputs, std::string& err);
 
@@ -99,6 +106,7 @@ public:
     int nodeCount(int b) const { return bodyCount_[b]; }
     int padNodes() const { return nPad_; }
     long long directedEntries() const { return m_; }
+    long long slicedEntries() const { return sellEntries_; }
 
     Settings& settings(int b) { return set_[b]; }
     PressState& press(int b) { return press_[b]; }
@@ -121,7 +129,8 @@ public:
     // engine stream; returns JSON {"label": [total_ms, launches], ...}.
     bool profile(int substeps, std::string& json, std::string& err);
 
-    float3* positionsDevice() { return x_.p; }
+    // Committed positions in the caller's node order (device memory).
+    float3* positionsDevice() { return permuted_ ? xOut_.p : x_.p; }
     hipStream_t stream() const { return stream_; }
     bool getState(int b, float* pos, float* vel);
     bool setState(int b, const float* pos, const float* vel);
@@ -167,6 +176,29 @@ private:
     void recordSubstep(Timer* t);
     bool rebuildGraph(std::string& err);
 
+    // caller <-> device node order: device local d of body b holds the
+    // caller's node ord_[b][d]; inv_[b] is the inverse
+    std::vector<std::vector<int>> ord_, inv_;
+    int nodeOrderMode_ = 1;
+    bool permuted_ = false;
+    DevBuf<int> callerPos_;      // [nPad] caller-order global index of each device node
+    DevBuf<float3> xOut_;        // committed positions in caller order (permuted only)
+    void refreshCallerPositions(int b);
+    template <typename T>
+    void toDevice(int b, const T* src, T* dst, int comps) const
+    {
+        const std::vector<int>& o = ord_[b];
+        for (size_t d = 0; d < o.size(); ++d)
+            for (int c = 0; c < comps; ++c) dst[d * comps + c] = src[(size_t)o[d] * comps + c];
+    }
+    template <typename T>
+    void toCaller(int b, const T* src, T* dst, int comps) const
+    {
+        const std::vector<int>& o = ord_[b];
+        for (size_t d = 0; d < o.size(); ++d)
+            for (int c = 0; c < comps; ++c) dst[(size_t)o[d] * comps + c] = src[d * comps + c];
+    }
+
     hipStream_t stream_ = nullptr;
     int nPad_ = 0;
     long long m_ = 0;
@@ -188,7 +220,13 @@ private:
     DevBuf<float4> p0_, p1_;
     DevBuf<float> invMass_, invMassSaved_, objW_;
     DevBuf<int> rowOff_, nbr_;
-    DevBuf<float> nrest_, nstiff_, nalpha_, lam_;
+    DevBuf<float> nstiff_;
+    // distance entries in sliced-ELL form (kernels.h SellArgs)
+    DevBuf<int> sliceOff_, sliceW_, sellNbr_;
+    DevBuf<float2> sellRA_;
+    DevBuf<float> sellLam_;
+    long long sellEntries_ = 0;
+    SellArgs sellArgs() const;
     DevBuf<int> tileBody_;
     DevBuf<BodyDev> bodies_;
     DevBuf<ColliderDev> cols_;

@@ -76,6 +76,7 @@ SelfColArgs Engine::scArgs() const
     a.accum = scAccum_.p;
     a.pushSums = pushSums_.p;
     a.pushT = pushT_.p;
+    a.callerPos = callerPos_.p;
     return a;
 }
 
@@ -146,9 +147,10 @@ bool Engine::ensureObjects(std::string& err)
     for (int b = 0; b < B; ++b) {
         const ObjTopo& t = objTopo_[b];
         const int nb = bodyBegin_[b], ob = objBase_[b];
-        for (int i = 0; i < bodyCount_[b]; ++i) {
-            hId[nb + i] = t.id[i] >= 0 ? ob + t.id[i] : -1;
-            for (int a = 0; a < 3; ++a) hRel[3 * (size_t)(nb + i) + a] = t.restRel[3 * (size_t)i + a];
+        for (int i = 0; i < bodyCount_[b]; ++i) {   // caller node i -> device node nb + d
+            const size_t d = (size_t)nb + inv_[b][i];
+            hId[d] = t.id[i] >= 0 ? ob + t.id[i] : -1;
+            for (int a = 0; a < 3; ++a) hRel[3 * d + a] = t.restRel[3 * (size_t)i + a];
         }
         for (int k = 0; k < t.K; ++k) {
             hBody[ob + k] = b;
@@ -289,27 +291,33 @@ bool Engine::selfCollisionLists(int b, unsigned int* keys, int* sorted, int* cou
         hk[hs[t]] = hks[t];
     }
     const unsigned int base = (unsigned int)scTableBase_[b], T = (unsigned int)scTable_[b];
+    const std::vector<int>& ord = ord_[b];
+    const std::vector<int>& inv = inv_[b];
+    auto inBody = [&](int t) { return hs[t] >= nb && hs[t] < nb + n; };
+    auto inTable = [&](int t) { return hks[t] >= base && hks[t] < base + T; };
     if (keys)
         for (int i = 0; i < n; ++i) {
-            const unsigned int k = hk[nb + i];
+            const unsigned int k = hk[nb + inv[i]];
             keys[i] = (k >= base && k < base + T) ? k - base : T;
         }
     if (sorted) {
         int w = 0;
-        for (int t = 0; t < nPad_; ++t) {
-            const int j = hs[t];
-            if (j >= nb && j < nb + n && hks[t] >= base && hks[t] < base + T) sorted[w++] = j - nb;
-        }
-        for (int t = 0; t < nPad_; ++t) {
-            const int j = hs[t];
-            if (j >= nb && j < nb + n && !(hks[t] >= base && hks[t] < base + T)) sorted[w++] = j - nb;
-        }
+        for (int t = 0; t < nPad_; ++t)
+            if (inBody(t) && inTable(t)) sorted[w++] = ord[hs[t] - nb];
+        for (int t = 0; t < nPad_; ++t)
+            if (inBody(t) && !inTable(t)) sorted[w++] = ord[hs[t] - nb];
     }
-    if (counts) std::copy(hc.begin(), hc.end(), counts);
+    if (counts)
+        for (int i = 0; i < n; ++i) counts[i] = hc[inv[i]];
     if (contacts)
-        for (size_t q = 0; q < hl.size(); ++q) {
-            const int code = hl[q];
-            contacts[q] = code >= 0 ? code - nb : -((-code - 1 - nb) + 1);
+        for (int i = 0; i < n; ++i) {
+            const int d = inv[i];
+            for (int k = 0; k < kSelfColCap; ++k) {
+                const int code = hl[(size_t)d * kSelfColCap + k];
+                int out = 0;   // slots past the count hold earlier builds' entries
+                if (k < hc[d]) out = code >= 0 ? ord[code - nb] : -(ord[-code - 1 - nb] + 1);
+                contacts[(size_t)i * kSelfColCap + k] = out;
+            }
         }
     return true;
 }

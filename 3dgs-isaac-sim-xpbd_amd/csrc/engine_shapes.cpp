@@ -3,6 +3,7 @@
 // updateContactShapes; native/xpbd_capi.cu:179-191, 248-260).
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "engine.h"
 
@@ -34,11 +35,15 @@ bool Engine::ensureShapeBuffers(std::string& err)
     for (int b = 0; b < numBodies(); ++b) {
         const BodyInput& in = *inputs_[b];
         const size_t nb = bodyBegin_[b], n = in.n;
-        XPBD_HIP(hipMemcpyAsync(gsScale_.p + 3 * nb, in.scales.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, stream_));
-        XPBD_HIP(hipMemcpyAsync(gsRot_.p + 4 * nb, in.rots.data(), sizeof(float) * 4 * n, hipMemcpyHostToDevice, stream_));
-        XPBD_HIP(hipMemcpyAsync(gsOpacity_.p + nb, in.opacity.data(), sizeof(float) * n, hipMemcpyHostToDevice, stream_));
+        std::vector<float> sc(3 * n), rt(4 * n), op(n);
+        toDevice(b, in.scales.data(), sc.data(), 3);
+        toDevice(b, in.rots.data(), rt.data(), 4);
+        toDevice(b, in.opacity.data(), op.data(), 1);
+        XPBD_HIP(hipMemcpyAsync(gsScale_.p + 3 * nb, sc.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, stream_));
+        XPBD_HIP(hipMemcpyAsync(gsRot_.p + 4 * nb, rt.data(), sizeof(float) * 4 * n, hipMemcpyHostToDevice, stream_));
+        XPBD_HIP(hipMemcpyAsync(gsOpacity_.p + nb, op.data(), sizeof(float) * n, hipMemcpyHostToDevice, stream_));
+        XPBD_HIP(hipStreamSynchronize(stream_));   // staging vectors go out of scope
     }
-    XPBD_HIP(hipStreamSynchronize(stream_));
     return true;
 }
 
@@ -83,13 +88,16 @@ bool Engine::computeShapes(int b, float deformEps, float* hostScales, float* hos
         launch_contact_pack(stream_, (int)nb, n, outScale_.p + 3 * nb, outQuat_.p + 4 * nb, gsOpacity_.p + nb,
                             contactTau_[b], cap_or_zero(contactCap_[b]), cM_.p, cRmax_.p);
     unsigned int c = 0;
+    std::vector<float> sc(hostScales ? 3 * (size_t)n : 0), qt(hostQuats ? 4 * (size_t)n : 0);
     XPBD_HIP(hipMemcpyAsync(&c, shapeCtr_.p, sizeof(unsigned int), hipMemcpyDeviceToHost, stream_));
     if (hostScales)
-        XPBD_HIP(hipMemcpyAsync(hostScales, outScale_.p + 3 * nb, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, stream_));
+        XPBD_HIP(hipMemcpyAsync(sc.data(), outScale_.p + 3 * nb, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, stream_));
     if (hostQuats)
-        XPBD_HIP(hipMemcpyAsync(hostQuats, outQuat_.p + 4 * nb, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, stream_));
+        XPBD_HIP(hipMemcpyAsync(qt.data(), outQuat_.p + 4 * nb, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, stream_));
     XPBD_HIP(hipStreamSynchronize(stream_));
     XPBD_HIP(hipGetLastError());
+    if (hostScales) toCaller(b, sc.data(), hostScales, 3);
+    if (hostQuats) toCaller(b, qt.data(), hostQuats, 4);
     deformed = (int)c;
     return true;
 }

@@ -96,12 +96,17 @@ bool Engine::ensureVolume(bool needAll, std::string& err)
         const VolumeTopo& t = volTopo_[b];
         const int nb = bodyBegin_[b], n = bodyCount_[b];
         const int padEnd = (b + 1 < B) ? bodyBegin_[b + 1] : nPad_;
-        for (int i = 0; i < n; ++i) {
-            hOff[nb + i] = (int)hIdx.size();
-            hRevOff[nb + i] = (int)hRev.size();
+        // device node d is the caller's node ord[d]; cluster member lists and
+        // the clusters containing a node keep the caller-order topology's order
+        const std::vector<int>& ord = ord_[b];
+        const std::vector<int>& inv = inv_[b];
+        for (int d = 0; d < n; ++d) {
+            const int i = ord[d];
+            hOff[nb + d] = (int)hIdx.size();
+            hRevOff[nb + d] = (int)hRev.size();
             if (t.memOff.empty()) continue;
-            for (int k = t.memOff[i]; k < t.memOff[i + 1]; ++k) hIdx.push_back(nb + t.memIdx[k]);
-            for (int k = t.revOff[i]; k < t.revOff[i + 1]; ++k) hRev.push_back(nb + t.revIdx[k]);
+            for (int k = t.memOff[i]; k < t.memOff[i + 1]; ++k) hIdx.push_back(nb + inv[t.memIdx[k]]);
+            for (int k = t.revOff[i]; k < t.revOff[i + 1]; ++k) hRev.push_back(nb + inv[t.revIdx[k]]);
         }
         for (int i = nb + n; i <= padEnd && i <= nPad_; ++i) {
             hOff[i] = (int)hIdx.size();
@@ -176,7 +181,8 @@ bool Engine::volumeStats(int b, float& jvw, float& jvwStd, std::string& err)
     XPBD_HIP(hipMemcpyAsync(J.data(), volJ_.p + nb, sizeof(float) * n, hipMemcpyDeviceToHost, stream_));
     XPBD_HIP(hipStreamSynchronize(stream_));
     double wSum = 0.0, wJ = 0.0, wJ2 = 0.0;
-    for (int i = 0; i < n; ++i) {
+    for (int c = 0; c < n; ++c) {   // caller order: the sums are order-sensitive
+        const int i = inv_[b][c];
         if (hMatType_[nb + i] != 0) continue;
         const float j = J[i];
         if (!std::isfinite(j)) continue;
@@ -207,8 +213,8 @@ bool Engine::volumeTopology(int b, int* leaders, int* memberCount, int* revCount
         if (leaders) leaders[i] = has ? t.isLeader[i] : 0;
         if (memberCount) memberCount[i] = has ? t.memOff[i + 1] - t.memOff[i] : 0;
         if (revCount) revCount[i] = has ? t.revOff[i + 1] - t.revOff[i] : 0;
-        if (matType) matType[i] = hMatType_[nb + i];
-        if (detRest) detRest[i] = dr[i];
+        if (matType) matType[i] = hMatType_[nb + inv_[b][i]];
+        if (detRest) detRest[i] = dr[inv_[b][i]];
     }
     return true;
 }

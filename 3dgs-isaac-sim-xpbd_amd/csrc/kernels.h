@@ -6,11 +6,24 @@
 
 namespace xpbd {
 
+// Grid size for kernels that remap blocks per XCD (a multiple of 8).
+inline int xcd_grid(int blocks) { return (blocks + 7) & ~7; }
+
 // Node-space launch description shared by every per-node kernel.
 struct NodeArgs {
     int nPad;                 // padded node count (multiple of kTile)
     const int* tileBody;      // [nPad / kTile]
     const BodyDev* bodies;    // [B]
+};
+
+// Directed distance entries in sliced-ELL form: slice t (= node tile t) holds
+// sliceW[t] columns of 64 lanes from entry sliceOff[t]; padding has nbr -1.
+struct SellArgs {
+    const int* sliceOff;      // [tiles]
+    const int* sliceW;        // [tiles]
+    const int* nbr;           // [entries]
+    float2* restAlpha;        // [entries] rest length, alpha tilde
+    float* lam;               // [entries]
 };
 
 // Buffers the per-node epilogue touches.
@@ -23,6 +36,8 @@ struct EpiArgs {
     const float* cM;          // contact ellipsoids k^2 Sigma [N x 6] or null
     const float* cRmax;       // [N] or null
     const float* rg;          // [N] ground support radius or null
+    const int* callerPos;     // [N] caller-order index of each node
+    float3* xCaller;          // committed positions in caller order, or null
 };
 
 // Volume-constraint buffers (node space; leaders own clusters).
@@ -91,6 +106,7 @@ struct SelfColArgs {
     float3* accum;
     float* pushSums;           // [4K]
     float* pushT;              // [4K]
+    const int* callerPos;      // [nPad] caller-order index (sort position of equal keys)
 };
 
 void launch_object_shape(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, int K, float4* P, const float3* rest,
@@ -104,7 +120,7 @@ void launch_selfcol_iteration(hipStream_t s, const NodeArgs& na, const SelfColAr
 void launch_contact_push(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, int K, float4* P, const EpiArgs& ea);
 int radix_sort_tiles(int n);
 bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0, unsigned int* k1, int* v1,
-                       unsigned int* hist);
+                       unsigned int* hist, bool iota);
 void launch_fill_double(hipStream_t s, int n, double* a);
 
 void launch_shape_match(hipStream_t s, const NodeArgs& na, const float4* Pin, float4* Pout, const float3* rest,
@@ -120,14 +136,16 @@ void launch_contact_ground(hipStream_t s, const NodeArgs& na, const float* M, fl
 
 void launch_predict(hipStream_t s, const NodeArgs& na, const float3* X, float3* V, const float* invMass, float4* P);
 void launch_distance_jacobi(hipStream_t s, const NodeArgs& na, bool first, bool last, const float4* Pin, float4* Pout,
-                            const int* rowOff, const int* nbr, const float* nrest, const float* nalpha, float* lam,
-                            unsigned epi, const EpiArgs& ea);
+                            const SellArgs& sa, unsigned epi, const EpiArgs& ea);
 void launch_epilogue(hipStream_t s, const NodeArgs& na, float4* P, unsigned epi, const EpiArgs& ea);
-void launch_distance_alpha(hipStream_t s, const NodeArgs& na, const int* rowOff, const float* nstiff, float* nalpha);
+void launch_distance_alpha(hipStream_t s, const NodeArgs& na, const int* rowOff, const float* nstiff, const SellArgs& sa);
 void launch_attach(hipStream_t s, int count, const int* idx, const float* pos, int nodeBegin, int nodeEnd, float3* X,
                    float3* V, float* invMass, float* objW, float weight);
 void launch_rigid_velocity(hipStream_t s, int nodeBegin, int nodeEnd, const float3* X, float3* V, const float* invMass,
                            float3 dv, float3 dw, float3 c);
+// xCaller[callerPos[i]] = X[i] over nodes [nodeBegin, nodeEnd).
+void launch_scatter_caller(hipStream_t s, int nodeBegin, int nodeEnd, const int* callerPos, const float3* X,
+                           float3* xCaller);
 void launch_fill_float(hipStream_t s, int n, float* a, float v);
 void launch_fill_int(hipStream_t s, int n, int* a, int v);
 

@@ -1,5 +1,5 @@
 // kernels_core.hip — per-substep node kernels of the MI355X XPBD solver:
-// predict, distance (Jacobi over the directed CSR) with a fused per-node
+// predict, distance (Jacobi over the directed graph, sliced ELL) with a fused per-node
 // epilogue (press plates, kinematic colliders, ground, velocity commit), and
 // the small state kernels behind the C ABI setters.
 //
@@ -39,21 +39,31 @@ __global__ void __launch_bounds__(256) predict_kernel(int nPad, const int* __res
 }
 
 // ---------------------------------------------------------------------------
-// Distance constraint, Jacobi over the directed CSR (forward.cu:6484-6571).
-// One thread per node walks its row in input order.  The multiplier is kept
-// per directed entry in place: the first iteration of a substep does not read
-// it (it is zero), the last does not write it (it is reset next substep).
+// Distance constraint, Jacobi (forward.cu:6484-6571) over the directed graph
+// in sliced-ELL form: one wave64 per slice of 64 nodes, entry k of the slice's
+// lane l at sliceOff + 64 k + l, so every per-entry load of a wave is one
+// contiguous 256 B (nbr, lambda) or 512 B (rest, alpha) request.  Rows keep
+// the reference's entry order (padding only at the end, nbr = -1), so the
+// per-node sums are the reference's bit for bit.  The multiplier is kept per
+// directed entry in place: the first iteration of a substep does not read it
+// (it is zero), the last does not write it (it is reset next substep).
+// Entries are processed four at a time: the loads and neighbour gathers of a
+// group are issued before its arithmetic, which stays in row order.
+// Per entry 20 B (nbr 4, rest+alpha 8, lambda 4 + 4) plus the 16 B neighbour
+// gather (cache resident); per node 32 B (p in, p out).
 // ---------------------------------------------------------------------------
+constexpr int kDistGroup = 4;
+
 template <bool kFirst, bool kLast>
 __global__ void __launch_bounds__(256) distance_jacobi_kernel(
-    int nPad, const int* __restrict__ tileBody, const BodyDev* __restrict__ bodies,
-    const float4* __restrict__ Pin, float4* __restrict__ Pout,
-    const int* __restrict__ rowOff, const int* __restrict__ nbr, const float* __restrict__ nrest,
-    const float* __restrict__ nalpha, float* __restrict__ lam, unsigned epi, EpiArgs ea)
+    int nTiles, const int* __restrict__ tileBody, const BodyDev* __restrict__ bodies,
+    const float4* __restrict__ Pin, float4* __restrict__ Pout, SellArgs sa, unsigned epi, EpiArgs ea)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nPad) return;
-    const BodyDev& B = bodies[wave_body(tileBody, i)];
+    const int tile = xcd_block() * 4 + (threadIdx.x >> 6);
+    if (tile >= nTiles) return;
+    const int lane = threadIdx.x & 63;
+    const int i = tile * kTile + lane;
+    const BodyDev& B = bodies[__builtin_amdgcn_readfirstlane(tileBody[tile])];
     if (i >= B.nodeEnd) return;
     const float4 pi4 = Pin[i];
     const float3 pi = xyz(pi4);
@@ -66,32 +76,51 @@ __global__ void __launch_bounds__(256) distance_jacobi_kernel(
     const float wi = pi4.w * B.invMassScale;
     float3 acc = f3(0.0f, 0.0f, 0.0f);
     int valid = 0;
-    const int e0 = rowOff[i], e1 = rowOff[i + 1];
-    for (int e = e0; e < e1; ++e) {
-        const int j = nbr[e];
-        float l = kFirst ? 0.0f : lam[e];
-        const float4 pj4 = Pin[j];
-        const float3 diff = sub3(pi, xyz(pj4));
-        const float d = len3(diff);
-        if (d < 1e-7f) {
-            if (!kLast && kFirst) lam[e] = l;
-            continue;
+    const int width = __builtin_amdgcn_readfirstlane(sa.sliceW[tile]);
+    const int base = __builtin_amdgcn_readfirstlane(sa.sliceOff[tile]) + lane;
+    for (int e0 = 0; e0 < width; e0 += kDistGroup) {
+        int jj[kDistGroup];
+        float2 ra[kDistGroup];
+        float ll[kDistGroup];
+        float4 pj[kDistGroup];
+#pragma unroll
+        for (int k = 0; k < kDistGroup; ++k) jj[k] = (e0 + k < width) ? sa.nbr[base + (e0 + k) * kTile] : -1;
+#pragma unroll
+        for (int k = 0; k < kDistGroup; ++k) {
+            if (jj[k] >= 0) {
+                const int slot = base + (e0 + k) * kTile;
+                ra[k] = sa.restAlpha[slot];
+                ll[k] = kFirst ? 0.0f : sa.lam[slot];
+                pj[k] = Pin[jj[k]];
+            }
         }
-        const float C = d - nrest[e];
-        const float3 grad = mul3(diff, 1.0f / d);
-        const float wj = pj4.w * B.invMassScale;
-        const float at = nalpha[e];
-        const float denom = (wi + wj) + at;
-        if (denom < 1e-8f) {
-            if (!kLast && kFirst) lam[e] = l;
-            continue;
-        }
-        const float dl = (-C - at * l) / denom;
-        l += dl;
-        if (!kLast) lam[e] = l;
-        if (wi > 0.0f) {
-            acc = add3(acc, mul3(grad, wi * dl));
-            valid++;
+#pragma unroll
+        for (int k = 0; k < kDistGroup; ++k) {
+            if (jj[k] < 0) continue;
+            const int slot = base + (e0 + k) * kTile;
+            float l = ll[k];
+            const float3 diff = sub3(pi, xyz(pj[k]));
+            const float d = len3(diff);
+            if (d < 1e-7f) {
+                if (!kLast && kFirst) sa.lam[slot] = l;
+                continue;
+            }
+            const float C = d - ra[k].x;
+            const float3 grad = mul3(diff, 1.0f / d);
+            const float wj = pj[k].w * B.invMassScale;
+            const float at = ra[k].y;
+            const float denom = (wi + wj) + at;
+            if (denom < 1e-8f) {
+                if (!kLast && kFirst) sa.lam[slot] = l;
+                continue;
+            }
+            const float dl = (-C - at * l) / denom;
+            l += dl;
+            if (!kLast) sa.lam[slot] = l;
+            if (wi > 0.0f) {
+                acc = add3(acc, mul3(grad, wi * dl));
+                valid++;
+            }
         }
     }
     float3 out = pi;
@@ -119,10 +148,10 @@ __global__ void __launch_bounds__(256) epilogue_kernel(int nPad, const int* __re
 }
 
 // alpha tilde per directed entry (the expression of forward.cu:6533-6545,
-// evaluated once per settings change instead of every iteration).
+// evaluated once per settings change instead of every iteration), written to
+// the sliced-ELL slot of the entry.
 __global__ void distance_alpha_kernel(int nPad, const int* __restrict__ tileBody, const BodyDev* __restrict__ bodies,
-                                      const int* __restrict__ rowOff, const float* __restrict__ nstiff,
-                                      float* __restrict__ nalpha)
+                                      const int* __restrict__ rowOff, const float* __restrict__ nstiff, SellArgs sa)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nPad) return;
@@ -130,10 +159,13 @@ __global__ void distance_alpha_kernel(int nPad, const int* __restrict__ tileBody
     if (i >= B.nodeEnd) return;
     const float compliance = B.distCompliance, stiffScale = B.stiffnessScale;
     const float dt2 = fmaxf(B.dt * B.dt, 1e-8f);
+    const int tile = i / kTile, lane = i % kTile;
+    const int base = sa.sliceOff[tile] + lane;
+    float* ra = reinterpret_cast<float*>(sa.restAlpha);
     for (int e = rowOff[i]; e < rowOff[i + 1]; ++e) {
         const float st = fmaxf(nstiff[e] * stiffScale, 1e-6f);
         const float alpha = compliance / st;
-        nalpha[e] = alpha / dt2;
+        ra[2 * (size_t)(base + (e - rowOff[i]) * kTile) + 1] = alpha / dt2;
     }
 }
 
@@ -165,6 +197,13 @@ __global__ void rigid_velocity_kernel(int nodeBegin, int nodeEnd, const float3* 
               v.z + dv.z + (dw.x * ry - dw.y * rx));
 }
 
+__global__ void scatter_caller_kernel(int nodeBegin, int nodeEnd, const int* __restrict__ callerPos,
+                                      const float3* __restrict__ X, float3* __restrict__ xCaller)
+{
+    const int i = nodeBegin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nodeEnd) xCaller[callerPos[i]] = X[i];
+}
+
 __global__ void fill_float_kernel(int n, float* a, float v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -188,18 +227,18 @@ void launch_predict(hipStream_t s, const NodeArgs& na, const float3* X, float3* 
 }
 
 void launch_distance_jacobi(hipStream_t s, const NodeArgs& na, bool first, bool last, const float4* Pin, float4* Pout,
-                            const int* rowOff, const int* nbr, const float* nrest, const float* nalpha, float* lam,
-                            unsigned epi, const EpiArgs& ea)
+                            const SellArgs& sa, unsigned epi, const EpiArgs& ea)
 {
-    const dim3 g(blocks_for(na.nPad, 256)), b(256);
+    const int tiles = na.nPad / kTile;
+    const dim3 g(xcd_grid(blocks_for(tiles, 4))), b(256);
     if (first && last)
-        distance_jacobi_kernel<true, true><<<g, b, 0, s>>>(na.nPad, na.tileBody, na.bodies, Pin, Pout, rowOff, nbr, nrest, nalpha, lam, epi, ea);
+        distance_jacobi_kernel<true, true><<<g, b, 0, s>>>(tiles, na.tileBody, na.bodies, Pin, Pout, sa, epi, ea);
     else if (first)
-        distance_jacobi_kernel<true, false><<<g, b, 0, s>>>(na.nPad, na.tileBody, na.bodies, Pin, Pout, rowOff, nbr, nrest, nalpha, lam, epi, ea);
+        distance_jacobi_kernel<true, false><<<g, b, 0, s>>>(tiles, na.tileBody, na.bodies, Pin, Pout, sa, epi, ea);
     else if (last)
-        distance_jacobi_kernel<false, true><<<g, b, 0, s>>>(na.nPad, na.tileBody, na.bodies, Pin, Pout, rowOff, nbr, nrest, nalpha, lam, epi, ea);
+        distance_jacobi_kernel<false, true><<<g, b, 0, s>>>(tiles, na.tileBody, na.bodies, Pin, Pout, sa, epi, ea);
     else
-        distance_jacobi_kernel<false, false><<<g, b, 0, s>>>(na.nPad, na.tileBody, na.bodies, Pin, Pout, rowOff, nbr, nrest, nalpha, lam, epi, ea);
+        distance_jacobi_kernel<false, false><<<g, b, 0, s>>>(tiles, na.tileBody, na.bodies, Pin, Pout, sa, epi, ea);
 }
 
 void launch_epilogue(hipStream_t s, const NodeArgs& na, float4* P, unsigned epi, const EpiArgs& ea)
@@ -207,10 +246,9 @@ void launch_epilogue(hipStream_t s, const NodeArgs& na, float4* P, unsigned epi,
     epilogue_kernel<<<blocks_for(na.nPad, 256), 256, 0, s>>>(na.nPad, na.tileBody, na.bodies, P, epi, ea);
 }
 
-void launch_distance_alpha(hipStream_t s, const NodeArgs& na, const int* rowOff, const float* nstiff, float* nalpha)
+void launch_distance_alpha(hipStream_t s, const NodeArgs& na, const int* rowOff, const float* nstiff, const SellArgs& sa)
 {
-    distance_alpha_kernel<<<blocks_for(na.nPad, 256), 256, 0, s>>>(na.nPad, na.tileBody, na.bodies, rowOff, nstiff,
-                                                                  nalpha);
+    distance_alpha_kernel<<<blocks_for(na.nPad, 256), 256, 0, s>>>(na.nPad, na.tileBody, na.bodies, rowOff, nstiff, sa);
 }
 
 void launch_attach(hipStream_t s, int count, const int* idx, const float* pos, int nodeBegin, int nodeEnd, float3* X,
@@ -226,6 +264,13 @@ void launch_rigid_velocity(hipStream_t s, int nodeBegin, int nodeEnd, const floa
     const int n = nodeEnd - nodeBegin;
     if (n <= 0) return;
     rigid_velocity_kernel<<<blocks_for(n, 256), 256, 0, s>>>(nodeBegin, nodeEnd, X, V, invMass, dv, dw, c);
+}
+
+void launch_scatter_caller(hipStream_t s, int nodeBegin, int nodeEnd, const int* callerPos, const float3* X,
+                           float3* xCaller)
+{
+    const int n = nodeEnd - nodeBegin;
+    if (n > 0) scatter_caller_kernel<<<blocks_for(n, 256), 256, 0, s>>>(nodeBegin, nodeEnd, callerPos, X, xCaller);
 }
 
 void launch_fill_float(hipStream_t s, int n, float* a, float v)

@@ -263,7 +263,9 @@ __global__ void sc_reach_kernel(int B, SelfColArgs a)
     a.scInvCell[b] = 1.0f / reach;
 }
 
-// Keys of the frame-start positions (forward.cu:7034-7044) in the body's table.
+// Keys of the frame-start positions (forward.cu:7034-7044) in the body's table,
+// stored at the node's caller-order position with the device index as value,
+// so that the stable sort breaks ties by the caller's node order.
 __global__ void __launch_bounds__(256) sc_keys_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -279,7 +281,9 @@ __global__ void __launch_bounds__(256) sc_keys_kernel(int nPad, SelfColArgs a, c
                   sc_hash(sc_cell(p.x, inv), sc_cell(p.y, inv), sc_cell(p.z, inv), (unsigned int)B.scTableMask);
         }
     }
-    a.keys[i] = key;
+    const int pos = a.callerPos[i];
+    a.keys[pos] = key;
+    a.idx[pos] = i;
 }
 
 __global__ void __launch_bounds__(256) sc_range_kernel(int nPad, SelfColArgs a, const unsigned int* __restrict__ keysSorted)
@@ -580,7 +584,7 @@ __global__ void __launch_bounds__(256) push_apply_kernel(int nPad, SelfColArgs a
             if (c < 0.0f) p = make_float3(p.x - c * B.gn[0], p.y - c * B.gn[1], p.z - c * B.gn[2]);
         }
     }
-    commit_node(i, p, p4.w, B, ea.x, ea.v, B.contactGround ? ea.rg : nullptr);
+    commit_node(i, p, p4.w, B, ea, B.contactGround ? ea.rg : nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -608,7 +612,7 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
     sc_active_disp_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P);
     sc_reach_kernel<<<nbs(numBodies, 64), 64, 0, s>>>(numBodies, a);
     sc_keys_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X);
-    const bool inAlt = launch_radix_sort(s, na.nPad, sortBits, a.keys, a.idx, keysAlt, idxAlt, hist);
+    const bool inAlt = launch_radix_sort(s, na.nPad, sortBits, a.keys, a.idx, keysAlt, idxAlt, hist, false);
     const unsigned int* ks = inAlt ? keysAlt : a.keys;
     const int* vs = inAlt ? idxAlt : a.idx;
     launch_fill_int(s, a.tableTotal, a.cellStart, -1);

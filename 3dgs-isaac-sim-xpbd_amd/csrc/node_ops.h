@@ -16,6 +16,16 @@ __device__ __forceinline__ float len3(float3 a) { return sqrtf(a.x * a.x + a.y *
 __device__ __forceinline__ float3 xyz(float4 a) { return f3(a.x, a.y, a.z); }
 
 // Body of the wave that owns node i (wave-uniform by construction).
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  With a grid of 8 * per blocks, this maps hardware block b to logical
+// block (b mod 8) * per + b / 8, so every XCD works on one contiguous range of
+// tiles and the neighbour gathers of adjacent tiles hit the same L2.
+__device__ __forceinline__ int xcd_block()
+{
+    const int per = gridDim.x >> 3;
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
 __device__ __forceinline__ int wave_body(const int* __restrict__ tileBody, int i)
 {
     return __builtin_amdgcn_readfirstlane(tileBody[i / kTile]);
@@ -108,8 +118,8 @@ __device__ __forceinline__ void collide_node(int i, float3& p, float3 x0, const 
 }
 
 // Velocity update and commit (forward.cu:6623-6661 and 7528-7599).
-__device__ __forceinline__ void commit_node(int i, float3 p, float w, const BodyDev& B, float3* __restrict__ X, float3* __restrict__ V,
-                            const float* __restrict__ rg)
+__device__ __forceinline__ void commit_xv(int i, float3 p, float w, const BodyDev& B, float3* __restrict__ X,
+                                          float3* __restrict__ V, const float* __restrict__ rg)
 {
     const float3 x = X[i];
     if (w <= 0.0f) {
@@ -166,6 +176,15 @@ __device__ __forceinline__ void commit_node(int i, float3 p, float w, const Body
     X[i] = p;
 }
 
+// Commit plus the caller-order copy of the position when the device order
+// differs from the caller's.
+__device__ __forceinline__ void commit_node(int i, float3 p, float w, const BodyDev& B, const EpiArgs& a,
+                                            const float* __restrict__ rg)
+{
+    commit_xv(i, p, w, B, a.x, a.v, rg);
+    if (a.xCaller) a.xCaller[a.callerPos[i]] = p;
+}
+
 // Per-node tail of an iteration.  Returns true when the node was committed
 // (velocity bit), in which case the predicted buffer need not be written.
 __device__ __forceinline__ bool epilogue_node(unsigned bits, int i, float3& p, float w, const BodyDev& B, const EpiArgs& a)
@@ -188,7 +207,7 @@ __device__ __forceinline__ bool epilogue_node(unsigned bits, int i, float3& p, f
         if (c < 0.0f) p = f3(p.x - c * B.gn[0], p.y - c * B.gn[1], p.z - c * B.gn[2]);
     }
     if (bits & kEpiVelocity) {
-        commit_node(i, p, w, B, a.x, a.v, rg);
+        commit_node(i, p, w, B, a, rg);
         return true;
     }
     return false;

@@ -11,8 +11,9 @@
 //           the four waves from LDS counts, across rounds from running
 //           per-digit counters.  Every key keeps its input order within its
 //           digit, so the sort is stable like an LSD sort must be.
-// Values start as the identity when `iota` is set.  Only the low `bits` bits
-// of the keys are sorted (keys must not have higher bits set).
+// Values start as the identity when `iota` is set, else as given in v0.
+// Only the low `bits` bits of the keys are sorted (keys must not have higher
+// bits set).
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -113,7 +114,7 @@ int radix_sort_tiles(int n) { return (n + kTileKeys - 1) / kTileKeys; }
 // in (k0, v0) when the number of passes is even, else in (k1, v1).  Returns
 // true when the result is in (k1, v1).
 bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0, unsigned int* k1, int* v1,
-                       unsigned int* hist)
+                       unsigned int* hist, bool iota)
 {
     if (n <= 0) return false;
     const int tiles = radix_sort_tiles(n);
@@ -126,7 +127,7 @@ bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0
         const int shift = 8 * p;
         radix_hist_kernel<<<tiles, kSortThreads, 0, s>>>(n, kin, shift, tiles, hist);
         radix_scan_kernel<<<1, 1024, 0, s>>>(256 * tiles, hist);
-        radix_scatter_kernel<<<tiles, kSortThreads, 0, s>>>(n, kin, vin, kout, vout, shift, tiles, hist, p == 0 ? 1 : 0);
+        radix_scatter_kernel<<<tiles, kSortThreads, 0, s>>>(n, kin, vin, kout, vout, shift, tiles, hist, (iota && p == 0) ? 1 : 0);
         unsigned int* tk = kin; kin = kout; kout = tk;
         int* tv = vin; vin = vout; vout = tv;
     }

@@ -118,6 +118,8 @@ def bind(lib):
     L.xpbd_ext_set_volume_clusters.restype = c_int
     L.xpbd_ext_volume_topology.argtypes = [ctypes.c_void_p] * 5 + [c_int]
     L.xpbd_ext_volume_topology.restype = c_int
+    L.xpbd_ext_set_node_order.argtypes = [c_int]
+    L.xpbd_ext_set_node_order.restype = c_int
     L.xpbd_ext_set_self_collision_grid.argtypes = [c_float, c_int]
     L.xpbd_ext_set_self_collision_grid.restype = c_int
     L.xpbd_ext_self_collision_lists.argtypes = [ctypes.c_void_p] * 4 + [c_int]
@@ -331,6 +333,10 @@ class XPBD:
         out = np.empty(self.n, np.float32)
         self._check(self.lib.xpbd_ext_get_inv_mass(out, self.n), "xpbd_ext_get_inv_mass")
         return out
+
+    def ext_set_node_order(self, mode):
+        """Device node layout of the next instance: 1 Morton (default), 0 caller order."""
+        self._check(self.lib.xpbd_ext_set_node_order(int(mode)), "xpbd_ext_set_node_order")
 
     def ext_set_self_collision_grid(self, cell_size=0.0, table_size=0):
         """Fixed self-collision hash cell (0 = displacement rule) and table size (0 = default)."""

@@ -172,6 +172,13 @@ int xpbd_ext_set_self_collision_grid(float cellSize, int tableSize);
  * (j, or -(j+1) for a fast cross-object pair).  Any pointer may be null. */
 int xpbd_ext_self_collision_lists(unsigned int* keys, int* sorted, int* counts, int* contacts, int n);
 
+/* Device memory layout of the next instance (applied at its first step):
+ * 1 (default) stores the nodes sorted along a Morton curve of the rest
+ * positions, so graph neighbours share cache lines; 0 keeps the caller's
+ * order.  Results are identical either way; every array crossing this ABI
+ * (including xpbd_positions_device) stays in the caller's order. */
+int xpbd_ext_set_node_order(int mode);
+
 /* Library identity: "hip-gfx950" for the GPU library, "oracle-cpu" for the oracle. */
 const char* xpbd_ext_backend(void);
 

@@ -158,6 +158,13 @@ void xpbd_self_collision_stats(int* out)
 
 // Extension: fixed hash grid (cell size > 0 replaces the displacement rule;
 // table size, a power of two >= 1024, replaces nextpow2(2N)); 0 keeps the rule.
+// Extension: the oracle keeps the caller's order; accepted for ABI parity.
+int xpbd_ext_set_node_order(int mode)
+{
+    if (mode != 0 && mode != 1) return fail("xpbd_ext_set_node_order: mode must be 0 (caller order) or 1 (Morton)");
+    return 0;
+}
+
 int xpbd_ext_set_self_collision_grid(float cellSize, int tableSize)
 {
     if (tableSize != 0 && (tableSize < 1024 || (tableSize & (tableSize - 1))))
