@@ -86,10 +86,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run a streaming kernel of known traffic (256 MiB read + 256 MiB written, x3) "
+                         "to scale the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.calibrate:
+        import torch
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 2.0, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "mul", "bytes_read": a.numel() * 4, "bytes_written": b.numel() * 4}))
+        return
 
     log(f"generating {args.n}-Gaussian scene")
     t = time.perf_counter()
