@@ -63,6 +63,8 @@ static const double kVolWaveMembers = 24.0;
 // (forward.cu:230, 262)
 static const float kScSearchScale = 1.5f;
 static const float kScFastRelScale = 0.0f;
+// node window (a multiple of the 64-node slice) sorted by row length
+static const int kSortWindow = 8 * kTile;
 
 void build_input(BodyInput& in, int n, const float* pos, const float* scales, const float* rots, const float* opacity,
                  int e, const int* edges, const float* rest, const float* stiff)
@@ -181,6 +183,15 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     if (base > 0x7fffffffLL - kTile || m > 0x7fffffffLL) { err = "engine: problem too large for int32 indexing"; return false; }
     nPad_ = (int)base;
     m_ = m;
+    uniformStiff_.assign(B, 1);
+    uniformStiffValue_.assign(B, 1.0f);
+    for (int b = 0; b < B; ++b) {
+        const std::vector<float>& st = inputs[b]->nstiff;
+        if (st.empty()) continue;
+        uniformStiffValue_[b] = st[0];
+        for (float v : st)
+            if (std::memcmp(&v, &st[0], sizeof(float)) != 0) { uniformStiff_[b] = 0; break; }
+    }
     set_.assign(B, Settings());
     press_.assign(B, PressState());
     attActive_.assign(B, false);
@@ -218,8 +229,17 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     for (int b = 0; b < B; ++b) {
         const BodyInput& in = *inputs[b];
         std::vector<int>& o = ord_[b];
-        if (nodeOrderMode_ == 1) morton_order(in, o);
-        else {
+        if (nodeOrderMode_ == 1) {
+            morton_order(in, o);
+            // longest rows first inside windows of 8 slices: the slices of the
+            // sliced-ELL distance entries then pad far less (SELL-C-sigma)
+            for (int w0 = 0; w0 < in.n; w0 += kSortWindow) {
+                const int w1 = std::min(in.n, w0 + kSortWindow);
+                std::stable_sort(o.begin() + w0, o.begin() + w1, [&](int x, int y) {
+                    return in.rowOff[x + 1] - in.rowOff[x] > in.rowOff[y + 1] - in.rowOff[y];
+                });
+            }
+        } else {
             o.resize(in.n);
             for (int i = 0; i < in.n; ++i) o[i] = i;
         }
@@ -307,19 +327,23 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
             if (total > 0x7fffffffLL) { err = "engine: sliced distance entries exceed int32 indexing"; return false; }
         }
         std::vector<int> hSellNbr(std::max<long long>(total, 1), -1);
-        std::vector<float2> hSellRA(std::max<long long>(total, 1), make_float2(0.0f, 0.0f));
+        std::vector<float> hSellRest(std::max<long long>(total, 1), 0.0f);
         for (int i = 0; i < nPad_; ++i) {
             const int t = i / kTile, l = i % kTile;
             for (int e = hrow[i]; e < hrow[i + 1]; ++e) {
                 const size_t slot = (size_t)hOff[t] + (size_t)(e - hrow[i]) * kTile + l;
                 hSellNbr[slot] = hnbr[e];
-                hSellRA[slot] = make_float2(hrest[e], 0.0f);
+                hSellRest[slot] = hrest[e];
             }
         }
         sellEntries_ = total;
         sliceOff_.alloc(std::max(tiles, 1)); sliceW_.alloc(std::max(tiles, 1));
-        sellNbr_.alloc(hSellNbr.size()); sellRA_.alloc(hSellRA.size()); sellLam_.alloc(hSellNbr.size());
-        if (!sliceOff_.p || !sliceW_.p || !sellNbr_.p || !sellRA_.p || !sellLam_.p) {
+        // per-edge alpha tilde only for bodies whose edges differ in stiffness
+        bool anyVarying = false;
+        for (int b = 0; b < B; ++b) anyVarying |= !uniformStiff_[b];
+        sellNbr_.alloc(hSellNbr.size()); sellRest_.alloc(hSellRest.size()); sellLam_.alloc(hSellNbr.size());
+        sellAlpha_.alloc(anyVarying ? hSellNbr.size() : 1);
+        if (!sliceOff_.p || !sliceW_.p || !sellNbr_.p || !sellRest_.p || !sellAlpha_.p || !sellLam_.p) {
             err = "engine: hipMalloc failed (out of device memory?)";
             return false;
         }
@@ -327,7 +351,7 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
         XPBD_HIP(hipMemcpyAsync(sliceW_.p, hW.data(), sizeof(int) * tiles, hipMemcpyHostToDevice, stream_));
         XPBD_HIP(hipMemcpyAsync(sellNbr_.p, hSellNbr.data(), sizeof(int) * hSellNbr.size(), hipMemcpyHostToDevice,
                                 stream_));
-        XPBD_HIP(hipMemcpyAsync(sellRA_.p, hSellRA.data(), sizeof(float2) * hSellRA.size(), hipMemcpyHostToDevice,
+        XPBD_HIP(hipMemcpyAsync(sellRest_.p, hSellRest.data(), sizeof(float) * hSellRest.size(), hipMemcpyHostToDevice,
                                 stream_));
         XPBD_HIP(hipStreamSynchronize(stream_));   // host staging goes out of scope
     }
@@ -538,6 +562,10 @@ void Engine::fillBodyDev(int b, BodyDev& d) const
     d.invMassScale = s.invMassScale;
     d.distCompliance = s.distCompliance;
     d.stiffnessScale = s.stiffnessScale;
+    // the per-entry expression of the alpha kernel, once for the body
+    d.distAlphaUniform = uniformStiff_[b] ? 1 : 0;
+    d.distAlphaU = (s.distCompliance / std::fmax(uniformStiffValue_[b] * s.stiffnessScale, 1e-6f)) /
+                   std::fmax(s.dt * s.dt, 1e-8f);
     // stepPhysicsOnly (forward.cu:13258-13262): gravity along -up, off during press
     const float g = (s.groundGravity > 0.0f && !ps.active) ? s.groundGravity : 0.0f;
     d.grav[0] = g > 0.0f ? -g * s.groundN[0] : 0.0f;
@@ -615,7 +643,7 @@ bool Engine::uploadParams(std::string& err)
         key[3 * b + 1] = set_[b].distCompliance;
         key[3 * b + 2] = set_[b].stiffnessScale;
     }
-    if (key != alphaKey_) {
+    if (key != alphaKey_ && sellAlpha_.n > 1) {
         NodeArgs na{nPad_, tileBody_.p, bodies_.p};
         launch_distance_alpha(stream_, na, rowOff_.p, nstiff_.p, sellArgs());
         alphaKey_ = key;
@@ -625,7 +653,7 @@ bool Engine::uploadParams(std::string& err)
 
 SellArgs Engine::sellArgs() const
 {
-    return SellArgs{sliceOff_.p, sliceW_.p, sellNbr_.p, sellRA_.p, sellLam_.p};
+    return SellArgs{sliceOff_.p, sliceW_.p, sellNbr_.p, sellRest_.p, sellAlpha_.p, sellLam_.p};
 }
 
 // Everything that changes the launch sequence (not the parameter values).

@@ -223,8 +223,9 @@ private:
     DevBuf<float> nstiff_;
     // distance entries in sliced-ELL form (kernels.h SellArgs)
     DevBuf<int> sliceOff_, sliceW_, sellNbr_;
-    DevBuf<float2> sellRA_;
-    DevBuf<float> sellLam_;
+    DevBuf<float> sellRest_, sellAlpha_, sellLam_;
+    std::vector<char> uniformStiff_;        // every edge of the body has one stiffness
+    std::vector<float> uniformStiffValue_;
     long long sellEntries_ = 0;
     SellArgs sellArgs() const;
     DevBuf<int> tileBody_;

@@ -22,7 +22,8 @@ struct SellArgs {
     const int* sliceOff;      // [tiles]
     const int* sliceW;        // [tiles]
     const int* nbr;           // [entries]
-    float2* restAlpha;        // [entries] rest length, alpha tilde
+    const float* rest;        // [entries] rest length
+    float* alpha;             // [entries] alpha tilde (bodies with per-edge stiffness)
     float* lam;               // [entries]
 };
 

@@ -49,8 +49,9 @@ __global__ void __launch_bounds__(256) predict_kernel(int nPad, const int* __res
 // (it is zero), the last does not write it (it is reset next substep).
 // Entries are processed four at a time: the loads and neighbour gathers of a
 // group are issued before its arithmetic, which stays in row order.
-// Per entry 20 B (nbr 4, rest+alpha 8, lambda 4 + 4) plus the 16 B neighbour
-// gather (cache resident); per node 32 B (p in, p out).
+// Per entry 20 B (nbr 4, rest 4, alpha 4, lambda 4 + 4; alpha is a per-body
+// scalar when the body has one stiffness) plus the 16 B neighbour gather
+// (cache resident); per node 32 B (p in, p out).
 // ---------------------------------------------------------------------------
 constexpr int kDistGroup = 4;
 
@@ -76,11 +77,12 @@ __global__ void __launch_bounds__(256) distance_jacobi_kernel(
     const float wi = pi4.w * B.invMassScale;
     float3 acc = f3(0.0f, 0.0f, 0.0f);
     int valid = 0;
+    const bool uniformAlpha = B.distAlphaUniform != 0;   // one stiffness for the whole body
     const int width = __builtin_amdgcn_readfirstlane(sa.sliceW[tile]);
     const int base = __builtin_amdgcn_readfirstlane(sa.sliceOff[tile]) + lane;
     for (int e0 = 0; e0 < width; e0 += kDistGroup) {
         int jj[kDistGroup];
-        float2 ra[kDistGroup];
+        float rl[kDistGroup], al[kDistGroup];
         float ll[kDistGroup];
         float4 pj[kDistGroup];
 #pragma unroll
@@ -89,7 +91,8 @@ __global__ void __launch_bounds__(256) distance_jacobi_kernel(
         for (int k = 0; k < kDistGroup; ++k) {
             if (jj[k] >= 0) {
                 const int slot = base + (e0 + k) * kTile;
-                ra[k] = sa.restAlpha[slot];
+                rl[k] = sa.rest[slot];
+                al[k] = uniformAlpha ? B.distAlphaU : sa.alpha[slot];
                 ll[k] = kFirst ? 0.0f : sa.lam[slot];
                 pj[k] = Pin[jj[k]];
             }
@@ -105,10 +108,10 @@ __global__ void __launch_bounds__(256) distance_jacobi_kernel(
                 if (!kLast && kFirst) sa.lam[slot] = l;
                 continue;
             }
-            const float C = d - ra[k].x;
+            const float C = d - rl[k];
             const float3 grad = mul3(diff, 1.0f / d);
             const float wj = pj[k].w * B.invMassScale;
-            const float at = ra[k].y;
+            const float at = al[k];
             const float denom = (wi + wj) + at;
             if (denom < 1e-8f) {
                 if (!kLast && kFirst) sa.lam[slot] = l;
@@ -156,16 +159,15 @@ __global__ void distance_alpha_kernel(int nPad, const int* __restrict__ tileBody
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nPad) return;
     const BodyDev& B = bodies[wave_body(tileBody, i)];
-    if (i >= B.nodeEnd) return;
+    if (i >= B.nodeEnd || B.distAlphaUniform) return;
     const float compliance = B.distCompliance, stiffScale = B.stiffnessScale;
     const float dt2 = fmaxf(B.dt * B.dt, 1e-8f);
     const int tile = i / kTile, lane = i % kTile;
     const int base = sa.sliceOff[tile] + lane;
-    float* ra = reinterpret_cast<float*>(sa.restAlpha);
     for (int e = rowOff[i]; e < rowOff[i + 1]; ++e) {
         const float st = fmaxf(nstiff[e] * stiffScale, 1e-6f);
         const float alpha = compliance / st;
-        ra[2 * (size_t)(base + (e - rowOff[i]) * kTile) + 1] = alpha / dt2;
+        sa.alpha[base + (e - rowOff[i]) * kTile] = alpha / dt2;
     }
 }
 

@@ -32,6 +32,8 @@ struct BodyDev {
     int distanceOn, iters;
     float dt, underRelax, velDamping, invMassScale;
     float distCompliance, stiffnessScale;
+    int distAlphaUniform;      // one stiffness on every edge: alpha tilde is distAlphaU
+    float distAlphaU;
     float grav[3];
     // ground (forward.cu:4430-4446); groundOn = enabled and no press active
     int groundOn;

@@ -51,6 +51,27 @@ def test_config1_per_frame_from_identical_state(scene, product_path, oracle_path
     cpu.destroy()
 
 
+def test_per_edge_stiffness_parity(scene, product_path, oracle_path):
+    # edges of different stiffness take the per-entry alpha stream instead of
+    # the per-body scalar
+    varied = dict(scene)
+    rng = np.random.default_rng(17)
+    varied["stiff"] = rng.uniform(0.2, 2.0, scene["stiff"].shape).astype(np.float32)
+    gpu, cpu = XPBD(product_path), XPBD(oracle_path)
+    setup_config1(gpu, varied)
+    setup_config1(cpu, varied)
+    extent = scenes.scene_extent(varied)
+    worst = 0.0
+    for f in range(30):
+        frame_config1(gpu, f)
+        frame_config1(cpu, f)
+        worst = max(worst, max_abs(gpu.positions(), cpu.positions()) / extent)
+    print(f"per-edge stiffness 30-frame drift: {worst:.3e}")
+    assert worst <= TOL_REL
+    gpu.destroy()
+    cpu.destroy()
+
+
 def test_config1_free_run_drift(scene, product_path, oracle_path):
     gpu, cpu = XPBD(product_path), XPBD(oracle_path)
     extent = scenes.scene_extent(scene)
