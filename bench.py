@@ -32,8 +32,23 @@ SUBSTEPS = 10
 ITERS = 4
 
 
+PROFILE = os.path.join(ROOT, "profiles", "distance_jacobi_1M.md")
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by the streaming calibration)."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def configure(sim):
@@ -160,7 +175,8 @@ def main():
                                f"(xpbd_step per substep)",
                    "gaussians": n, "edges": e, "substeps_per_step": SUBSTEPS, "iters": ITERS},
         "roofline": {"bound": "hbm", "kernel": key, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "traffic_source": os.path.relpath(PROFILE, ROOT),
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_s * 1e6},
         "cpu_baseline": cpu,
         "kernel_ms": prof,

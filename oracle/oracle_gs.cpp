@@ -140,6 +140,7 @@ void volume_gs_build()
 {
     vgs = VolumeGS();
     vgs.built = true;
+    vgs.version = vol.version;
     const int n = g.n;
     double work = 0.0;
     for (int j = 0; j < n; ++j) {

@@ -19,6 +19,7 @@ struct DistanceGS {
 // buildVolumeGSColors (forward.cu:9745-9812): leaders in colour-major order.
 struct VolumeGS {
     bool built = false, skipped = false;
+    long long version = -1;             // volume topology the colouring belongs to
     int lowerBound = 0;
     std::vector<int> order, colorOff;
 };

@@ -251,7 +251,9 @@ void volume_build_reference(int k, int cap, int leaderR)
 void volume_set_topology(const std::vector<char>& isLeader, const std::vector<std::vector<int>>& members,
                          int minMembers)
 {
+    static long long versions = 0;
     const int n = g.n;
+    vol.version = ++versions;
     vol.isLeader = isLeader;
     vol.minMembers = minMembers;
     vol.memOff.assign(n + 1, 0);

@@ -13,6 +13,7 @@ struct VolumeState {
     int minMembers = 4;                 // precompute gate (forward.cu:7745 rejects < 4)
     int k = -1, cap = -1, r = -1;       // parameters the topology was built with
     bool explicitTopo = false;
+    long long version = 0;              // bumped by every topology (re)build
     std::vector<char> isLeader;
     std::vector<int> memOff, memIdx, revOff, revIdx;
     std::vector<float> detRest, Vrest, restLen, lam, cS, coef;

@@ -15,6 +15,7 @@
 #include "oracle_volume.h"
 #include "oracle_shapes.h"
 #include "oracle_selfcol.h"
+#include "oracle_gs.h"
 
 namespace xo {
 
@@ -38,22 +39,44 @@ void extensions_free()
     g_cM.clear(); g_cRmax.clear(); g_cRg.clear();
     obj = ObjectState();
     sc = SelfColState();
+    dgs = DistanceGS();
+    vgs = VolumeGS();
 }
 void extensions_on_reset() {}
+// The colourings are built on first use after being enabled, and the volume
+// colouring again after every topology rebuild (9745-9812 runs inside
+// rebuildVolumeClusters).
+static void gs_ensure()
+{
+    if (s.distGS && !dgs.built) distance_gs_build();
+    if (s.volGS && vol.built && (!vgs.built || vgs.version != vol.version)) volume_gs_build();
+}
 void extensions_begin_step()
 {
     if (s.useVolume) {
         volume_ensure();
         std::fill(vol.lam.begin(), vol.lam.end(), 0.0f);
     }
+    gs_ensure();
+    std::fill(dgs.lam.begin(), dgs.lam.end(), 0.0f);   // 12067-12069
 }
+static bool volume_gs_ready() { return s.volGS && vgs.built && !vgs.skipped && vgs.colorOff.size() > 1; }
 void extensions_after_predict_impl();
 void extensions_after_predict() { extensions_after_predict_impl(); }
 void extensions_before_iterations() { contact_ground_radius_update(); }
-bool extensions_distance(std::vector<V3>&, std::vector<V3>&, int) { return false; }
+// Distance GS per colour in place (12250-12262) when enabled and edges exist.
+bool extensions_distance(std::vector<V3>& pin, std::vector<V3>&, int)
+{
+    if (!(s.useDistance && s.distGS && dgs.built && !dgs.a.empty())) return false;
+    distance_gs_iteration(pin, s.dt);
+    return true;
+}
 void extensions_iteration(std::vector<V3>*& pin, std::vector<V3>*& pout, int)
 {
-    if (s.useVolume && vol.built) volume_iteration(pin, pout);
+    if (s.useVolume && vol.built) {
+        if (volume_gs_ready()) volume_gs_iteration(*pin, s.dt);   // 12310-12324
+        else volume_iteration(pin, pout);
+    }
     if (s.useShape) {
         shape_matching(*pin, *pout);
         std::swap(pin, pout);
@@ -156,8 +179,6 @@ void xpbd_self_collision_stats(int* out)
     out[2] = sc.overflow;
 }
 
-// Extension: fixed hash grid (cell size > 0 replaces the displacement rule;
-// table size, a power of two >= 1024, replaces nextpow2(2N)); 0 keeps the rule.
 // Extension: the oracle keeps the caller's order; accepted for ABI parity.
 int xpbd_ext_set_node_order(int mode)
 {
@@ -165,6 +186,59 @@ int xpbd_ext_set_node_order(int mode)
     return 0;
 }
 
+// Extension: graph-coloured Gauss-Seidel passes (forward.cu:3780-3810; the
+// reference reaches them through FORWARD::setDistanceGSEnabled /
+// setVolumeGSEnabled, not through its C ABI).
+int xpbd_ext_set_gauss_seidel(int distance, int volume)
+{
+    s.distGS = distance != 0;
+    s.volGS = volume != 0;
+    return 0;
+}
+
+// Extension: sizes of the colourings: distance edges, distance colours,
+// one-way edges, volume clusters, volume colours, volume colouring skipped.
+int xpbd_ext_gs_info(int* info)
+{
+    if (!g.stepped) return fail("xpbd_ext_gs_info: no step yet");
+    if (!info) return fail("xpbd_ext_gs_info: null output");
+    gs_ensure();
+    info[0] = (int)dgs.a.size();
+    info[1] = dgs.colorOff.empty() ? 0 : (int)dgs.colorOff.size() - 1;
+    info[2] = dgs.oneWay;
+    info[3] = (int)vgs.order.size();
+    info[4] = vgs.colorOff.empty() ? 0 : (int)vgs.colorOff.size() - 1;
+    info[5] = vgs.skipped ? 1 : 0;
+    return 0;
+}
+
+// Extension: the distance colouring (edge ends ab[2E], rest[E], stiff[E],
+// colour offsets colorOff[C+1]); any pointer may be null.
+int xpbd_ext_distance_gs_edges(int* ab, float* rest, float* stiff, int* colorOff)
+{
+    if (!g.stepped) return fail("xpbd_ext_distance_gs_edges: no step yet");
+    gs_ensure();
+    for (size_t e = 0; e < dgs.a.size(); ++e) {
+        if (ab) { ab[2 * e] = dgs.a[e]; ab[2 * e + 1] = dgs.b[e]; }
+        if (rest) rest[e] = dgs.rest[e];
+        if (stiff) stiff[e] = dgs.stiff[e];
+    }
+    if (colorOff) std::copy(dgs.colorOff.begin(), dgs.colorOff.end(), colorOff);
+    return 0;
+}
+
+// Extension: the volume colouring (leaders in colour-major order, colour offsets).
+int xpbd_ext_volume_gs_order(int* order, int* colorOff)
+{
+    if (!g.stepped) return fail("xpbd_ext_volume_gs_order: no step yet");
+    gs_ensure();
+    if (order) std::copy(vgs.order.begin(), vgs.order.end(), order);
+    if (colorOff) std::copy(vgs.colorOff.begin(), vgs.colorOff.end(), colorOff);
+    return 0;
+}
+
+// Extension: fixed hash grid (cell size > 0 replaces the displacement rule;
+// table size, a power of two >= 1024, replaces nextpow2(2N)); 0 keeps the rule.
 int xpbd_ext_set_self_collision_grid(float cellSize, int tableSize)
 {
     if (tableSize != 0 && (tableSize < 1024 || (tableSize & (tableSize - 1))))

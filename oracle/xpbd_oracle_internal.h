@@ -61,6 +61,8 @@ struct Settings {
     float scRadiusScale = 1.5f, scExcludeScale = 2.0f, scSearchScale = 1.5f, scFastRel = 0.0f;
     float scCellOverride = 0.0f;   // extension: fixed cell size instead of the displacement rule
     int scTableOverride = 0;       // extension: fixed hash table size (power of two)
+    // graph-coloured Gauss-Seidel passes (forward.cu:310-313, off by default)
+    bool distGS = false, volGS = false;
     // squash / press (forward.cu:535-575)
     bool sqActive = false, sqPress = false, sqPressNext = false, sqPressFromLo = false, sqSlip = true;
     bool sqAutoLog = true;
