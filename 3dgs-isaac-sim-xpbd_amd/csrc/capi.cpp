@@ -522,6 +522,44 @@ int xpbd_ext_set_node_order(int mode)
     return 0;
 }
 
+int xpbd_ext_set_gauss_seidel(int distance, int volume)
+{
+    Settings& s = G().set;
+    s.distGS = distance != 0;
+    s.volGS = volume != 0;
+    return 0;
+}
+
+int xpbd_ext_gs_info(int* info)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_ext_gs_info: no step yet");
+    if (!info) return fail("xpbd_ext_gs_info: null output");
+    std::string err;
+    g.engine->settings(0) = g.set;
+    if (!g.engine->gsInfo(0, info, err)) return fail("xpbd_ext_gs_info: " + err);
+    return 0;
+}
+
+int xpbd_ext_distance_gs_edges(int* ab, float* rest, float* stiff, int* colorOff)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_ext_distance_gs_edges: no step yet");
+    std::string err;
+    if (!g.engine->distanceGSEdges(0, ab, rest, stiff, colorOff, err)) return fail("xpbd_ext_distance_gs_edges: " + err);
+    return 0;
+}
+
+int xpbd_ext_volume_gs_order(int* order, int* colorOff)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_ext_volume_gs_order: no step yet");
+    std::string err;
+    g.engine->settings(0) = g.set;
+    if (!g.engine->volumeGSOrder(0, order, colorOff, err)) return fail("xpbd_ext_volume_gs_order: " + err);
+    return 0;
+}
+
 int xpbd_ext_set_self_collision_grid(float cellSize, int tableSize)
 {
     Global& g = G();

@@ -56,6 +56,7 @@ template struct DevBuf<float2>;
 template struct DevBuf<unsigned int>;
 template struct DevBuf<unsigned char>;
 template struct DevBuf<double>;
+template struct DevBuf<int2>;
 
 // wave64-per-cluster volume solve from this mean member count up
 static const double kVolWaveMembers = 24.0;
@@ -221,6 +222,12 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     scDeviceValid_ = false;
     scSortedIdx_ = nullptr;
     scSortedKeys_ = nullptr;
+    dgs_.assign(B, DistanceGSTopo());
+    vgs_.assign(B, VolumeGSTopo());
+    dgsIn_.clear(); vgsIn_.clear();
+    dgsColorOff_.clear(); vgsColorOff_.clear();
+    dgsDevValid_ = false; vgsDevValid_ = false;
+    dgsAlphaKey_.clear();
 
     // device node order of every body (caller order or Morton order)
     ord_.assign(B, std::vector<int>());
@@ -554,7 +561,10 @@ void Engine::fillBodyDev(int b, BodyDev& d) const
     std::memset(&d, 0, sizeof(d));
     d.nodeBegin = bodyBegin_[b];
     d.nodeEnd = bodyBegin_[b] + bodyCount_[b];
-    d.distanceOn = s.useDistance ? 1 : 0;
+    // distance: the coloured Gauss-Seidel pass replaces Jacobi when enabled
+    // (forward.cu:12065, 12251-12286)
+    d.distGSOn = (s.useDistance && s.distGS && dgsDevValid_ && b < (int)dgsIn_.size() && dgsIn_[b]) ? 1 : 0;
+    d.distanceOn = (s.useDistance && !s.distGS) ? 1 : 0;
     d.iters = s.iters;
     d.dt = s.dt;
     d.underRelax = s.underRelax;
@@ -588,7 +598,10 @@ void Engine::fillBodyDev(int b, BodyDev& d) const
     d.pressHi = ps.hi - (ps.fromLo ? 0.0f : ps.curDisp);
     d.contactColliders = 0;
     d.contactGround = 0;
-    d.volumeOn = s.useVolume ? 1 : 0;
+    // volume: the coloured pass replaces the gather pass unless its colouring
+    // was skipped (forward.cu:12066, 12310-12324)
+    d.volGSOn = (s.useVolume && s.volGS && vgsDevValid_ && b < (int)vgsIn_.size() && vgsIn_[b]) ? 1 : 0;
+    d.volumeOn = (s.useVolume && !d.volGSOn) ? 1 : 0;
     d.volCompliance = s.volCompliance;
     d.shapeOn = s.useShape ? 1 : 0;
     d.shapeRobust = s.shapeRobust ? 1 : 0;
@@ -648,6 +661,10 @@ bool Engine::uploadParams(std::string& err)
         launch_distance_alpha(stream_, na, rowOff_.p, nstiff_.p, sellArgs());
         alphaKey_ = key;
     }
+    if (dgsDevValid_ && dgsColorOff_.size() > 1 && dgsColorOff_.back() > 0 && key != dgsAlphaKey_) {
+        launch_dist_gs_alpha(stream_, dgsColorOff_.back(), dgsAB_.p, dgsStiff_.p, tileBody_.p, bodies_.p, dgsAlpha_.p);
+        dgsAlphaKey_ = key;
+    }
     return true;
 }
 
@@ -662,24 +679,28 @@ std::vector<int> Engine::planSignature() const
     const int B = numBodies();
     int iters = set_[0].iters;
     int anyDist = 0, anyCol = 0, anyGround = 0, anyPress = 0, anyVol = 0, anyShape = 0, anyCGround = 0;
-    int anyObj = 0, anyBoxes = 0, anySc = 0;
+    int anyObj = 0, anyBoxes = 0, anySc = 0, anyDistGS = 0, anyVolGS = 0;
     for (int b = 0; b < B; ++b) {
         const BodyDev& d = hostBodies_[b];
         anyObj |= d.objOn;
         anyBoxes |= d.objBoxes;
         anySc |= d.scOn;
-        anyDist |= set_[b].useDistance ? 1 : 0;
+        anyDist |= d.distanceOn;
+        anyDistGS |= d.distGSOn;
+        anyVolGS |= d.volGSOn;
         anyCol |= colCount_[b] > 0 ? 1 : 0;
         anyGround |= set_[b].groundEnabled ? 1 : 0;
         anyPress |= press_[b].active ? 1 : 0;
-        anyVol |= set_[b].useVolume ? 1 : 0;
+        anyVol |= d.volumeOn;
         anyShape |= set_[b].useShape ? 1 : 0;
         anyCGround |= (cM_.p && contactGround_[b] && set_[b].groundEnabled) ? 1 : 0;
     }
     const int volWave = (anyVol && volAvgMembers_ >= kVolWaveMembers) ? 1 : 0;
+    const int nDistGS = (anyDistGS && dgsDevValid_) ? (int)dgsColorOff_.size() - 1 : 0;
+    const int nVolGS = (anyVolGS && vgsDevValid_) ? (int)vgsColorOff_.size() - 1 : 0;
     return {iters, anyDist, anyCol, anyGround, anyPress, nPad_, anyVol, volWave, anyVol ? volActive_ : 0, anyShape,
             anyCGround, anyObj, anyBoxes, anySc, anyObj ? objTotal_ : 0, anySc ? scTableTotal_ : 0,
-            anySc ? scSortBits_ : 0, device_alloc_epoch()};
+            anySc ? scSortBits_ : 0, nDistGS, nVolGS, device_alloc_epoch()};
 }
 
 // Event pairs around kernel launches (profile mode only).
@@ -765,18 +786,45 @@ void Engine::recordSubstep(Timer* t)
         // with self-collision the velocity commit moves behind the contact push
         if (last && !scPass) epi |= kEpiVelocity;
         const bool volPass = sig[6] != 0;
-        const bool constraintPass = anyDist || volPass || shapePass;
+        // constraint passes in order: distance GS, distance Jacobi, volume GS,
+        // volume gather, shape matching (forward.cu:12250-12470).  The
+        // epilogue runs on the last of them when that pass visits every node;
+        // after a Gauss-Seidel pass (which visits only its colours' nodes) it
+        // runs as its own pass.
+        const int nDistGS = sig[17], nVolGS = sig[18];
+        enum { kNone, kDistGS, kJacobi, kVolGS, kVolGather, kShape };
+        int lastPass = kNone;
+        if (nDistGS) lastPass = kDistGS;
+        if (anyDist) lastPass = kJacobi;
+        if (nVolGS) lastPass = kVolGS;
+        if (volPass) lastPass = kVolGather;
+        if (shapePass) lastPass = kShape;
         // the press slab precedes self-collision; colliders and ground follow it
         // (forward.cu:12474-12560)
-        const unsigned preEpi = scPass ? (constraintPass ? (epi & kEpiPress) : 0u) : epi;
-        const unsigned scEpi = constraintPass ? (epi & ~kEpiPress) : epi;
+        const unsigned preEpi = scPass ? (lastPass != kNone ? (epi & kEpiPress) : 0u) : epi;
+        const unsigned scEpi = lastPass != kNone ? (epi & ~kEpiPress) : epi;
+        if (nDistGS) {
+            XPBD_T_BEGIN("distance_gs");
+            for (int c = 0; c < nDistGS; ++c)
+                launch_dist_gs(stream_, it == 0, last, dgsColorOff_[c], dgsColorOff_[c + 1] - dgsColorOff_[c], dgsAB_.p,
+                               dgsRest_.p, dgsAlpha_.p, dgsLam_.p, pin, tileBody_.p, bodies_.p);
+            XPBD_T_END();
+        }
         if (anyDist) {
             XPBD_T_BEGIN(it == 0 ? (last ? "distance_jacobi_only" : "distance_jacobi_first")
                                  : (last ? "distance_jacobi_last" : "distance_jacobi_mid"));
             launch_distance_jacobi(stream_, na, it == 0, last, pin, pout, sellArgs(),
-                                   (volPass || shapePass) ? 0u : preEpi, ea);
+                                   lastPass == kJacobi ? preEpi : 0u, ea);
             XPBD_T_END();
             std::swap(pin, pout);
+        }
+        if (nVolGS) {
+            const VolumeArgs va = volArgs();
+            XPBD_T_BEGIN("volume_gs");
+            for (int c = 0; c < nVolGS; ++c)
+                launch_vol_gs(stream_, it == 0, last, vgsColorOff_[c], vgsColorOff_[c + 1] - vgsColorOff_[c],
+                              vgsOrder_.p, pin, va, tileBody_.p, bodies_.p);
+            XPBD_T_END();
         }
         if (volPass) {
             const VolumeArgs va = volArgs();
@@ -784,7 +832,7 @@ void Engine::recordSubstep(Timer* t)
             launch_vol_solve(stream_, na, sig[7] != 0, it == 0, last, pin, va);
             XPBD_T_END();
             XPBD_T_BEGIN("volume_apply");
-            launch_vol_apply(stream_, na, pin, pout, va, shapePass ? 0u : preEpi, ea);
+            launch_vol_apply(stream_, na, pin, pout, va, lastPass == kVolGather ? preEpi : 0u, ea);
             XPBD_T_END();
             std::swap(pin, pout);
         }
@@ -794,12 +842,17 @@ void Engine::recordSubstep(Timer* t)
             XPBD_T_END();
             std::swap(pin, pout);
         }
+        if ((lastPass == kDistGS || lastPass == kVolGS) && preEpi) {
+            XPBD_T_BEGIN("epilogue");
+            launch_epilogue(stream_, na, pin, preEpi, ea);
+            XPBD_T_END();
+        }
         if (scPass) {
             XPBD_T_BEGIN("selfcol_iteration");
             launch_selfcol_iteration(stream_, na, sa, pin, pout, x_.p, rest_.p, scEpi, ea);
             XPBD_T_END();
             std::swap(pin, pout);
-        } else if (!constraintPass && epi) {
+        } else if (lastPass == kNone && epi) {
             XPBD_T_BEGIN("epilogue");
             launch_epilogue(stream_, na, pin, epi, ea);
             XPBD_T_END();
@@ -837,11 +890,16 @@ bool Engine::prepare(std::string& err)
     }
     if (anyObj && !ensureObjects(err)) return false;
     if (anySc && !ensureSelfCol(err)) return false;
-    if (!uploadParams(err)) return false;
-    bool anyVol = false;
-    for (int b = 0; b < numBodies(); ++b) anyVol |= set_[b].useVolume;
+    bool anyVol = false, anyVolGS = false, anyDistGS = false;
+    for (int b = 0; b < numBodies(); ++b) {
+        anyVol |= set_[b].useVolume;
+        anyVolGS |= set_[b].useVolume && set_[b].volGS;
+        anyDistGS |= set_[b].useDistance && set_[b].distGS;
+    }
     if (anyVol && !ensureVolume(false, err)) return false;
-    return true;
+    if ((anyVolGS || vgsDevValid_) && !ensureVolumeGS(err)) return false;
+    if ((anyDistGS || dgsDevValid_) && !ensureDistanceGS(err)) return false;
+    return uploadParams(err);
 }
 
 // The object shift of a body that ran object shape matching is a valid

@@ -39,6 +39,7 @@ struct Settings {
     bool objShapeGPU = true;
     bool scEnabled = false, scWithinBody = false;
     float scRadiusScale = 1.5f, scExcludeScale = 2.0f;
+    bool distGS = false, volGS = false;   // forward.cu:310-313 (FORWARD::set*GSEnabled)
 };
 
 // One body's input graph in the reference's loadGraph form (local ids).
@@ -62,8 +63,27 @@ struct VolumeTopo {
     int minMembers = 4;       // rest-precompute gate (forward.cu:7745)
     int k = -1, cap = -1, r = -1;
     bool explicitClusters = false;
+    long long version = 0;    // changes with every finish(): colourings follow it
     void finish(int n);       // builds the reverse CSR from isLeader / members
 };
+
+// Gauss-Seidel colourings of one body in caller ids.
+// Distance (forward.cu:10407-10517): undirected edges a < b, colour-major.
+struct DistanceGSTopo {
+    bool built = false;
+    std::vector<int> a, b, colorOff;
+    std::vector<float> rest, stiff;
+    int oneWay = 0;
+};
+// Volume (forward.cu:9745-9812): leaders in colour-major order.
+struct VolumeGSTopo {
+    bool built = false, skipped = false;
+    long long version = -1;
+    int lowerBound = 0;
+    std::vector<int> order, colorOff;
+};
+void build_distance_gs(const BodyInput& in, DistanceGSTopo& t);
+void build_volume_gs(const VolumeTopo& v, int n, VolumeGSTopo& t);
 
 // k-ring clusters by the reference rule (forward.cu:9814-10240).
 void build_volume_reference(const BodyInput& in, int k, int cap, int leaderR, VolumeTopo& t);
@@ -160,7 +180,25 @@ public:
     void setSelfCollisionGrid(int b, float cellSize, int tableSize);
     bool selfCollisionLists(int b, unsigned int* keys, int* sorted, int* counts, int* contacts, std::string& err);
 
+    // Gauss-Seidel colourings (built on demand, caller ids): info = distance
+    // edges, distance colours, one-way edges, volume clusters, volume colours,
+    // volume colouring skipped.
+    bool gsInfo(int b, int info[6], std::string& err);
+    bool distanceGSEdges(int b, int* ab, float* rest, float* stiff, int* colorOff, std::string& err);
+    bool volumeGSOrder(int b, int* order, int* colorOff, std::string& err);
+
 private:
+    bool ensureDistanceGS(std::string& err);
+    bool ensureVolumeGS(std::string& err);
+    std::vector<DistanceGSTopo> dgs_;
+    std::vector<VolumeGSTopo> vgs_;
+    std::vector<char> dgsIn_, vgsIn_;       // bodies merged into the device colourings
+    std::vector<int> dgsColorOff_, vgsColorOff_;
+    bool dgsDevValid_ = false, vgsDevValid_ = false;
+    std::vector<float> dgsAlphaKey_;
+    DevBuf<int2> dgsAB_;
+    DevBuf<float> dgsRest_, dgsStiff_, dgsAlpha_, dgsLam_;
+    DevBuf<int> vgsOrder_;
     bool ensureShapeBuffers(std::string& err);
     bool ensureObjects(std::string& err);
     bool ensureSelfCol(std::string& err);

@@ -67,6 +67,14 @@ void launch_vol_apply(hipStream_t s, const NodeArgs& na, const float4* Pin, floa
                       unsigned epi, const EpiArgs& ea);
 void launch_vol_J(hipStream_t s, const NodeArgs& na, const float3* X, const VolumeArgs& va, float* outJ);
 
+// Graph-coloured Gauss-Seidel: one launch per colour, in place on P.
+void launch_dist_gs_alpha(hipStream_t s, int m, const int2* ab, const float* stiff, const int* tileBody,
+                          const BodyDev* bodies, float* alpha);
+void launch_dist_gs(hipStream_t s, bool first, bool last, int begin, int count, const int2* ab, const float* rest,
+                    const float* alpha, float* lam, float4* P, const int* tileBody, const BodyDev* bodies);
+void launch_vol_gs(hipStream_t s, bool first, bool last, int begin, int count, const int* order, float4* P,
+                   const VolumeArgs& va, const int* tileBody, const BodyDev* bodies);
+
 constexpr int kSelfColCap = 32;   // forward.cu:223 SELFCOL_MAX_CONTACTS
 
 // Object-shape and self-collision buffers.  Objects are numbered globally

@@ -101,6 +101,8 @@ void ball_members(const BodyInput& in, int i, int k, int cap, BallScratch& w, st
 
 void VolumeTopo::finish(int n)
 {
+    static long long versions = 0;
+    version = ++versions;
     revOff.assign(n + 1, 0);
     std::vector<int> cnt(n, 0);
     for (int i = 0; i < n; ++i) {

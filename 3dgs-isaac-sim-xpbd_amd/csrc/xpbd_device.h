@@ -34,6 +34,7 @@ struct BodyDev {
     float distCompliance, stiffnessScale;
     int distAlphaUniform;      // one stiffness on every edge: alpha tilde is distAlphaU
     float distAlphaU;
+    int distGSOn, volGSOn;     // graph-coloured Gauss-Seidel instead of Jacobi / gather
     float grav[3];
     // ground (forward.cu:4430-4446); groundOn = enabled and no press active
     int groundOn;

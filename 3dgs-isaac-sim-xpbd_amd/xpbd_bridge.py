@@ -120,6 +120,14 @@ def bind(lib):
     L.xpbd_ext_volume_topology.restype = c_int
     L.xpbd_ext_set_node_order.argtypes = [c_int]
     L.xpbd_ext_set_node_order.restype = c_int
+    L.xpbd_ext_set_gauss_seidel.argtypes = [c_int, c_int]
+    L.xpbd_ext_set_gauss_seidel.restype = c_int
+    L.xpbd_ext_gs_info.argtypes = [ctypes.c_void_p]
+    L.xpbd_ext_gs_info.restype = c_int
+    L.xpbd_ext_distance_gs_edges.argtypes = [ctypes.c_void_p] * 4
+    L.xpbd_ext_distance_gs_edges.restype = c_int
+    L.xpbd_ext_volume_gs_order.argtypes = [ctypes.c_void_p] * 2
+    L.xpbd_ext_volume_gs_order.restype = c_int
     L.xpbd_ext_set_self_collision_grid.argtypes = [c_float, c_int]
     L.xpbd_ext_set_self_collision_grid.restype = c_int
     L.xpbd_ext_self_collision_lists.argtypes = [ctypes.c_void_p] * 4 + [c_int]
@@ -337,6 +345,37 @@ class XPBD:
     def ext_set_node_order(self, mode):
         """Device node layout of the next instance: 1 Morton (default), 0 caller order."""
         self._check(self.lib.xpbd_ext_set_node_order(int(mode)), "xpbd_ext_set_node_order")
+
+    def ext_set_gauss_seidel(self, distance, volume):
+        """Graph-coloured Gauss-Seidel instead of the Jacobi distance / gather volume passes."""
+        self._check(self.lib.xpbd_ext_set_gauss_seidel(int(distance), int(volume)), "xpbd_ext_set_gauss_seidel")
+
+    def ext_gs_info(self):
+        """dict of the colouring sizes (after the first step)."""
+        info = np.zeros(6, np.int32)
+        self._check(self.lib.xpbd_ext_gs_info(_ptr(info)), "xpbd_ext_gs_info")
+        keys = ("edges", "edge_colors", "one_way", "clusters", "cluster_colors", "cluster_skipped")
+        return dict(zip(keys, (int(v) for v in info)))
+
+    def ext_distance_gs_edges(self):
+        """(ab[E, 2], rest[E], stiff[E], colorOff[C+1]) of the distance colouring."""
+        info = self.ext_gs_info()
+        m, c = info["edges"], info["edge_colors"]
+        ab = np.zeros((max(m, 1), 2), np.int32)
+        rest = np.zeros(max(m, 1), np.float32)
+        stiff = np.zeros(max(m, 1), np.float32)
+        off = np.zeros(c + 1, np.int32)
+        self._check(self.lib.xpbd_ext_distance_gs_edges(_ptr(ab), _ptr(rest), _ptr(stiff), _ptr(off)),
+                    "xpbd_ext_distance_gs_edges")
+        return ab[:m], rest[:m], stiff[:m], off
+
+    def ext_volume_gs_order(self):
+        """(leaders in colour-major order, colorOff[C+1]) of the volume colouring."""
+        info = self.ext_gs_info()
+        order = np.zeros(max(info["clusters"], 1), np.int32)
+        off = np.zeros(info["cluster_colors"] + 1, np.int32)
+        self._check(self.lib.xpbd_ext_volume_gs_order(_ptr(order), _ptr(off)), "xpbd_ext_volume_gs_order")
+        return order[:info["clusters"]], off
 
     def ext_set_self_collision_grid(self, cell_size=0.0, table_size=0):
         """Fixed self-collision hash cell (0 = displacement rule) and table size (0 = default)."""

@@ -179,6 +179,26 @@ int xpbd_ext_self_collision_lists(unsigned int* keys, int* sorted, int* counts, 
  * (including xpbd_positions_device) stays in the caller's order. */
 int xpbd_ext_set_node_order(int mode);
 
+/* Graph-coloured Gauss-Seidel instead of the Jacobi distance pass and the
+ * gather volume pass (the reference's FORWARD::setDistanceGSEnabled /
+ * setVolumeGSEnabled, forward.cu:3780-3810, off by default).  Settings persist
+ * like the solver settings. */
+int xpbd_ext_set_gauss_seidel(int distance, int volume);
+
+/* Colouring sizes (after the first step): info[0] distance edges, [1]
+ * distance colours, [2] one-way edges, [3] volume clusters, [4] volume
+ * colours, [5] 1 when the volume colouring was skipped as too large. */
+int xpbd_ext_gs_info(int* info);
+
+/* The distance colouring in colour-major order: edge ends ab[2E] (a < b,
+ * caller ids), rest[E], stiff[E], colour offsets colorOff[C+1].  Any pointer
+ * may be null. */
+int xpbd_ext_distance_gs_edges(int* ab, float* rest, float* stiff, int* colorOff);
+
+/* The volume colouring: leaders in colour-major order order[L], colour
+ * offsets colorOff[C+1].  Either pointer may be null. */
+int xpbd_ext_volume_gs_order(int* order, int* colorOff);
+
 /* Library identity: "hip-gfx950" for the GPU library, "oracle-cpu" for the oracle. */
 const char* xpbd_ext_backend(void);
 

@@ -38,3 +38,18 @@ def product_path():
     if not os.path.isfile(PRODUCT_LIB):
         pytest.fail("libxpbd_isaac.so missing: run 3dgs-isaac-sim-xpbd_amd/build_lib.py product")
     return PRODUCT_LIB
+
+
+@pytest.fixture(autouse=True)
+def reset_extension_settings():
+    """Extension settings persist in each library like the reference's solver
+    settings; put them back to their defaults after every test."""
+    yield
+    import ctypes
+    for path in (ORACLE_LIB, PRODUCT_LIB):
+        if not os.path.isfile(path):
+            continue
+        lib = ctypes.CDLL(path)
+        lib.xpbd_ext_set_gauss_seidel(0, 0)
+        lib.xpbd_ext_set_node_order(1)
+        lib.xpbd_ext_set_self_collision_grid(ctypes.c_float(0.0), 0)
