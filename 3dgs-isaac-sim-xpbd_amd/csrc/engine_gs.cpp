@@ -228,10 +228,12 @@ bool Engine::ensureVolumeGS(std::string& err)
 
 bool Engine::gsInfo(int b, int info[6], std::string& err)
 {
+    // both colourings exist whether or not GS is enabled, like the reference's
+    // (forward.cu:10834, 10245); the volume one while the constraint is on
     if (!dgs_[b].built) build_distance_gs(*inputs_[b], dgs_[b]);
-    const Settings& s = set_[b];
-    if (s.useVolume || s.volGS) {
-        if (!ensureVolume(true, err)) return false;
+    const bool volOn = set_[b].useVolume;
+    if (volOn) {
+        if (!ensureVolume(false, err)) return false;
         if (!vgs_[b].built || vgs_[b].version != volTopo_[b].version)
             build_volume_gs(volTopo_[b], bodyCount_[b], vgs_[b]);
     }
@@ -240,9 +242,9 @@ bool Engine::gsInfo(int b, int info[6], std::string& err)
     info[0] = (int)d.a.size();
     info[1] = d.colorOff.empty() ? 0 : (int)d.colorOff.size() - 1;
     info[2] = d.oneWay;
-    info[3] = (int)v.order.size();
-    info[4] = v.colorOff.empty() ? 0 : (int)v.colorOff.size() - 1;
-    info[5] = v.skipped ? 1 : 0;
+    info[3] = volOn ? (int)v.order.size() : 0;
+    info[4] = (volOn && !v.colorOff.empty()) ? (int)v.colorOff.size() - 1 : 0;
+    info[5] = (volOn && v.skipped) ? 1 : 0;
     return true;
 }
 
@@ -261,7 +263,8 @@ bool Engine::distanceGSEdges(int b, int* ab, float* rest, float* stiff, int* col
 
 bool Engine::volumeGSOrder(int b, int* order, int* colorOff, std::string& err)
 {
-    if (!ensureVolume(true, err)) return false;
+    if (!set_[b].useVolume) return true;
+    if (!ensureVolume(false, err)) return false;
     if (!vgs_[b].built || vgs_[b].version != volTopo_[b].version)
         build_volume_gs(volTopo_[b], bodyCount_[b], vgs_[b]);
     const VolumeGSTopo& t = vgs_[b];

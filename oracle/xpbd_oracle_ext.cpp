@@ -202,13 +202,17 @@ int xpbd_ext_gs_info(int* info)
 {
     if (!g.stepped) return fail("xpbd_ext_gs_info: no step yet");
     if (!info) return fail("xpbd_ext_gs_info: null output");
-    gs_ensure();
+    // the reference builds both colourings whether or not GS is enabled
+    // (10834 at graph upload, 10245 with every cluster rebuild)
+    if (!dgs.built) distance_gs_build();
+    const bool volOn = s.useVolume && vol.built;
+    if (volOn && (!vgs.built || vgs.version != vol.version)) volume_gs_build();
     info[0] = (int)dgs.a.size();
     info[1] = dgs.colorOff.empty() ? 0 : (int)dgs.colorOff.size() - 1;
     info[2] = dgs.oneWay;
-    info[3] = (int)vgs.order.size();
-    info[4] = vgs.colorOff.empty() ? 0 : (int)vgs.colorOff.size() - 1;
-    info[5] = vgs.skipped ? 1 : 0;
+    info[3] = volOn ? (int)vgs.order.size() : 0;
+    info[4] = (volOn && !vgs.colorOff.empty()) ? (int)vgs.colorOff.size() - 1 : 0;
+    info[5] = (volOn && vgs.skipped) ? 1 : 0;
     return 0;
 }
 
@@ -217,7 +221,7 @@ int xpbd_ext_gs_info(int* info)
 int xpbd_ext_distance_gs_edges(int* ab, float* rest, float* stiff, int* colorOff)
 {
     if (!g.stepped) return fail("xpbd_ext_distance_gs_edges: no step yet");
-    gs_ensure();
+    if (!dgs.built) distance_gs_build();
     for (size_t e = 0; e < dgs.a.size(); ++e) {
         if (ab) { ab[2 * e] = dgs.a[e]; ab[2 * e + 1] = dgs.b[e]; }
         if (rest) rest[e] = dgs.rest[e];
@@ -231,7 +235,8 @@ int xpbd_ext_distance_gs_edges(int* ab, float* rest, float* stiff, int* colorOff
 int xpbd_ext_volume_gs_order(int* order, int* colorOff)
 {
     if (!g.stepped) return fail("xpbd_ext_volume_gs_order: no step yet");
-    gs_ensure();
+    if (!(s.useVolume && vol.built)) return 0;
+    if (!vgs.built || vgs.version != vol.version) volume_gs_build();
     if (order) std::copy(vgs.order.begin(), vgs.order.end(), order);
     if (colorOff) std::copy(vgs.colorOff.begin(), vgs.colorOff.end(), colorOff);
     return 0;
