@@ -38,6 +38,7 @@ struct Global {
     float scCell = 0.0f;
     int scTable = 0;
     int nodeOrder = 1;   // device node order of the next engine (1 = Morton)
+    std::vector<int> shapeClusters;     // cluster of every node, empty = 1-ring
     int n = 0;
     bool stepped = false;
     bool contactOn = false;
@@ -66,6 +67,7 @@ void free_all()
     g.volExplicit = false;
     g.volOff.clear();
     g.volMembers.clear();
+    g.shapeClusters.clear();
     g.n = 0;
     g.stepped = false;
     g.contactOn = false;
@@ -284,6 +286,10 @@ int xpbd_step(void)
         }
         push_colliders();
         g.engine->setSelfCollisionGrid(0, g.scCell, g.scTable);
+        if (!g.shapeClusters.empty() && !g.engine->setShapeClusters(0, g.shapeClusters.data(), err)) {
+            g.engine.reset();
+            return fail("xpbd_step: " + err);
+        }
         if (g.volExplicit &&
             !g.engine->setVolumeClusters(0, g.volOff.data(), g.volMembers.data(), g.volMinMembers, err)) {
             g.engine.reset();
@@ -519,6 +525,23 @@ int xpbd_ext_set_node_order(int mode)
 {
     if (mode != 0 && mode != 1) return fail("xpbd_ext_set_node_order: mode must be 0 (caller order) or 1 (Morton)");
     G().nodeOrder = mode;
+    return 0;
+}
+
+int xpbd_ext_set_shape_clusters(const int* clusterOf, int n)
+{
+    Global& g = G();
+    if (!g.input) return fail("xpbd_ext_set_shape_clusters: call xpbd_create first");
+    if (clusterOf && n != g.n) return fail("xpbd_ext_set_shape_clusters: size mismatch");
+    if (clusterOf)
+        for (int i = 0; i < n; ++i)
+            if (clusterOf[i] < -1 || clusterOf[i] >= n) return fail("xpbd_ext_set_shape_clusters: cluster id out of range");
+    if (clusterOf) g.shapeClusters.assign(clusterOf, clusterOf + n);
+    else g.shapeClusters.clear();
+    if (g.engine) {
+        std::string err;
+        if (!g.engine->setShapeClusters(0, clusterOf, err)) return fail("xpbd_ext_set_shape_clusters: " + err);
+    }
     return 0;
 }
 

@@ -228,6 +228,9 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     dgsColorOff_.clear(); vgsColorOff_.clear();
     dgsDevValid_ = false; vgsDevValid_ = false;
     dgsAlphaKey_.clear();
+    shapeClusterOf_.assign(B, std::vector<int>());
+    shcDevValid_ = true;
+    shcK_ = 0;
 
     // device node order of every body (caller order or Morton order)
     ord_.assign(B, std::vector<int>());
@@ -604,6 +607,7 @@ void Engine::fillBodyDev(int b, BodyDev& d) const
     d.volumeOn = (s.useVolume && !d.volGSOn) ? 1 : 0;
     d.volCompliance = s.volCompliance;
     d.shapeOn = s.useShape ? 1 : 0;
+    d.shapeClusters = (shcDevValid_ && !shapeClusterOf_[b].empty()) ? 1 : 0;
     d.shapeRobust = s.shapeRobust ? 1 : 0;
     d.shapeCompliance = s.shapeCompliance;
     d.shapeBlend = s.shapeBlend;
@@ -700,7 +704,7 @@ std::vector<int> Engine::planSignature() const
     const int nVolGS = (anyVolGS && vgsDevValid_) ? (int)vgsColorOff_.size() - 1 : 0;
     return {iters, anyDist, anyCol, anyGround, anyPress, nPad_, anyVol, volWave, anyVol ? volActive_ : 0, anyShape,
             anyCGround, anyObj, anyBoxes, anySc, anyObj ? objTotal_ : 0, anySc ? scTableTotal_ : 0,
-            anySc ? scSortBits_ : 0, nDistGS, nVolGS, device_alloc_epoch()};
+            anySc ? scSortBits_ : 0, nDistGS, nVolGS, device_alloc_epoch(), (anyShape && shcDevValid_) ? shcK_ : 0};
 }
 
 // Event pairs around kernel launches (profile mode only).
@@ -838,7 +842,9 @@ void Engine::recordSubstep(Timer* t)
         }
         if (shapePass) {
             XPBD_T_BEGIN("shape_match");
-            launch_shape_match(stream_, na, pin, pout, rest_.p, rowOff_.p, nbr_.p, preEpi, ea);
+            ShapeClusterArgs sca = shcArgs();
+            if (!sig[20]) sca.K = 0;
+            launch_shape_match(stream_, na, pin, pout, rest_.p, rowOff_.p, nbr_.p, sca, preEpi, ea);
             XPBD_T_END();
             std::swap(pin, pout);
         }
@@ -899,6 +905,7 @@ bool Engine::prepare(std::string& err)
     if (anyVol && !ensureVolume(false, err)) return false;
     if ((anyVolGS || vgsDevValid_) && !ensureVolumeGS(err)) return false;
     if ((anyDistGS || dgsDevValid_) && !ensureDistanceGS(err)) return false;
+    if (!ensureShapeClusters(err)) return false;
     return uploadParams(err);
 }
 

@@ -187,7 +187,18 @@ public:
     bool distanceGSEdges(int b, int* ab, float* rest, float* stiff, int* colorOff, std::string& err);
     bool volumeGSOrder(int b, int* order, int* colorOff, std::string& err);
 
+    // Shape matching over node clusters (caller ids, -1 = none; null = 1-ring).
+    bool setShapeClusters(int b, const int* clusterOf, std::string& err);
+
 private:
+    std::vector<std::vector<int>> shapeClusterOf_;
+    bool shcDevValid_ = true;
+    int shcK_ = 0;
+    DevBuf<int> shcOff_, shcMem_, shcOf_;
+    DevBuf<float3> shcRest_;
+    DevBuf<float> shcParams_;
+    bool ensureShapeClusters(std::string& err);
+    ShapeClusterArgs shcArgs() const;
     bool ensureDistanceGS(std::string& err);
     bool ensureVolumeGS(std::string& err);
     std::vector<DistanceGSTopo> dgs_;

@@ -103,3 +103,84 @@ bool Engine::computeShapes(int b, float deformEps, float* hostScales, float* hos
 }
 
 }  // namespace xpbd
+
+namespace xpbd {
+
+// Shape-matching clusters (extension).  Members of a cluster are listed in
+// increasing caller order and its rest centroid is summed in that order, so
+// the result is independent of the device node order.
+bool Engine::setShapeClusters(int b, const int* clusterOf, std::string& err)
+{
+    const int n = bodyCount_[b];
+    std::vector<int> of;
+    if (clusterOf) {
+        of.assign(clusterOf, clusterOf + n);
+        for (int v : of)
+            if (v < -1 || v >= n) { err = "shape clusters: cluster id out of range"; return false; }
+    }
+    shapeClusterOf_[b] = std::move(of);
+    shcDevValid_ = false;
+    return true;
+}
+
+ShapeClusterArgs Engine::shcArgs() const
+{
+    return ShapeClusterArgs{shcK_, shcOff_.p, shcMem_.p, shcRest_.p, shcOf_.p, shcParams_.p};
+}
+
+bool Engine::ensureShapeClusters(std::string& err)
+{
+    if (shcDevValid_) return true;
+    XPBD_HIP(hipStreamSynchronize(stream_));   // queued substeps may still use the old buffers
+    std::vector<int> hOff(1, 0), hMem, hOf(nPad_, -1);
+    std::vector<float3> hRest;
+    for (int b = 0; b < numBodies(); ++b) {
+        const std::vector<int>& of = shapeClusterOf_[b];
+        if (of.empty()) continue;
+        const BodyInput& in = *inputs_[b];
+        const int n = bodyCount_[b], nb = bodyBegin_[b];
+        int K = 0;
+        for (int v : of) K = std::max(K, v + 1);
+        std::vector<int> cnt(K, 0), off(K + 1, 0);
+        for (int v : of)
+            if (v >= 0) ++cnt[v];
+        for (int k = 0; k < K; ++k) off[k + 1] = off[k] + cnt[k];
+        std::vector<int> mem(off[K]), fill(off.begin(), off.end() - 1);
+        for (int i = 0; i < n; ++i)
+            if (of[i] >= 0) mem[fill[of[i]]++] = i;
+        const int g0 = (int)hOff.size() - 1;
+        for (int k = 0; k < K; ++k) {
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            for (int t = off[k]; t < off[k + 1]; ++t) {
+                const int i = mem[t];
+                sx = sx + in.pos[3 * i]; sy = sy + in.pos[3 * i + 1]; sz = sz + in.pos[3 * i + 2];
+                hMem.push_back(nb + inv_[b][i]);
+            }
+            const int c = off[k + 1] - off[k];
+            const float ic = c > 0 ? 1.0f / (float)c : 0.0f;
+            hRest.push_back(make_float3(sx * ic, sy * ic, sz * ic));
+            hOff.push_back((int)hMem.size());
+        }
+        for (int i = 0; i < n; ++i)
+            if (of[i] >= 0) hOf[nb + inv_[b][i]] = g0 + of[i];
+    }
+    shcK_ = (int)hOff.size() - 1;
+    const size_t K = std::max(shcK_, 1);
+    shcOff_.alloc(K + 1); shcMem_.alloc(std::max<size_t>(hMem.size(), 1)); shcOf_.alloc(nPad_);
+    shcRest_.alloc(K); shcParams_.alloc(16 * K);
+    if (!shcOff_.p || !shcMem_.p || !shcOf_.p || !shcRest_.p || !shcParams_.p) {
+        err = "shape clusters: hipMalloc failed";
+        return false;
+    }
+    XPBD_HIP(hipMemcpyAsync(shcOff_.p, hOff.data(), sizeof(int) * hOff.size(), hipMemcpyHostToDevice, stream_));
+    if (!hMem.empty())
+        XPBD_HIP(hipMemcpyAsync(shcMem_.p, hMem.data(), sizeof(int) * hMem.size(), hipMemcpyHostToDevice, stream_));
+    XPBD_HIP(hipMemcpyAsync(shcOf_.p, hOf.data(), sizeof(int) * nPad_, hipMemcpyHostToDevice, stream_));
+    if (!hRest.empty())
+        XPBD_HIP(hipMemcpyAsync(shcRest_.p, hRest.data(), sizeof(float3) * hRest.size(), hipMemcpyHostToDevice, stream_));
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    shcDevValid_ = true;
+    return true;
+}
+
+}  // namespace xpbd

@@ -132,8 +132,20 @@ bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0
                        unsigned int* hist, bool iota);
 void launch_fill_double(hipStream_t s, int n, double* a);
 
+// Shape-matching node clusters (K clusters; members in increasing caller
+// order, device ids); params per cluster: centroid, rotation, valid flag.
+struct ShapeClusterArgs {
+    int K;
+    const int* off;           // [K + 1]
+    const int* mem;           // [members]
+    const float3* comRest;    // [K]
+    const int* clusterOf;     // [nPad] cluster of each node or -1
+    float* params;            // [16 K]
+};
+
 void launch_shape_match(hipStream_t s, const NodeArgs& na, const float4* Pin, float4* Pout, const float3* rest,
-                        const int* rowOff, const int* nbr, unsigned epi, const EpiArgs& ea);
+                        const int* rowOff, const int* nbr, const ShapeClusterArgs& sca, unsigned epi,
+                        const EpiArgs& ea);
 // scales / rots / outScale / outQuat are the body's own arrays (local index);
 // node arrays are indexed from nodeBegin.
 void launch_deformed_shapes(hipStream_t s, int nodeBegin, int n, const float3* rest, const float3* cur,

@@ -51,11 +51,101 @@ __device__ bool polar_rotation_f32(const M3& A, M3& R)
     return true;
 }
 
-// Per-node shape matching over the graph 1-ring, plus the iteration epilogue.
+__device__ __forceinline__ float lane_f(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Cluster shape matching, solve: one wave64 per cluster.  Members are spread
+// over the lanes (64 at a time); the centroid and A = sum (p - c)(q - c0)^T
+// are summed serially in member order by reading lanes one after the other,
+// which keeps the float result of a serial loop.  A is normalised by its
+// Frobenius norm and its rotation (fp64 polar when robust) stored with the
+// centroid for the per-node projection.
+__global__ void __launch_bounds__(256) shape_cluster_kernel(int K, ShapeClusterArgs a, const float4* __restrict__ P,
+                                                           const float3* __restrict__ rest,
+                                                           const int* __restrict__ tileBody,
+                                                           const BodyDev* __restrict__ bodies)
+{
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (k >= K) return;   // whole waves
+    const int lane = threadIdx.x & 63;
+    const int b0 = a.off[k], cnt = a.off[k + 1] - b0;
+    float* prm = &a.params[16 * (size_t)k];
+    if (cnt < 4) {
+        if (lane == 0) prm[15] = 0.0f;
+        return;
+    }
+    const BodyDev& B = bodies[tileBody[a.mem[b0] / kTile]];
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (int base = 0; base < cnt; base += 64) {
+        const int t = base + lane;
+        const float4 p = (t < cnt) ? P[a.mem[b0 + t]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const int m = min(64, cnt - base);
+        for (int u = 0; u < m; ++u) {
+            sx = sx + lane_f(p.x, u);
+            sy = sy + lane_f(p.y, u);
+            sz = sz + lane_f(p.z, u);
+        }
+    }
+    const float invC = 1.0f / (float)cnt;
+    const float cx = sx * invC, cy = sy * invC, cz = sz * invC;
+    const float3 cr = a.comRest[k];
+    M3 A = m3_zero();
+    for (int base = 0; base < cnt; base += 64) {
+        const int t = base + lane;
+        float pv[3] = {0.0f, 0.0f, 0.0f}, qv[3] = {0.0f, 0.0f, 0.0f};
+        if (t < cnt) {
+            const int j = a.mem[b0 + t];
+            const float4 p = P[j];
+            const float3 q = rest[j];
+            pv[0] = p.x - cx; pv[1] = p.y - cy; pv[2] = p.z - cz;
+            qv[0] = q.x - cr.x; qv[1] = q.y - cr.y; qv[2] = q.z - cr.z;
+        }
+        float prod[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) prod[c][r] = pv[r] * qv[c];
+        const int m = min(64, cnt - base);
+        for (int u = 0; u < m; ++u) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) A.m[c][r] += lane_f(prod[c][r], u);
+        }
+    }
+    float nsq = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) nsq += A.m[c][r] * A.m[c][r];
+    const float an = sqrtf(nsq);
+    bool ok = isfinite(an) && !(an < 1e-8f);
+    M3 R = m3_diag(1.0f, 1.0f, 1.0f);
+    if (ok) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) A.m[c][r] /= an;
+        ok = B.shapeRobust ? polar_rotation_f64(A, R) : polar_rotation_f32(A, R);
+    }
+    if (lane == 0) {
+        prm[0] = cx; prm[1] = cy; prm[2] = cz;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) prm[3 + 3 * c + r] = R.m[c][r];
+        prm[15] = ok ? 1.0f : 0.0f;
+    }
+}
+
+// Per-node shape matching over the graph 1-ring (or the node's cluster),
+// plus the iteration epilogue.
 __global__ void __launch_bounds__(256) shape_match_kernel(
     int nPad, const int* __restrict__ tileBody, const BodyDev* __restrict__ bodies, const float4* __restrict__ Pin,
     float4* __restrict__ Pout, const float3* __restrict__ rest, const int* __restrict__ rowOff,
-    const int* __restrict__ nbr, unsigned epi, EpiArgs ea)
+    const int* __restrict__ nbr, ShapeClusterArgs sca, unsigned epi, EpiArgs ea)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nPad) return;
@@ -65,7 +155,32 @@ __global__ void __launch_bounds__(256) shape_match_kernel(
     float3 out = make_float3(p4.x, p4.y, p4.z);
     const float wi = p4.w * B.invMassScale;
     const int off = rowOff[idx], cnt = rowOff[idx + 1] - off;
-    while (B.shapeOn && wi > 0.0f && cnt >= 4) {
+    // cluster goal (the cluster pass above stored centroid and rotation)
+    const int kc = (B.shapeOn && B.shapeClusters && wi > 0.0f) ? sca.clusterOf[idx] : -1;
+    while (kc >= 0) {
+        const float* prm = &sca.params[16 * (size_t)kc];
+        if (prm[15] == 0.0f) break;
+        M3 R;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) R.m[c][r] = prm[3 + 3 * c + r];
+        const float3 r0 = rest[idx];
+        const float3 cr = sca.comRest[kc];
+        const float3 rq = m3_mul_vec(R, r0.x - cr.x, r0.y - cr.y, r0.z - cr.z);
+        const float gx = prm[0] + rq.x, gy = prm[1] + rq.y, gz = prm[2] + rq.z;
+        const float ddx = gx - p4.x, ddy = gy - p4.y, ddz = gz - p4.z;
+        const float gd2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        if (!isfinite(gd2) || gd2 < 1e-18f) break;
+        const float dt2 = fmaxf(B.dt * B.dt, 1e-8f);
+        const float at = B.shapeCompliance / dt2;
+        const float denom = wi + at;
+        if (denom < 1e-8f) break;
+        const float sc = B.shapeBlend * (wi / denom);
+        out = make_float3(p4.x + ddx * sc, p4.y + ddy * sc, p4.z + ddz * sc);
+        break;
+    }
+    while (B.shapeOn && !B.shapeClusters && wi > 0.0f && cnt >= 4) {
         const float3 r0 = rest[idx];
         float crx = r0.x, cry = r0.y, crz = r0.z;
         float ccx = p4.x, ccy = p4.y, ccz = p4.z;
@@ -349,10 +464,13 @@ __global__ void contact_ground_kernel(int nPad, const int* __restrict__ tileBody
 static inline int nbk(int n, int bs) { return (n + bs - 1) / bs; }
 
 void launch_shape_match(hipStream_t s, const NodeArgs& na, const float4* Pin, float4* Pout, const float3* rest,
-                        const int* rowOff, const int* nbr, unsigned epi, const EpiArgs& ea)
+                        const int* rowOff, const int* nbr, const ShapeClusterArgs& sca, unsigned epi,
+                        const EpiArgs& ea)
 {
+    if (sca.K > 0)
+        shape_cluster_kernel<<<nbk(sca.K, 4), 256, 0, s>>>(sca.K, sca, Pin, rest, na.tileBody, na.bodies);
     shape_match_kernel<<<nbk(na.nPad, 256), 256, 0, s>>>(na.nPad, na.tileBody, na.bodies, Pin, Pout, rest, rowOff,
-                                                        nbr, epi, ea);
+                                                        nbr, sca, epi, ea);
 }
 
 void launch_deformed_shapes(hipStream_t s, int nodeBegin, int n, const float3* rest, const float3* cur,

@@ -53,6 +53,7 @@ struct BodyDev {
     float volCompliance;
     // per-node shape matching (forward.cu:6088-6383)
     int shapeOn, shapeRobust;
+    int shapeClusters;         // node clusters replace the 1-ring (extension)
     float shapeCompliance, shapeBlend;
     // object (connected component) shape matching (forward.cu:6801-6937)
     int objOn, objBase, objK, objWeighted, objBoxes, objActiveAllowed;

@@ -72,6 +72,30 @@ def make_ellipsoid_scene(n, k, semi=(0.25, 0.15, 0.10), seed=1):
     }
 
 
+def kmeans_clusters(pos, size=64, iters=20, seed=0):
+    """Seeded Lloyd k-means with K = N / size clusters; returns the cluster of
+    every node (int32).  Assignment by a k-d tree over the centroids, so one
+    iteration is O(N log K)."""
+    from scipy.spatial import cKDTree
+    p = np.asarray(pos, np.float64)
+    n = p.shape[0]
+    k = max(1, n // size)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cent = p[rng.choice(n, size=k, replace=False)]
+    lab = np.zeros(n, np.int64)
+    for _ in range(iters):
+        lab = cKDTree(cent).query(p)[1]
+        cnt = np.bincount(lab, minlength=k).astype(np.float64)
+        for a in range(3):
+            s = np.bincount(lab, weights=p[:, a], minlength=k)
+            cent[:, a] = np.where(cnt > 0, s / np.maximum(cnt, 1), cent[:, a])
+    # empty clusters are dropped and the ids compacted
+    used = np.unique(lab)
+    remap = np.full(k, -1, np.int64)
+    remap[used] = np.arange(len(used))
+    return remap[lab].astype(np.int32)
+
+
 def merge_scenes(parts, offsets):
     """Several scenes as one graph (disjoint components), part p moved by
     offsets[p].  The components become the objects of object shape matching

@@ -120,6 +120,8 @@ def bind(lib):
     L.xpbd_ext_volume_topology.restype = c_int
     L.xpbd_ext_set_node_order.argtypes = [c_int]
     L.xpbd_ext_set_node_order.restype = c_int
+    L.xpbd_ext_set_shape_clusters.argtypes = [ctypes.c_void_p, c_int]
+    L.xpbd_ext_set_shape_clusters.restype = c_int
     L.xpbd_ext_set_gauss_seidel.argtypes = [c_int, c_int]
     L.xpbd_ext_set_gauss_seidel.restype = c_int
     L.xpbd_ext_gs_info.argtypes = [ctypes.c_void_p]
@@ -345,6 +347,14 @@ class XPBD:
     def ext_set_node_order(self, mode):
         """Device node layout of the next instance: 1 Morton (default), 0 caller order."""
         self._check(self.lib.xpbd_ext_set_node_order(int(mode)), "xpbd_ext_set_node_order")
+
+    def ext_set_shape_clusters(self, cluster_of=None):
+        """Shape matching over node clusters (cluster id per node, -1 = none); None = 1-ring."""
+        if cluster_of is None:
+            self._check(self.lib.xpbd_ext_set_shape_clusters(None, self.n), "xpbd_ext_set_shape_clusters")
+            return
+        c = np.ascontiguousarray(cluster_of, np.int32)
+        self._check(self.lib.xpbd_ext_set_shape_clusters(_ptr(c), int(c.size)), "xpbd_ext_set_shape_clusters")
 
     def ext_set_gauss_seidel(self, distance, volume):
         """Graph-coloured Gauss-Seidel instead of the Jacobi distance / gather volume passes."""

@@ -179,6 +179,13 @@ int xpbd_ext_self_collision_lists(unsigned int* keys, int* sorted, int* counts, 
  * (including xpbd_positions_device) stays in the caller's order. */
 int xpbd_ext_set_node_order(int mode);
 
+/* Shape matching over node clusters (clusterOf[n]: cluster id, -1 = none;
+ * e.g. k-means clusters of ~64 nodes) instead of the graph 1-ring, with the
+ * same compliance / blend projection (forward.cu:4972-4988).  Clusters of fewer
+ * than 4 nodes do not act.  NULL returns to the 1-ring.  Call after
+ * xpbd_create. */
+int xpbd_ext_set_shape_clusters(const int* clusterOf, int n);
+
 /* Graph-coloured Gauss-Seidel instead of the Jacobi distance pass and the
  * gather volume pass (the reference's FORWARD::setDistanceGSEnabled /
  * setVolumeGSEnabled, forward.cu:3780-3810, off by default).  Settings persist

@@ -276,6 +276,96 @@ void shape_matching(const std::vector<V3>& pin, std::vector<V3>& pout)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Cluster shape matching (extension; the reference's shape constraint with
+// node clusters, e.g. k-means, in place of the graph 1-ring).  Per cluster of
+// >= 4 members (members in increasing node order): rest centroid (once),
+// current centroid, A = sum (p - c)(q - c0)^T normalised by its Frobenius norm,
+// polar rotation (fp64 when robust), then every member is projected toward
+// c + R (q - c0) with xpbdProjectTowardGoal (forward.cu:4972-4988), like
+// the per-node pass (6088-6383).  Sums run in member order.
+// ---------------------------------------------------------------------------
+ShapeClusters shc;
+
+void shape_clusters_set(const int* clusterOf, int n)
+{
+    shc = ShapeClusters();
+    if (!clusterOf) return;
+    int K = 0;
+    for (int i = 0; i < n; ++i) K = std::max(K, clusterOf[i] + 1);
+    std::vector<int> cnt(K, 0);
+    for (int i = 0; i < n; ++i)
+        if (clusterOf[i] >= 0) ++cnt[clusterOf[i]];
+    shc.off.assign(K + 1, 0);
+    for (int k = 0; k < K; ++k) shc.off[k + 1] = shc.off[k] + cnt[k];
+    shc.mem.resize(shc.off[K]);
+    std::vector<int> fill(shc.off.begin(), shc.off.end() - 1);
+    for (int i = 0; i < n; ++i)
+        if (clusterOf[i] >= 0) shc.mem[fill[clusterOf[i]]++] = i;
+    shc.clusterOf.assign(clusterOf, clusterOf + n);
+    shc.comRest.assign(K, V3{0, 0, 0});
+    for (int k = 0; k < K; ++k) {
+        const int c = shc.off[k + 1] - shc.off[k];
+        if (c <= 0) continue;
+        V3 sr{0, 0, 0};
+        for (int t = shc.off[k]; t < shc.off[k + 1]; ++t) sr = add(sr, g.rest[shc.mem[t]]);
+        shc.comRest[k] = mul(sr, 1.0f / (float)c);
+    }
+    shc.on = true;
+}
+
+void shape_cluster_matching(const std::vector<V3>& pin, std::vector<V3>& pout)
+{
+    const int n = g.n, K = (int)shc.comRest.size();
+    std::vector<V3> comCurr(K);
+    std::vector<M3> rot(K);
+    std::vector<char> ok(K, 0);
+    for (int k = 0; k < K; ++k) {
+        const int b0 = shc.off[k], cnt = shc.off[k + 1] - b0;
+        if (cnt < 4) continue;
+        V3 sc{0, 0, 0};
+        for (int t = 0; t < cnt; ++t) sc = add(sc, pin[shc.mem[b0 + t]]);
+        const V3 cc = mul(sc, 1.0f / (float)cnt);
+        const V3 cr = shc.comRest[k];
+        M3 A = mzero();
+        for (int t = 0; t < cnt; ++t) {
+            const int j = shc.mem[b0 + t];
+            const V3 p = sub(pin[j], cc), q = sub(g.rest[j], cr);
+            const float pv[3] = {p.x, p.y, p.z}, qv[3] = {q.x, q.y, q.z};
+            for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) A.m[c][r] += pv[r] * qv[c];
+        }
+        float nsq = 0.0f;
+        for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) nsq += A.m[c][r] * A.m[c][r];
+        const float an = std::sqrt(nsq);
+        if (!std::isfinite(an) || an < 1e-8f) continue;
+        for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) A.m[c][r] /= an;
+        M3 R = mdiag(1, 1, 1);
+        if (s.shapeRobust) { if (!polar_double(A, R)) continue; }
+        else if (!polar_float(A, R)) continue;
+        comCurr[k] = cc;
+        rot[k] = R;
+        ok[k] = 1;
+    }
+    for (int idx = 0; idx < n; ++idx) {
+        const V3 pi = pin[idx];
+        pout[idx] = pi;
+        const float wi = g.invMass[idx] * s.invMassScale;
+        const int k = shc.clusterOf[idx];
+        if (wi <= 0.0f || k < 0 || !ok[k]) continue;
+        const V3 rq = mvec(rot[k], sub(g.rest[idx], shc.comRest[k]));
+        const V3 goal = add(comCurr[k], rq);
+        const V3 gd = sub(goal, pi);
+        const float gd2 = len2(gd);
+        if (!std::isfinite(gd2) || gd2 < 1e-18f) continue;
+        const float dt2 = std::fmax(s.dt * s.dt, 1e-8f);
+        const float at = s.shapeCompliance / dt2;
+        const float denom = wi + at;
+        if (denom < 1e-8f) continue;
+        const float scale = s.shapeBlend * (wi / denom);
+        pout[idx] = add(pi, mul(sub(goal, pi), scale));
+    }
+}
+
 // symEig3Double (13022-13061)
 static void sym_eig3_double(double a[3][3], double ev[3], double V[3][3])
 {

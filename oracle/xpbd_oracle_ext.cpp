@@ -41,6 +41,7 @@ void extensions_free()
     sc = SelfColState();
     dgs = DistanceGS();
     vgs = VolumeGS();
+    shc = ShapeClusters();
 }
 void extensions_on_reset() {}
 // The colourings are built on first use after being enabled, and the volume
@@ -78,7 +79,8 @@ void extensions_iteration(std::vector<V3>*& pin, std::vector<V3>*& pout, int)
         else volume_iteration(pin, pout);
     }
     if (s.useShape) {
-        shape_matching(*pin, *pout);
+        if (shc.on) shape_cluster_matching(*pin, *pout);
+        else shape_matching(*pin, *pout);
         std::swap(pin, pout);
     }
 }
@@ -177,6 +179,19 @@ void xpbd_self_collision_stats(int* out)
     out[0] = (int)sc.contactsTotal;
     out[1] = sc.activeCount;
     out[2] = sc.overflow;
+}
+
+// Extension: shape matching over node clusters (clusterOf[n], -1 = none)
+// instead of the graph 1-ring; null returns to the 1-ring.  After create.
+int xpbd_ext_set_shape_clusters(const int* clusterOf, int n)
+{
+    if (g.n <= 0) return fail("xpbd_ext_set_shape_clusters: call xpbd_create first");
+    if (clusterOf && n != g.n) return fail("xpbd_ext_set_shape_clusters: size mismatch");
+    if (clusterOf)
+        for (int i = 0; i < n; ++i)
+            if (clusterOf[i] < -1 || clusterOf[i] >= n) return fail("xpbd_ext_set_shape_clusters: cluster id out of range");
+    shape_clusters_set(clusterOf, n);
+    return 0;
 }
 
 // Extension: the oracle keeps the caller's order; accepted for ABI parity.

@@ -179,3 +179,60 @@ def test_contact_pack_closed_form(oracle_path, scene):
     assert z == pytest.approx(np.sqrt(M[2, 2]), rel=1e-5)
     sim.destroy()
     assert n == 1
+
+
+def cluster_sim(oracle_path, scene, pos, labels, robust=True, iters=1):
+    sim = XPBD(oracle_path)
+    sim.create(scene)
+    sim.ext_set_shape_clusters(labels)
+    sim.set_solver(iters=iters, dt=1 / 600, vel_damping=0.0, shape_compliance=0.0, shape_blend=1.0)
+    sim.set_constraints(distance=False, shape=True, angle=False, volume=False)
+    sim.set_shape_robust(robust)
+    sim.set_ground(False, (0, 0, 1), 0.0, gravity=0.0)
+    sim.step()
+    sim.ext_set_state(pos, np.zeros_like(pos))
+    return sim
+
+
+def test_kmeans_clusters_partition(scene):
+    lab = scenes.kmeans_clusters(scene["pos"], size=64, seed=3)
+    n = scene["pos"].shape[0]
+    assert lab.shape == (n,) and lab.min() == 0
+    sizes = np.bincount(lab)
+    assert (sizes > 0).all() and len(sizes) <= n // 64
+    # each node is nearer its own centroid than any other (Lloyd fixed point up to the last move)
+    cent = np.stack([np.bincount(lab, weights=scene["pos"][:, a]) / sizes for a in range(3)], 1)
+    d = np.linalg.norm(scene["pos"][:, None, :] - cent[None], axis=2)
+    assert (np.argmin(d, 1) == lab).mean() > 0.97
+
+
+@pytest.mark.parametrize("robust", [True, False])
+def test_cluster_shape_matching_goals(oracle_path, scene, robust):
+    lab = scenes.kmeans_clusters(scene["pos"], size=64, seed=3)
+    rest = scene["pos"].astype(np.float64)
+    # a rigid motion leaves every cluster where it is
+    R = rot([0.3, -0.5, 0.8], 0.7)
+    p = (rest @ R.T + [0.02, 0.0, 0.05]).astype(np.float32)
+    sim = cluster_sim(oracle_path, scene, p, lab, robust)
+    sim.step()
+    ext = scenes.scene_extent(scene)
+    assert np.max(np.abs(sim.positions() - p)) / ext < 2e-6
+    sim.destroy()
+    # an affine map moves every cluster onto its best rigid fit (blend 1)
+    A = np.array([[1.15, 0.1, 0.0], [0.0, 0.9, 0.08], [0.05, 0.0, 1.05]])
+    p = (rest @ A.T).astype(np.float32)
+    sim = cluster_sim(oracle_path, scene, p, lab, robust)
+    sim.step()
+    got = sim.positions().astype(np.float64)
+    worst = 0.0
+    for k in range(lab.max() + 1):
+        m = lab == k
+        if m.sum() < 4:
+            continue
+        q = rest[m] - rest[m].mean(0)
+        c = p[m].astype(np.float64).mean(0)
+        U, _, Vt = np.linalg.svd((p[m] - c).T @ q)
+        Rk = U @ np.diag([1, 1, np.sign(np.linalg.det(U @ Vt))]) @ Vt
+        worst = max(worst, np.abs(got[m] - (c + q @ Rk.T)).max())
+    assert worst / ext < (2e-5 if robust else 2e-4)
+    sim.destroy()

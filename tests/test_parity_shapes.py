@@ -118,3 +118,18 @@ def test_grippers_contact_shapes_parity(scene, product_path, oracle_path):
     np.testing.assert_allclose(pg, pc, rtol=1e-3, atol=1e-6)
     gpu.destroy()
     cpu.destroy()
+
+
+@pytest.mark.parametrize("robust", [True, False])
+def test_cluster_shape_matching_parity(scene, product_path, oracle_path, robust):
+    lab = scenes.kmeans_clusters(scene["pos"], size=64, seed=5)
+
+    def configure(sim):
+        base(sim, scene, robust=robust)
+        sim.ext_set_shape_clusters(lab)
+
+    gpu, cpu, worst = run_pair(scene, product_path, oracle_path, 20, configure)
+    print(f"cluster shape matching robust={robust} ({lab.max() + 1} clusters): per-frame max |dx|/extent {worst:.3e}")
+    assert worst <= TOL_REL
+    gpu.destroy()
+    cpu.destroy()
