@@ -528,6 +528,16 @@ int xpbd_ext_set_node_order(int mode)
     return 0;
 }
 
+int xpbd_ext_self_collision_grid_info(float* info)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_ext_self_collision_grid_info: no step yet");
+    if (!info) return fail("xpbd_ext_self_collision_grid_info: null output");
+    std::string err;
+    if (!g.engine->selfCollisionGridInfo(0, info, err)) return fail("xpbd_ext_self_collision_grid_info: " + err);
+    return 0;
+}
+
 int xpbd_ext_set_shape_clusters(const int* clusterOf, int n)
 {
     Global& g = G();

@@ -178,6 +178,7 @@ public:
     // and the build's artefacts in body-local ids.
     bool selfCollisionStats(int b, int out[3], std::string& err);
     void setSelfCollisionGrid(int b, float cellSize, int tableSize);
+    bool selfCollisionGridInfo(int b, float info[3], std::string& err);   // cell, d_c, search
     bool selfCollisionLists(int b, unsigned int* keys, int* sorted, int* counts, int* contacts, std::string& err);
 
     // Gauss-Seidel colourings (built on demand, caller ids): info = distance
@@ -338,6 +339,7 @@ private:
     DevBuf<float> scBodyDisp_, scInvCell_;
     DevBuf<unsigned char> scActive_;
     DevBuf<float3> scDp_, scDpCross_, scAccum_;
+    DevBuf<float4> scSortP_, scSortX_, scSortR_;
 
     // graph
     hipGraph_t graph_ = nullptr;

@@ -77,6 +77,9 @@ SelfColArgs Engine::scArgs() const
     a.pushSums = pushSums_.p;
     a.pushT = pushT_.p;
     a.callerPos = callerPos_.p;
+    a.sortP = scSortP_.p;
+    a.sortX = scSortX_.p;
+    a.sortR = scSortR_.p;
     return a;
 }
 
@@ -239,9 +242,11 @@ bool Engine::ensureSelfCol(std::string& err)
     cellStart_.alloc(scTableTotal_); cellEnd_.alloc(scTableTotal_);
     contacts_.alloc(n * kSelfColCap); counts_.alloc(n);
     scActive_.alloc(n); scDp_.alloc(n); scDpCross_.alloc(n); scAccum_.alloc(n);
+    scSortP_.alloc(n); scSortX_.alloc(n); scSortR_.alloc(n);
     if (!scMax2_.p || !scCounters_.p || !scUseActive_.p || !scBodyDisp_.p || !scInvCell_.p || !scKeys_.p ||
         !scKeysAlt_.p || !scIdx_.p || !scIdxAlt_.p || !scHist_.p || !cellStart_.p || !cellEnd_.p || !contacts_.p ||
-        !counts_.p || !scActive_.p || !scDp_.p || !scDpCross_.p || !scAccum_.p) {
+        !counts_.p || !scActive_.p || !scDp_.p || !scDpCross_.p || !scAccum_.p || !scSortP_.p || !scSortX_.p ||
+        !scSortR_.p) {
         err = "self-collision: hipMalloc failed";
         return false;
     }
@@ -253,6 +258,21 @@ bool Engine::ensureSelfCol(std::string& err)
     scSortedIdx_ = nullptr;
     scSortedKeys_ = nullptr;
     scDeviceValid_ = true;
+    return true;
+}
+
+static const float kScSearchScaleHost = 1.5f;   // forward.cu:230
+
+bool Engine::selfCollisionGridInfo(int b, float info[3], std::string& err)
+{
+    info[0] = info[1] = info[2] = 0.0f;
+    if (!scDeviceValid_) return true;
+    float inv = 0.0f;
+    XPBD_HIP(hipMemcpyAsync(&inv, scInvCell_.p + b, sizeof(float), hipMemcpyDeviceToHost, stream_));
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    info[0] = inv > 0.0f ? 1.0f / inv : 0.0f;
+    info[1] = set_[b].scRadiusScale * scSpacing_[b];
+    info[2] = kScSearchScaleHost * info[1];
     return true;
 }
 

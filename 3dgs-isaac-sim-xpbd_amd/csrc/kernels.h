@@ -116,6 +116,9 @@ struct SelfColArgs {
     float* pushSums;           // [4K]
     float* pushT;              // [4K]
     const int* callerPos;      // [nPad] caller-order index (sort position of equal keys)
+    float4* sortP;             // [nPad] sorted order: p, object id bits
+    float4* sortX;             // [nPad] sorted order: x, node id bits
+    float4* sortR;             // [nPad] sorted order: rest
 };
 
 void launch_object_shape(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, int K, float4* P, const float3* rest,

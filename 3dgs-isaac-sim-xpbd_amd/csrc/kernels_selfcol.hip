@@ -296,10 +296,39 @@ __global__ void __launch_bounds__(256) sc_range_kernel(int nPad, SelfColArgs a, 
     if (i == nPad - 1 || keysSorted[i + 1] != k) a.cellEnd[k] = i + 1;
 }
 
-// Candidate lists (forward.cu:7079-7180).
-__global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
+// Slot of the worst candidate: highest tier, then largest priority, the first
+// such slot (the scan of forward.cu:7160-7172).
+constexpr int kBuildThreads = 128;   // 32 KB of candidate slots per block
+
+__device__ __forceinline__ int sc_worst(const float* prio, const int* tier)
+{
+    int w = 0;
+    for (int m = 1; m < kSelfColCap; ++m) {
+        const int tm = tier[m * kBuildThreads], tw = tier[w * kBuildThreads];
+        if (tm > tw || (tm == tw && prio[m * kBuildThreads] > prio[w * kBuildThreads])) w = m;
+    }
+    return w;
+}
+
+// Sorted-order copies of what the candidate scan reads, so that each cell of
+// the stencil is one contiguous run: (p, object id), (x, node id), rest.
+__global__ void __launch_bounds__(256) sc_stage_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
                                                       const float4* __restrict__ P, const float3* __restrict__ rest,
                                                       const int* __restrict__ sortedIdx)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nPad) return;
+    const int j = sortedIdx[t];
+    const float4 p = P[j];
+    const float3 x = X[j], r = rest[j];
+    a.sortP[t] = make_float4(p.x, p.y, p.z, __int_as_float(a.objId[j]));
+    a.sortX[t] = make_float4(x.x, x.y, x.z, __int_as_float(j));
+    a.sortR[t] = make_float4(r.x, r.y, r.z, 0.0f);
+}
+
+// Candidate lists (forward.cu:7079-7180).
+__global__ void __launch_bounds__(kBuildThreads) sc_build_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
+                                                      const float4* __restrict__ P, const float3* __restrict__ rest)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nPad) return;
@@ -320,35 +349,38 @@ __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, 
         const unsigned int mask = (unsigned int)B.scTableMask;
         const int cx = sc_cell(x.x, inv), cy = sc_cell(x.y, inv), cz = sc_cell(x.z, inv);
         int* out = &a.contacts[(size_t)i * CAP];
-        unsigned int hs[27];
-        float prio[CAP];
-        int tier[CAP];
-        int nh = 0;
-        for (int oz = -1; oz <= 1; ++oz)
-            for (int oy = -1; oy <= 1; ++oy)
-                for (int ox = -1; ox <= 1; ++ox) {
-                    const unsigned int h = sc_hash(cx + ox, cy + oy, cz + oz, mask);
-                    bool dup = false;
-                    for (int m = 0; m < nh; ++m)
-                        if (hs[m] == h) { dup = true; break; }
-                    if (dup) continue;
-                    hs[nh++] = h;
+        // per-thread candidate priorities in LDS (slot m of thread t at m * kBuildThreads + t)
+        __shared__ float sPrio[CAP * kBuildThreads];
+        __shared__ int sTier[CAP * kBuildThreads];
+        float* prio = &sPrio[threadIdx.x];
+        int* tier = &sTier[threadIdx.x];
+        int worst = -1;   // current worst slot once the list is full
+        for (int c = 0; c < 27; ++c) {
+            const int oz = c / 9 - 1, oy = (c / 3) % 3 - 1, ox = c % 3 - 1;
+            const unsigned int h = sc_hash(cx + ox, cy + oy, cz + oz, mask);
+            // a cell reached through an earlier stencil offset is scanned once
+            bool dup = false;
+            for (int c2 = 0; c2 < c && !dup; ++c2)
+                dup = sc_hash(cx + c2 % 3 - 1, cy + (c2 / 3) % 3 - 1, cz + c2 / 9 - 1, mask) == h;
+            if (dup) continue;
+            {
                     const int s = a.cellStart[B.scTableBase + h];
                     if (s < 0) continue;
                     const int e = a.cellEnd[B.scTableBase + h];
                     for (int t = s; t < e; ++t) {
-                        const int j = sortedIdx[t];
+                        const float4 xj4 = a.sortX[t];
+                        const int j = __float_as_int(xj4.w);
                         if (j == i) continue;
-                        const bool cross = a.objId[j] != oi;
+                        const float4 pj = a.sortP[t];
+                        const bool cross = __float_as_int(pj.w) != oi;
                         if (!cross && !B.scWithin) continue;
-                        const float4 pj = P[j];
-                        const float3 qj = rest[j];
+                        const float4 qj = a.sortR[t];
                         const float qx = q.x - qj.x, qy = q.y - qj.y, qz = q.z - qj.z;
                         const float q2 = qx * qx + qy * qy + qz * qz;
                         float pr;
                         int code = j;
                         if (cross) {
-                            const float3 xj = X[j];
+                            const float4 xj = xj4;
                             const float ex = x.x - xj.x, ey = x.y - xj.y, ez = x.z - xj.z;
                             const float distX = sqrtf(ex * ex + ey * ey + ez * ez);
                             const float rx = dix - (pj.x - xj.x), ry = diy - (pj.y - xj.y), rz = diz - (pj.z - xj.z);
@@ -367,22 +399,27 @@ __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, 
                         const int tr = cross ? 0 : 1;
                         if (cnt < CAP) {
                             out[cnt] = code;
-                            prio[cnt] = pr;
-                            tier[cnt] = tr;
+                            prio[cnt * kBuildThreads] = pr;
+                            tier[cnt * kBuildThreads] = tr;
                             ++cnt;
                         } else {
-                            int worst = 0;
-                            for (int m = 1; m < CAP; ++m)
-                                if (tier[m] > tier[worst] || (tier[m] == tier[worst] && prio[m] > prio[worst])) worst = m;
-                            if (tr < tier[worst] || (tr == tier[worst] && pr < prio[worst])) {
+                            // replace the worst entry (highest tier, then largest
+                            // priority, first slot among equals); the worst slot
+                            // only changes when an entry is replaced
+                            if (worst < 0) worst = sc_worst(prio, tier);
+                            const int tw = tier[worst * kBuildThreads];
+                            const float pw = prio[worst * kBuildThreads];
+                            if (tr < tw || (tr == tw && pr < pw)) {
                                 out[worst] = code;
-                                prio[worst] = pr;
-                                tier[worst] = tr;
+                                prio[worst * kBuildThreads] = pr;
+                                tier[worst * kBuildThreads] = tr;
+                                worst = sc_worst(prio, tier);
                             }
                             ++overflow;
                         }
                     }
-                }
+            }
+        }
     }
     a.counts[i] = cnt;
     if (cnt > 0) atomicAdd(&a.scCounters[3 * b], cnt);
@@ -617,7 +654,8 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
     const int* vs = inAlt ? idxAlt : a.idx;
     launch_fill_int(s, a.tableTotal, a.cellStart, -1);
     sc_range_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, ks);
-    sc_build_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
+    sc_stage_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
+    sc_build_kernel<<<nbs(na.nPad, kBuildThreads), kBuildThreads, 0, s>>>(na.nPad, a, X, P, rest);
     sc_fill_accum_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a.accum);
     *sortedIdxOut = vs;
     return inAlt;

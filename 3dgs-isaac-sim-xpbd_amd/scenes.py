@@ -96,6 +96,36 @@ def kmeans_clusters(pos, size=64, iters=20, seed=0):
     return remap[lab].astype(np.int32)
 
 
+def full_recipe(scene, cluster_size=64, seed=0):
+    """The full single-object recipe: one volume cluster per node (the node and
+    its 3 nearest neighbours, a tetrahedron), k-means shape clusters of about
+    `cluster_size` nodes, self-collision within the body."""
+    from scipy.spatial import cKDTree
+    pos = scene["pos"]
+    n = pos.shape[0]
+    _, nn = cKDTree(pos).query(pos, 4)
+    tet_off = np.arange(0, 3 * n + 1, 3, dtype=np.int32)
+    tet_mem = np.ascontiguousarray(nn[:, 1:].astype(np.int32).ravel())
+    table = 1 << 21 if n <= 300_000 else 1 << 23
+    return {"tet_off": tet_off, "tet_mem": tet_mem,
+            "clusters": kmeans_clusters(pos, size=cluster_size, seed=seed), "table": table}
+
+
+def configure_full(sim, recipe, iters=4, substeps=10):
+    """Solver settings of the full recipe (after create, before the first step).
+    Self-collision keeps the reference's cell rule (search radius + twice the
+    largest predicted move); the table size follows the recipe."""
+    sim.set_solver(iters=iters, dt=1.0 / (60.0 * substeps), under_relax=0.6, vel_damping=0.01,
+                   dist_compliance=1e-6, shape_compliance=0.0, shape_blend=0.1)
+    sim.set_constraints(distance=True, shape=True, angle=False, volume=True)
+    sim.ext_set_volume_clusters(recipe["tet_off"], recipe["tet_mem"], 3)
+    sim.ext_set_shape_clusters(recipe["clusters"])
+    sim.set_ground(True, (0.0, 0.0, 1.0), 0.0, friction=0.5, restitution=0.0, gravity=9.81)
+    sim.set_object_shape(0.0)
+    sim.ext_set_self_collision_grid(0.0, recipe["table"])
+    sim.set_self_collision(True, 1.5, 2.0, True)
+
+
 def merge_scenes(parts, offsets):
     """Several scenes as one graph (disjoint components), part p moved by
     offsets[p].  The components become the objects of object shape matching

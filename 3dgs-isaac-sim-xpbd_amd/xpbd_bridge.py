@@ -120,6 +120,8 @@ def bind(lib):
     L.xpbd_ext_volume_topology.restype = c_int
     L.xpbd_ext_set_node_order.argtypes = [c_int]
     L.xpbd_ext_set_node_order.restype = c_int
+    L.xpbd_ext_self_collision_grid_info.argtypes = [ctypes.c_void_p]
+    L.xpbd_ext_self_collision_grid_info.restype = c_int
     L.xpbd_ext_set_shape_clusters.argtypes = [ctypes.c_void_p, c_int]
     L.xpbd_ext_set_shape_clusters.restype = c_int
     L.xpbd_ext_set_gauss_seidel.argtypes = [c_int, c_int]
@@ -263,6 +265,16 @@ class XPBD:
             self._check(-1, "xpbd_compute_shapes")
         return sc, q, k
 
+    def shapes_device(self, deform_eps=1e-2):
+        """compute_shapes with the outputs left in device memory (null host
+        pointers, as the ABI allows); returns the deformed count."""
+        fn = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int)(
+            ("xpbd_compute_shapes", self.lib))
+        k = fn(float(deform_eps), None, None, self.n)
+        if k < 0:
+            self._check(-1, "xpbd_compute_shapes")
+        return k
+
     def positions_device_ptr(self):
         return int(self.lib.xpbd_positions_device())
 
@@ -347,6 +359,12 @@ class XPBD:
     def ext_set_node_order(self, mode):
         """Device node layout of the next instance: 1 Morton (default), 0 caller order."""
         self._check(self.lib.xpbd_ext_set_node_order(int(mode)), "xpbd_ext_set_node_order")
+
+    def ext_self_collision_grid_info(self):
+        """(cell edge, d_c, search radius) of the last self-collision build."""
+        info = np.zeros(3, np.float32)
+        self._check(self.lib.xpbd_ext_self_collision_grid_info(_ptr(info)), "xpbd_ext_self_collision_grid_info")
+        return tuple(float(v) for v in info)
 
     def ext_set_shape_clusters(self, cluster_of=None):
         """Shape matching over node clusters (cluster id per node, -1 = none); None = 1-ring."""

@@ -1,11 +1,16 @@
 """Benchmark: Gaussian-substeps/s of the XPBD step on one MI355X.
 
-Workload (the 1M-Gaussian roofline recipe): 1M Gaussians uniform in a
-0.5 x 0.3 x 0.2 m ellipsoid on the z=0 plane, symmetric kNN (k=12) distance
-constraints (compliance 1e-6), gravity and ground, two capsule grippers
-(r 0.02 m, half-length 0.04 m) that attach 500 nodes each, pull 0.1 m and
-twist 30 deg over 60 frames then release; dt = 1/60 s per frame as 10
-substeps x 4 iterations.  One bench step = one frame = 10 substeps.
+Default workload ("full", the 1M roofline configuration): 1M Gaussians uniform
+in a 0.5 x 0.3 x 0.2 m ellipsoid on the z=0 plane (PCG64 seed 3), symmetric
+kNN (k=12) distance constraints (compliance 1e-6), one volume cluster per node
+(the node and its 3 nearest neighbours), k-means shape clusters of ~64 nodes
+(shape blend 0.1), self-collision within the body (2^23 hash cells, the
+reference's cell rule), gravity and ground, two capsule grippers (r 0.02 m,
+half-length 0.04 m) that attach 500 nodes each, pull 0.1 m and twist 30 deg
+over 60 frames then release, and the per-Gaussian scale/orientation update
+once per frame (outputs left in device memory).  dt = 1/60 s per frame as 10
+substeps x 4 iterations; one bench step = one frame.  "--workload distance"
+keeps only distance + ground + grippers.
 
 Prints one JSON line (see the repository contract).  --gpus N>1 is refused:
 this build runs on one GPU.
@@ -27,38 +32,40 @@ from xpbd_bridge import XPBD  # noqa: E402
 
 LIB = os.path.join(PKG, "libxpbd_isaac.so")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle_xpbd.so")
+PROFILE = os.path.join(ROOT, "profiles", "distance_jacobi_1M.md")
 HBM_PEAK_GBS = 8000.0
 SUBSTEPS = 10
 ITERS = 4
-
-
-PROFILE = os.path.join(ROOT, "profiles", "distance_jacobi_1M.md")
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic():
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (scaled by the streaming calibration)."""
     try:
         for line in open(PROFILE):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
                 return int(line.split(":")[1])
     except OSError:
         pass
     return None
 
 
-def configure(sim):
-    sim.set_solver(iters=ITERS, dt=1.0 / (60.0 * SUBSTEPS), under_relax=0.6, vel_damping=0.01,
-                   dist_compliance=1e-6)
-    sim.set_constraints(distance=True, shape=False, angle=False, volume=False)
-    sim.set_ground(True, (0.0, 0.0, 1.0), 0.0, friction=0.5, restitution=0.0, gravity=9.81)
+def configure(sim, workload, recipe):
+    if workload == "full":
+        scenes.configure_full(sim, recipe, iters=ITERS, substeps=SUBSTEPS)
+    else:
+        sim.set_solver(iters=ITERS, dt=1.0 / (60.0 * SUBSTEPS), under_relax=0.6, vel_damping=0.01,
+                       dist_compliance=1e-6)
+        sim.set_constraints(distance=True, shape=False, angle=False, volume=False)
+        sim.set_ground(True, (0.0, 0.0, 1.0), 0.0, friction=0.5, restitution=0.0, gravity=9.81)
+        sim.set_self_collision(False)
 
 
-def run_frame(sim, script, frame, first):
+def run_frame(sim, script, frame, first, shapes):
     t, p, d = script.colliders(frame)
     sim.set_colliders(t, p, d)
     att = script.attached(frame)
@@ -69,29 +76,70 @@ def run_frame(sim, script, frame, first):
             elif frame == script.ramp:
                 sim.clear_attached()
         sim.step()
+    if shapes:
+        sim.shapes_device(1e-2)
 
 
-def cpu_baseline(scene, script, seconds_budget=20.0):
-    """Oracle (serial restatement) on a bounded sample of the same workload."""
+def cpu_baseline(scene, script, workload, recipe, seconds_budget=20.0):
+    """Oracle (serial restatement) on a bounded sample of the same workload:
+    substeps of the first frame(s) until the time budget is spent."""
     import multiprocessing
     sim = XPBD(ORACLE_LIB)
     sim.create(scene)
-    configure(sim)
+    configure(sim, workload, recipe)
+    # one untimed substep builds the oracle's topology caches
+    t, p, d = script.colliders(0)
+    sim.set_colliders(t, p, d)
+    sim.step()
     t0 = time.perf_counter()
     substeps = 0
     frame = 0
-    while True:
-        run_frame(sim, script, frame, frame == 0)
-        substeps += SUBSTEPS
+    done = False
+    while not done:
+        t, p, d = script.colliders(frame)
+        sim.set_colliders(t, p, d)
+        att = script.attached(frame)
+        for s in range(SUBSTEPS):
+            if s == (1 if frame == 0 else 0) and att is not None:
+                sim.set_attached(att[0], att[1], 1.0)
+            sim.step()
+            substeps += 1
+            if time.perf_counter() - t0 > seconds_budget:
+                done = True
+                break
         frame += 1
-        if time.perf_counter() - t0 > seconds_budget or frame >= 3:
+        if frame >= 3:
             break
     dt = time.perf_counter() - t0
     sim.destroy()
     n = scene["pos"].shape[0]
     return {"value": n * substeps / dt, "unit": "Gaussian-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle, same {n}-Gaussian scene and script, first {frame} frame(s) = {substeps} substeps, "
+            "sample": f"oracle, same {n}-Gaussian scene, recipe and script, first {substeps} substep(s), "
                       f"single thread, {dt:.1f} s; host has {multiprocessing.cpu_count()} logical CPUs"}
+
+
+def kernel_bytes(workload, n, m, sc_table, sc_passes, vol):
+    """Algorithmic bytes per launch (compulsory traffic) of each profiled kernel
+    label: N nodes, M directed distance entries, L leaders, S members,
+    R = S + L reverse entries, T hash cells."""
+    L, S = vol
+    R = S + L
+    out = {
+        "predict": 52 * n,
+        "distance_jacobi_first": 36 * n + 20 * m,
+        "distance_jacobi_mid": 36 * n + 20 * m,
+        "distance_jacobi_last": 36 * n + 20 * m,
+    }
+    if workload == "full":
+        out.update({
+            "volume_solve_thread": 100 * L + 4 * S,
+            "volume_solve_wave": 100 * L + 4 * S,
+            "volume_apply": 56 * n + 4 * R,
+            "shape_match": 60 * n,
+            "selfcol_build": 150 * n + 4 * sc_table + 16 * n * sc_passes,
+            "contact_push_commit": 64 * n,
+        })
+    return out
 
 
 def main():
@@ -100,6 +148,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--workload", choices=("full", "distance"), default="full")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run a streaming kernel of known traffic (256 MiB read + 256 MiB written, x3) "
@@ -118,45 +167,65 @@ def main():
         print(json.dumps({"calibration": "mul", "bytes_read": a.numel() * 4, "bytes_written": b.numel() * 4}))
         return
 
-    log(f"generating {args.n}-Gaussian scene")
+    log(f"generating {args.n}-Gaussian scene ({args.workload})")
     t = time.perf_counter()
     scene = scenes.make_ellipsoid_scene(args.n, 12, seed=3)
     script = scenes.GripperScript(scene, seed=7)
+    recipe = scenes.full_recipe(scene, seed=3) if args.workload == "full" else None
     n = scene["pos"].shape[0]
     e = scene["edges"].shape[0]
-    log(f"scene ready in {time.perf_counter() - t:.1f}s: N={n} E={e} M={2 * e}")
+    m = 2 * e
+    log(f"scene ready in {time.perf_counter() - t:.1f}s: N={n} E={e} M={m}")
 
     sim = XPBD(LIB)
     if sim.backend != "hip-gfx950":
         raise SystemExit("bench.py needs the HIP library")
     sim.create(scene)
-    configure(sim)
+    configure(sim, args.workload, recipe)
+    shapes = args.workload == "full"
     frame = 0
     for _ in range(args.warmup):
-        run_frame(sim, script, frame, frame == 0)
+        run_frame(sim, script, frame, frame == 0, shapes)
         frame += 1
     log(f"warmup {args.warmup} frames done")
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        run_frame(sim, script, frame, frame == 0)
+        run_frame(sim, script, frame, frame == 0, shapes)
         frame += 1
     t1 = time.perf_counter()
     elapsed = t1 - t0
     value = n * SUBSTEPS * args.steps / elapsed
     log(f"timed {args.steps} frames in {elapsed:.3f}s -> {value / 1e9:.3f} G Gaussian-substeps/s")
+    stats = sim.self_collision_stats() if shapes else (0, 0, 0)
+    grid = sim.ext_self_collision_grid_info() if shapes else (0.0, 0.0, 0.0)
+    log(f"self-collision: candidates/active/overflow {stats}, cell/d_c/search {grid}")
 
     # per-kernel HIP-event timing on the library stream, same state and script
     prof = sim.ext_profile(SUBSTEPS)
     log("kernel ms (total, launches): " + json.dumps(prof))
-    key = "distance_jacobi_mid"
-    tot, cnt = prof[key]
-    avg_s = tot / cnt / 1e3
-    m = 2 * e
-    alg_bytes = 36 * n + 20 * m      # predicted pos in/out + row offsets + 5 streams per directed entry
-    achieved = alg_bytes / avg_s / 1e9
+    vol = (n, 3 * n) if shapes else (0, 0)
+    table = recipe["table"] if shapes else 0
+    passes = (int(np.ceil(np.log2(table + 1))) + 7) // 8 if shapes else 0
+    formulas = kernel_bytes(args.workload, n, m, table, passes, vol)
+    kernels = {}
+    for k, (tot, cnt) in prof.items():
+        if k in formulas and cnt > 0:
+            avg = tot / cnt / 1e3
+            kernels[k] = {"ms_total": tot, "launches": cnt, "avg_us": avg * 1e6, "alg_bytes": formulas[k],
+                          "GBps": formulas[k] / avg / 1e9}
+    dominant = max(kernels, key=lambda k: kernels[k]["ms_total"])
+    dk = kernels[dominant]
+    achieved = dk["GBps"]
     sim.destroy()
 
-    cpu = None if args.no_cpu else cpu_baseline(scene, script)
+    cpu = None if args.no_cpu else cpu_baseline(scene, script, args.workload, recipe)
+    if args.workload == "full":
+        wl = (f"1M-Gaussian ellipsoid full solver: kNN k=12 distance, node+3NN volume clusters, k-means shape "
+              f"clusters of 64, self-collision within the body (2^23 cells), ground+gravity, 2 capsule grippers "
+              f"pull/twist, per-frame shape update; {SUBSTEPS} substeps x {ITERS} iters per step; plain C-ABI path")
+    else:
+        wl = (f"1M-Gaussian ellipsoid, kNN k=12 distance, ground+gravity, 2 capsule grippers pull/twist; "
+              f"{SUBSTEPS} substeps x {ITERS} iters per step; plain C-ABI path")
     out = {
         "metric": "Gaussian-substeps/s at 1M",
         "value": value,
@@ -170,16 +239,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"1M-Gaussian ellipsoid, kNN k=12 distance, ground+gravity, 2 capsule grippers "
-                               f"pull/twist; {SUBSTEPS} substeps x {ITERS} iters per step; plain C-ABI path "
-                               f"(xpbd_step per substep)",
-                   "gaussians": n, "edges": e, "substeps_per_step": SUBSTEPS, "iters": ITERS},
-        "roofline": {"bound": "hbm", "kernel": key, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+        "config": {"workload": wl, "gaussians": n, "edges": e, "substeps_per_step": SUBSTEPS, "iters": ITERS,
+                   "shape_clusters": int(recipe["clusters"].max() + 1) if shapes else 0,
+                   "selfcol_stats_last": list(stats), "selfcol_cell_dc_search": list(grid)},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
                      "traffic_source": os.path.relpath(PROFILE, ROOT),
-                     "alg_bytes_per_launch": alg_bytes, "avg_launch_us": avg_s * 1e6},
+                     "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_us": dk["avg_us"]},
         "cpu_baseline": cpu,
-        "kernel_ms": prof,
+        "kernels": kernels,
     }
     print(json.dumps(out), flush=True)
 

@@ -179,6 +179,10 @@ int xpbd_ext_self_collision_lists(unsigned int* keys, int* sorted, int* counts, 
  * (including xpbd_positions_device) stays in the caller's order. */
 int xpbd_ext_set_node_order(int mode);
 
+/* Grid of the last self-collision build (after the first step): info[0] cell
+ * edge, info[1] contact distance d_c, info[2] search radius. */
+int xpbd_ext_self_collision_grid_info(float* info);
+
 /* Shape matching over node clusters (clusterOf[n]: cluster id, -1 = none;
  * e.g. k-means clusters of ~64 nodes) instead of the graph 1-ring, with the
  * same compliance / blend projection (forward.cu:4972-4988).  Clusters of fewer

@@ -194,6 +194,17 @@ int xpbd_ext_set_shape_clusters(const int* clusterOf, int n)
     return 0;
 }
 
+// Extension: grid of the last self-collision build: cell edge, d_c, search radius.
+int xpbd_ext_self_collision_grid_info(float* info)
+{
+    if (!g.stepped) return fail("xpbd_ext_self_collision_grid_info: no step yet");
+    if (!info) return fail("xpbd_ext_self_collision_grid_info: null output");
+    info[0] = sc.invCell > 0.0f ? 1.0f / sc.invCell : 0.0f;
+    info[1] = s.scRadiusScale * sc.spacing;
+    info[2] = s.scSearchScale * info[1];
+    return 0;
+}
+
 // Extension: the oracle keeps the caller's order; accepted for ABI parity.
 int xpbd_ext_set_node_order(int mode)
 {
