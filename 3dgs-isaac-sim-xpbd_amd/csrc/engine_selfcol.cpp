@@ -298,7 +298,10 @@ bool Engine::selfCollisionLists(int b, unsigned int* keys, int* sorted, int* cou
     if (!scDeviceValid_ || !scSortedIdx_) { err = "no self-collision build yet"; return false; }
     const int nb = bodyBegin_[b], n = bodyCount_[b];
     std::vector<unsigned int> hk(nPad_), hks(nPad_);
-    std::vector<int> hs(nPad_), hc(n), hl((size_t)n * kSelfColCap);
+    // lists are slot-major per 64-node tile: slot k of tile-local node l at
+    // tile * 64 * 32 + 64 k + l (the body starts on a tile boundary)
+    const size_t tiles = ((size_t)n + kTile - 1) / kTile;
+    std::vector<int> hs(nPad_), hc(n), hl(tiles * kTile * kSelfColCap);
     XPBD_HIP(hipMemcpyAsync(hks.data(), scSortedKeys_, sizeof(unsigned int) * nPad_, hipMemcpyDeviceToHost, stream_));
     XPBD_HIP(hipMemcpyAsync(hs.data(), scSortedIdx_, sizeof(int) * nPad_, hipMemcpyDeviceToHost, stream_));
     XPBD_HIP(hipMemcpyAsync(hc.data(), counts_.p + nb, sizeof(int) * n, hipMemcpyDeviceToHost, stream_));
@@ -333,7 +336,7 @@ bool Engine::selfCollisionLists(int b, unsigned int* keys, int* sorted, int* cou
         for (int i = 0; i < n; ++i) {
             const int d = inv[i];
             for (int k = 0; k < kSelfColCap; ++k) {
-                const int code = hl[(size_t)d * kSelfColCap + k];
+                const int code = hl[(size_t)(d / kTile) * kTile * kSelfColCap + (size_t)k * kTile + d % kTile];
                 int out = 0;   // slots past the count hold earlier builds' entries
                 if (k < hc[d]) out = code >= 0 ? ord[code - nb] : -(ord[-code - 1 - nb] + 1);
                 contacts[(size_t)i * kSelfColCap + k] = out;

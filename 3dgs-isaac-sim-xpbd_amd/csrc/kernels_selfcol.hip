@@ -296,20 +296,6 @@ __global__ void __launch_bounds__(256) sc_range_kernel(int nPad, SelfColArgs a, 
     if (i == nPad - 1 || keysSorted[i + 1] != k) a.cellEnd[k] = i + 1;
 }
 
-// Slot of the worst candidate: highest tier, then largest priority, the first
-// such slot (the scan of forward.cu:7160-7172).
-constexpr int kBuildThreads = 128;   // 32 KB of candidate slots per block
-
-__device__ __forceinline__ int sc_worst(const float* prio, const int* tier)
-{
-    int w = 0;
-    for (int m = 1; m < kSelfColCap; ++m) {
-        const int tm = tier[m * kBuildThreads], tw = tier[w * kBuildThreads];
-        if (tm > tw || (tm == tw && prio[m * kBuildThreads] > prio[w * kBuildThreads])) w = m;
-    }
-    return w;
-}
-
 // Sorted-order copies of what the candidate scan reads, so that each cell of
 // the stencil is one contiguous run: (p, object id), (x, node id), rest.
 __global__ void __launch_bounds__(256) sc_stage_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
@@ -322,109 +308,199 @@ __global__ void __launch_bounds__(256) sc_stage_kernel(int nPad, SelfColArgs a, 
     const float4 p = P[j];
     const float3 x = X[j], r = rest[j];
     a.sortP[t] = make_float4(p.x, p.y, p.z, __int_as_float(a.objId[j]));
-    a.sortX[t] = make_float4(x.x, x.y, x.z, __int_as_float(j));
-    a.sortR[t] = make_float4(r.x, r.y, r.z, 0.0f);
+    a.sortX[t] = make_float4(x.x, x.y, x.z, 0.0f);
+    a.sortR[t] = make_float4(r.x, r.y, r.z, __int_as_float(j));
 }
 
-// Candidate lists (forward.cu:7079-7180).
-__global__ void __launch_bounds__(kBuildThreads) sc_build_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
+// Distance from x to the stencil cell at offset o along one axis, given the
+// position f in [0, 1) of x inside its own cell of edge e.
+__device__ __forceinline__ float sc_axis_gap(int o, float f, float e)
+{
+    return o < 0 ? f * e : (o > 0 ? (1.0f - f) * e : 0.0f);
+}
+
+__device__ __forceinline__ int lane_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+// Candidate lists are slot-major inside each 64-node tile: slot k of node i
+// at tile * 64 * 32 + 64 k + (i mod 64), so a wave reads one slot of its 64
+// nodes as one contiguous 256 B request.
+__device__ __forceinline__ size_t sc_slot(int i, int k)
+{
+    return (size_t)(i / kTile) * (kTile * kSelfColCap) + (size_t)k * kTile + (i % kTile);
+}
+__device__ __forceinline__ float lane_fl(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Lane holding the worst of the 32 list slots (lanes 0-31): highest tier, then
+// largest priority, then the lowest slot (the scan of forward.cu:7160-7172).
+__device__ __forceinline__ int sc_worst_lane(int tier, float prio, int lane)
+{
+    int wl = lane < kSelfColCap ? lane : 0;
+    int wt = lane < kSelfColCap ? tier : -1;
+    float wp = prio;
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        const int ol = __shfl_xor(wl, m, 64), ot = __shfl_xor(wt, m, 64);
+        const float op = __shfl_xor(wp, m, 64);
+        const bool take = ot > wt || (ot == wt && (op > wp || (op == wp && ol < wl)));
+        if (take) { wl = ol; wt = ot; wp = op; }
+    }
+    return __builtin_amdgcn_readfirstlane(wl);
+}
+
+// Candidate lists (forward.cu:7079-7180), one wave64 per node at a time.  The
+// lanes test the candidates of a hash bucket 64 at a time; the accepted ones
+// are inserted in lane order (= the reference's scan order) into the 32 slots
+// held by lanes 0-31, replacing the worst slot once the list is full.
+constexpr int kBuildNodesPerWave = 8;
+
+__global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
                                                       const float4* __restrict__ P, const float3* __restrict__ rest)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nPad) return;
-    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
-    const BodyDev& B = a.bodies[b];
-    if (!B.scOn || i >= B.nodeEnd) return;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
     constexpr int CAP = kSelfColCap;
-    const float3 x = X[i];
-    const float4 p4 = P[i];
-    int cnt = 0, overflow = 0;
-    if (a.scActive[i] && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p4.x) && isfinite(p4.y) &&
-        isfinite(p4.z)) {
-        const float3 q = rest[i];
-        const float dix = p4.x - x.x, diy = p4.y - x.y, diz = p4.z - x.z;
-        const int oi = a.objId[i];
-        const float search = B.scSearch, search2 = search * search;
-        const float inv = a.scInvCell[b];
-        const unsigned int mask = (unsigned int)B.scTableMask;
-        const int cx = sc_cell(x.x, inv), cy = sc_cell(x.y, inv), cz = sc_cell(x.z, inv);
-        int* out = &a.contacts[(size_t)i * CAP];
-        // per-thread candidate priorities in LDS (slot m of thread t at m * kBuildThreads + t)
-        __shared__ float sPrio[CAP * kBuildThreads];
-        __shared__ int sTier[CAP * kBuildThreads];
-        float* prio = &sPrio[threadIdx.x];
-        int* tier = &sTier[threadIdx.x];
-        int worst = -1;   // current worst slot once the list is full
-        for (int c = 0; c < 27; ++c) {
-            const int oz = c / 9 - 1, oy = (c / 3) % 3 - 1, ox = c % 3 - 1;
+    const int i0 = wave * kBuildNodesPerWave;
+    if (i0 >= nPad) return;   // whole waves
+    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i0 / kTile]);
+    const BodyDev& B = a.bodies[b];
+    long long candTotal = 0, overTotal = 0;
+    for (int ii = 0; ii < kBuildNodesPerWave; ++ii) {
+        const int i = i0 + ii;
+        if (!B.scOn || i >= B.nodeEnd) break;
+        const float3 x = X[i];
+        const float4 p4 = P[i];
+        int cnt = 0, overflow = 0;
+        int slotCode = 0, slotTier = 0;
+        float slotPrio = 0.0f;
+        int worst = -1, worstTier = 0;
+        float worstPrio = 0.0f;
+        if (a.scActive[i] && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p4.x) && isfinite(p4.y) &&
+            isfinite(p4.z)) {
+            const float3 q = rest[i];
+            const float dix = p4.x - x.x, diy = p4.y - x.y, diz = p4.z - x.z;
+            const int oi = a.objId[i];
+            const float search = B.scSearch, search2 = search * search;
+            const float inv = a.scInvCell[b];
+            const unsigned int mask = (unsigned int)B.scTableMask;
+            const int cx = sc_cell(x.x, inv), cy = sc_cell(x.y, inv), cz = sc_cell(x.z, inv);
+            // Stencil cells that cannot hold a candidate are skipped: a node j
+            // passes either test only if |x_i - x_j| < search + |p_i - x_i| +
+            // max |p - x| (the body's largest predicted move), so a cell farther
+            // than that bound (with a relative margin for rounding) holds none.
+            // A hash bucket is still scanned, at its first stencil offset,
+            // whenever any offset mapping to it is needed, so the candidate
+            // order is that of the full stencil scan.
+            const float edge = 1.0f / inv;
+            const float tx = fminf(fmaxf(x.x * inv, -1.0e9f), 1.0e9f), ty = fminf(fmaxf(x.y * inv, -1.0e9f), 1.0e9f),
+                        tz = fminf(fmaxf(x.z * inv, -1.0e9f), 1.0e9f);
+            const float fx = tx - floorf(tx), fy = ty - floorf(ty), fz = tz - floorf(tz);
+            const float di = sqrtf(dix * dix + diy * diy + diz * diz);
+            const float maxMove = sqrtf(fmaxf(__uint_as_float(a.scMax2[b]), 0.0f));
+            const float bound = (search + di + maxMove) * 1.0001f + edge * 1.0e-4f + 1.0e-6f;
+            const float bound2 = bound * bound;
+            // lane c < 27: hash of stencil offset c, first occurrence, needed
+            const int c = lane < 27 ? lane : 26;
+            const int ox = c % 3 - 1, oy = (c / 3) % 3 - 1, oz = c / 9 - 1;
             const unsigned int h = sc_hash(cx + ox, cy + oy, cz + oz, mask);
-            // a cell reached through an earlier stencil offset is scanned once
-            bool dup = false;
-            for (int c2 = 0; c2 < c && !dup; ++c2)
-                dup = sc_hash(cx + c2 % 3 - 1, cy + (c2 / 3) % 3 - 1, cz + c2 / 9 - 1, mask) == h;
-            if (dup) continue;
-            {
-                    const int s = a.cellStart[B.scTableBase + h];
-                    if (s < 0) continue;
-                    const int e = a.cellEnd[B.scTableBase + h];
-                    for (int t = s; t < e; ++t) {
-                        const float4 xj4 = a.sortX[t];
-                        const int j = __float_as_int(xj4.w);
-                        if (j == i) continue;
+            const float gx = sc_axis_gap(ox, fx, edge), gy = sc_axis_gap(oy, fy, edge), gz = sc_axis_gap(oz, fz, edge);
+            const bool need = lane < 27 && gx * gx + gy * gy + gz * gz <= bound2;
+            bool first = lane < 27, scan = false;
+            for (int c2 = 0; c2 < 27; ++c2) {
+                const unsigned int h2 = (unsigned int)lane_i((int)h, c2);
+                const bool need2 = (__ballot(need) >> c2) & 1ull;
+                if (h2 == h && c2 < c) first = false;
+                if (h2 == h && c2 >= c && need2) scan = true;
+            }
+            const unsigned long long todo = __ballot(first && scan);
+            for (int cs = 0; cs < 27; ++cs) {
+                if (!((todo >> cs) & 1ull)) continue;
+                const unsigned int hb = (unsigned int)lane_i((int)h, cs);
+                const int s = a.cellStart[B.scTableBase + hb];
+                if (s < 0) continue;
+                const int e = a.cellEnd[B.scTableBase + hb];
+                for (int base = s; base < e; base += 64) {
+                    const int t = base + lane;
+                    bool acc = false;
+                    int code = 0, tr = 0;
+                    float pr = 0.0f;
+                    if (t < e) {
                         const float4 pj = a.sortP[t];
                         const bool cross = __float_as_int(pj.w) != oi;
-                        if (!cross && !B.scWithin) continue;
-                        const float4 qj = a.sortR[t];
-                        const float qx = q.x - qj.x, qy = q.y - qj.y, qz = q.z - qj.z;
-                        const float q2 = qx * qx + qy * qy + qz * qz;
-                        float pr;
-                        int code = j;
-                        if (cross) {
-                            const float4 xj = xj4;
-                            const float ex = x.x - xj.x, ey = x.y - xj.y, ez = x.z - xj.z;
-                            const float distX = sqrtf(ex * ex + ey * ey + ez * ez);
-                            const float rx = dix - (pj.x - xj.x), ry = diy - (pj.y - xj.y), rz = diz - (pj.z - xj.z);
-                            const float rel = sqrtf(rx * rx + ry * ry + rz * rz);
-                            if (!(distX < search + rel)) continue;
-                            if (!(q2 >= B.scDc * B.scDc)) continue;
-                            pr = distX - rel;
-                            if (rel > B.scFastRel) code = -(j + 1);
-                        } else {
-                            const float ex = p4.x - pj.x, ey = p4.y - pj.y, ez = p4.z - pj.z;
-                            const float d2 = ex * ex + ey * ey + ez * ez;
-                            if (!(d2 < search2)) continue;
-                            if (!(q2 >= B.scExcl2)) continue;
-                            pr = sqrtf(d2);
+                        if (cross || B.scWithin) {
+                            float d2 = 0.0f, distX = 0.0f, rel = 0.0f;
+                            bool pass;
+                            if (cross) {
+                                const float4 xj = a.sortX[t];
+                                const float ex = x.x - xj.x, ey = x.y - xj.y, ez = x.z - xj.z;
+                                distX = sqrtf(ex * ex + ey * ey + ez * ez);
+                                const float rx = dix - (pj.x - xj.x), ry = diy - (pj.y - xj.y),
+                                            rz = diz - (pj.z - xj.z);
+                                rel = sqrtf(rx * rx + ry * ry + rz * rz);
+                                pass = distX < search + rel;
+                            } else {
+                                const float ex = p4.x - pj.x, ey = p4.y - pj.y, ez = p4.z - pj.z;
+                                d2 = ex * ex + ey * ey + ez * ez;
+                                pass = d2 < search2;
+                            }
+                            if (pass) {
+                                const float4 qj = a.sortR[t];
+                                const int j = __float_as_int(qj.w);
+                                const float qx = q.x - qj.x, qy = q.y - qj.y, qz = q.z - qj.z;
+                                const float q2 = qx * qx + qy * qy + qz * qz;
+                                if (j != i) {
+                                    if (cross) {
+                                        acc = q2 >= B.scDc * B.scDc;
+                                        pr = distX - rel;
+                                        code = (rel > B.scFastRel) ? -(j + 1) : j;
+                                    } else {
+                                        acc = q2 >= B.scExcl2;
+                                        pr = sqrtf(d2);
+                                        code = j;
+                                    }
+                                    tr = cross ? 0 : 1;
+                                }
+                            }
                         }
-                        const int tr = cross ? 0 : 1;
+                    }
+                    unsigned long long am = __ballot(acc);
+                    while (am) {
+                        const int l = __builtin_ctzll(am);
+                        am &= am - 1ull;
+                        const int cc = lane_i(code, l), ct = lane_i(tr, l);
+                        const float cp = lane_fl(pr, l);
                         if (cnt < CAP) {
-                            out[cnt] = code;
-                            prio[cnt * kBuildThreads] = pr;
-                            tier[cnt * kBuildThreads] = tr;
+                            if (lane == cnt) { slotCode = cc; slotTier = ct; slotPrio = cp; }
                             ++cnt;
                         } else {
-                            // replace the worst entry (highest tier, then largest
-                            // priority, first slot among equals); the worst slot
-                            // only changes when an entry is replaced
-                            if (worst < 0) worst = sc_worst(prio, tier);
-                            const int tw = tier[worst * kBuildThreads];
-                            const float pw = prio[worst * kBuildThreads];
-                            if (tr < tw || (tr == tw && pr < pw)) {
-                                out[worst] = code;
-                                prio[worst * kBuildThreads] = pr;
-                                tier[worst * kBuildThreads] = tr;
-                                worst = sc_worst(prio, tier);
+                            // the worst slot only changes when a slot is replaced
+                            if (worst < 0) {
+                                worst = sc_worst_lane(slotTier, slotPrio, lane);
+                                worstTier = lane_i(slotTier, worst);
+                                worstPrio = lane_fl(slotPrio, worst);
+                            }
+                            if (ct < worstTier || (ct == worstTier && cp < worstPrio)) {
+                                if (lane == worst) { slotCode = cc; slotTier = ct; slotPrio = cp; }
+                                worst = -1;
                             }
                             ++overflow;
                         }
                     }
+                }
             }
         }
+        if (lane < cnt) a.contacts[sc_slot(i, lane)] = slotCode;
+        if (lane == 0) a.counts[i] = cnt;
+        candTotal += cnt;
+        overTotal += overflow;
     }
-    a.counts[i] = cnt;
-    if (cnt > 0) atomicAdd(&a.scCounters[3 * b], cnt);
-    if (overflow > 0) atomicAdd(&a.scCounters[3 * b + 1], overflow);
+    if (lane == 0 && candTotal > 0) atomicAdd(&a.scCounters[3 * b], (int)candTotal);
+    if (lane == 0 && overTotal > 0) atomicAdd(&a.scCounters[3 * b + 1], (int)overTotal);
 }
+
+// Candidate lists (forward.cu:7079-7180).
 
 __global__ void sc_fill_accum_kernel(int nPad, float3* accum)
 {
@@ -466,12 +542,12 @@ __global__ void __launch_bounds__(256) sc_solve_kernel(int nPad, SelfColArgs a, 
     float ax = 0.0f, ay = 0.0f, az = 0.0f, cpx = 0.0f, cpy = 0.0f, cpz = 0.0f;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, wFast = 0.0f;
     int act = 0, nFast = 0;
-    const int* list = &a.contacts[(size_t)i * CAP];
+    const int* list = &a.contacts[sc_slot(i, 0)];   // slot k at list[k * kTile]
     const float3 xi = X[i], qi = rest[i];
     if (wi > 0.0f) {
         const float dc2 = dc * dc;
         for (int k = 0; k < cnt; ++k) {
-            const int code = list[k];
+            const int code = list[k * kTile];
             if (code < 0) {
                 const int j = -code - 1;
                 float nx, ny, nz;
@@ -516,7 +592,7 @@ __global__ void __launch_bounds__(256) sc_solve_kernel(int nPad, SelfColArgs a, 
             const float bx = sx / sl, by = sy / sl, bz = sz / sl;
             float delta = 0.0f;
             for (int k = 0; k < cnt; ++k) {
-                const int code = list[k];
+                const int code = list[k * kTile];
                 if (code >= 0) continue;
                 const int j = -code - 1;
                 float nx, ny, nz;
@@ -655,7 +731,8 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
     launch_fill_int(s, a.tableTotal, a.cellStart, -1);
     sc_range_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, ks);
     sc_stage_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
-    sc_build_kernel<<<nbs(na.nPad, kBuildThreads), kBuildThreads, 0, s>>>(na.nPad, a, X, P, rest);
+    const int waves = nbs(na.nPad, kBuildNodesPerWave);
+    sc_build_kernel<<<nbs(waves, 4), 256, 0, s>>>(na.nPad, a, X, P, rest);
     sc_fill_accum_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a.accum);
     *sortedIdxOut = vs;
     return inAlt;
