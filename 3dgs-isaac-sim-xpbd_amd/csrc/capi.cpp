@@ -121,64 +121,29 @@ void xpbd_set_solver(int iters, float dt, float underRelax, float velDamping, fl
                      float invMassScale, float distCompliance, float shapeCompliance, float shapeBlend,
                      float angleCompliance, float angleBlend)
 {
-    Settings& s = G().set;
-    s.iters = iters < 1 ? 1 : iters;
-    s.dt = std::fmax(1e-5f, dt);
-    s.underRelax = std::fmax(0.0f, underRelax);
-    s.velDamping = std::fmax(0.0f, velDamping);
-    s.stiffnessScale = std::fmax(0.0f, stiffnessScale);
-    s.invMassScale = std::fmax(0.0f, invMassScale);
-    s.distCompliance = std::fmax(0.0f, distCompliance);
-    s.shapeCompliance = std::fmax(0.0f, shapeCompliance);
-    s.shapeBlend = std::fmin(std::fmax(0.0f, shapeBlend), 1.0f);
-    s.angleCompliance = std::fmax(0.0f, angleCompliance);
-    s.angleBlend = std::fmin(std::fmax(0.0f, angleBlend), 1.0f);
+    settings_solver(G().set, iters, dt, underRelax, velDamping, stiffnessScale, invMassScale, distCompliance,
+                    shapeCompliance, shapeBlend, angleCompliance, angleBlend);
 }
 
 void xpbd_set_constraints(int distance, int shape, int angle, int volume)
 {
-    Settings& s = G().set;
-    s.useDistance = distance != 0;
-    s.useShape = shape != 0;
-    s.useAngle = angle != 0;
-    s.useVolume = volume != 0;
+    settings_constraints(G().set, distance, shape, angle, volume);
 }
 
 void xpbd_set_volume(float compliance, int ringK, int maxMembers, int leaderMinHop)
 {
-    Settings& s = G().set;
-    s.volCompliance = compliance;
-    s.volRingK = std::max(1, std::min(8, ringK));
-    s.volMaxMembers = std::max(8, std::min(8192, maxMembers));
-    s.volLeaderMinHop = std::max(1, std::min(8, leaderMinHop));
-    s.volMemberSelect = 1;
+    settings_volume(G().set, compliance, ringK, maxMembers, leaderMinHop);
 }
 
 void xpbd_set_ground(int enabled, float ux, float uy, float uz, float height, float friction,
                      float restitution, float contactRadius, float contactSlop, float gravity)
 {
-    Settings& s = G().set;
-    float nx = ux, ny = uy, nz = uz;
-    const float l = std::sqrt(nx * nx + ny * ny + nz * nz);
-    if (!(l > 1e-8f) || !std::isfinite(l)) { nx = 0; ny = 0; nz = 1; }
-    else { nx /= l; ny /= l; nz /= l; }
-    s.groundN[0] = nx; s.groundN[1] = ny; s.groundN[2] = nz;
-    s.groundEnabled = enabled != 0;
-    s.groundHeight = std::isfinite(height) ? height : 0.0f;
-    s.groundFriction = std::fmax(0.0f, friction);
-    s.groundRestitution = std::fmin(std::fmax(restitution, 0.0f), 1.0f);
-    s.groundRadius = std::fmax(0.0f, contactRadius);
-    s.groundSlop = std::fmax(0.0f, contactSlop);
-    s.groundGravity = std::fmax(0.0f, gravity);
+    settings_ground(G().set, enabled, ux, uy, uz, height, friction, restitution, contactRadius, contactSlop, gravity);
 }
 
 void xpbd_set_object_shape(float stiffness)
 {
-    Settings& s = G().set;
-    float v = std::fmin(std::fmax(stiffness, 0.0f), 1.0f);
-    if (!std::isfinite(v)) v = 0.0f;
-    s.objShapeStiffness = v;
-    s.objMinComponent = 50;
+    settings_object_shape(G().set, stiffness);
 }
 
 void xpbd_set_object_shape_gpu(int enabled) { G().set.objShapeGPU = enabled != 0; }
@@ -187,24 +152,9 @@ void xpbd_set_colliders(int count, const int* types, const float* poses, const f
                         float friction)
 {
     Global& g = G();
-    if (count < 0) count = 0;
-    if (count > kMaxColliders) {
-        std::printf("[Collider] %d colliders requested, using the first %d\n", count, kMaxColliders);
-        count = kMaxColliders;
-    }
-    const bool samePrev = (count == g.colCount);
-    for (int c = 0; c < count; ++c) {
-        ColliderDev& K = g.cols[c];
-        const bool keep = samePrev && K.type == types[c];
-        for (int a = 0; a < 9; ++a) K.Rp[a] = keep ? K.R[a] : poses[12 * c + a];
-        for (int a = 0; a < 3; ++a) K.tp[a] = keep ? K.t[a] : poses[12 * c + 9 + a];
-        K.type = types[c];
-        for (int a = 0; a < 9; ++a) K.R[a] = poses[12 * c + a];
-        for (int a = 0; a < 3; ++a) { K.t[a] = poses[12 * c + 9 + a]; K.h[a] = std::fmax(dims[3 * c + a], 0.0f); }
-    }
-    g.colCount = count;
-    g.colMargin = (std::isfinite(margin) && margin > 0.0f) ? margin : 0.0f;
-    g.colFriction = (std::isfinite(friction) && friction > 0.0f) ? friction : 0.0f;
+    g.colCount = collider_list_update(g.cols, g.colCount, count, types, poses, dims);
+    g.colMargin = positive_or_zero(margin);
+    g.colFriction = positive_or_zero(friction);
     push_colliders();
 }
 
@@ -253,11 +203,7 @@ int xpbd_set_attached(int count, const int* idx, const float* pos, float weight)
 
 void xpbd_set_self_collision(int enabled, float radiusScale, float excludeScale, int withinBody)
 {
-    Settings& s = G().set;
-    s.scEnabled = enabled != 0;
-    s.scWithinBody = withinBody != 0;
-    s.scRadiusScale = std::fmin(std::fmax(std::isfinite(radiusScale) ? radiusScale : 1.5f, 0.1f), 10.0f);
-    s.scExcludeScale = std::fmin(std::fmax(std::isfinite(excludeScale) ? excludeScale : 2.0f, 1.05f), 20.0f);
+    settings_self_collision(G().set, enabled, radiusScale, excludeScale, withinBody);
 }
 
 void xpbd_self_collision_stats(int* out)

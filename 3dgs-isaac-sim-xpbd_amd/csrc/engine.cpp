@@ -19,16 +19,25 @@ namespace xpbd {
         }                                                                                 \
     } while (0)
 
-// Bumped by every device allocation: a captured graph holds raw pointers, so
-// any reallocation must trigger a recapture.
+// A captured graph holds raw pointers, so any reallocation must trigger a
+// recapture.  Allocations made inside an Engine call bump that engine's own
+// counter (an AllocScope names it for the calling thread); any other
+// allocation bumps the global counter, which every engine's plan includes.
+// Engines that build on one thread while others step on another therefore do
+// not force each other's graphs to be recaptured.
 static std::atomic<int> g_allocEpoch{0};
+static thread_local int* t_allocEpoch = nullptr;
 static int device_alloc_epoch() { return g_allocEpoch.load(); }
+
+AllocScope::AllocScope(int* e) : prev(t_allocEpoch) { t_allocEpoch = e; }
+AllocScope::~AllocScope() { t_allocEpoch = prev; }
 
 template <typename T>
 void DevBuf<T>::alloc(size_t count)
 {
     free();
-    ++g_allocEpoch;
+    if (t_allocEpoch) ++*t_allocEpoch;
+    else ++g_allocEpoch;
     n = count;
     if (count) {
         if (hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count) != hipSuccess) {
@@ -143,6 +152,20 @@ static void morton_order(const BodyInput& in, std::vector<int>& ord)
     for (int i = 0; i < n; ++i) ord[i] = key[i].second;
 }
 
+// A node listed more than once takes its last target, as a serial loop over
+// the list leaves it: earlier repeats become -1 (skipped by the kernel), so
+// the parallel attach writes each node once.
+void keep_last_of_repeats(std::vector<int>& ids)
+{
+    std::vector<std::pair<int, int>> s;
+    s.reserve(ids.size());
+    for (size_t k = 0; k < ids.size(); ++k)
+        if (ids[k] >= 0) s.emplace_back(ids[k], (int)k);
+    std::sort(s.begin(), s.end());
+    for (size_t k = 0; k + 1 < s.size(); ++k)
+        if (s[k].first == s[k + 1].first) ids[s[k].second] = -1;
+}
+
 // Caller-order copy of body b's committed positions after a host-side change.
 void Engine::refreshCallerPositions(int b)
 {
@@ -169,6 +192,7 @@ Engine::~Engine()
 
 bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err)
 {
+    XPBD_ALLOC_SCOPE;
     const int B = (int)inputs.size();
     if (B <= 0) { err = "engine: no bodies"; return false; }
     inputs_ = inputs;
@@ -387,21 +411,9 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
 void Engine::setColliders(int b, int count, const int* types, const float* poses, const float* dims, float margin,
                           float friction)
 {
-    if (count < 0) count = 0;
-    if (count > kMaxColliders) count = kMaxColliders;
-    const bool samePrev = (count == colCount_[b]);
-    for (int c = 0; c < count; ++c) {
-        ColliderDev& K = colHost_[(size_t)b * kMaxColliders + c];
-        const bool keep = samePrev && K.type == types[c];
-        for (int a = 0; a < 9; ++a) K.Rp[a] = keep ? K.R[a] : poses[12 * c + a];
-        for (int a = 0; a < 3; ++a) K.tp[a] = keep ? K.t[a] : poses[12 * c + 9 + a];
-        K.type = types[c];
-        for (int a = 0; a < 9; ++a) K.R[a] = poses[12 * c + a];
-        for (int a = 0; a < 3; ++a) { K.t[a] = poses[12 * c + 9 + a]; K.h[a] = std::fmax(dims[3 * c + a], 0.0f); }
-    }
-    colCount_[b] = count;
-    colMargin_[b] = (std::isfinite(margin) && margin > 0.0f) ? margin : 0.0f;
-    colFriction_[b] = (std::isfinite(friction) && friction > 0.0f) ? friction : 0.0f;
+    colCount_[b] = collider_list_update(&colHost_[(size_t)b * kMaxColliders], colCount_[b], count, types, poses, dims);
+    colMargin_[b] = positive_or_zero(margin);
+    colFriction_[b] = positive_or_zero(friction);
     colDirty_ = true;
 }
 
@@ -443,6 +455,7 @@ int Engine::colliderStats(int b, int* hits, float* push, float* centroid, int ma
 // setAttachedParticles (forward.cu:6759-6788)
 bool Engine::attach(int b, int count, const int* idx, const float* pos, float weight)
 {
+    XPBD_ALLOC_SCOPE;
     const int nb = bodyBegin_[b], nn = bodyCount_[b];
     if (attActive_[b])
         (void)hipMemcpyAsync(invMass_.p + nb, invMassSaved_.p + nb, sizeof(float) * nn, hipMemcpyDeviceToDevice, stream_);
@@ -461,6 +474,7 @@ bool Engine::attach(int b, int count, const int* idx, const float* pos, float we
     std::vector<int> dIdx(idx, idx + count);
     for (int& k : dIdx)
         if (k >= 0 && k < nn) k = inv_[b][k];
+    keep_last_of_repeats(dIdx);
     (void)hipMemcpyAsync(attIdx_.p, dIdx.data(), sizeof(int) * count, hipMemcpyHostToDevice, stream_);
     (void)hipMemcpyAsync(attPos_.p, pos, sizeof(float) * 3 * count, hipMemcpyHostToDevice, stream_);
     launch_attach(stream_, count, attIdx_.p, attPos_.p, nb, nb + nn, x_.p, v_.p, invMass_.p, objW_.p,
@@ -606,6 +620,7 @@ void Engine::fillBodyDev(int b, BodyDev& d) const
     d.volGSOn = (s.useVolume && s.volGS && vgsDevValid_ && b < (int)vgsIn_.size() && vgsIn_[b]) ? 1 : 0;
     d.volumeOn = (s.useVolume && !d.volGSOn) ? 1 : 0;
     d.volCompliance = s.volCompliance;
+    d.volMinMembers = volTopo_[b].minMembers;
     d.shapeOn = s.useShape ? 1 : 0;
     d.shapeClusters = (shcDevValid_ && !shapeClusterOf_[b].empty()) ? 1 : 0;
     d.shapeRobust = s.shapeRobust ? 1 : 0;
@@ -699,12 +714,15 @@ std::vector<int> Engine::planSignature() const
         anyShape |= set_[b].useShape ? 1 : 0;
         anyCGround |= (cM_.p && contactGround_[b] && set_[b].groundEnabled) ? 1 : 0;
     }
-    const int volWave = (anyVol && volAvgMembers_ >= kVolWaveMembers) ? 1 : 0;
+    const int volWave = anyVol ? (tune_.volMode >= 0 ? (tune_.volMode ? 1 : 0) : (volAvgMembers_ >= kVolWaveMembers ? 1 : 0))
+                               : 0;
     const int nDistGS = (anyDistGS && dgsDevValid_) ? (int)dgsColorOff_.size() - 1 : 0;
     const int nVolGS = (anyVolGS && vgsDevValid_) ? (int)vgsColorOff_.size() - 1 : 0;
     return {iters, anyDist, anyCol, anyGround, anyPress, nPad_, anyVol, volWave, anyVol ? volActive_ : 0, anyShape,
             anyCGround, anyObj, anyBoxes, anySc, anyObj ? objTotal_ : 0, anySc ? scTableTotal_ : 0,
-            anySc ? scSortBits_ : 0, nDistGS, nVolGS, device_alloc_epoch(), (anyShape && shcDevValid_) ? shcK_ : 0};
+            anySc ? scSortBits_ : 0, nDistGS, nVolGS, device_alloc_epoch(), (anyShape && shcDevValid_) ? shcK_ : 0,
+            allocEpoch_, tune_.jacobiTiles, tune_.volMode, tune_.shapeClusterWaves, tune_.scBuildWaves,
+            tune_.scBuildNodes};
 }
 
 // Event pairs around kernel launches (profile mode only).
@@ -767,7 +785,7 @@ void Engine::recordSubstep(Timer* t)
         XPBD_T_BEGIN("selfcol_build");
         const int* sorted = nullptr;
         const bool inAlt = launch_selfcol_build(stream_, na, numBodies(), sa, x_.p, p0_.p, rest_.p, scSortBits_,
-                                                scKeysAlt_.p, scIdxAlt_.p, scHist_.p, &sorted);
+                                                scKeysAlt_.p, scIdxAlt_.p, scHist_.p, &sorted, tune_);
         XPBD_T_END();
         scSortedIdx_ = sorted;
         scSortedKeys_ = inAlt ? scKeysAlt_.p : scKeys_.p;
@@ -818,7 +836,7 @@ void Engine::recordSubstep(Timer* t)
             XPBD_T_BEGIN(it == 0 ? (last ? "distance_jacobi_only" : "distance_jacobi_first")
                                  : (last ? "distance_jacobi_last" : "distance_jacobi_mid"));
             launch_distance_jacobi(stream_, na, it == 0, last, pin, pout, sellArgs(),
-                                   lastPass == kJacobi ? preEpi : 0u, ea);
+                                   lastPass == kJacobi ? preEpi : 0u, ea, tune_);
             XPBD_T_END();
             std::swap(pin, pout);
         }
@@ -844,7 +862,7 @@ void Engine::recordSubstep(Timer* t)
             XPBD_T_BEGIN("shape_match");
             ShapeClusterArgs sca = shcArgs();
             if (!sig[20]) sca.K = 0;
-            launch_shape_match(stream_, na, pin, pout, rest_.p, rowOff_.p, nbr_.p, sca, preEpi, ea);
+            launch_shape_match(stream_, na, pin, pout, rest_.p, rowOff_.p, nbr_.p, sca, preEpi, ea, tune_);
             XPBD_T_END();
             std::swap(pin, pout);
         }
@@ -886,9 +904,11 @@ bool Engine::rebuildGraph(std::string& err)
     return true;
 }
 
-bool Engine::prepare(std::string& err)
+bool Engine::prepare(std::string& err, bool advance)
 {
-    for (int b = 0; b < numBodies(); ++b) pressAdvance(b);
+    XPBD_ALLOC_SCOPE;
+    if (advance)
+        for (int b = 0; b < numBodies(); ++b) pressAdvance(b);
     bool anyObj = false, anySc = false;
     for (int b = 0; b < numBodies(); ++b) {
         anyObj |= set_[b].objShapeStiffness > 0.0f || set_[b].scEnabled;
@@ -917,8 +937,23 @@ void Engine::afterSubstep()
         if (hostBodies_[b].objOn) objShiftValid_[b] = 1;
 }
 
+bool Engine::warm(std::string& err)
+{
+    XPBD_ALLOC_SCOPE;
+    for (int b = 1; b < numBodies(); ++b)
+        if (set_[b].iters != set_[0].iters) { err = "engine: every body of one batch needs the same iteration count"; return false; }
+    if (!prepare(err, false)) return false;
+    if (!graphExec_ || planSignature() != graphSig_) {
+        if (!rebuildGraph(err)) return false;
+    }
+    return true;
+}
+
 bool Engine::step(int substeps, std::string& err)
 {
+    XPBD_ALLOC_SCOPE;
+    for (int b = 1; b < numBodies(); ++b)
+        if (set_[b].iters != set_[0].iters) { err = "engine: every body of one batch needs the same iteration count"; return false; }
     for (int s = 0; s < substeps; ++s) {
         if (!prepare(err)) return false;
         if (!graphExec_ || planSignature() != graphSig_) {
@@ -933,6 +968,7 @@ bool Engine::step(int substeps, std::string& err)
 
 bool Engine::profile(int substeps, std::string& json, std::string& err)
 {
+    XPBD_ALLOC_SCOPE;
     Timer t;
     t.s = stream_;
     for (int s = 0; s < substeps; ++s) {

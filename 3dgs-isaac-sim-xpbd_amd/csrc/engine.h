@@ -42,6 +42,23 @@ struct Settings {
     bool distGS = false, volGS = false;   // forward.cu:310-313 (FORWARD::set*GSEnabled)
 };
 
+// Setter clamps of the reference (settings.cpp), shared by both C APIs.
+void settings_solver(Settings& s, int iters, float dt, float underRelax, float velDamping, float stiffnessScale,
+                     float invMassScale, float distCompliance, float shapeCompliance, float shapeBlend,
+                     float angleCompliance, float angleBlend);
+void settings_constraints(Settings& s, int distance, int shape, int angle, int volume);
+void settings_volume(Settings& s, float compliance, int ringK, int maxMembers, int leaderMinHop);
+void settings_ground(Settings& s, int enabled, float ux, float uy, float uz, float height, float friction,
+                     float restitution, float contactRadius, float contactSlop, float gravity);
+void settings_object_shape(Settings& s, float stiffness);
+void settings_self_collision(Settings& s, int enabled, float radiusScale, float excludeScale, int withinBody);
+// Updates cols[0 .. count) in place; returns the clamped count.
+int collider_list_update(ColliderDev* cols, int prevCount, int count, const int* types, const float* poses,
+                         const float* dims);
+float positive_or_zero(float v);
+// Attachment lists: earlier repeats of a node id become -1 (the last wins).
+void keep_last_of_repeats(std::vector<int>& ids);
+
 // One body's input graph in the reference's loadGraph form (local ids).
 struct BodyInput {
     int n = 0;
@@ -95,6 +112,17 @@ struct PressState {
     float ramp = 0.15f, maxDisp = 0.30f, curDisp = 0.0f, lo = 0.0f, hi = 0.0f;
 };
 
+// While alive, device allocations made by this thread bump *e (an engine's
+// allocation counter) instead of the process-wide one.
+struct AllocScope {
+    int* prev;
+    explicit AllocScope(int* e);
+    ~AllocScope();
+    AllocScope(const AllocScope&) = delete;
+    AllocScope& operator=(const AllocScope&) = delete;
+};
+#define XPBD_ALLOC_SCOPE ::xpbd::AllocScope allocScope_(&allocEpoch_)
+
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
@@ -142,9 +170,40 @@ public:
     void launchFromRest(int b, const float* lin, const float* ang, const float* tilt);
     void releaseInvMass(int b);   // invMass = 1 on the body (squashStop)
 
-    // Advance every body by `substeps` substeps (one graph replay each).
+    // Advance every body by `substeps` substeps (one graph replay each).  The
+    // launches are queued on the engine stream; sync() waits for them.  Every
+    // body of one engine runs the same iteration count.
     bool step(int substeps, std::string& err);
     bool sync(std::string& err);
+    // All host-side set-up of the next substep (topologies, colourings, device
+    // buffers, parameter upload, graph capture) without launching it.
+    bool warm(std::string& err);
+
+    // Copy the dynamic state of body sb of `src` into body b of this engine
+    // (both built from the same input with the same node order): positions,
+    // velocities, inverse masses (live and saved), attachment weights and
+    // flags, object-shape shifts, collider poses, press state and settings.
+    // A body continues bit for bit where it left off.
+    bool copyBodyFrom(int b, Engine& src, int sb, std::string& err);
+
+    // Attach several bodies at once: request r pins idx[off[r] .. off[r+1])
+    // (caller ids of body bodies[r]) to pos; a request with no nodes releases.
+    bool attachMany(int count, const int* bodies, const int* off, const int* idx, const float* pos, float weight,
+                    std::string& err);
+
+    // Deformed shapes of every body into device memory (device node order;
+    // callerPosDevice maps a device node to its caller-order global index).
+    bool computeShapesAll(float deformEps, std::string& err);
+    const float* shapeScalesDevice() const { return outScale_.p; }
+    const float* shapeQuatsDevice() const { return outQuat_.p; }
+    const int* callerPosDevice() const { return callerPos_.p; }
+
+    // Launch shapes (kernels.h Tuning).  autotune() times each candidate on
+    // `substeps` substeps from the current state, restores the state after
+    // each trial and keeps the fastest; results do not depend on the choice.
+    Tuning& tuning() { return tune_; }
+    // flags bit 0 also tunes the self-collision hash-table sizes.
+    bool autotune(int substeps, int flags, std::string& report, std::string& err);
     // Run `substeps` substeps eagerly with HIP events around every kernel on the
     // engine stream; returns JSON {"label": [total_ms, launches], ...}.
     bool profile(int substeps, std::string& json, std::string& err);
@@ -216,7 +275,9 @@ private:
     bool ensureSelfCol(std::string& err);
     SelfColArgs scArgs() const;
     bool ensureVolume(bool needAll, std::string& err);
-    bool prepare(std::string& err);   // per-substep host work before launching
+    bool prepare(std::string& err, bool advance = true);   // per-substep host work before launching
+    Tuning tune_;
+    int allocEpoch_ = 0;              // device allocations made by this engine
     void afterSubstep();              // host bookkeeping after a substep is queued
     void pressAdvance(int b);
     void fillBodyDev(int b, BodyDev& d) const;

@@ -228,6 +228,7 @@ bool Engine::ensureVolumeGS(std::string& err)
 
 bool Engine::gsInfo(int b, int info[6], std::string& err)
 {
+    XPBD_ALLOC_SCOPE;
     // both colourings exist whether or not GS is enabled, like the reference's
     // (forward.cu:10834, 10245); the volume one while the constraint is on
     if (!dgs_[b].built) build_distance_gs(*inputs_[b], dgs_[b]);
@@ -250,6 +251,7 @@ bool Engine::gsInfo(int b, int info[6], std::string& err)
 
 bool Engine::distanceGSEdges(int b, int* ab, float* rest, float* stiff, int* colorOff, std::string&)
 {
+    XPBD_ALLOC_SCOPE;
     if (!dgs_[b].built) build_distance_gs(*inputs_[b], dgs_[b]);
     const DistanceGSTopo& t = dgs_[b];
     for (size_t e = 0; e < t.a.size(); ++e) {
@@ -263,6 +265,7 @@ bool Engine::distanceGSEdges(int b, int* ab, float* rest, float* stiff, int* col
 
 bool Engine::volumeGSOrder(int b, int* order, int* colorOff, std::string& err)
 {
+    XPBD_ALLOC_SCOPE;
     if (!set_[b].useVolume) return true;
     if (!ensureVolume(false, err)) return false;
     if (!vgs_[b].built || vgs_[b].version != volTopo_[b].version)

@@ -265,6 +265,7 @@ static const float kScSearchScaleHost = 1.5f;   // forward.cu:230
 
 bool Engine::selfCollisionGridInfo(int b, float info[3], std::string& err)
 {
+    XPBD_ALLOC_SCOPE;
     info[0] = info[1] = info[2] = 0.0f;
     if (!scDeviceValid_) return true;
     float inv = 0.0f;
@@ -278,6 +279,7 @@ bool Engine::selfCollisionGridInfo(int b, float info[3], std::string& err)
 
 bool Engine::selfCollisionStats(int b, int out[3], std::string& err)
 {
+    XPBD_ALLOC_SCOPE;
     out[0] = out[1] = out[2] = 0;
     if (!scDeviceValid_ || !set_[b].scEnabled) return true;
     int c[3] = {0, 0, 0}, use = 0;
@@ -295,6 +297,7 @@ bool Engine::selfCollisionStats(int b, int out[3], std::string& err)
 // and contact lists [n x 32] with local ids.
 bool Engine::selfCollisionLists(int b, unsigned int* keys, int* sorted, int* counts, int* contacts, std::string& err)
 {
+    XPBD_ALLOC_SCOPE;
     if (!scDeviceValid_ || !scSortedIdx_) { err = "no self-collision build yet"; return false; }
     const int nb = bodyBegin_[b], n = bodyCount_[b];
     std::vector<unsigned int> hk(nPad_), hks(nPad_);

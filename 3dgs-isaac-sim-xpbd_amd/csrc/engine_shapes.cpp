@@ -51,6 +51,7 @@ static float cap_or_zero(float c) { return (std::isfinite(c) && c > 0.0f) ? c : 
 
 bool Engine::setContactShape(int b, int enabled, float tau, float radiusCap, std::string& err)
 {
+    XPBD_ALLOC_SCOPE;
     if (!ensureShapeBuffers(err)) return false;
     contactOn_[b] = enabled != 0;
     contactTau_[b] = tau;
@@ -78,6 +79,7 @@ bool Engine::setContactShape(int b, int enabled, float tau, float radiusCap, std
 
 bool Engine::computeShapes(int b, float deformEps, float* hostScales, float* hostQuats, int& deformed, std::string& err)
 {
+    XPBD_ALLOC_SCOPE;
     if (!ensureShapeBuffers(err)) return false;
     const size_t nb = bodyBegin_[b];
     const int n = bodyCount_[b];

@@ -87,11 +87,6 @@ bool Engine::ensureVolume(bool needAll, std::string& err)
 
     // concatenate into the padded node space (bodies without clusters get empty rows)
     std::vector<int> hOff(nPad_ + 1, 0), hRevOff(nPad_ + 1, 0), hIdx, hRev;
-    int minMembers = 4;
-    for (int b = 0; b < B; ++b) {
-        const VolumeTopo& t = volTopo_[b];
-        if (!t.memOff.empty()) minMembers = std::min(minMembers, t.minMembers);
-    }
     for (int b = 0; b < B; ++b) {
         const VolumeTopo& t = volTopo_[b];
         const int nb = bodyBegin_[b], n = bodyCount_[b];
@@ -111,10 +106,6 @@ bool Engine::ensureVolume(bool needAll, std::string& err)
         for (int i = nb + n; i <= padEnd && i <= nPad_; ++i) {
             hOff[i] = (int)hIdx.size();
             hRevOff[i] = (int)hRev.size();
-        }
-        if (t.explicitClusters && t.minMembers != minMembers) {
-            err = "volume clusters: bodies in one batch must share the member gate";
-            return false;
         }
     }
     hOff[nPad_] = (int)hIdx.size();
@@ -145,7 +136,7 @@ bool Engine::ensureVolume(bool needAll, std::string& err)
     if (!uploadParams(perr)) { err = perr; return false; }
     const NodeArgs na{nPad_, tileBody_.p, bodies_.p};
     VolumeArgs va = volArgs();
-    launch_vol_precompute(stream_, na, rest_.p, rowOff_.p, nbr_.p, va, minMembers, volAniso_);
+    launch_vol_precompute(stream_, na, rest_.p, rowOff_.p, nbr_.p, va, volAniso_);
     hMatType_.resize(nPad_);
     hVrest_.resize(nPad_);
     XPBD_HIP(hipMemcpyAsync(hMatType_.data(), matType_.p, sizeof(int) * nPad_, hipMemcpyDeviceToHost, stream_));
@@ -172,6 +163,7 @@ bool Engine::ensureVolume(bool needAll, std::string& err)
 // Volume-weighted J of the committed positions (forward.cu:12658-12700).
 bool Engine::volumeStats(int b, float& jvw, float& jvwStd, std::string& err)
 {
+    XPBD_ALLOC_SCOPE;
     if (!ensureVolume(true, err)) return false;
     const NodeArgs na{nPad_, tileBody_.p, bodies_.p};
     VolumeArgs va = volArgs();
@@ -200,6 +192,7 @@ bool Engine::volumeStats(int b, float& jvw, float& jvwStd, std::string& err)
 bool Engine::volumeTopology(int b, int* leaders, int* memberCount, int* revCount, int* matType, float* detRest,
                             std::string& err)
 {
+    XPBD_ALLOC_SCOPE;
     if (!ensureVolume(true, err)) return false;
     const VolumeTopo& t = volTopo_[b];
     const int nb = bodyBegin_[b], n = bodyCount_[b];

@@ -9,6 +9,17 @@ namespace xpbd {
 // Grid size for kernels that remap blocks per XCD (a multiple of 8).
 inline int xcd_grid(int blocks) { return (blocks + 7) & ~7; }
 
+// Launch shapes the autotuner may change.  None of them changes a result:
+// each only regroups independent waves into blocks or picks between two
+// kernels that sum in the same order.
+struct Tuning {
+    int jacobiTiles = 4;        // 64-node tiles (one wave each) per distance Jacobi block: 1, 2, 4
+    int volMode = -1;           // volume cluster solve: -1 by mean member count, 0 thread, 1 wave per cluster
+    int shapeClusterWaves = 4;  // clusters (one wave each) per cluster shape-matching block: 1, 2, 4
+    int scBuildWaves = 4;       // waves per self-collision candidate-build block: 1, 2, 4
+    int scBuildNodes = 8;       // nodes per wave of the candidate build: 4, 8, 16
+};
+
 // Node-space launch description shared by every per-node kernel.
 struct NodeArgs {
     int nPad;                 // padded node count (multiple of kTile)
@@ -60,7 +71,7 @@ struct VolumeArgs {
 };
 
 void launch_vol_precompute(hipStream_t s, const NodeArgs& na, const float3* rest, const int* rowOff, const int* nbr,
-                           const VolumeArgs& va, int minMembers, float aniso);
+                           const VolumeArgs& va, float aniso);
 void launch_vol_solve(hipStream_t s, const NodeArgs& na, bool wave, bool first, bool last, const float4* P,
                       const VolumeArgs& va);
 void launch_vol_apply(hipStream_t s, const NodeArgs& na, const float4* Pin, float4* Pout, const VolumeArgs& va,
@@ -126,7 +137,7 @@ void launch_object_shape(hipStream_t s, const NodeArgs& na, const SelfColArgs& a
 // Returns true when the sorted (keys, idx) ended in the alternate buffers.
 bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, const SelfColArgs& a, const float3* X,
                           const float4* P, const float3* rest, int sortBits, unsigned int* keysAlt, int* idxAlt,
-                          unsigned int* hist, const int** sortedIdxOut);
+                          unsigned int* hist, const int** sortedIdxOut, const Tuning& tu);
 void launch_selfcol_iteration(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, const float4* Pin,
                               float4* Pout, const float3* X, const float3* rest, unsigned epi, const EpiArgs& ea);
 void launch_contact_push(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, int K, float4* P, const EpiArgs& ea);
@@ -148,7 +159,7 @@ struct ShapeClusterArgs {
 
 void launch_shape_match(hipStream_t s, const NodeArgs& na, const float4* Pin, float4* Pout, const float3* rest,
                         const int* rowOff, const int* nbr, const ShapeClusterArgs& sca, unsigned epi,
-                        const EpiArgs& ea);
+                        const EpiArgs& ea, const Tuning& tu);
 // scales / rots / outScale / outQuat are the body's own arrays (local index);
 // node arrays are indexed from nodeBegin.
 void launch_deformed_shapes(hipStream_t s, int nodeBegin, int n, const float3* rest, const float3* cur,
@@ -160,7 +171,7 @@ void launch_contact_ground(hipStream_t s, const NodeArgs& na, const float* M, fl
 
 void launch_predict(hipStream_t s, const NodeArgs& na, const float3* X, float3* V, const float* invMass, float4* P);
 void launch_distance_jacobi(hipStream_t s, const NodeArgs& na, bool first, bool last, const float4* Pin, float4* Pout,
-                            const SellArgs& sa, unsigned epi, const EpiArgs& ea);
+                            const SellArgs& sa, unsigned epi, const EpiArgs& ea, const Tuning& tu);
 void launch_epilogue(hipStream_t s, const NodeArgs& na, float4* P, unsigned epi, const EpiArgs& ea);
 void launch_distance_alpha(hipStream_t s, const NodeArgs& na, const int* rowOff, const float* nstiff, const SellArgs& sa);
 void launch_attach(hipStream_t s, int count, const int* idx, const float* pos, int nodeBegin, int nodeEnd, float3* X,

@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(256) distance_jacobi_kernel(
     int nTiles, const int* __restrict__ tileBody, const BodyDev* __restrict__ bodies,
     const float4* __restrict__ Pin, float4* __restrict__ Pout, SellArgs sa, unsigned epi, EpiArgs ea)
 {
-    const int tile = xcd_block() * 4 + (threadIdx.x >> 6);
+    const int tile = xcd_block() * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
     if (tile >= nTiles) return;
     const int lane = threadIdx.x & 63;
     const int i = tile * kTile + lane;
@@ -229,10 +229,11 @@ void launch_predict(hipStream_t s, const NodeArgs& na, const float3* X, float3* 
 }
 
 void launch_distance_jacobi(hipStream_t s, const NodeArgs& na, bool first, bool last, const float4* Pin, float4* Pout,
-                            const SellArgs& sa, unsigned epi, const EpiArgs& ea)
+                            const SellArgs& sa, unsigned epi, const EpiArgs& ea, const Tuning& tu)
 {
     const int tiles = na.nPad / kTile;
-    const dim3 g(xcd_grid(blocks_for(tiles, 4))), b(256);
+    const int per = (tu.jacobiTiles == 1 || tu.jacobiTiles == 2) ? tu.jacobiTiles : 4;
+    const dim3 g(xcd_grid(blocks_for(tiles, per))), b(64 * per);
     if (first && last)
         distance_jacobi_kernel<true, true><<<g, b, 0, s>>>(tiles, na.tileBody, na.bodies, Pin, Pout, sa, epi, ea);
     else if (first)

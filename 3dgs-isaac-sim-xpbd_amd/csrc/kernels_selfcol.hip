@@ -353,21 +353,22 @@ __device__ __forceinline__ int sc_worst_lane(int tier, float prio, int lane)
 // Candidate lists (forward.cu:7079-7180), one wave64 per node at a time.  The
 // lanes test the candidates of a hash bucket 64 at a time; the accepted ones
 // are inserted in lane order (= the reference's scan order) into the 32 slots
-// held by lanes 0-31, replacing the worst slot once the list is full.
-constexpr int kBuildNodesPerWave = 8;
-
+// held by lanes 0-31, replacing the worst slot once the list is full.  Each
+// wave takes NPW consecutive nodes (a divisor of the 64-node tile, so a wave
+// never spans two bodies).
+template <int NPW>
 __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
                                                       const float4* __restrict__ P, const float3* __restrict__ rest)
 {
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     constexpr int CAP = kSelfColCap;
-    const int i0 = wave * kBuildNodesPerWave;
+    const int i0 = wave * NPW;
     if (i0 >= nPad) return;   // whole waves
     const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i0 / kTile]);
     const BodyDev& B = a.bodies[b];
     long long candTotal = 0, overTotal = 0;
-    for (int ii = 0; ii < kBuildNodesPerWave; ++ii) {
+    for (int ii = 0; ii < NPW; ++ii) {
         const int i = i0 + ii;
         if (!B.scOn || i >= B.nodeEnd) break;
         const float3 x = X[i];
@@ -719,7 +720,7 @@ void launch_object_shape(hipStream_t s, const NodeArgs& na, const SelfColArgs& a
 
 bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, const SelfColArgs& a, const float3* X,
                           const float4* P, const float3* rest, int sortBits, unsigned int* keysAlt, int* idxAlt,
-                          unsigned int* hist, const int** sortedIdxOut)
+                          unsigned int* hist, const int** sortedIdxOut, const Tuning& tu)
 {
     sc_prepare_kernel<<<nbs(numBodies, 64), 64, 0, s>>>(numBodies, a);
     sc_active_disp_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P);
@@ -731,8 +732,13 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
     launch_fill_int(s, a.tableTotal, a.cellStart, -1);
     sc_range_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, ks);
     sc_stage_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
-    const int waves = nbs(na.nPad, kBuildNodesPerWave);
-    sc_build_kernel<<<nbs(waves, 4), 256, 0, s>>>(na.nPad, a, X, P, rest);
+    const int wpb = (tu.scBuildWaves == 1 || tu.scBuildWaves == 2) ? tu.scBuildWaves : 4;
+    if (tu.scBuildNodes == 4)
+        sc_build_kernel<4><<<nbs(nbs(na.nPad, 4), wpb), 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest);
+    else if (tu.scBuildNodes == 16)
+        sc_build_kernel<16><<<nbs(nbs(na.nPad, 16), wpb), 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest);
+    else
+        sc_build_kernel<8><<<nbs(nbs(na.nPad, 8), wpb), 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest);
     sc_fill_accum_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a.accum);
     *sortedIdxOut = vs;
     return inAlt;

@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(256) shape_cluster_kernel(int K, ShapeClusterA
                                                            const int* __restrict__ tileBody,
                                                            const BodyDev* __restrict__ bodies)
 {
-    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.x * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
     if (k >= K) return;   // whole waves
     const int lane = threadIdx.x & 63;
     const int b0 = a.off[k], cnt = a.off[k + 1] - b0;
@@ -465,10 +465,11 @@ static inline int nbk(int n, int bs) { return (n + bs - 1) / bs; }
 
 void launch_shape_match(hipStream_t s, const NodeArgs& na, const float4* Pin, float4* Pout, const float3* rest,
                         const int* rowOff, const int* nbr, const ShapeClusterArgs& sca, unsigned epi,
-                        const EpiArgs& ea)
+                        const EpiArgs& ea, const Tuning& tu)
 {
+    const int per = (tu.shapeClusterWaves == 1 || tu.shapeClusterWaves == 2) ? tu.shapeClusterWaves : 4;
     if (sca.K > 0)
-        shape_cluster_kernel<<<nbk(sca.K, 4), 256, 0, s>>>(sca.K, sca, Pin, rest, na.tileBody, na.bodies);
+        shape_cluster_kernel<<<nbk(sca.K, per), 64 * per, 0, s>>>(sca.K, sca, Pin, rest, na.tileBody, na.bodies);
     shape_match_kernel<<<nbk(na.nPad, 256), 256, 0, s>>>(na.nPad, na.tileBody, na.bodies, Pin, Pout, rest, rowOff,
                                                         nbr, sca, epi, ea);
 }

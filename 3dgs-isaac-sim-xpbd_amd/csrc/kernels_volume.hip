@@ -30,7 +30,7 @@ __device__ __forceinline__ const BodyDev& body_of(const int* __restrict__ tileBo
 __global__ void vol_precompute_kernel(int nPad, const int* __restrict__ tileBody, const BodyDev* __restrict__ bodies,
                                       const float3* __restrict__ rest, const int* __restrict__ rowOff,
                                       const int* __restrict__ nbr, const int* __restrict__ volOff,
-                                      const int* __restrict__ volIdx, int minMembers, float aniso,
+                                      const int* __restrict__ volIdx, float aniso,
                                       float* __restrict__ detRest, float* __restrict__ Vrest,
                                       float* __restrict__ restLen, int* __restrict__ matType)
 {
@@ -54,7 +54,7 @@ __global__ void vol_precompute_kernel(int nPad, const int* __restrict__ tileBody
     }
     restLen[idx] = (dmin < 3.402823466e+38f) ? dmin : 0.0f;
     const int off = volOff[idx], cnt = volOff[idx + 1] - off;
-    if (cnt < minMembers) return;
+    if (cnt < B.volMinMembers) return;
     float cx = p0.x, cy = p0.y, cz = p0.z;
     for (int k = 0; k < cnt; ++k) {
         const float3 q = rest[volIdx[off + k]];
@@ -373,10 +373,10 @@ __global__ void vol_J_kernel(int nPad, const int* __restrict__ tileBody, const B
 static inline int nb(int n, int bs) { return (n + bs - 1) / bs; }
 
 void launch_vol_precompute(hipStream_t s, const NodeArgs& na, const float3* rest, const int* rowOff, const int* nbr,
-                           const VolumeArgs& va, int minMembers, float aniso)
+                           const VolumeArgs& va, float aniso)
 {
     vol_precompute_kernel<<<nb(na.nPad, 256), 256, 0, s>>>(na.nPad, na.tileBody, na.bodies, rest, rowOff, nbr,
-                                                          va.volOff, va.volIdx, minMembers, aniso, va.detRest,
+                                                          va.volOff, va.volIdx, aniso, va.detRest,
                                                           va.Vrest, va.restLen, va.matType);
 }
 

@@ -101,7 +101,7 @@ void ball_members(const BodyInput& in, int i, int k, int cap, BallScratch& w, st
 
 void VolumeTopo::finish(int n)
 {
-    static long long versions = 0;
+    static std::atomic<long long> versions{0};   // engines may build on several threads
     version = ++versions;
     revOff.assign(n + 1, 0);
     std::vector<int> cnt(n, 0);

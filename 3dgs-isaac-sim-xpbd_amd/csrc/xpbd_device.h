@@ -51,6 +51,7 @@ struct BodyDev {
     // volume constraint (forward.cu:8531-9243)
     int volumeOn;
     float volCompliance;
+    int volMinMembers;         // rest-precompute gate of the body's clusters (forward.cu:7745)
     // per-node shape matching (forward.cu:6088-6383)
     int shapeOn, shapeRobust;
     int shapeClusters;         // node clusters replace the 1-ring (extension)

@@ -161,6 +161,54 @@ def scene_extent(scene):
 
 
 # ----------------------------------------------------------------------------
+# Many-object workloads
+# ----------------------------------------------------------------------------
+
+def log_uniform_sizes(count, nmin, nmax, rng):
+    return np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=count)).astype(np.int64)
+
+
+def object_request(n, seed, k=12, cluster_size=64, semi_range=(0.05, 0.30), table=None):
+    """One object of the many-object workloads: an ellipsoid of n Gaussians with
+    semi-axes uniform in semi_range (seeded), the full recipe (table: hash
+    cells, default nextpow2(2n) as the reference sizes it) and its own gripper
+    pair."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    semi = rng.uniform(semi_range[0], semi_range[1], size=3)
+    scene = make_ellipsoid_scene(int(n), k, semi=tuple(semi), seed=seed)
+    recipe = full_recipe(scene, cluster_size=cluster_size, seed=seed)
+    if table is None:
+        t = 1024
+        while t < 2 * int(n):
+            t <<= 1
+        table = t
+    recipe["table"] = table
+    script = GripperScript(scene, seed=seed + 1, attach=min(500, max(4, int(n) // 10)))
+    return scene, recipe, script
+
+
+def object_mix(count=64, nmin=5_000, nmax=200_000, seed=11, **kw):
+    """`count` objects with sizes log-uniform in [nmin, nmax] (the batched
+    workload: 64 objects, 5k-200k Gaussians, ~3.4M in total)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = log_uniform_sizes(count, nmin, nmax, rng)
+    return [object_request(int(n), seed * 1000 + j, **kw) for j, n in enumerate(sizes)]
+
+
+def configure_full_batch(batch, obj, recipe, iters=4, substeps=10):
+    """configure_full for object `obj` of an XPBDBatch."""
+    batch.set_solver(obj, iters=iters, dt=1.0 / (60.0 * substeps), under_relax=0.6, vel_damping=0.01,
+                     dist_compliance=1e-6, shape_compliance=0.0, shape_blend=0.1)
+    batch.set_constraints(obj, distance=True, shape=True, angle=False, volume=True)
+    batch.set_volume_clusters(obj, recipe["tet_off"], recipe["tet_mem"], 3)
+    batch.set_shape_clusters(obj, recipe["clusters"])
+    batch.set_ground(obj, True, (0.0, 0.0, 1.0), 0.0, friction=0.5, restitution=0.0, gravity=9.81)
+    batch.set_object_shape(obj, 0.0)
+    batch.set_self_collision_grid(obj, 0.0, recipe["table"])
+    batch.set_self_collision(obj, True, 1.5, 2.0, True)
+
+
+# ----------------------------------------------------------------------------
 # Scripted scene drivers
 # ----------------------------------------------------------------------------
 

@@ -1,0 +1,348 @@
+"""Many-object stepping and the serving loop over the batched API.
+
+``step_frame`` advances every object of one batch by one frame of its own
+gripper script.  ``Server`` is the continuous-batching request loop: requests
+(objects with a lifetime in frames) arrive at any tick, are admitted while the
+HBM budget (and, optionally, a tick-time budget) allows, are built into a new
+batch on a background thread while the live batches keep stepping, join at the
+next tick boundary, and leave when their lifetime is over.  Batches whose
+objects have mostly left are merged: the survivors' state moves into a new
+batch with ``copy_object`` and continues bit for bit.  Every tick advances all
+live objects by one frame; its latency is the wall time from the tick's start
+to every live object's frame (positions and shapes) being done.
+
+The reference runs one object through a global instance and has no serving
+path (native/xpbd_capi.cu:6, 17-29; src/runtime_loop.py:874-982 steps it from
+the simulator callbacks); a plain per-object loop over single-object batches
+stands in for it where the two are compared.
+"""
+import collections
+import threading
+import time
+
+import numpy as np
+
+from batch import XPBDBatch
+import scenes
+
+SUBSTEPS = 10
+ITERS = 4
+
+
+def step_frame(batch, entries, substeps=SUBSTEPS, shapes=True):
+    """Queue one frame of every object of `batch`.  entries[k] is (script,
+    frame) for a live object, None for one that only keeps its slot.  As in
+    the single-object drivers, attachments are set before the first substep,
+    except on an object's first frame, where they follow it (the reference
+    refuses attachments before the first step, native/xpbd_capi.cu:195-200).
+    Returns without waiting for the device (batch.sync() does)."""
+    before, after = [], []
+    for k, e in enumerate(entries):
+        if e is None:
+            continue
+        script, f = e
+        t, p, d = script.colliders(f)
+        batch.set_colliders(k, t, p, d)
+        att = script.attached(f)
+        if att is not None:
+            (after if f == 0 else before).append((k, att[0], att[1]))
+        elif f == script.ramp:
+            before.append((k, (), ()))
+    if before:
+        batch.set_attached(before)
+    if after:
+        batch.step_async(1)
+        batch.set_attached(after)
+        batch.step_async(substeps - 1)
+    else:
+        batch.step_async(substeps)
+    if shapes:
+        batch.shapes_device()
+
+
+def build_batch(requests, lib=None, iters=ITERS, substeps=SUBSTEPS, tuning=None):
+    """A batch of `requests` (objects with .scene and .recipe) configured with
+    the full recipe and prepared (device buffers and graph ready)."""
+    b = XPBDBatch([r.scene for r in requests], lib)
+    for k, r in enumerate(requests):
+        scenes.configure_full_batch(b, k, r.recipe, iters=iters, substeps=substeps)
+    if tuning is not None:
+        b.set_tuning(tuning)
+    b.prepare()
+    return b
+
+
+def freeze(batch, k):
+    """Switch every constraint, contact and force of object k off: a departed
+    object keeps its slot until its batch is merged away, at the cost of a
+    pass over its nodes that does no work."""
+    batch.set_constraints(k, distance=False, shape=False, angle=False, volume=False)
+    batch.set_self_collision(k, False)
+    batch.set_ground(k, False, (0.0, 0.0, 1.0), 0.0, gravity=0.0)
+    batch.set_colliders(k, [], [], [])
+    batch.set_attached([(k, (), ())])
+
+
+class Request:
+    """One object to simulate for `life` frames (scene, recipe, script)."""
+
+    def __init__(self, rid, scene, recipe, script, life, arrival_tick=None):
+        self.id = rid
+        self.scene, self.recipe, self.script = scene, recipe, script
+        self.n = int(scene["pos"].shape[0])
+        self.table = int(recipe["table"])
+        self.life = int(life)
+        self.arrival = arrival_tick
+        self.admitted = None      # tick
+        self.started = None       # tick of its first frame
+        self.frame = 0
+        self.done = None
+
+
+def hbm_bytes(n, table):
+    """Device memory of one object: ~1.1 KB per Gaussian (state, sliced
+    distance entries, volume and shape clusters, self-collision buffers and
+    candidate lists, shapes) plus 8 B per hash cell."""
+    return 1100 * n + 8 * table
+
+
+class _Segment:
+    def __init__(self, batch, objs):
+        self.batch = batch
+        self.objs = list(objs)          # Request per slot
+        self.alive = [True] * len(objs)
+        self.merging = False
+
+    def nodes(self):
+        return sum(r.n for r in self.objs)
+
+    def live_nodes(self):
+        return sum(r.n for r, a in zip(self.objs, self.alive) if a)
+
+
+class Server:
+    """Continuous-batching serving loop (one GPU)."""
+
+    def __init__(self, hbm_budget=200e9, tick_budget_ms=None, max_segments=6, waste_limit=0.35, shapes=True,
+                 record=False, lib=None, tuning=None):
+        self.hbm_budget = hbm_budget
+        self.tick_budget_ms = tick_budget_ms
+        self.max_segments = max_segments
+        self.waste_limit = waste_limit
+        self.shapes = shapes
+        self.record = record
+        self.lib = lib
+        self.tuning = tuning
+        self.queue = collections.deque()
+        self.segments = []
+        self.building = []              # requests inside a pending build
+        self.hbm_used = 0.0
+        self.tick_no = 0
+        self.rate = None                # measured Gaussian-substeps per ms
+        self.trajectories = {}
+        self.finished = []
+        self.log = []
+        self.merges = 0
+        self._jobs = []
+        self._pending_jobs = collections.deque()
+        self._lock = threading.Lock()
+        self._worker = None
+
+    # -- background builds -----------------------------------------------------
+    def _run_jobs(self):
+        while True:
+            with self._lock:
+                if not self._pending_jobs:
+                    self._worker = None
+                    return
+                job = self._pending_jobs.popleft()
+            kind, payload = job
+            try:
+                b = build_batch(payload["objs"], self.lib, tuning=self.tuning)
+                payload["batch"] = b
+            except Exception as e:  # surfaced on the next tick
+                payload["error"] = e
+            with self._lock:
+                self._jobs.append(job)
+
+    def _submit_job(self, kind, payload):
+        with self._lock:
+            self._pending_jobs.append((kind, payload))
+            if self._worker is None:
+                self._worker = threading.Thread(target=self._run_jobs, daemon=True)
+                self._worker.start()
+
+    def _collect_jobs(self):
+        with self._lock:
+            done, self._jobs = self._jobs, []
+        for kind, payload in done:
+            if "error" in payload:
+                raise RuntimeError(f"background {kind} build failed") from payload["error"]
+            b = payload["batch"]
+            if kind == "admit":
+                for r in payload["objs"]:
+                    self.building.remove(r)
+                self.segments.append(_Segment(b, payload["objs"]))
+            else:
+                olds = payload["segments"]
+                seg = _Segment(b, payload["objs"])
+                for k, r in enumerate(payload["objs"]):
+                    src, slot = payload["from"][k]
+                    if src.alive[slot]:
+                        b.copy_object(k, src.batch, slot)
+                    else:
+                        seg.alive[k] = False
+                        freeze(b, k)
+                for o in olds:
+                    self.segments.remove(o)
+                    o.batch.close()
+                self.segments.append(seg)
+
+    # -- admission ---------------------------------------------------------------
+    def submit(self, req):
+        req.arrival = self.tick_no if req.arrival is None else req.arrival
+        self.queue.append(req)
+        self._admit()
+
+    def _live_nodes(self):
+        return sum(s.live_nodes() for s in self.segments) + sum(r.n for r in self.building)
+
+    def _admit(self):
+        take = []
+        while self.queue:
+            r = self.queue[0]
+            need = hbm_bytes(r.n, r.table)
+            if self.hbm_used + need > self.hbm_budget:
+                break
+            if self.tick_budget_ms is not None and self.rate is not None:
+                live = self._live_nodes() + sum(x.n for x in take) + r.n
+                if live * SUBSTEPS / self.rate > 0.9 * self.tick_budget_ms and (self._live_nodes() or take):
+                    break
+            self.queue.popleft()
+            self.hbm_used += need
+            r.admitted = self.tick_no
+            take.append(r)
+        if take:
+            self.building.extend(take)
+            self._submit_job("admit", {"objs": take})
+
+    def _maybe_merge(self):
+        cand = [s for s in self.segments if not s.merging]
+        if not cand:
+            return
+        wasteful = [s for s in cand if s.nodes() and 1.0 - s.live_nodes() / s.nodes() > self.waste_limit]
+        pick = list(wasteful)
+        if len(self.segments) > self.max_segments:
+            small = sorted((s for s in cand if s not in pick), key=lambda s: s.live_nodes())
+            pick += small[:max(2, len(self.segments) - self.max_segments + 1)]
+        pick = [s for s in pick if any(s.alive)] + [s for s in pick if not any(s.alive)]
+        if not pick:
+            return
+        empty = [s for s in pick if not any(s.alive)]
+        for s in empty:                       # nothing to carry over
+            self.segments.remove(s)
+            s.batch.close()
+        pick = [s for s in pick if any(s.alive)]
+        if len(pick) < 2 and not any(s in wasteful for s in pick):
+            return
+        objs, frm = [], []
+        for s in pick:
+            s.merging = True
+            for k, (r, a) in enumerate(zip(s.objs, s.alive)):
+                if a:
+                    objs.append(r)
+                    frm.append((s, k))
+        self.merges += 1
+        self._submit_job("merge", {"objs": objs, "from": frm, "segments": pick})
+
+    # -- one tick ------------------------------------------------------------------
+    def tick(self):
+        t0 = time.perf_counter()
+        self._collect_jobs()
+        live_gs = 0
+        for s in self.segments:
+            entries = []
+            for k, (r, a) in enumerate(zip(s.objs, s.alive)):
+                if not a:
+                    entries.append(None)
+                    continue
+                if r.started is None:
+                    r.started = self.tick_no
+                entries.append((r.script, r.frame))
+                live_gs += r.n
+            step_frame(s.batch, entries, shapes=self.shapes)
+        for s in self.segments:
+            s.batch.sync()
+        t1 = time.perf_counter()
+        ms = (t1 - t0) * 1e3
+        if live_gs:
+            rate = live_gs * SUBSTEPS / ms
+            self.rate = rate if self.rate is None else 0.8 * self.rate + 0.2 * rate
+        # bookkeeping after the results are ready
+        for s in self.segments:
+            for k, (r, a) in enumerate(zip(s.objs, s.alive)):
+                if not a:
+                    continue
+                if self.record:
+                    self.trajectories.setdefault(r.id, []).append(s.batch.positions(k))
+                r.frame += 1
+                if r.frame >= r.life:
+                    r.done = self.tick_no
+                    s.alive[k] = False
+                    freeze(s.batch, k)
+                    self.hbm_used -= hbm_bytes(r.n, r.table)
+                    self.finished.append(r)
+        self._admit()
+        self._maybe_merge()
+        rec = {"tick": self.tick_no, "ms": ms, "live_objects": sum(sum(s.alive) for s in self.segments),
+               "live_gaussians": live_gs, "segments": len(self.segments), "queued": len(self.queue),
+               "building": len(self.building)}
+        self.log.append(rec)
+        self.tick_no += 1
+        return rec
+
+    def _any_alive(self):
+        return any(any(s.alive) for s in self.segments)
+
+    def run(self, requests, max_ticks=100000, on_tick=None):
+        """Submit each request at its arrival tick and tick until all have
+        finished (or max_ticks).  While nothing is live and a build is
+        outstanding the loop waits for it instead of ticking empty."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        while max_ticks > 0 and (pending or self.queue or self.building or self._any_alive()):
+            while pending and pending[0].arrival <= self.tick_no:
+                self.submit(pending.pop(0))
+            if not self._any_alive() and self.building:
+                self._collect_jobs()
+                if not self._any_alive():
+                    time.sleep(0.002)
+                    continue
+            rec = self.tick()
+            if on_tick is not None:
+                on_tick(rec)
+            max_ticks -= 1
+
+    def close(self):
+        while True:
+            with self._lock:
+                busy = self._worker is not None
+            if not busy:
+                break
+            time.sleep(0.01)
+        self._collect_jobs()
+        for s in self.segments:
+            s.batch.close()
+        self.segments = []
+
+
+def run_alone(req, lib=None, shapes=True, tuning=None):
+    """The request simulated alone in a one-object batch, same script and
+    frames: positions after every frame."""
+    b = build_batch([req], lib, tuning=tuning)
+    out = []
+    for f in range(req.life):
+        step_frame(b, [(req.script, f)], shapes=shapes)
+        b.sync()
+        out.append(b.positions(0))
+    b.close()
+    return out

@@ -213,6 +213,111 @@ int xpbd_ext_volume_gs_order(int* order, int* colorOff);
 /* Library identity: "hip-gfx950" for the GPU library, "oracle-cpu" for the oracle. */
 const char* xpbd_ext_backend(void);
 
+/* ---------------------------------------------------------------------------
+ * Part 3: batched objects (no counterpart in the reference, whose state is one
+ * global instance, native/xpbd_capi.cu:6, 17-29).  A batch holds many
+ * independent objects in one device node space and steps all of them with one
+ * launch sequence per substep, captured in a HIP graph.  Each object keeps
+ * its own settings, colliders, attachments, clusters and hash table, and
+ * advances exactly as it would alone (same launches per node, same summation
+ * orders).  Objects of one batch share the iteration count.  Functions return
+ * 0 / -1 (message in xpbd_batch_last_error) unless stated; `obj` < 0 in a
+ * setter means every object.  Handles are independent: different batches may
+ * be used from different threads at once; one batch from one thread at a time.
+ * ------------------------------------------------------------------------- */
+typedef struct xpbd_batch xpbd_batch;
+
+/* count objects back to back: object k owns nodes nodeOff[k] .. nodeOff[k+1]
+ * of pos[*3], scales[*3], rots[*4], opacity[*] and edges edgeOff[k] ..
+ * edgeOff[k+1] of edges[*2], rest[*], stiff[*], whose ends are node ids local
+ * to object k.  Every object starts at rest with the reference defaults.
+ * Returns null on failure (message in xpbd_batch_last_error(null)). */
+xpbd_batch* xpbd_batch_create(int count, const int* nodeOff, const float* pos, const float* scales, const float* rots,
+                              const float* opacity, const int* edgeOff, const int* edges, const float* rest,
+                              const float* stiff);
+void xpbd_batch_destroy(xpbd_batch* b);
+/* Last error of the batch; for null, the last failed xpbd_batch_create of the calling thread. */
+const char* xpbd_batch_last_error(const xpbd_batch* b);
+int xpbd_batch_objects(const xpbd_batch* b);
+/* Nodes of object obj (obj < 0: of the whole batch). */
+int xpbd_batch_count(const xpbd_batch* b, int obj);
+
+/* Per-object settings with the clamps of the Part 1 setters. */
+int xpbd_batch_set_solver(xpbd_batch* b, int obj, int iters, float dt, float underRelax, float velDamping,
+                          float stiffnessScale, float invMassScale, float distCompliance, float shapeCompliance,
+                          float shapeBlend, float angleCompliance, float angleBlend);
+int xpbd_batch_set_constraints(xpbd_batch* b, int obj, int distance, int shape, int angle, int volume);
+int xpbd_batch_set_volume(xpbd_batch* b, int obj, float compliance, int ringK, int maxMembers, int leaderMinHop);
+int xpbd_batch_set_ground(xpbd_batch* b, int obj, int enabled, float ux, float uy, float uz, float height,
+                          float friction, float restitution, float contactRadius, float contactSlop, float gravity);
+int xpbd_batch_set_object_shape(xpbd_batch* b, int obj, float stiffness);
+int xpbd_batch_set_self_collision(xpbd_batch* b, int obj, int enabled, float radiusScale, float excludeScale,
+                                  int withinBody);
+int xpbd_batch_set_self_collision_grid(xpbd_batch* b, int obj, float cellSize, int tableSize);
+int xpbd_batch_set_gauss_seidel(xpbd_batch* b, int obj, int distance, int volume);
+int xpbd_batch_set_shape_robust(xpbd_batch* b, int obj, int enabled);
+/* Explicit volume clusters / shape clusters of one object (as the Part 2
+ * functions, local ids; null returns to the reference rule). */
+int xpbd_batch_set_volume_clusters(xpbd_batch* b, int obj, const int* off, const int* members, int minMembers);
+int xpbd_batch_set_shape_clusters(xpbd_batch* b, int obj, const int* clusterOf, int n);
+/* Colliders of one object (as xpbd_set_colliders). */
+int xpbd_batch_set_colliders(xpbd_batch* b, int obj, int count, const int* types, const float* poses,
+                             const float* dims, float margin, float friction);
+/* Attachments of several objects in one call: request r pins nodes
+ * idx[off[r] .. off[r+1]) (local ids of object objs[r]) to pos[3*k]; a request
+ * with no nodes releases that object (as xpbd_set_attached). */
+int xpbd_batch_set_attached(xpbd_batch* b, int requests, const int* objs, const int* off, const int* idx,
+                            const float* pos, float weight);
+int xpbd_batch_add_rigid_velocity(xpbd_batch* b, int obj, const float* dv, const float* dw, const float* center);
+int xpbd_batch_reset(xpbd_batch* b, int obj);
+
+/* Host-side set-up of the next substep (topologies, colourings, buffers, graph
+ * capture) without stepping; safe to run on another thread than the one
+ * stepping other batches. */
+int xpbd_batch_prepare(xpbd_batch* b);
+/* `substeps` substeps of every object; xpbd_batch_step returns after device
+ * completion, xpbd_batch_step_async after queueing (xpbd_batch_sync waits). */
+int xpbd_batch_step(xpbd_batch* b, int substeps);
+int xpbd_batch_step_async(xpbd_batch* b, int substeps);
+int xpbd_batch_sync(xpbd_batch* b);
+
+/* Device pointer to float3 committed positions of every object in caller
+ * order (object k from node nodeOff[k]); valid until destroy. */
+unsigned long long xpbd_batch_positions_device(xpbd_batch* b);
+int xpbd_batch_positions_host(xpbd_batch* b, int obj, float* out, int n);
+int xpbd_batch_get_state(xpbd_batch* b, int obj, float* pos, float* vel, int n);
+int xpbd_batch_set_state(xpbd_batch* b, int obj, const float* pos, const float* vel, int n);
+/* Deformed shapes (as xpbd_compute_shapes) of object obj; obj < 0 updates
+ * every object in one launch and leaves the results in device memory
+ * (returns 0, see xpbd_batch_shapes_device). */
+int xpbd_batch_compute_shapes(xpbd_batch* b, int obj, float deformEps, float* hostScales, float* hostQuats, int n);
+/* ptrs[0] scales float[3*P], ptrs[1] quaternions float[4*P] (w,x,y,z), both in
+ * device node order, ptrs[2] int[P]: the caller-order node index (nodeOff[k] +
+ * local id) held by each device node, P = padded node count returned. */
+int xpbd_batch_shapes_device(xpbd_batch* b, unsigned long long* ptrs);
+int xpbd_batch_self_collision_stats(xpbd_batch* b, int obj, int* out);
+int xpbd_batch_collider_stats(xpbd_batch* b, int obj, int* hits, float* push, float* centroid, int maxCount);
+
+/* Continue object srcObj of batch src as object dstObj of batch dst (both
+ * created from the same object data): settings, colliders, attachments,
+ * clusters and dynamic state move over and the object advances bit for bit as
+ * if it had stayed.  Used to regroup live objects into new batches. */
+int xpbd_batch_copy_object(xpbd_batch* dst, int dstObj, xpbd_batch* src, int srcObj);
+
+/* Launch-shape autotuner: times candidates of each tunable over `substeps`
+ * substeps from the current state (restored afterwards) and keeps the
+ * fastest; writes a JSON report.  Results do not depend on these choices.
+ * tuning[5]: Jacobi tiles per block, volume mode (-1 auto, 0 thread, 1 wave),
+ * cluster shape waves per block, self-collision build waves per block and
+ * nodes per wave.  flags bit 0 also picks the self-collision hash-table size
+ * (1, 4 or 16 x nextpow2(2N) cells per object), which can reorder candidate
+ * lists that overflow their 32 slots. */
+int xpbd_batch_autotune(xpbd_batch* b, int substeps, int flags, char* json, int len);
+int xpbd_batch_get_tuning(xpbd_batch* b, int* tuning);
+int xpbd_batch_set_tuning(xpbd_batch* b, const int* tuning);
+/* Per-kernel HIP-event timing of `substeps` eager substeps (JSON). */
+int xpbd_batch_profile(xpbd_batch* b, int substeps, char* json, int len);
+
 #ifdef __cplusplus
 }
 #endif
