@@ -38,6 +38,7 @@ struct Global {
     float scCell = 0.0f;
     int scTable = 0;
     int nodeOrder = 1;   // device node order of the next engine (1 = Morton)
+    bool scFine = true;  // candidate search of one-object bodies on the predicted-position grid
     std::vector<int> shapeClusters;     // cluster of every node, empty = 1-ring
     int n = 0;
     bool stepped = false;
@@ -232,6 +233,7 @@ int xpbd_step(void)
         }
         push_colliders();
         g.engine->setSelfCollisionGrid(0, g.scCell, g.scTable);
+        g.engine->setSelfCollisionFine(g.scFine);
         if (!g.shapeClusters.empty() && !g.engine->setShapeClusters(0, g.shapeClusters.data(), err)) {
             g.engine.reset();
             return fail("xpbd_step: " + err);
@@ -444,6 +446,21 @@ int xpbd_ext_set_state(const float* pos, const float* vel, int n)
     return 0;
 }
 
+int xpbd_ext_autotune(int substeps, int flags, char* json, int len)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_ext_autotune: no step yet");
+    std::string out, err;
+    Engine& E = *g.engine;
+    E.settings(0) = g.set;
+    if (!E.autotune(substeps, flags, out, err)) return fail("xpbd_ext_autotune: " + err);
+    if (json && len > 0) {
+        if ((int)out.size() + 1 > len) return fail("xpbd_ext_autotune: buffer too small");
+        std::memcpy(json, out.c_str(), out.size() + 1);
+    }
+    return 0;
+}
+
 int xpbd_ext_profile(int substeps, char* json, int len)
 {
     Global& g = G();
@@ -547,6 +564,14 @@ int xpbd_ext_set_self_collision_grid(float cellSize, int tableSize)
     g.scCell = (std::isfinite(cellSize) && cellSize > 0.0f) ? cellSize : 0.0f;
     g.scTable = tableSize;
     if (g.engine) g.engine->setSelfCollisionGrid(0, g.scCell, g.scTable);
+    return 0;
+}
+
+int xpbd_ext_set_self_collision_fine(int enabled)
+{
+    Global& g = G();
+    g.scFine = enabled != 0;
+    if (g.engine) g.engine->setSelfCollisionFine(g.scFine);
     return 0;
 }
 

@@ -242,6 +242,9 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     objDeviceValid_ = false;
     scSpacing_.assign(B, 0.0f); scCellOverride_.assign(B, 0.0f);
     scTable_.assign(B, 0); scTableBase_.assign(B, 0); scTableOverride_.assign(B, 0);
+    objSingle_.assign(B, 0);
+    fineTable_.assign(B, 0); fineBase_.assign(B, 0);
+    fineTotal_ = 0;
     scTableTotal_ = 0; scSortBits_ = 0;
     scDeviceValid_ = false;
     scSortedIdx_ = nullptr;
@@ -651,6 +654,14 @@ void Engine::fillBodyDev(int b, BodyDev& d) const
     d.scExcl2 = excl * excl;
     d.scFastRel = kScFastRelScale * dc;
     d.scCellOverride = scCellOverride_[b];
+    // the predicted-position grid serves bodies whose candidates are all
+    // same-object ones (self-contact on, every node in one object)
+    const bool fine = scDeviceValid_ && fineTable_[b] > 0 && s.scWithinBody && b < (int)objSingle_.size() &&
+                      objSingle_[b] && scFineOn_;
+    d.scFine = fine ? 1 : 0;
+    d.scFineBase = fine ? fineBase_[b] : 0;
+    d.scFineMask = fine ? fineTable_[b] - 1 : 0;
+    d.scFineInv = 1.0f / (d.scSearch * 1.01f + 1e-6f);
 }
 
 bool Engine::uploadParams(std::string& err)
@@ -698,12 +709,13 @@ std::vector<int> Engine::planSignature() const
     const int B = numBodies();
     int iters = set_[0].iters;
     int anyDist = 0, anyCol = 0, anyGround = 0, anyPress = 0, anyVol = 0, anyShape = 0, anyCGround = 0;
-    int anyObj = 0, anyBoxes = 0, anySc = 0, anyDistGS = 0, anyVolGS = 0;
+    int anyObj = 0, anyBoxes = 0, anySc = 0, anyDistGS = 0, anyVolGS = 0, anyFine = 0;
     for (int b = 0; b < B; ++b) {
         const BodyDev& d = hostBodies_[b];
         anyObj |= d.objOn;
         anyBoxes |= d.objBoxes;
         anySc |= d.scOn;
+        anyFine |= d.scOn && d.scFine;
         anyDist |= d.distanceOn;
         anyDistGS |= d.distGSOn;
         anyVolGS |= d.volGSOn;
@@ -722,7 +734,7 @@ std::vector<int> Engine::planSignature() const
             anyCGround, anyObj, anyBoxes, anySc, anyObj ? objTotal_ : 0, anySc ? scTableTotal_ : 0,
             anySc ? scSortBits_ : 0, nDistGS, nVolGS, device_alloc_epoch(), (anyShape && shcDevValid_) ? shcK_ : 0,
             allocEpoch_, tune_.jacobiTiles, tune_.volMode, tune_.shapeClusterWaves, tune_.scBuildWaves,
-            tune_.scBuildNodes};
+            tune_.scBuildNodes, tune_.scBuildXcd, anyFine};
 }
 
 // Event pairs around kernel launches (profile mode only).

@@ -237,6 +237,10 @@ public:
     // and the build's artefacts in body-local ids.
     bool selfCollisionStats(int b, int out[3], std::string& err);
     void setSelfCollisionGrid(int b, float cellSize, int tableSize);
+    // Candidate search of one-object bodies through the predicted-position
+    // grid (on by default; off = the coarse scan for every body).  The lists
+    // are the same either way.
+    void setSelfCollisionFine(bool on) { scFineOn_ = on; }
     bool selfCollisionGridInfo(int b, float info[3], std::string& err);   // cell, d_c, search
     bool selfCollisionLists(int b, unsigned int* keys, int* sorted, int* counts, int* contacts, std::string& err);
 
@@ -389,6 +393,16 @@ private:
     DevBuf<double> objSumQ_, objShift_, objRestRel_, objSums_;
 
     // self-collision: one hash table per body inside shared cell arrays
+    // candidate search through a second grid on the predicted positions for
+    // bodies whose nodes form one object (same lists as the coarse scan)
+    std::vector<char> objSingle_;           // every node of the body in one object
+    std::vector<int> fineTable_, fineBase_;
+    int fineTotal_ = 0, fineBits_ = 0;
+    bool scFineOn_ = true;
+    DevBuf<unsigned int> fkeys_, fkeysAlt_;
+    DevBuf<int> fidx_, fidxAlt_, fcellStart_, fcellEnd_, fineRank_, coarseRank_, accCount_, accOff_, accUsed_;
+    DevBuf<float4> fineP_, fineR_;
+    DevBuf<int2> accList_;
     std::vector<float> scSpacing_, scCellOverride_;
     std::vector<int> scTable_, scTableBase_, scTableOverride_;
     int scTableTotal_ = 0, scSortBits_ = 0;
@@ -397,6 +411,7 @@ private:
     const unsigned int* scSortedKeys_ = nullptr;
     DevBuf<unsigned int> scMax2_, scKeys_, scKeysAlt_, scHist_;
     DevBuf<int> scCounters_, scUseActive_, scIdx_, scIdxAlt_, cellStart_, cellEnd_, contacts_, counts_;
+    DevBuf<int> scOver_, scStats_;   // per-node overflow; statistics read-back
     DevBuf<float> scBodyDisp_, scInvCell_;
     DevBuf<unsigned char> scActive_;
     DevBuf<float3> scDp_, scDpCross_, scAccum_;

@@ -252,6 +252,7 @@ bool Engine::autotune(int substeps, int flags, std::string& report, std::string&
         {"shape_cluster_waves_per_block", &tune_.shapeClusterWaves, {4, 2, 1}, anyClusters},
         {"selfcol_nodes_per_wave", &tune_.scBuildNodes, {8, 4, 16}, anySc},
         {"selfcol_waves_per_block", &tune_.scBuildWaves, {4, 2, 1}, anySc},
+        {"selfcol_xcd_runs", &tune_.scBuildXcd, {1, 0}, anySc},
     };
     std::string rep = "{";
     float base = 0.0f;

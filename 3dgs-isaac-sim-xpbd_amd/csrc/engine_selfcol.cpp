@@ -71,6 +71,7 @@ SelfColArgs Engine::scArgs() const
     a.cellEnd = cellEnd_.p;
     a.contacts = contacts_.p;
     a.counts = counts_.p;
+    a.overflow = scOver_.p;
     a.dp = scDp_.p;
     a.dpCross = scDpCross_.p;
     a.accum = scAccum_.p;
@@ -80,6 +81,25 @@ SelfColArgs Engine::scArgs() const
     a.sortP = scSortP_.p;
     a.sortX = scSortX_.p;
     a.sortR = scSortR_.p;
+    bool fine = false;
+    for (int b = 0; b < numBodies() && !fine; ++b) fine = hostBodies_[b].scFine != 0;
+    a.coarseRank = fine ? coarseRank_.p : nullptr;
+    a.fineTotal = fine ? fineTotal_ : 0;
+    a.fineBits = fineBits_;
+    a.fkeys = fkeys_.p;
+    a.fidx = fidx_.p;
+    a.fkeysAlt = fkeysAlt_.p;
+    a.fidxAlt = fidxAlt_.p;
+    a.fcellStart = fcellStart_.p;
+    a.fcellEnd = fcellEnd_.p;
+    a.fineP = fineP_.p;
+    a.fineR = fineR_.p;
+    a.fineRank = fineRank_.p;
+    a.accList = accList_.p;
+    a.accOff = accOff_.p;
+    a.accCount = fine ? accCount_.p : nullptr;
+    a.accUsed = fine ? accUsed_.p : nullptr;
+    a.accCap = (long long)accList_.n;
     return a;
 }
 
@@ -138,6 +158,13 @@ bool Engine::ensureObjects(std::string& err)
         changed = true;
     }
     if (!changed) return true;
+    objSingle_.assign(B, 0);
+    for (int b = 0; b < B; ++b) {
+        const std::vector<int>& id = objTopo_[b].id;
+        bool same = true;
+        for (size_t i = 1; i < id.size() && same; ++i) same = id[i] == id[0];
+        objSingle_[b] = same ? 1 : 0;
+    }
     XPBD_HIP(hipStreamSynchronize(stream_));   // queued substeps may still use the old buffers
     objBase_.assign(B, 0);
     int K = 0;
@@ -238,19 +265,48 @@ bool Engine::ensureSelfCol(std::string& err)
     const size_t n = nPad_;
     scMax2_.alloc(B); scCounters_.alloc(3 * (size_t)B); scUseActive_.alloc(B); scBodyDisp_.alloc(B); scInvCell_.alloc(B);
     scKeys_.alloc(n); scKeysAlt_.alloc(n); scIdx_.alloc(n); scIdxAlt_.alloc(n);
-    scHist_.alloc(256 * (size_t)radix_sort_tiles(nPad_));
+    scHist_.alloc(radix_sort_hist_size(nPad_));
     cellStart_.alloc(scTableTotal_); cellEnd_.alloc(scTableTotal_);
-    contacts_.alloc(n * kSelfColCap); counts_.alloc(n);
+    contacts_.alloc(n * kSelfColCap); counts_.alloc(n); scOver_.alloc(n);
     scActive_.alloc(n); scDp_.alloc(n); scDpCross_.alloc(n); scAccum_.alloc(n);
     scSortP_.alloc(n); scSortX_.alloc(n); scSortR_.alloc(n);
     if (!scMax2_.p || !scCounters_.p || !scUseActive_.p || !scBodyDisp_.p || !scInvCell_.p || !scKeys_.p ||
         !scKeysAlt_.p || !scIdx_.p || !scIdxAlt_.p || !scHist_.p || !cellStart_.p || !cellEnd_.p || !contacts_.p ||
-        !counts_.p || !scActive_.p || !scDp_.p || !scDpCross_.p || !scAccum_.p || !scSortP_.p || !scSortX_.p ||
-        !scSortR_.p) {
+        !counts_.p || !scOver_.p || !scActive_.p || !scDp_.p || !scDpCross_.p || !scAccum_.p || !scSortP_.p ||
+        !scSortX_.p || !scSortR_.p) {
+        err = "self-collision: hipMalloc failed";
+        return false;
+    }
+    // predicted-position grid: one table of nextpow2(2N) cells per body
+    fineTable_.assign(B, 0);
+    fineBase_.assign(B, 0);
+    long long fineTotal = 0;
+    for (int b = 0; b < B; ++b) {
+        int t = 1024;
+        while (t < 2 * bodyCount_[b]) t <<= 1;
+        fineTable_[b] = t;
+        fineBase_[b] = (int)fineTotal;
+        fineTotal += t;
+    }
+    if (fineTotal >= 0x7fffffffLL) { err = "self-collision: hash tables too large"; return false; }
+    fineTotal_ = (int)fineTotal;
+    fineBits_ = 0;
+    while ((1LL << fineBits_) <= fineTotal) ++fineBits_;
+    fkeys_.alloc(n); fkeysAlt_.alloc(n); fidx_.alloc(n); fidxAlt_.alloc(n);
+    fcellStart_.alloc(fineTotal_); fcellEnd_.alloc(fineTotal_);
+    fineP_.alloc(n); fineR_.alloc(n); fineRank_.alloc(n); coarseRank_.alloc(n);
+    // sorted candidate lists handed to the one-thread-per-node slot kernel:
+    // room for 64 per node on average (a node that finds the pool full fills
+    // its slots in the build kernel)
+    accList_.alloc(n * 64); accOff_.alloc(n); accCount_.alloc(n); accUsed_.alloc(1);
+    if (!fkeys_.p || !fkeysAlt_.p || !fidx_.p || !fidxAlt_.p || !fcellStart_.p || !fcellEnd_.p || !fineP_.p ||
+        !fineR_.p || !fineRank_.p || !coarseRank_.p || !accList_.p || !accOff_.p || !accCount_.p || !accUsed_.p) {
         err = "self-collision: hipMalloc failed";
         return false;
     }
     XPBD_HIP(hipMemsetAsync(counts_.p, 0, sizeof(int) * n, stream_));
+    XPBD_HIP(hipMemsetAsync(scOver_.p, 0, sizeof(int) * n, stream_));
+    XPBD_HIP(hipMemsetAsync(scActive_.p, 0, n, stream_));
     XPBD_HIP(hipMemsetAsync(scCounters_.p, 0, sizeof(int) * 3 * B, stream_));
     XPBD_HIP(hipMemsetAsync(scUseActive_.p, 0, sizeof(int) * B, stream_));
     XPBD_HIP(hipMemsetAsync(scAccum_.p, 0, sizeof(float3) * n, stream_));
@@ -282,12 +338,19 @@ bool Engine::selfCollisionStats(int b, int out[3], std::string& err)
     XPBD_ALLOC_SCOPE;
     out[0] = out[1] = out[2] = 0;
     if (!scDeviceValid_ || !set_[b].scEnabled) return true;
-    int c[3] = {0, 0, 0}, use = 0;
-    XPBD_HIP(hipMemcpyAsync(c, scCounters_.p + 3 * b, sizeof(int) * 3, hipMemcpyDeviceToHost, stream_));
-    XPBD_HIP(hipMemcpyAsync(&use, scUseActive_.p + b, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    if (!scStats_.p) {
+        scStats_.alloc(3);
+        if (!scStats_.p) { err = "self-collision: hipMalloc failed"; return false; }
+    }
+    // totals over the body's nodes: list lengths, overflow, active mask (all
+    // ones when the active restriction is off)
+    launch_selfcol_stats(stream_, bodyBegin_[b], bodyBegin_[b] + bodyCount_[b], scArgs(), scStats_.p);
+    int c[3] = {0, 0, 0};
+    XPBD_HIP(hipMemcpyAsync(c, scStats_.p, sizeof(int) * 3, hipMemcpyDeviceToHost, stream_));
     XPBD_HIP(hipStreamSynchronize(stream_));
+    XPBD_HIP(hipGetLastError());
     out[0] = c[0];
-    out[1] = use ? c[2] : bodyCount_[b];
+    out[1] = c[2];
     out[2] = c[1];
     return true;
 }

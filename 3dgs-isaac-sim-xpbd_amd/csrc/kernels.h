@@ -18,6 +18,7 @@ struct Tuning {
     int shapeClusterWaves = 4;  // clusters (one wave each) per cluster shape-matching block: 1, 2, 4
     int scBuildWaves = 4;       // waves per self-collision candidate-build block: 1, 2, 4
     int scBuildNodes = 8;       // nodes per wave of the candidate build: 4, 8, 16
+    int scBuildXcd = 1;         // candidate build: contiguous node runs per XCD (1) or round-robin blocks (0)
 };
 
 // Node-space launch description shared by every per-node kernel.
@@ -121,6 +122,7 @@ struct SelfColArgs {
     int* cellEnd;
     int* contacts;             // [nPad * kSelfColCap]
     int* counts;               // [nPad]
+    int* overflow;             // [nPad] accepted candidates beyond the 32 slots
     float3* dp;
     float3* dpCross;
     float3* accum;
@@ -130,6 +132,24 @@ struct SelfColArgs {
     float4* sortP;             // [nPad] sorted order: p, object id bits
     float4* sortX;             // [nPad] sorted order: x, node id bits
     float4* sortR;             // [nPad] sorted order: rest
+    int* coarseRank;           // [nPad] sorted position of each node
+    // second grid on the predicted positions (cell edge >= search), used for
+    // bodies whose nodes all belong to one object
+    int fineTotal, fineBits;   // cells of all fine tables; key bits of the sort
+    unsigned int* fkeys;       // [nPad] fine key of each node (fineTotal = none)
+    int* fidx;                 // [nPad]
+    unsigned int* fkeysAlt;    // [nPad] radix-sort ping-pong
+    int* fidxAlt;
+    int* fcellStart;           // [fineTotal]
+    int* fcellEnd;
+    float4* fineP;             // [nPad] fine order: p, node id bits
+    float4* fineR;             // [nPad] fine order: rest, coarse key bits
+    int* fineRank;             // [nPad] fine order: coarse sorted position
+    int2* accList;             // [accCap] pool of sorted accepted-candidate lists (code, priority bits)
+    int* accOff;               // [nPad] start of a node's list in the pool
+    int* accCount;             // [nPad] list length, -1 when the build kernel filled the slots itself
+    int* accUsed;              // [1] pool entries taken this build
+    long long accCap;
 };
 
 void launch_object_shape(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, int K, float4* P, const float3* rest,
@@ -141,7 +161,10 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
 void launch_selfcol_iteration(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, const float4* Pin,
                               float4* Pout, const float3* X, const float3* rest, unsigned epi, const EpiArgs& ea);
 void launch_contact_push(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, int K, float4* P, const EpiArgs& ea);
+// out[3] = candidates, overflow, active nodes of the last build over [nodeBegin, nodeEnd).
+void launch_selfcol_stats(hipStream_t s, int nodeBegin, int nodeEnd, const SelfColArgs& a, int* out);
 int radix_sort_tiles(int n);
+size_t radix_sort_hist_size(int n);   // entries of the sort's histogram buffer
 bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0, unsigned int* k1, int* v1,
                        unsigned int* hist, bool iota);
 void launch_fill_double(hipStream_t s, int n, double* a);

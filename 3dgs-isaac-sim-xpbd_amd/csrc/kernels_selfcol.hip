@@ -202,6 +202,7 @@ __global__ void sc_prepare_kernel(int B, SelfColArgs a)
     const BodyDev& D = a.bodies[b];
     a.scMax2[b] = 0u;
     a.scCounters[3 * b] = 0; a.scCounters[3 * b + 1] = 0; a.scCounters[3 * b + 2] = 0;
+    if (b == 0 && a.accUsed) *a.accUsed = 0;   // the deferred-list pool starts empty
     float maxDisp = 0.0f;
     for (int k = 0; k < D.objK; ++k) maxDisp = fmaxf(maxDisp, a.objDisp[D.objBase + k]);
     a.scUseActive[b] = (D.scOn && D.objActiveAllowed && isfinite(maxDisp)) ? 1 : 0;
@@ -209,42 +210,77 @@ __global__ void sc_prepare_kernel(int B, SelfColArgs a)
 }
 
 // Active mask (forward.cu:6998-7030) and max |p - x|^2 over active nodes (7058-7068).
-__global__ void __launch_bounds__(256) sc_active_disp_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
-                                                            const float4* __restrict__ P)
+// Blocks of 1024 threads: each wave (one 64-node tile of one body) reduces its
+// largest move, then the block combines its waves, one atomic per run of
+// waves of the same body (atomics on one address serialise).
+constexpr int kDispThreads = 1024;
+
+__global__ void __launch_bounds__(kDispThreads) sc_active_disp_kernel(int nPad, SelfColArgs a,
+                                                                     const float3* __restrict__ X,
+                                                                     const float4* __restrict__ P)
 {
+    __shared__ unsigned int wmax[kDispThreads / 64];
+    __shared__ int wbody[kDispThreads / 64];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nPad) return;
-    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
-    const BodyDev& B = a.bodies[b];
-    if (!B.scOn || i >= B.nodeEnd) return;
-    const float3 x = X[i];
-    const float4 p = P[i];
-    unsigned char act = 1;
-    if (a.scUseActive[b]) {
-        const int id = a.objId[i];
-        act = 0;
-        if (id < 0) {
-            act = 1;
+    const int w = threadIdx.x >> 6;
+    int b = -1;
+    unsigned int m = 0u;
+    if (i < nPad) {   // nPad is a multiple of 64: whole waves
+        b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
+        const BodyDev& B = a.bodies[b];
+        if (B.scOn) {
+            const bool node = i < B.nodeEnd;
+            float3 x = make_float3(0.0f, 0.0f, 0.0f);
+            float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (node) { x = X[i]; p = P[i]; }
+            unsigned char act = node ? 1 : 0;
+            if (node && a.scUseActive[b]) {
+                const int id = a.objId[i];
+                act = 0;
+                if (id < 0) {
+                    act = 1;
+                } else {
+                    const float inflate = B.scSearch + 2.0f * a.scBodyDisp[b] + 2.0f * B.scDc;
+                    for (int k = 0; k < B.objK; ++k) {
+                        const int gk = B.objBase + k;
+                        if (gk == id) continue;
+                        const float* bx = &a.objBoxesArr[6 * (size_t)gk];
+                        const float lx = bx[0] - inflate, ly = bx[1] - inflate, lz = bx[2] - inflate;
+                        const float hx = bx[3] + inflate, hy = bx[4] + inflate, hz = bx[5] + inflate;
+                        const bool inP = p.x >= lx && p.x <= hx && p.y >= ly && p.y <= hy && p.z >= lz && p.z <= hz;
+                        const bool inX = x.x >= lx && x.x <= hx && x.y >= ly && x.y <= hy && x.z >= lz && x.z <= hz;
+                        if (inP || inX) { act = 1; break; }
+                    }
+                }
+            }
+            a.scActive[i] = act;
+            const float dx = p.x - x.x, dy = p.y - x.y, dz = p.z - x.z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            // non-negative floats order like their bit patterns
+            m = (act && isfinite(d2) && d2 > 0.0f) ? __float_as_uint(d2) : 0u;
         } else {
-            const float inflate = B.scSearch + 2.0f * a.scBodyDisp[b] + 2.0f * B.scDc;
-            for (int k = 0; k < B.objK; ++k) {
-                const int gk = B.objBase + k;
-                if (gk == id) continue;
-                const float* bx = &a.objBoxesArr[6 * (size_t)gk];
-                const float lx = bx[0] - inflate, ly = bx[1] - inflate, lz = bx[2] - inflate;
-                const float hx = bx[3] + inflate, hy = bx[4] + inflate, hz = bx[5] + inflate;
-                const bool inP = p.x >= lx && p.x <= hx && p.y >= ly && p.y <= hy && p.z >= lz && p.z <= hz;
-                const bool inX = x.x >= lx && x.x <= hx && x.y >= ly && x.y <= hy && x.z >= lz && x.z <= hz;
-                if (inP || inX) { act = 1; break; }
+            b = -1;
+        }
+    }
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) m = max(m, (unsigned int)__shfl_xor((int)m, s, 64));
+    if ((threadIdx.x & 63) == 0) {
+        wmax[w] = m;
+        wbody[w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (int)(blockDim.x >> 6);
+        unsigned int run = 0u;
+        for (int k = 0; k < nw; ++k) {
+            if (wbody[k] < 0) continue;
+            run = max(run, wmax[k]);
+            if (k + 1 == nw || wbody[k + 1] != wbody[k]) {
+                if (run > 0u) atomicMax(&a.scMax2[wbody[k]], run);
+                run = 0u;
             }
         }
-        if (act) atomicAdd(&a.scCounters[3 * b + 2], 1);
     }
-    a.scActive[i] = act;
-    if (!act) return;
-    const float dx = p.x - x.x, dy = p.y - x.y, dz = p.z - x.z;
-    const float d2 = dx * dx + dy * dy + dz * dz;
-    if (isfinite(d2) && d2 > 0.0f) atomicMax(&a.scMax2[b], __float_as_uint(d2));
 }
 
 // Cell size per body: search + 2 max displacement, capped at search + 16 d_c
@@ -310,6 +346,60 @@ __global__ void __launch_bounds__(256) sc_stage_kernel(int nPad, SelfColArgs a, 
     a.sortP[t] = make_float4(p.x, p.y, p.z, __int_as_float(a.objId[j]));
     a.sortX[t] = make_float4(x.x, x.y, x.z, 0.0f);
     a.sortR[t] = make_float4(r.x, r.y, r.z, __int_as_float(j));
+    if (a.coarseRank) a.coarseRank[j] = t;
+}
+
+// Keys of the predicted positions in the fine grid (bodies with scFine): the
+// nodes that have a coarse key (active, finite x) and a finite p.
+__global__ void __launch_bounds__(256) sc_fine_keys_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
+                                                          const float4* __restrict__ P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;
+    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
+    const BodyDev& B = a.bodies[b];
+    unsigned int key = (unsigned int)a.fineTotal;
+    if (B.scOn && B.scFine && i < B.nodeEnd && a.scActive[i]) {
+        const float3 x = X[i];
+        const float4 p = P[i];
+        if (isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p.x) && isfinite(p.y) && isfinite(p.z)) {
+            const float g = B.scFineInv;
+            key = (unsigned int)B.scFineBase +
+                  sc_hash(sc_cell(p.x, g), sc_cell(p.y, g), sc_cell(p.z, g), (unsigned int)B.scFineMask);
+        }
+    }
+    a.fkeys[i] = key;
+    a.fidx[i] = i;
+}
+
+__global__ void __launch_bounds__(256) sc_fine_range_kernel(int nPad, SelfColArgs a,
+                                                           const unsigned int* __restrict__ keysSorted)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;
+    const unsigned int k = keysSorted[i];
+    if (k >= (unsigned int)a.fineTotal) return;
+    if (i == 0 || keysSorted[i - 1] != k) a.fcellStart[k] = i;
+    if (i == nPad - 1 || keysSorted[i + 1] != k) a.fcellEnd[k] = i + 1;
+}
+
+// Fine-order copies for the fine scan: (p, node id), (rest, coarse key) and
+// the coarse sorted position (the candidate's place in the coarse scan).
+__global__ void __launch_bounds__(256) sc_fine_stage_kernel(int nPad, SelfColArgs a, const float4* __restrict__ P,
+                                                           const float3* __restrict__ rest,
+                                                           const unsigned int* __restrict__ keysSorted,
+                                                           const int* __restrict__ sortedIdx,
+                                                           const unsigned int* __restrict__ coarseSorted)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nPad || keysSorted[t] >= (unsigned int)a.fineTotal) return;
+    const int j = sortedIdx[t];
+    const float4 p = P[j];
+    const float3 r = rest[j];
+    const int rk = a.coarseRank[j];
+    a.fineP[t] = make_float4(p.x, p.y, p.z, __int_as_float(j));
+    a.fineR[t] = make_float4(r.x, r.y, r.z, __uint_as_float(coarseSorted[rk]));   // j's coarse key
+    a.fineRank[t] = rk;
 }
 
 // Distance from x to the stencil cell at offset o along one axis, given the
@@ -350,34 +440,118 @@ __device__ __forceinline__ int sc_worst_lane(int tier, float prio, int lane)
     return __builtin_amdgcn_readfirstlane(wl);
 }
 
-// Candidate lists (forward.cu:7079-7180), one wave64 per node at a time.  The
-// lanes test the candidates of a hash bucket 64 at a time; the accepted ones
-// are inserted in lane order (= the reference's scan order) into the 32 slots
-// held by lanes 0-31, replacing the worst slot once the list is full.  Each
+// Candidate-list slots of one node: lanes 0-31 hold slot (code, tier,
+// priority); the worst slot (highest tier, then largest priority, then lowest
+// slot, the scan of forward.cu:7160-7172) is cached until a slot changes.
+struct ScSlots {
+    int cnt = 0, overflow = 0;
+    int code = 0, tier = 0;
+    float prio = 0.0f;
+    int worst = -1, worstTier = 0;
+    float worstPrio = 0.0f;
+
+    // One accepted candidate, in scan order (wave-uniform arguments).
+    __device__ __forceinline__ void insert(int cc, int ct, float cp, int lane)
+    {
+        if (cnt < kSelfColCap) {
+            if (lane == cnt) { code = cc; tier = ct; prio = cp; }
+            ++cnt;
+            return;
+        }
+        if (worst < 0) {
+            worst = sc_worst_lane(tier, prio, lane);
+            worstTier = lane_i(tier, worst);
+            worstPrio = lane_fl(prio, worst);
+        }
+        if (ct < worstTier || (ct == worstTier && cp < worstPrio)) {
+            if (lane == worst) { code = cc; tier = ct; prio = cp; }
+            worst = -1;
+        }
+        ++overflow;
+    }
+};
+
+// Walk of a virtual run of up to 27 bucket ranges (lane c < 27 holds range c
+// as start rs and length rn): returns the exclusive prefix of the lengths in
+// `pre` and the run length.
+__device__ __forceinline__ int sc_run_prefix(int rn, int lane, int& pre)
+{
+    pre = rn;
+#pragma unroll
+    for (int m = 1; m < 32; m <<= 1) {
+        const int o = __shfl_up(pre, m, 64);
+        if (lane >= m) pre += o;
+    }
+    const int total = lane_i(pre, 26);
+    pre -= rn;
+    return total;
+}
+
+// Sorted-array position of run position v: the last range c with pre_c <= v.
+__device__ __forceinline__ int sc_run_pos(int v, int rs, int pre)
+{
+    int c0 = 0;
+#pragma unroll
+    for (int step = 16; step >= 1; step >>= 1) {
+        const int cand = c0 + step;
+        const int pc = __shfl(pre, cand < 27 ? cand : 26, 64);
+        if (cand < 27 && pc <= v) c0 = cand;
+    }
+    return __shfl(rs, c0, 64) + (v - __shfl(pre, c0, 64));
+}
+
+constexpr int kFineList = 128;   // accepted candidates a wave sorts in LDS (a multiple of 64)
+constexpr int kAccChunk = 1024;  // list-pool entries a wave reserves at once (>= kFineList)
+
+// Candidate lists (forward.cu:7079-7180), one wave64 per node at a time.  Each
 // wave takes NPW consecutive nodes (a divisor of the 64-node tile, so a wave
 // never spans two bodies).
+//
+// Coarse scan (the reference's own walk): the 27 stencil cells of the
+// frame-start position, each hash bucket once at its first stencil offset, in
+// sorted order, tested 64 candidates at a time; accepted candidates enter the
+// slots in lane order, which is the scan order.  Stencil cells that cannot
+// hold a candidate are skipped: a node j passes either test only if
+// |x_i - x_j| < search + |p_i - x_i| + max |p - x|, so a cell farther than that
+// bound (with a relative margin for rounding) holds none; a bucket is still
+// scanned at its first offset whenever any offset mapping to it is needed.
+//
+// Fine scan (bodies whose nodes all belong to one object, with self-contact
+// inside it on): the only test is |p_i - p_j| < search, so every candidate
+// lies in the 27 cells of edge >= search around p_i in a second grid built on
+// the predicted positions.  Those few cells are tested instead of the coarse
+// stencil; a passing j is a reference candidate when its coarse bucket is one
+// of the stencil's buckets, and it takes the scan position (first stencil
+// offset of that bucket, sorted position inside it).  The accepted ones are
+// sorted by that position in LDS and enter the slots in that order, so the
+// lists and counters equal the coarse scan's.  A node with more than
+// kFineList accepted candidates takes the coarse scan.
 template <int NPW>
 __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
-                                                      const float4* __restrict__ P, const float3* __restrict__ rest)
+                                                      const float4* __restrict__ P, const float3* __restrict__ rest,
+                                                      int xcd)
 {
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    __shared__ int4 lacc[4][kFineList];
+    __shared__ int2 lsort[4][kFineList];
+    // xcd: blocks of one XCD take one contiguous run of nodes (Morton order,
+    // so neighbouring cells), which keeps their buckets in that XCD's L2
+    const int blk = xcd ? xcd_block() : (int)blockIdx.x;
+    const int wave = (blk * (int)blockDim.x + (int)threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    constexpr int CAP = kSelfColCap;
+    const int wl = threadIdx.x >> 6;
     const int i0 = wave * NPW;
     if (i0 >= nPad) return;   // whole waves
     const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i0 / kTile]);
     const BodyDev& B = a.bodies[b];
-    long long candTotal = 0, overTotal = 0;
+    bool anyCand = false;
+    int chunkPos = 0, chunkEnd = 0;   // the wave's share of the deferred-list pool
     for (int ii = 0; ii < NPW; ++ii) {
         const int i = i0 + ii;
         if (!B.scOn || i >= B.nodeEnd) break;
         const float3 x = X[i];
         const float4 p4 = P[i];
-        int cnt = 0, overflow = 0;
-        int slotCode = 0, slotTier = 0;
-        float slotPrio = 0.0f;
-        int worst = -1, worstTier = 0;
-        float worstPrio = 0.0f;
+        ScSlots S;
+        int deferred = -1;   // >= 0: sorted list of that length handed to sc_slots_kernel
         if (a.scActive[i] && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p4.x) && isfinite(p4.y) &&
             isfinite(p4.z)) {
             const float3 q = rest[i];
@@ -387,48 +561,166 @@ __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, 
             const float inv = a.scInvCell[b];
             const unsigned int mask = (unsigned int)B.scTableMask;
             const int cx = sc_cell(x.x, inv), cy = sc_cell(x.y, inv), cz = sc_cell(x.z, inv);
-            // Stencil cells that cannot hold a candidate are skipped: a node j
-            // passes either test only if |x_i - x_j| < search + |p_i - x_i| +
-            // max |p - x| (the body's largest predicted move), so a cell farther
-            // than that bound (with a relative margin for rounding) holds none.
-            // A hash bucket is still scanned, at its first stencil offset,
-            // whenever any offset mapping to it is needed, so the candidate
-            // order is that of the full stencil scan.
-            const float edge = 1.0f / inv;
-            const float tx = fminf(fmaxf(x.x * inv, -1.0e9f), 1.0e9f), ty = fminf(fmaxf(x.y * inv, -1.0e9f), 1.0e9f),
-                        tz = fminf(fmaxf(x.z * inv, -1.0e9f), 1.0e9f);
-            const float fx = tx - floorf(tx), fy = ty - floorf(ty), fz = tz - floorf(tz);
-            const float di = sqrtf(dix * dix + diy * diy + diz * diz);
-            const float maxMove = sqrtf(fmaxf(__uint_as_float(a.scMax2[b]), 0.0f));
-            const float bound = (search + di + maxMove) * 1.0001f + edge * 1.0e-4f + 1.0e-6f;
-            const float bound2 = bound * bound;
-            // lane c < 27: hash of stencil offset c, first occurrence, needed
+            // lane c < 27: stencil offset c and its (global) coarse bucket
             const int c = lane < 27 ? lane : 26;
             const int ox = c % 3 - 1, oy = (c / 3) % 3 - 1, oz = c / 9 - 1;
             const unsigned int h = sc_hash(cx + ox, cy + oy, cz + oz, mask);
-            const float gx = sc_axis_gap(ox, fx, edge), gy = sc_axis_gap(oy, fy, edge), gz = sc_axis_gap(oz, fz, edge);
-            const bool need = lane < 27 && gx * gx + gy * gy + gz * gz <= bound2;
-            bool first = lane < 27, scan = false;
-            for (int c2 = 0; c2 < 27; ++c2) {
-                const unsigned int h2 = (unsigned int)lane_i((int)h, c2);
-                const bool need2 = (__ballot(need) >> c2) & 1ull;
-                if (h2 == h && c2 < c) first = false;
-                if (h2 == h && c2 >= c && need2) scan = true;
+            const unsigned int hg = (unsigned int)B.scTableBase + h;
+            bool coarse = true;
+            if (B.scFine) {
+                // ---- fine scan ----
+                const float ginv = B.scFineInv;
+                const unsigned int fmask = (unsigned int)B.scFineMask;
+                const unsigned int fh = sc_hash(sc_cell(p4.x, ginv) + ox, sc_cell(p4.y, ginv) + oy,
+                                                sc_cell(p4.z, ginv) + oz, fmask);
+                bool ffirst = lane < 27;
+                for (int c2 = 0; c2 < 27; ++c2)
+                    if ((unsigned int)lane_i((int)fh, c2) == fh && c2 < c) ffirst = false;
+                int rs = 0, rn = 0;
+                if (ffirst) {
+                    rs = a.fcellStart[B.scFineBase + fh];
+                    rn = rs >= 0 ? a.fcellEnd[B.scFineBase + fh] - rs : 0;
+                }
+                int pre;
+                const int total = sc_run_prefix(rn, lane, pre);
+                int nacc = 0;
+                for (int base = 0; base < total; base += 64) {
+                    const int v = base + lane;
+                    const int t = sc_run_pos(v, rs, pre);
+                    bool acc = false;
+                    float pr = 0.0f;
+                    int k1 = 27, j = 0, rk = 0;
+                    unsigned int ck = 0xffffffffu;
+                    if (v < total) {
+                        const float4 pj = a.fineP[t];
+                        const float4 qj = a.fineR[t];
+                        rk = a.fineRank[t];
+                        ck = __float_as_uint(qj.w);
+                        j = __float_as_int(pj.w);
+                        if (j != i) {
+                            const float ex = p4.x - pj.x, ey = p4.y - pj.y, ez = p4.z - pj.z;
+                            const float d2 = ex * ex + ey * ey + ez * ez;
+                            if (d2 < search2) {
+                                const float qx = q.x - qj.x, qy = q.y - qj.y, qz = q.z - qj.z;
+                                const float q2 = qx * qx + qy * qy + qz * qz;
+                                if (q2 >= B.scExcl2) {
+                                    pr = sqrtf(d2);
+                                    acc = true;
+                                }
+                            }
+                        }
+                    }
+                    // scan position: first stencil offset of the candidate's coarse
+                    // bucket; lanes 0-26 hold the stencil buckets, so each
+                    // accepted candidate is matched against all of them at once
+                    unsigned long long pend = __ballot(acc);
+                    while (pend) {
+                        const int l = __builtin_ctzll(pend);
+                        pend &= pend - 1ull;
+                        const unsigned int ckl = (unsigned int)lane_i((int)ck, l);
+                        const unsigned long long hit = __ballot(lane < 27 && hg == ckl);
+                        if (lane == l) k1 = hit ? __builtin_ctzll(hit) : 27;
+                    }
+                    acc = acc && k1 < 27;
+                    const unsigned long long am = __ballot(acc);
+                    const int slot = nacc + (int)__builtin_amdgcn_mbcnt_hi((unsigned int)(am >> 32),
+                                                                          __builtin_amdgcn_mbcnt_lo((unsigned int)am, 0u));
+                    if (acc && slot < kFineList) lacc[wl][slot] = make_int4(j, __float_as_int(pr), k1, rk);
+                    nacc += __popcll(am);
+                }
+                if (nacc <= kFineList) {
+                    coarse = false;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    // rank of each accepted entry by (first offset, sorted position)
+                    constexpr int R = kFineList / 64;
+                    int4 e[R];
+                    int rk[R];
+#pragma unroll
+                    for (int k = 0; k < R; ++k) {
+                        e[k] = (k * 64 + lane < nacc) ? lacc[wl][k * 64 + lane] : make_int4(0, 0, 27, 0);
+                        rk[k] = 0;
+                    }
+                    for (int m = 0; m < nacc; ++m) {
+                        const int4 em = lacc[wl][m];
+#pragma unroll
+                        for (int k = 0; k < R; ++k)
+                            rk[k] += (em.z < e[k].z || (em.z == e[k].z && em.w < e[k].w)) ? 1 : 0;
+                    }
+                    // the sorted list goes to the slot kernel (one thread per node)
+                    // when the shared list pool has room; the wave takes pool
+                    // space kAccChunk entries at a time
+                    if (chunkPos + nacc > chunkEnd) {
+                        int off = 0;
+                        if (lane == 0) off = atomicAdd(a.accUsed, kAccChunk);
+                        chunkPos = __builtin_amdgcn_readfirstlane(off);
+                        chunkEnd = chunkPos + kAccChunk;
+                    }
+                    const int off = chunkPos;
+                    if ((long long)chunkEnd <= a.accCap) {
+                        chunkPos += nacc;
+#pragma unroll
+                        for (int k = 0; k < R; ++k)
+                            if (k * 64 + lane < nacc) a.accList[(size_t)off + rk[k]] = make_int2(e[k].x, e[k].y);
+                        if (lane == 0) a.accOff[i] = off;
+                        deferred = nacc;
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < R; ++k)
+                            if (k * 64 + lane < nacc) lsort[wl][rk[k]] = make_int2(e[k].x, e[k].y);
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                        for (int r = 0; r < nacc; ++r) {
+                            const int2 e = lsort[wl][r];
+                            S.insert(__builtin_amdgcn_readfirstlane(e.x), 1,
+                                     __int_as_float(__builtin_amdgcn_readfirstlane(e.y)), lane);
+                        }
+                    }
+                    __builtin_amdgcn_wave_barrier();   // the next node reuses the wave's LDS rows
+                }
             }
-            const unsigned long long todo = __ballot(first && scan);
-            for (int cs = 0; cs < 27; ++cs) {
-                if (!((todo >> cs) & 1ull)) continue;
-                const unsigned int hb = (unsigned int)lane_i((int)h, cs);
-                const int s = a.cellStart[B.scTableBase + hb];
-                if (s < 0) continue;
-                const int e = a.cellEnd[B.scTableBase + hb];
-                for (int base = s; base < e; base += 64) {
-                    const int t = base + lane;
+            if (coarse) {
+                // ---- coarse scan ----
+                const float edge = 1.0f / inv;
+                const float tx = fminf(fmaxf(x.x * inv, -1.0e9f), 1.0e9f), ty = fminf(fmaxf(x.y * inv, -1.0e9f), 1.0e9f),
+                            tz = fminf(fmaxf(x.z * inv, -1.0e9f), 1.0e9f);
+                const float fx = tx - floorf(tx), fy = ty - floorf(ty), fz = tz - floorf(tz);
+                const float di = sqrtf(dix * dix + diy * diy + diz * diz);
+                const float maxMove = sqrtf(fmaxf(__uint_as_float(a.scMax2[b]), 0.0f));
+                const float bound = (search + di + maxMove) * 1.0001f + edge * 1.0e-4f + 1.0e-6f;
+                const float bound2 = bound * bound;
+                const float gx = sc_axis_gap(ox, fx, edge), gy = sc_axis_gap(oy, fy, edge), gz = sc_axis_gap(oz, fz, edge);
+                // nodes outside the 27 cells are at least one edge away; when
+                // the edge is shorter than the bound (a capped or fixed cell)
+                // such a node could pass through a colliding bucket, so no
+                // cell is skipped
+                const bool cull = edge >= bound;
+                const bool need = lane < 27 && (!cull || gx * gx + gy * gy + gz * gz <= bound2);
+                bool first = lane < 27, scan = false;
+                for (int c2 = 0; c2 < 27; ++c2) {
+                    const unsigned int h2 = (unsigned int)lane_i((int)h, c2);
+                    const bool need2 = (__ballot(need) >> c2) & 1ull;
+                    if (h2 == h && c2 < c) first = false;
+                    if (h2 == h && c2 >= c && need2) scan = true;
+                }
+                int rs = 0, rn = 0;
+                if (first && scan) {
+                    rs = a.cellStart[hg];
+                    rn = rs >= 0 ? a.cellEnd[hg] - rs : 0;
+                }
+                int pre;
+                const int total = sc_run_prefix(rn, lane, pre);
+                for (int base = 0; base < total; base += 64) {
+                    const int v = base + lane;
+                    const int t = sc_run_pos(v, rs, pre);
                     bool acc = false;
                     int code = 0, tr = 0;
                     float pr = 0.0f;
-                    if (t < e) {
+                    if (v < total) {
                         const float4 pj = a.sortP[t];
+                        const float4 qj = a.sortR[t];   // issued with pj: no dependent second trip
                         const bool cross = __float_as_int(pj.w) != oi;
                         if (cross || B.scWithin) {
                             float d2 = 0.0f, distX = 0.0f, rel = 0.0f;
@@ -447,7 +739,6 @@ __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, 
                                 pass = d2 < search2;
                             }
                             if (pass) {
-                                const float4 qj = a.sortR[t];
                                 const int j = __float_as_int(qj.w);
                                 const float qx = q.x - qj.x, qy = q.y - qj.y, qz = q.z - qj.z;
                                 const float q2 = qx * qx + qy * qy + qz * qz;
@@ -470,38 +761,108 @@ __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, 
                     while (am) {
                         const int l = __builtin_ctzll(am);
                         am &= am - 1ull;
-                        const int cc = lane_i(code, l), ct = lane_i(tr, l);
-                        const float cp = lane_fl(pr, l);
-                        if (cnt < CAP) {
-                            if (lane == cnt) { slotCode = cc; slotTier = ct; slotPrio = cp; }
-                            ++cnt;
-                        } else {
-                            // the worst slot only changes when a slot is replaced
-                            if (worst < 0) {
-                                worst = sc_worst_lane(slotTier, slotPrio, lane);
-                                worstTier = lane_i(slotTier, worst);
-                                worstPrio = lane_fl(slotPrio, worst);
-                            }
-                            if (ct < worstTier || (ct == worstTier && cp < worstPrio)) {
-                                if (lane == worst) { slotCode = cc; slotTier = ct; slotPrio = cp; }
-                                worst = -1;
-                            }
-                            ++overflow;
-                        }
+                        S.insert(lane_i(code, l), lane_i(tr, l), lane_fl(pr, l), lane);
                     }
                 }
             }
         }
-        if (lane < cnt) a.contacts[sc_slot(i, lane)] = slotCode;
-        if (lane == 0) a.counts[i] = cnt;
-        candTotal += cnt;
-        overTotal += overflow;
+        if (a.accCount && lane == 0) a.accCount[i] = deferred;
+        if (deferred < 0) {
+            if (lane < S.cnt) a.contacts[sc_slot(i, lane)] = S.code;
+            if (lane == 0) {
+                a.counts[i] = S.cnt;
+                a.overflow[i] = S.overflow;
+            }
+            anyCand |= S.cnt > 0;
+        }
     }
-    if (lane == 0 && candTotal > 0) atomicAdd(&a.scCounters[3 * b], (int)candTotal);
-    if (lane == 0 && overTotal > 0) atomicAdd(&a.scCounters[3 * b + 1], (int)overTotal);
+    // the contact push of object shape matching needs to know whether the body
+    // has any candidate; the totals are summed only when asked for
+    if (lane == 0 && anyCand && B.objOn) a.scCounters[3 * b] = 1;
 }
 
-// Candidate lists (forward.cu:7079-7180).
+// Slots of the nodes whose sorted candidate list the fine scan deferred, one
+// thread per node: the first 32 fill the slots in order, each later one
+// replaces the worst slot (largest priority, lowest slot among equals) when
+// strictly better, as the wave insertion does (all fine candidates are
+// same-object ones, one tier).
+__global__ void __launch_bounds__(256) sc_slots_kernel(int nPad, SelfColArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;   // nPad is a multiple of 64: whole waves
+    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
+    const BodyDev& B = a.bodies[b];
+    if (!B.scOn || !B.scFine) return;
+    const int n = i < B.nodeEnd ? a.accCount[i] : -1;
+    const int2* list = a.accList + (n > 0 ? a.accOff[i] : 0);
+    constexpr int CAP = kSelfColCap;
+    int code[CAP];
+    float prio[CAP];
+    int cnt = 0, over = 0;
+    for (int r = 0; r < n; ++r) {
+        const int2 e = list[r];
+        const float p = __int_as_float(e.y);
+        if (cnt < CAP) {
+#pragma unroll
+            for (int s = 0; s < CAP; ++s)
+                if (s == cnt) { code[s] = e.x; prio[s] = p; }
+            ++cnt;
+        } else {
+            int w = 0;
+            float wp = prio[0];
+#pragma unroll
+            for (int s = 1; s < CAP; ++s)
+                if (prio[s] > wp) { wp = prio[s]; w = s; }
+            if (p < wp) {
+#pragma unroll
+                for (int s = 0; s < CAP; ++s)
+                    if (s == w) { code[s] = e.x; prio[s] = p; }
+            }
+            ++over;
+        }
+    }
+    if (n >= 0) {
+#pragma unroll
+        for (int s = 0; s < CAP; ++s)
+            if (s < cnt) a.contacts[sc_slot(i, s)] = code[s];
+        a.counts[i] = cnt;
+        a.overflow[i] = over;
+    }
+    const bool any = __ballot(n > 0 && cnt > 0) != 0ull;
+    if ((threadIdx.x & 63) == 0 && any && B.objOn) a.scCounters[3 * b] = 1;
+}
+
+// Statistics of one body's last build (forward.cu:11837-11849 reports them):
+// candidates (sum of list lengths), overflow (accepted beyond the 32 slots)
+// and active nodes, summed over [nodeBegin, nodeEnd) into out[3].
+__global__ void __launch_bounds__(256) sc_stats_kernel(int nodeBegin, int nodeEnd, SelfColArgs a, int* out)
+{
+    const int i = nodeBegin + blockIdx.x * blockDim.x + threadIdx.x;
+    int c = 0, o = 0, act = 0;
+    if (i < nodeEnd) {
+        c = a.counts[i];
+        o = a.overflow[i];
+        act = a.scActive[i] ? 1 : 0;
+    }
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) {
+        c += __shfl_xor(c, s, 64);
+        o += __shfl_xor(o, s, 64);
+        act += __shfl_xor(act, s, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (c) atomicAdd(&out[0], c);
+        if (o) atomicAdd(&out[1], o);
+        if (act) atomicAdd(&out[2], act);
+    }
+}
+
+void launch_selfcol_stats(hipStream_t s, int nodeBegin, int nodeEnd, const SelfColArgs& a, int* out)
+{
+    launch_fill_int(s, 3, out, 0);
+    const int n = nodeEnd - nodeBegin;
+    if (n > 0) sc_stats_kernel<<<(n + 255) / 256, 256, 0, s>>>(nodeBegin, nodeEnd, a, out);
+}
 
 __global__ void sc_fill_accum_kernel(int nPad, float3* accum)
 {
@@ -723,7 +1084,7 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
                           unsigned int* hist, const int** sortedIdxOut, const Tuning& tu)
 {
     sc_prepare_kernel<<<nbs(numBodies, 64), 64, 0, s>>>(numBodies, a);
-    sc_active_disp_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P);
+    sc_active_disp_kernel<<<nbs(na.nPad, kDispThreads), kDispThreads, 0, s>>>(na.nPad, a, X, P);
     sc_reach_kernel<<<nbs(numBodies, 64), 64, 0, s>>>(numBodies, a);
     sc_keys_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X);
     const bool inAlt = launch_radix_sort(s, na.nPad, sortBits, a.keys, a.idx, keysAlt, idxAlt, hist, false);
@@ -732,13 +1093,27 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
     launch_fill_int(s, a.tableTotal, a.cellStart, -1);
     sc_range_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, ks);
     sc_stage_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
+    if (a.fineTotal > 0) {
+        sc_fine_keys_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P);
+        const bool fAlt = launch_radix_sort(s, na.nPad, a.fineBits, a.fkeys, a.fidx, a.fkeysAlt, a.fidxAlt, hist, false);
+        const unsigned int* fks = fAlt ? a.fkeysAlt : a.fkeys;
+        const int* fvs = fAlt ? a.fidxAlt : a.fidx;
+        launch_fill_int(s, a.fineTotal, a.fcellStart, -1);
+        sc_fine_range_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, fks);
+        sc_fine_stage_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, P, rest, fks, fvs, ks);
+    }
     const int wpb = (tu.scBuildWaves == 1 || tu.scBuildWaves == 2) ? tu.scBuildWaves : 4;
-    if (tu.scBuildNodes == 4)
-        sc_build_kernel<4><<<nbs(nbs(na.nPad, 4), wpb), 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest);
-    else if (tu.scBuildNodes == 16)
-        sc_build_kernel<16><<<nbs(nbs(na.nPad, 16), wpb), 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest);
+    const int npw = (tu.scBuildNodes == 4 || tu.scBuildNodes == 16) ? tu.scBuildNodes : 8;
+    const int blocks = nbs(nbs(na.nPad, npw), wpb);
+    const int xcd = tu.scBuildXcd ? 1 : 0;
+    const int grid = xcd ? xcd_grid(blocks) : blocks;   // the XCD remap needs a multiple of 8
+    if (npw == 4)
+        sc_build_kernel<4><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);
+    else if (npw == 16)
+        sc_build_kernel<16><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);
     else
-        sc_build_kernel<8><<<nbs(nbs(na.nPad, 8), wpb), 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest);
+        sc_build_kernel<8><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);
+    if (a.fineTotal > 0) sc_slots_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a);
     sc_fill_accum_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a.accum);
     *sortedIdxOut = vs;
     return inAlt;

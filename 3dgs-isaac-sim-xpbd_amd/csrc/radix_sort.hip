@@ -3,8 +3,8 @@
 // 8-bit digits, one three-kernel pass per digit:
 //   hist    one block per tile of kTileKeys keys: digit histogram in LDS,
 //           written digit-major hist[digit * tiles + tile];
-//   scan    one block: exclusive scan of hist (digit-major order makes the
-//           global order digit, then tile, then position: stable);
+//   scan    one block per digit: exclusive scan of its tile counts plus the
+//           digit total (digit, then tile, then position: stable);
 //   scatter one block per tile: keys are ranked in rounds of 256 (one per
 //           thread, in index order); within a wave64 the rank among equal
 //           digits comes from 8 ballots (match mask) and a popcount, across
@@ -39,39 +39,59 @@ __global__ void __launch_bounds__(kSortThreads) radix_hist_kernel(int n, const u
     hist[threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of `count` entries by one block of 1024 threads.
-__global__ void __launch_bounds__(1024) radix_scan_kernel(int count, unsigned int* __restrict__ data)
+// Per digit d (one block each): exclusive scan of its `tiles` counts
+// hist[d * tiles + tile] in place, and the digit's total in dsum[d].  The
+// scatter adds the digits' exclusive offsets, so the global order is digit,
+// then tile, then position (stable).
+__global__ void __launch_bounds__(kSortThreads) radix_scan_kernel(int tiles, unsigned int* __restrict__ hist,
+                                                                 unsigned int* __restrict__ dsum)
 {
-    __shared__ unsigned int part[1024];
-    const int t = threadIdx.x;
-    const int per = (count + 1023) / 1024;
-    const int b0 = t * per, b1 = min(count, b0 + per);
-    unsigned int s = 0;
-    for (int i = b0; i < b1; ++i) s += data[i];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const unsigned int v = (t >= off) ? part[t - off] : 0u;
+    __shared__ unsigned int wsum[kSortThreads / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    unsigned int* h = hist + (size_t)blockIdx.x * tiles;
+    unsigned int carry = 0;
+    for (int base = 0; base < tiles; base += kSortThreads) {
+        const int i = base + t;
+        const unsigned int v = i < tiles ? h[i] : 0u;
+        unsigned int x = v;   // inclusive scan inside the wave
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+            const unsigned int o = __shfl_up(x, m, 64);
+            if (lane >= m) x += o;
+        }
+        if (lane == 63) wsum[wave] = x;
         __syncthreads();
-        part[t] += v;
+        unsigned int before = carry;
+        for (int w = 0; w < wave; ++w) before += wsum[w];
+        if (i < tiles) h[i] = before + x - v;
+        unsigned int chunk = 0;
+        for (int w = 0; w < kSortThreads / 64; ++w) chunk += wsum[w];
+        carry += chunk;
         __syncthreads();
     }
-    unsigned int run = (t == 0) ? 0u : part[t - 1];
-    for (int i = b0; i < b1; ++i) {
-        const unsigned int v = data[i];
-        data[i] = run;
-        run += v;
-    }
+    if (t == 0) dsum[blockIdx.x] = carry;
 }
 
 __global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(
     int n, const unsigned int* __restrict__ keysIn, const int* __restrict__ valsIn, unsigned int* __restrict__ keysOut,
-    int* __restrict__ valsOut, int shift, int tiles, const unsigned int* __restrict__ offsets, int iota)
+    int* __restrict__ valsOut, int shift, int tiles, const unsigned int* __restrict__ offsets,
+    const unsigned int* __restrict__ dsum, int iota)
 {
     __shared__ unsigned int run[256];          // keys of each digit already placed by this tile
     __shared__ unsigned int wcnt[4][256];      // per-wave digit counts of the current round
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    run[t] = offsets[t * tiles + blockIdx.x];
+    // exclusive offsets of the digits (scan of the 256 digit totals)
+    run[t] = dsum[t];
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const unsigned int v = (t >= off) ? run[t - off] : 0u;
+        __syncthreads();
+        run[t] += v;
+        __syncthreads();
+    }
+    const unsigned int digitBase = run[t] - dsum[t];
+    __syncthreads();
+    run[t] = digitBase + offsets[t * tiles + blockIdx.x];
     const int base = blockIdx.x * kTileKeys;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int r = 0; r < kSortRounds; ++r) {
@@ -109,6 +129,7 @@ __global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(
 }
 
 int radix_sort_tiles(int n) { return (n + kTileKeys - 1) / kTileKeys; }
+size_t radix_sort_hist_size(int n) { return 256 * (size_t)radix_sort_tiles(n) + 256; }
 
 // keys/vals ping-pong between (k0, v0) and (k1, v1); the sorted result ends
 // in (k0, v0) when the number of passes is even, else in (k1, v1).  Returns
@@ -126,8 +147,9 @@ bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         radix_hist_kernel<<<tiles, kSortThreads, 0, s>>>(n, kin, shift, tiles, hist);
-        radix_scan_kernel<<<1, 1024, 0, s>>>(256 * tiles, hist);
-        radix_scatter_kernel<<<tiles, kSortThreads, 0, s>>>(n, kin, vin, kout, vout, shift, tiles, hist, (iota && p == 0) ? 1 : 0);
+        radix_scan_kernel<<<256, kSortThreads, 0, s>>>(tiles, hist, hist + 256 * (size_t)tiles);
+        radix_scatter_kernel<<<tiles, kSortThreads, 0, s>>>(n, kin, vin, kout, vout, shift, tiles, hist,
+                                                            hist + 256 * (size_t)tiles, (iota && p == 0) ? 1 : 0);
         unsigned int* tk = kin; kin = kout; kout = tk;
         int* tv = vin; vin = vout; vout = tv;
     }

@@ -63,6 +63,9 @@ struct BodyDev {
     // [scTableBase, scTableBase + scTableMask + 1) of the shared cell arrays
     int scOn, scWithin, scTableBase, scTableMask;
     float scDc, scSearch, scExcl2, scFastRel, scCellOverride;
+    // candidate search through the predicted-position grid (extension; same lists)
+    int scFine, scFineBase, scFineMask;
+    float scFineInv;
 };
 
 // forward.cu:128-136 KinCollider, same field meaning.

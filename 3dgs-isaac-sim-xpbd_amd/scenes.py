@@ -257,7 +257,7 @@ class GripperScript:
     normal, linearly over `ramp_frames` frames, then release."""
 
     def __init__(self, scene, seed=7, grippers=2, attach=500, pull=0.1, twist_deg=30.0, ramp_frames=60,
-                 radius=0.02, half_length=0.04):
+                 radius=0.02, half_length=0.04, tree=None):
         rng = np.random.Generator(np.random.PCG64(seed))
         pos = scene["pos"].astype(np.float64)
         semi = np.asarray(scene["semi"], np.float64)
@@ -266,7 +266,7 @@ class GripperScript:
         self.radius, self.half_length = radius, half_length
         self.pull, self.twist = pull, np.deg2rad(twist_deg)
         self.ramp = ramp_frames
-        tree = cKDTree(pos)
+        tree = cKDTree(pos) if tree is None else tree   # a tree of scene["pos"] may be shared
         self.g = []
         for _ in range(grippers):
             d = rng.standard_normal(3)

@@ -346,3 +346,54 @@ def run_alone(req, lib=None, shapes=True, tuning=None):
         out.append(b.positions(0))
     b.close()
     return out
+
+
+class TemplatePool:
+    """Request payloads drawn from a pool of prepared objects: `count` objects
+    with sizes log-uniform in [nmin, nmax] (each with its full recipe and a
+    shared k-d tree); a request takes a random template and its own gripper
+    script.  Stands in for clients that upload their own graphs."""
+
+    def __init__(self, count, nmin, nmax, seed=5):
+        from scipy.spatial import cKDTree
+        rng = np.random.Generator(np.random.PCG64(seed))
+        sizes = scenes.log_uniform_sizes(count, nmin, nmax, rng)
+        self.items = []
+        for j, n in enumerate(sizes):
+            scene, recipe, _ = scenes.object_request(int(n), seed * 7919 + j)
+            self.items.append((scene, recipe, cKDTree(scene["pos"].astype(np.float64))))
+
+    def request(self, rid, rng, life, arrival):
+        scene, recipe, tree = self.items[int(rng.integers(len(self.items)))]
+        n = scene["pos"].shape[0]
+        script = scenes.GripperScript(scene, seed=int(rng.integers(1 << 30)), attach=min(500, max(4, n // 10)),
+                                      tree=tree)
+        return Request(rid, scene, recipe, script, life, arrival)
+
+
+def poisson_stream(pool, rate_per_s, ticks, life=(60, 600), tick_hz=60.0, seed=3, steady_start=True):
+    """Requests arriving as a Poisson process of `rate_per_s` over `ticks`
+    ticks, lifetimes uniform in `life` frames.  steady_start adds the
+    population a stream that had been running for long would hold at tick 0
+    (Little's law: rate x mean life objects, each with a residual lifetime
+    drawn from the equilibrium distribution, density proportional to
+    P(life > t))."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    lo, hi = life
+    if steady_start:
+        mean_life = 0.5 * (lo + hi)
+        count = rng.poisson(rate_per_s * mean_life / tick_hz)
+        for _ in range(count):
+            while True:        # residual life t ~ P(L > t) / E[L] by rejection
+                t = rng.uniform(1, hi)
+                if rng.uniform() * (hi - lo) <= min(hi - lo, hi - t):
+                    break
+            out.append(pool.request(len(out), rng, int(t), 0))
+    t = 0.0
+    while True:
+        t += rng.exponential(tick_hz / rate_per_s)
+        if t >= ticks:
+            break
+        out.append(pool.request(len(out), rng, int(rng.integers(lo, hi + 1)), int(t)))
+    return out

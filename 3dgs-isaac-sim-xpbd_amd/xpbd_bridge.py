@@ -329,6 +329,25 @@ class XPBD:
         self._check(self.lib.xpbd_ext_profile(int(substeps), buf, len(buf)), "xpbd_ext_profile")
         return {k: (float(v[0]), int(v[1])) for k, v in json.loads(buf.value.decode()).items()}
 
+    def ext_set_self_collision_fine(self, enabled):
+        """Candidate search of one-object bodies through the predicted-position
+        grid (True, default) or the reference's coarse walk (False); same lists."""
+        fn = self.lib.xpbd_ext_set_self_collision_fine
+        fn.argtypes = [ctypes.c_int]
+        fn.restype = ctypes.c_int
+        self._check(fn(int(bool(enabled))), "xpbd_ext_set_self_collision_fine")
+
+    def ext_autotune(self, substeps=2, hash_table=False):
+        """Pick the fastest launch shapes on the current state (restored
+        afterwards; results unchanged).  Returns the tuner's report."""
+        import json
+        fn = self.lib.xpbd_ext_autotune
+        fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        fn.restype = ctypes.c_int
+        buf = ctypes.create_string_buffer(4096)
+        self._check(fn(int(substeps), 1 if hash_table else 0, buf, len(buf)), "xpbd_ext_autotune")
+        return json.loads(buf.value.decode())
+
     def ext_set_volume_clusters(self, off=None, members=None, min_members=4):
         """Explicit volume clusters: node i leads {i} + members[off[i]:off[i+1]].
         None returns to the k-ring rule."""

@@ -142,14 +142,180 @@ def kernel_bytes(workload, n, m, sc_table, sc_passes, vol):
     return out
 
 
+def plain_frame(single, script, f):
+    """One frame of one object the way the singleton ABI runs it: every
+    substep a synchronous call (native/xpbd_capi.cu:218-225), attachments
+    after the first step on the first frame, then the shape update."""
+    t, p, d = script.colliders(f)
+    single.set_colliders(0, t, p, d)
+    att = script.attached(f)
+    for s in range(SUBSTEPS):
+        if s == (1 if f == 0 else 0):
+            if att is not None:
+                single.set_attached([(0, att[0], att[1])])
+            elif f == script.ramp:
+                single.set_attached([(0, (), ())])
+        single.step(1)
+    # xpbd_compute_shapes with the outputs left in device memory (null host
+    # pointers): one object, the deformed count read back
+    single.lib.xpbd_batch_compute_shapes(single.h, 0, 1e-2, None, None, single.counts[0])
+
+
+def bench_batched(args):
+    """Many objects in one batch against the same objects one per call."""
+    import serving
+    log(f"generating {args.objects} objects, sizes log-uniform in [5k, 200k]")
+    t = time.perf_counter()
+    objs = scenes.object_mix(args.objects, 5_000, 200_000, seed=11)
+    reqs = [serving.Request(j, *o, life=10 ** 6) for j, o in enumerate(objs)]
+    total = sum(r.n for r in reqs)
+    log(f"objects ready in {time.perf_counter() - t:.1f}s: {len(reqs)} objects, {total} Gaussians")
+
+    b = serving.build_batch(reqs)
+    frame = 0
+    for _ in range(max(args.warmup, 1)):
+        serving.step_frame(b, [(r.script, frame) for r in reqs])
+        b.sync()
+        frame += 1
+    tune = b.autotune(substeps=2, hash_tables=args.tune_hash)
+    log("autotune: " + json.dumps(tune))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        serving.step_frame(b, [(r.script, frame) for r in reqs])
+        b.sync()
+        frame += 1
+    dt_b = time.perf_counter() - t0
+    rate_b = total * SUBSTEPS * args.steps / dt_b
+    log(f"batched: {args.steps} frames in {dt_b:.3f}s -> {rate_b / 1e6:.1f} M Gaussian-substeps/s")
+    prof = b.profile(SUBSTEPS)
+    tuning = b.tuning()
+    b.close()
+
+    singles = []
+    for r in reqs:
+        s = serving.build_batch([r])
+        singles.append(s)
+    frame = 0
+    for _ in range(max(args.warmup, 1)):
+        for s, r in zip(singles, reqs):
+            plain_frame(s, r.script, frame)
+        frame += 1
+    plain_steps = max(1, args.steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        for s, r in zip(singles, reqs):
+            plain_frame(s, r.script, frame)
+        frame += 1
+    dt_p = time.perf_counter() - t0
+    rate_p = total * SUBSTEPS * plain_steps / dt_p
+    log(f"plain: {plain_steps} frames in {dt_p:.3f}s -> {rate_p / 1e6:.1f} M Gaussian-substeps/s")
+    for s in singles:
+        s.close()
+    out = {
+        "metric": "batched vs plain GPU speedup",
+        "value": rate_b / rate_p,
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt_b / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(reqs)} objects, N log-uniform in [5k, 200k] (seed 11), ellipsoids with semi-axes "
+                               f"U[0.05, 0.30] m, full recipe per object (kNN k=12 distance, node+3NN volume, k-means "
+                               f"shape clusters of 64, self-collision within the body, nextpow2(2N) cells), own "
+                               f"gripper pair per object, shapes every frame; {SUBSTEPS} substeps x {ITERS} iters",
+                   "objects": len(reqs), "gaussians": total},
+        "batched": {"gaussian_substeps_per_s": rate_b, "ms_per_frame": 1e3 * dt_b / args.steps, "tuning": tuning,
+                    "autotune": tune, "kernels_ms": prof},
+        "plain": {"gaussian_substeps_per_s": rate_p, "ms_per_frame": 1e3 * dt_p / plain_steps, "frames": plain_steps,
+                  "path": "one single-object batch per object, one synchronous call per substep, objects in turn"},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_served(args):
+    """The request loop: Poisson arrivals, HBM-budget admission, one frame of
+    every live object per tick."""
+    import serving
+    sys.setswitchinterval(0.0005)   # the builder thread must not hold the tick loop back
+    log(f"preparing {args.templates} object templates, N log-uniform in [2k, 500k]")
+    t = time.perf_counter()
+    pool = serving.TemplatePool(args.templates, 2_000, 500_000, seed=5)
+    reqs = serving.poisson_stream(pool, 20.0, args.ticks, seed=3, steady_start=not args.cold)
+    log(f"templates ready in {time.perf_counter() - t:.1f}s; {len(reqs)} requests "
+        f"({sum(r.arrival == 0 for r in reqs)} at tick 0)")
+    budget_ms = args.tick_budget if args.tick_budget > 0 else None
+    srv = serving.Server(hbm_budget=200e9, tick_budget_ms=budget_ms, max_segments=args.max_segments)
+    last = [time.perf_counter()]
+
+    def progress(rec):
+        if time.perf_counter() - last[0] > 20:
+            last[0] = time.perf_counter()
+            log(f"tick {rec['tick']}: {rec['ms']:.1f} ms, {rec['live_objects']} live objects, "
+                f"{rec['live_gaussians']} Gaussians, {rec['segments']} batches, {rec['queued']} queued")
+
+    t0 = time.perf_counter()
+    srv.run([r for r in reqs], max_ticks=args.ticks, on_tick=progress)
+    wall = time.perf_counter() - t0
+    srv.close()
+    recs = srv.log[args.skip:]
+    ms = np.array([r["ms"] for r in recs])
+    gs = np.array([r["live_gaussians"] for r in recs], np.float64)
+    started = [r for r in reqs if r.started is not None]
+    waits = np.array([(r.started - r.arrival) for r in started], np.float64)
+    out = {
+        "metric": "p99 frame latency",
+        "value": float(np.percentile(ms, 99)),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": len(recs),
+        "warmup": args.skip,
+        "ms_per_step": float(ms.mean()),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "served stream: Poisson arrivals 20/s, N log-uniform [2k, 500k] "
+                               f"({args.templates} templates), life U[60, 600] frames, full recipe, own gripper "
+                               "script per request, 200 GB HBM budget, one frame of every live object per tick"
+                               + (f", admission also capped by a {budget_ms} ms tick budget" if budget_ms else ""),
+                   "ticks": args.ticks, "requests": len(reqs), "steady_start": not args.cold},
+        "p50_ms": float(np.percentile(ms, 50)),
+        "p99_ms": float(np.percentile(ms, 99)),
+        "max_ms": float(ms.max()),
+        "gaussian_substeps_per_s": float(gs.sum() * SUBSTEPS / (ms.sum() / 1e3)),
+        "mean_live_objects": float(np.mean([r["live_objects"] for r in recs])),
+        "mean_live_gaussians": float(gs.mean()),
+        "admission_wait_ticks_p50": float(np.percentile(waits, 50)) if waits.size else None,
+        "admission_wait_ticks_p99": float(np.percentile(waits, 99)) if waits.size else None,
+        "merges": srv.merges,
+        "wall_s": wall,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--workload", choices=("full", "distance"), default="full")
+    ap.add_argument("--workload", choices=("full", "distance", "batched", "served"), default="full")
+    ap.add_argument("--objects", type=int, default=64, help="batched: object count")
+    ap.add_argument("--tune-hash", action="store_true", help="batched: let the autotuner size the hash tables")
+    ap.add_argument("--templates", type=int, default=40, help="served: prepared objects requests draw from")
+    ap.add_argument("--ticks", type=int, default=150, help="served: ticks to run")
+    ap.add_argument("--skip", type=int, default=10, help="served: ticks left out of the latency figures")
+    ap.add_argument("--tick-budget", type=float, default=0.0, help="served: admit only while ticks stay under this (ms)")
+    ap.add_argument("--max-segments", type=int, default=6, help="served: batches before merging")
+    ap.add_argument("--cold", action="store_true", help="served: start empty instead of at the steady population")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run a streaming kernel of known traffic (256 MiB read + 256 MiB written, x3) "
                          "to scale the FETCH_SIZE / WRITE_SIZE counters")
@@ -166,6 +332,10 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "mul", "bytes_read": a.numel() * 4, "bytes_written": b.numel() * 4}))
         return
+    if args.workload == "batched":
+        return bench_batched(args)
+    if args.workload == "served":
+        return bench_served(args)
 
     log(f"generating {args.n}-Gaussian scene ({args.workload})")
     t = time.perf_counter()
@@ -188,6 +358,9 @@ def main():
         run_frame(sim, script, frame, frame == 0, shapes)
         frame += 1
     log(f"warmup {args.warmup} frames done")
+    tune = sim.ext_autotune(2) if not args.no_tune else None
+    if tune is not None:
+        log("autotune: " + json.dumps(tune))
     t0 = time.perf_counter()
     for _ in range(args.steps):
         run_frame(sim, script, frame, frame == 0, shapes)
@@ -241,7 +414,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl, "gaussians": n, "edges": e, "substeps_per_step": SUBSTEPS, "iters": ITERS,
                    "shape_clusters": int(recipe["clusters"].max() + 1) if shapes else 0,
-                   "selfcol_stats_last": list(stats), "selfcol_cell_dc_search": list(grid)},
+                   "selfcol_stats_last": list(stats), "selfcol_cell_dc_search": list(grid), "autotune": tune},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
                      "traffic_source": os.path.relpath(PROFILE, ROOT),

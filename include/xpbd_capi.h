@@ -146,6 +146,11 @@ int xpbd_ext_get_inv_mass(float* invMass, int n);
  * into json[len].  The state advances exactly as with xpbd_step. */
 int xpbd_ext_profile(int substeps, char* json, int len);
 
+/* Launch-shape autotuner of the instance (after the first step), as
+ * xpbd_batch_autotune: the state is restored and results do not depend on
+ * the launch shapes it picks; flags bit 0 also sizes the hash table. */
+int xpbd_ext_autotune(int substeps, int flags, char* json, int len);
+
 /* Volume clusters given explicitly instead of by the k-ring rule: node i leads
  * the cluster {i} + members[off[i] .. off[i+1]) (an empty range = no cluster).
  * minMembers is the rest-precompute gate (the reference rule rejects clusters
@@ -165,6 +170,14 @@ int xpbd_ext_volume_topology(int* leaders, int* memberCount, int* revCount, int*
  * 0 keeps nextpow2(max(1024, 2n)) cells, else a power of two >= 1024.  Applies
  * to later builds; persists like the solver settings. */
 int xpbd_ext_set_self_collision_grid(float cellSize, int tableSize);
+
+/* How candidate lists are found for a body whose nodes all belong to one
+ * object with self-contact on: 1 (default) through a second grid on the
+ * predicted positions (cells of edge >= the search radius) with the results
+ * put in the reference's scan order, 0 by the reference's own walk of the 27
+ * frame-start cells.  The lists, counts and statistics are identical; only
+ * the work differs.  The oracle accepts and ignores it. */
+int xpbd_ext_set_self_collision_fine(int enabled);
 
 /* Artefacts of the last self-collision build (after the first step): cell key
  * of every node (table size = inactive / non-finite), nodes in stable key
@@ -307,9 +320,9 @@ int xpbd_batch_copy_object(xpbd_batch* dst, int dstObj, xpbd_batch* src, int src
 /* Launch-shape autotuner: times candidates of each tunable over `substeps`
  * substeps from the current state (restored afterwards) and keeps the
  * fastest; writes a JSON report.  Results do not depend on these choices.
- * tuning[5]: Jacobi tiles per block, volume mode (-1 auto, 0 thread, 1 wave),
- * cluster shape waves per block, self-collision build waves per block and
- * nodes per wave.  flags bit 0 also picks the self-collision hash-table size
+ * tuning[6]: Jacobi tiles per block, volume mode (-1 auto, 0 thread, 1 wave),
+ * cluster shape waves per block, self-collision build waves per block, nodes
+ * per wave and XCD node runs (1) or round-robin blocks (0).  flags bit 0 also picks the self-collision hash-table size
  * (1, 4 or 16 x nextpow2(2N) cells per object), which can reorder candidate
  * lists that overflow their 32 slots. */
 int xpbd_batch_autotune(xpbd_batch* b, int substeps, int flags, char* json, int len);

@@ -280,6 +280,10 @@ int xpbd_ext_set_self_collision_grid(float cellSize, int tableSize)
     return 0;
 }
 
+// Extension: how the GPU library finds candidates; the serial walk here is the
+// reference's in either case.
+int xpbd_ext_set_self_collision_fine(int) { return 0; }
+
 // Extension: the last build's artefacts: cell keys (by node), node order
 // after the stable key sort, candidate counts and candidate lists [n x 32].
 int xpbd_ext_self_collision_lists(unsigned int* keys, int* sorted, int* counts, int* contacts, int n)

@@ -53,6 +53,7 @@ def reset_extension_settings():
         lib.xpbd_ext_set_gauss_seidel(0, 0)
         lib.xpbd_ext_set_node_order(1)
         lib.xpbd_ext_set_self_collision_grid(ctypes.c_float(0.0), 0)
+        lib.xpbd_ext_set_self_collision_fine(1)
         lib.xpbd_set_self_collision(0, ctypes.c_float(1.5), ctypes.c_float(2.0), 0)
         lib.xpbd_set_object_shape(ctypes.c_float(0.0))
         lib.xpbd_set_colliders(0, None, None, None, ctypes.c_float(0.0), ctypes.c_float(0.0))

@@ -158,7 +158,7 @@ def test_autotune_keeps_results(product_path):
         for f in range(frames):
             assert np.array_equal(got[k][f], ref[k][f]), f"object {k} frame {f}"
     # every tuned shape also gives the same results
-    alt, _ = batch_run(objs, frames, tuning=[1, 1, 1, 1, 16])
+    alt, _ = batch_run(objs, frames, tuning=[1, 1, 1, 1, 16, 0])
     for k in range(len(objs)):
         assert np.array_equal(alt[k][-1], ref[k][-1])
     print("autotune report", report)

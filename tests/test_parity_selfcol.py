@@ -118,3 +118,65 @@ def test_falling_bodies_drift_report(product_path, oracle_path):
     assert np.all(np.isfinite(gpu.positions()))
     gpu.destroy()
     cpu.destroy()
+
+
+def squeezed(sim, scene, cell, table, fine=True):
+    """One object with self-contact inside it, squeezed to 25 % along z (many
+    candidates, lists that overflow their 32 slots) and moving."""
+    sim.create(scene)
+    sim.set_solver(iters=2, dt=1 / 600, vel_damping=0.0)
+    sim.set_constraints(distance=False, shape=False, angle=False, volume=False)
+    sim.set_ground(False, (0, 0, 1), 0.0, gravity=0.0)
+    sim.set_colliders([], [], [])
+    sim.set_object_shape(0.0)
+    sim.ext_set_self_collision_grid(cell, table)
+    if sim.backend == "hip-gfx950":
+        sim.ext_set_self_collision_fine(fine)
+    sim.set_self_collision(True, 1.5, 2.0, True)
+
+
+@pytest.mark.parametrize("cell,table", [
+    (0.0, 0),          # the reference rule: cell = search + 2 max move
+    (0.0, 1024),       # a small table: stencil buckets collide
+    (0.012, 4096),     # a fixed cell shorter than the reach, colliding buckets
+])
+def test_fine_scan_lists_bit_exact(product_path, oracle_path, cell, table):
+    """The predicted-position grid path gives the reference's lists: against
+    the oracle and against the GPU's own coarse walk, from identical states."""
+    scene = scenes.make_ellipsoid_scene(12000, 8, semi=(0.08, 0.06, 0.05), seed=53)
+    gpu, cpu = XPBD(product_path), XPBD(oracle_path)
+    for sim in (gpu, cpu):
+        squeezed(sim, scene, cell, table)
+        sim.step()
+    rng = np.random.Generator(np.random.PCG64(3))
+    pos = scene["pos"].copy()
+    pos[:, 2] *= 0.25
+    vel = rng.normal(0.0, 0.4, size=pos.shape).astype(np.float32)
+    over = 0
+    for it in range(3):
+        for sim in (gpu, cpu):
+            sim.ext_set_state(pos, vel)
+        gpu.ext_set_self_collision_fine(True)
+        gpu.step()
+        cpu.step()
+        kf, of, cf, lf = gpu.ext_self_collision_lists()
+        kc, oc, cc, lc = cpu.ext_self_collision_lists()
+        np.testing.assert_array_equal(kf, kc)
+        np.testing.assert_array_equal(of, oc)
+        np.testing.assert_array_equal(cf, cc)
+        np.testing.assert_array_equal(lf, lc)
+        assert gpu.self_collision_stats() == cpu.self_collision_stats()
+        over += cpu.self_collision_stats()[2]
+        # the same state through the coarse walk
+        gpu.ext_set_state(pos, vel)
+        gpu.ext_set_self_collision_fine(False)
+        gpu.step()
+        _, _, cg, lg = gpu.ext_self_collision_lists()
+        np.testing.assert_array_equal(cg, cf)
+        np.testing.assert_array_equal(lg, lf)
+        pos, vel = cpu.ext_get_state()
+    print(f"fine scan cell={cell} table={table}: lists equal the oracle's and the coarse walk's; "
+          f"stats {cpu.self_collision_stats()}, overflow seen {over}")
+    assert over > 0
+    gpu.destroy()
+    cpu.destroy()
