@@ -100,10 +100,15 @@ class Request:
 
 
 def hbm_bytes(n, table):
-    """Device memory of one object: ~1.1 KB per Gaussian (state, sliced
-    distance entries, volume and shape clusters, self-collision buffers and
-    candidate lists, shapes) plus 8 B per hash cell."""
-    return 1100 * n + 8 * table
+    """Device memory of one object: ~1.7 KB per Gaussian (state, sliced
+    distance entries, volume and shape clusters, self-collision buffers,
+    candidate lists and the sorted-candidate pool, shapes) plus 8 B per cell of
+    its two hash tables (the coarse one and the nextpow2(2n) predicted-position
+    one)."""
+    fine = 1024
+    while fine < 2 * n:
+        fine <<= 1
+    return 1700 * n + 8 * (table + fine)
 
 
 class _Segment:
@@ -144,33 +149,40 @@ class Server:
         self.log = []
         self.merges = 0
         self._jobs = []
-        self._pending_jobs = collections.deque()
+        self._pending_jobs = {"admit": collections.deque(), "merge": collections.deque()}
         self._lock = threading.Lock()
-        self._worker = None
+        self._worker = {}
+        self._closers = []
 
     # -- background builds -----------------------------------------------------
-    def _run_jobs(self):
+    # One worker thread per kind of build (admissions, merges), so a merge
+    # never holds new requests back.
+    def _run_jobs(self, kind):
         while True:
             with self._lock:
-                if not self._pending_jobs:
-                    self._worker = None
+                q = self._pending_jobs[kind]
+                if not q:
+                    self._worker[kind] = None
                     return
-                job = self._pending_jobs.popleft()
-            kind, payload = job
+                payload = q.popleft()
             try:
-                b = build_batch(payload["objs"], self.lib, tuning=self.tuning)
-                payload["batch"] = b
+                payload["batch"] = build_batch(payload["objs"], self.lib, tuning=self.tuning)
             except Exception as e:  # surfaced on the next tick
                 payload["error"] = e
             with self._lock:
-                self._jobs.append(job)
+                self._jobs.append((kind, payload))
+
+    def _close_later(self, batches):
+        t = threading.Thread(target=lambda: [b.close() for b in batches], daemon=True)
+        t.start()
+        self._closers.append(t)
 
     def _submit_job(self, kind, payload):
         with self._lock:
-            self._pending_jobs.append((kind, payload))
-            if self._worker is None:
-                self._worker = threading.Thread(target=self._run_jobs, daemon=True)
-                self._worker.start()
+            self._pending_jobs[kind].append(payload)
+            if self._worker.get(kind) is None:
+                self._worker[kind] = threading.Thread(target=self._run_jobs, args=(kind,), daemon=True)
+                self._worker[kind].start()
 
     def _collect_jobs(self):
         with self._lock:
@@ -195,14 +207,16 @@ class Server:
                         freeze(b, k)
                 for o in olds:
                     self.segments.remove(o)
-                    o.batch.close()
+                # freeing device memory can wait for the device: off the tick path
+                self._close_later([o.batch for o in olds])
                 self.segments.append(seg)
 
     # -- admission ---------------------------------------------------------------
     def submit(self, req):
+        """Queue a request; admission runs at the tick boundary, so the
+        requests that arrive together are built into one batch."""
         req.arrival = self.tick_no if req.arrival is None else req.arrival
         self.queue.append(req)
-        self._admit()
 
     def _live_nodes(self):
         return sum(s.live_nodes() for s in self.segments) + sum(r.n for r in self.building)
@@ -241,7 +255,8 @@ class Server:
         empty = [s for s in pick if not any(s.alive)]
         for s in empty:                       # nothing to carry over
             self.segments.remove(s)
-            s.batch.close()
+        if empty:
+            self._close_later([s.batch for s in empty])
         pick = [s for s in pick if any(s.alive)]
         if len(pick) < 2 and not any(s in wasteful for s in pick):
             return
@@ -312,6 +327,7 @@ class Server:
         while max_ticks > 0 and (pending or self.queue or self.building or self._any_alive()):
             while pending and pending[0].arrival <= self.tick_no:
                 self.submit(pending.pop(0))
+            self._admit()
             if not self._any_alive() and self.building:
                 self._collect_jobs()
                 if not self._any_alive():
@@ -325,11 +341,13 @@ class Server:
     def close(self):
         while True:
             with self._lock:
-                busy = self._worker is not None
+                busy = any(w is not None for w in self._worker.values())
             if not busy:
                 break
             time.sleep(0.01)
         self._collect_jobs()
+        for t in self._closers:
+            t.join()
         for s in self.segments:
             s.batch.close()
         self.segments = []
