@@ -567,6 +567,18 @@ int xpbd_ext_set_self_collision_grid(float cellSize, int tableSize)
     return 0;
 }
 
+int xpbd_ext_angle_pairs(int* pairNext, float* restCos, int m)
+{
+    Global& g = G();
+    if (!g.input) return fail("xpbd_ext_angle_pairs: call xpbd_create first");
+    if (m != g.input->rowOff[g.n]) return fail("xpbd_ext_angle_pairs: size mismatch");
+    AngleTopo t;
+    build_angle_pairs(*g.input, t);
+    if (pairNext && m) std::memcpy(pairNext, t.pairNext.data(), sizeof(int) * m);
+    if (restCos && m) std::memcpy(restCos, t.restCos.data(), sizeof(float) * m);
+    return 0;
+}
+
 int xpbd_ext_set_self_collision_fine(int enabled)
 {
     Global& g = G();

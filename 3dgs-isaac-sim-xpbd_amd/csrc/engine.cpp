@@ -257,6 +257,9 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     dgsAlphaKey_.clear();
     shapeClusterOf_.assign(B, std::vector<int>());
     shcDevValid_ = true;
+    angTopo_.assign(B, AngleTopo());
+    angBodies_.assign(B, 0);
+    angDevValid_ = false;
     shcK_ = 0;
 
     // device node order of every body (caller order or Morton order)
@@ -622,6 +625,9 @@ void Engine::fillBodyDev(int b, BodyDev& d) const
     // was skipped (forward.cu:12066, 12310-12324)
     d.volGSOn = (s.useVolume && s.volGS && vgsDevValid_ && b < (int)vgsIn_.size() && vgsIn_[b]) ? 1 : 0;
     d.volumeOn = (s.useVolume && !d.volGSOn) ? 1 : 0;
+    d.angleOn = (s.useAngle && angDevValid_) ? 1 : 0;
+    d.angleCompliance = s.angleCompliance;
+    d.angleBlend = s.angleBlend;
     d.volCompliance = s.volCompliance;
     d.volMinMembers = volTopo_[b].minMembers;
     d.shapeOn = s.useShape ? 1 : 0;
@@ -709,7 +715,7 @@ std::vector<int> Engine::planSignature() const
     const int B = numBodies();
     int iters = set_[0].iters;
     int anyDist = 0, anyCol = 0, anyGround = 0, anyPress = 0, anyVol = 0, anyShape = 0, anyCGround = 0;
-    int anyObj = 0, anyBoxes = 0, anySc = 0, anyDistGS = 0, anyVolGS = 0, anyFine = 0;
+    int anyObj = 0, anyBoxes = 0, anySc = 0, anyDistGS = 0, anyVolGS = 0, anyFine = 0, anyAngle = 0;
     for (int b = 0; b < B; ++b) {
         const BodyDev& d = hostBodies_[b];
         anyObj |= d.objOn;
@@ -724,6 +730,7 @@ std::vector<int> Engine::planSignature() const
         anyPress |= press_[b].active ? 1 : 0;
         anyVol |= d.volumeOn;
         anyShape |= set_[b].useShape ? 1 : 0;
+        anyAngle |= d.angleOn;
         anyCGround |= (cM_.p && contactGround_[b] && set_[b].groundEnabled) ? 1 : 0;
     }
     const int volWave = anyVol ? (tune_.volMode >= 0 ? (tune_.volMode ? 1 : 0) : (volAvgMembers_ >= kVolWaveMembers ? 1 : 0))
@@ -734,7 +741,7 @@ std::vector<int> Engine::planSignature() const
             anyCGround, anyObj, anyBoxes, anySc, anyObj ? objTotal_ : 0, anySc ? scTableTotal_ : 0,
             anySc ? scSortBits_ : 0, nDistGS, nVolGS, device_alloc_epoch(), (anyShape && shcDevValid_) ? shcK_ : 0,
             allocEpoch_, tune_.jacobiTiles, tune_.volMode, tune_.shapeClusterWaves, tune_.scBuildWaves,
-            tune_.scBuildNodes, tune_.scBuildXcd, anyFine};
+            tune_.scBuildNodes, tune_.scBuildXcd, anyFine, anyAngle};
 }
 
 // Event pairs around kernel launches (profile mode only).
@@ -826,13 +833,15 @@ void Engine::recordSubstep(Timer* t)
         // after a Gauss-Seidel pass (which visits only its colours' nodes) it
         // runs as its own pass.
         const int nDistGS = sig[17], nVolGS = sig[18];
-        enum { kNone, kDistGS, kJacobi, kVolGS, kVolGather, kShape };
+        enum { kNone, kDistGS, kJacobi, kVolGS, kVolGather, kShape, kAngle };
+        const bool anglePass = sig[29] != 0;
         int lastPass = kNone;
         if (nDistGS) lastPass = kDistGS;
         if (anyDist) lastPass = kJacobi;
         if (nVolGS) lastPass = kVolGS;
         if (volPass) lastPass = kVolGather;
         if (shapePass) lastPass = kShape;
+        if (anglePass) lastPass = kAngle;
         // the press slab precedes self-collision; colliders and ground follow it
         // (forward.cu:12474-12560)
         const unsigned preEpi = scPass ? (lastPass != kNone ? (epi & kEpiPress) : 0u) : epi;
@@ -874,7 +883,15 @@ void Engine::recordSubstep(Timer* t)
             XPBD_T_BEGIN("shape_match");
             ShapeClusterArgs sca = shcArgs();
             if (!sig[20]) sca.K = 0;
-            launch_shape_match(stream_, na, pin, pout, rest_.p, rowOff_.p, nbr_.p, sca, preEpi, ea, tune_);
+            launch_shape_match(stream_, na, pin, pout, rest_.p, rowOff_.p, nbr_.p, sca,
+                               lastPass == kShape ? preEpi : 0u, ea, tune_);
+            XPBD_T_END();
+            std::swap(pin, pout);
+        }
+        // umbrella angle constraint after shape matching (forward.cu:12486-12517)
+        if (anglePass) {
+            XPBD_T_BEGIN("angle");
+            launch_angle(stream_, na, pin, pout, angArgs(), preEpi, ea);
             XPBD_T_END();
             std::swap(pin, pout);
         }
@@ -938,6 +955,12 @@ bool Engine::prepare(std::string& err, bool advance)
     if ((anyVolGS || vgsDevValid_) && !ensureVolumeGS(err)) return false;
     if ((anyDistGS || dgsDevValid_) && !ensureDistanceGS(err)) return false;
     if (!ensureShapeClusters(err)) return false;
+    // angle arrays cover exactly the bodies with the constraint on
+    std::vector<char> angOn(numBodies(), 0);
+    bool anyAngle = false;
+    for (int b = 0; b < numBodies(); ++b) { angOn[b] = set_[b].useAngle ? 1 : 0; anyAngle |= set_[b].useAngle; }
+    if (angOn != angBodies_) { angBodies_ = angOn; angDevValid_ = false; }
+    if (anyAngle && !ensureAngle(err)) return false;
     return uploadParams(err);
 }
 

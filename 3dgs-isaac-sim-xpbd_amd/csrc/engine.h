@@ -99,6 +99,15 @@ struct VolumeGSTopo {
     int lowerBound = 0;
     std::vector<int> order, colorOff;
 };
+// Umbrella angle pairing of one body (forward.cu:10708-10819), per directed
+// CSR entry in caller order: paired neighbour (-1 none) and rest cosine (2 none).
+struct AngleTopo {
+    bool built = false;
+    std::vector<int> pairNext;
+    std::vector<float> restCos;
+};
+void build_angle_pairs(const BodyInput& in, AngleTopo& t);
+
 void build_distance_gs(const BodyInput& in, DistanceGSTopo& t);
 void build_volume_gs(const VolumeTopo& v, int n, VolumeGSTopo& t);
 
@@ -254,7 +263,21 @@ public:
     // Shape matching over node clusters (caller ids, -1 = none; null = 1-ring).
     bool setShapeClusters(int b, const int* clusterOf, std::string& err);
 
+    // Umbrella angle pairing of body b (directed entries in caller CSR order).
+    bool anglePairs(int b, int* pairNext, float* restCos, std::string& err);
+
 private:
+    // angle constraint (forward.cu:4990-5130): pairing per body, device
+    // entry arrays, and per node the contributions it receives in the
+    // serial order (centre caller index, entry, role)
+    std::vector<AngleTopo> angTopo_;
+    std::vector<char> angBodies_;           // bodies the device arrays were built for
+    bool angDevValid_ = false;
+    DevBuf<int> angPair_, angRevOff_, angRevEnt_;
+    DevBuf<float> angRestCos_;
+    DevBuf<float4> angDp_;
+    bool ensureAngle(std::string& err);
+    AngleArgs angArgs() const;
     std::vector<std::vector<int>> shapeClusterOf_;
     bool shcDevValid_ = true;
     int shcK_ = 0;

@@ -79,6 +79,22 @@ void launch_vol_apply(hipStream_t s, const NodeArgs& na, const float4* Pin, floa
                       unsigned epi, const EpiArgs& ea);
 void launch_vol_J(hipStream_t s, const NodeArgs& na, const float3* X, const VolumeArgs& va, float* outJ);
 
+// Umbrella angle constraint over the device CSR: entry e of centre i pairs
+// nbr[e] with pair[e]; dp[3e + role] holds the contribution to the centre
+// (role 0), nbr[e] (1) and pair[e] (2) with w = 1 when it counts; node j
+// sums the entries revEnt[revOff[j] .. revOff[j+1]) in that order.
+struct AngleArgs {
+    const int* rowOff;        // [nPad + 1]
+    const int* nbr;           // [M]
+    const int* pair;          // [M] device id or -1
+    const float* restCos;     // [M]
+    float4* dp;               // [3M]
+    const int* revOff;        // [nPad + 1]
+    const int* revEnt;        // [contributions] 3e + role
+};
+void launch_angle(hipStream_t s, const NodeArgs& na, const float4* Pin, float4* Pout, const AngleArgs& aa,
+                  unsigned epi, const EpiArgs& ea);
+
 // Graph-coloured Gauss-Seidel: one launch per colour, in place on P.
 void launch_dist_gs_alpha(hipStream_t s, int m, const int2* ab, const float* stiff, const int* tileBody,
                           const BodyDev* bodies, float* alpha);

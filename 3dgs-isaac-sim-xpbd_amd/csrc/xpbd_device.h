@@ -48,6 +48,9 @@ struct BodyDev {
     float pressLo, pressHi;
     // contact ellipsoids (forward.cu:7358-7399)
     int contactColliders, contactGround;
+    // umbrella angle constraint (forward.cu:4990-5130)
+    int angleOn;
+    float angleCompliance, angleBlend;
     // volume constraint (forward.cu:8531-9243)
     int volumeOn;
     float volCompliance;

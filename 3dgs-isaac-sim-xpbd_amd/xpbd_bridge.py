@@ -174,6 +174,8 @@ class XPBD:
         self._check(self.lib.xpbd_create(inp["pos"].shape[0], inp["pos"].ravel(), inp["scales"].ravel(),
                                          inp["rots"].ravel(), inp["opacity"], e, edges, rest, stiff), "xpbd_create")
         self.n = inp["pos"].shape[0]
+        ed = np.asarray(inp["edges"]).reshape(-1, 2)
+        self._edges_valid = int(np.count_nonzero((ed[:, 0] != ed[:, 1]) & (ed >= 0).all(1) & (ed < self.n).all(1)))
 
     def set_solver(self, iters=20, dt=1 / 60, under_relax=0.6, vel_damping=0.01, stiffness_scale=1.0,
                    inv_mass_scale=1.0, dist_compliance=0.0, shape_compliance=0.0, shape_blend=0.1,
@@ -328,6 +330,18 @@ class XPBD:
         buf = ctypes.create_string_buffer(1 << 16)
         self._check(self.lib.xpbd_ext_profile(int(substeps), buf, len(buf)), "xpbd_ext_profile")
         return {k: (float(v[0]), int(v[1])) for k, v in json.loads(buf.value.decode()).items()}
+
+    def ext_angle_pairs(self):
+        """Umbrella angle pairing per directed entry (reference CSR order):
+        (pairNext int32[m], restCos float32[m])."""
+        fn = self.lib.xpbd_ext_angle_pairs
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        fn.restype = ctypes.c_int
+        m = 2 * int(self._edges_valid)
+        pair = np.empty(max(m, 1), np.int32)
+        cos = np.empty(max(m, 1), np.float32)
+        self._check(fn(_ptr(pair), _ptr(cos), m), "xpbd_ext_angle_pairs")
+        return pair[:m], cos[:m]
 
     def ext_set_self_collision_fine(self, enabled):
         """Candidate search of one-object bodies through the predicted-position

@@ -179,6 +179,12 @@ int xpbd_ext_set_self_collision_grid(float cellSize, int tableSize);
  * the work differs.  The oracle accepts and ignores it. */
 int xpbd_ext_set_self_collision_fine(int enabled);
 
+/* Umbrella angle pairing (forward.cu:10708-10819) of the instance, per
+ * directed graph entry in the reference's CSR order (both directions of each
+ * edge, input order): the paired neighbour pairNext[m] (-1 none) and the rest
+ * cosine restCos[m] (2 none); m = 2 x the number of valid edges. */
+int xpbd_ext_angle_pairs(int* pairNext, float* restCos, int m);
+
 /* Artefacts of the last self-collision build (after the first step): cell key
  * of every node (table size = inactive / non-finite), nodes in stable key
  * order, candidate counts counts[n] and candidate lists contacts[n*32]

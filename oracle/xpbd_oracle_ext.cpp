@@ -16,6 +16,7 @@
 #include "oracle_shapes.h"
 #include "oracle_selfcol.h"
 #include "oracle_gs.h"
+#include "oracle_angle.h"
 
 namespace xo {
 
@@ -42,6 +43,7 @@ void extensions_free()
     dgs = DistanceGS();
     vgs = VolumeGS();
     shc = ShapeClusters();
+    ang = AnglePairs();
 }
 void extensions_on_reset() {}
 // The colourings are built on first use after being enabled, and the volume
@@ -81,6 +83,13 @@ void extensions_iteration(std::vector<V3>*& pin, std::vector<V3>*& pout, int)
     if (s.useShape) {
         if (shc.on) shape_cluster_matching(*pin, *pout);
         else shape_matching(*pin, *pout);
+        std::swap(pin, pout);
+    }
+    // angle constraint after shape matching (forward.cu:12486-12517); the
+    // pairing uses the positions the graph was uploaded with
+    if (s.useAngle) {
+        if (!ang.built) angle_build(g.rest);
+        angle_pass(*pin, *pout);
         std::swap(pin, pout);
     }
 }
@@ -283,6 +292,17 @@ int xpbd_ext_set_self_collision_grid(float cellSize, int tableSize)
 // Extension: how the GPU library finds candidates; the serial walk here is the
 // reference's in either case.
 int xpbd_ext_set_self_collision_fine(int) { return 0; }
+
+// Extension: the umbrella angle pairing (forward.cu:10708-10819), per entry.
+int xpbd_ext_angle_pairs(int* pairNext, float* restCos, int m)
+{
+    if (g.n <= 0) return fail("xpbd_ext_angle_pairs: call xpbd_create first");
+    if (m != g.off[g.n]) return fail("xpbd_ext_angle_pairs: size mismatch");
+    if (!ang.built) angle_build(g.rest);
+    if (pairNext && m) std::memcpy(pairNext, ang.pairNext.data(), sizeof(int) * m);
+    if (restCos && m) std::memcpy(restCos, ang.restCos.data(), sizeof(float) * m);
+    return 0;
+}
 
 // Extension: the last build's artefacts: cell keys (by node), node order
 // after the stable key sort, candidate counts and candidate lists [n x 32].
