@@ -66,6 +66,7 @@ template struct DevBuf<unsigned int>;
 template struct DevBuf<unsigned char>;
 template struct DevBuf<double>;
 template struct DevBuf<int2>;
+template struct DevBuf<int4>;
 
 // wave64-per-cluster volume solve from this mean member count up
 static const double kVolWaveMembers = 24.0;
@@ -661,9 +662,10 @@ void Engine::fillBodyDev(int b, BodyDev& d) const
     d.scFastRel = kScFastRelScale * dc;
     d.scCellOverride = scCellOverride_[b];
     // the predicted-position grid serves bodies whose candidates are all
-    // same-object ones (self-contact on, every node in one object)
+    // same-object ones (self-contact on, every node in one object); its scan
+    // order key packs the coarse sorted position into 27 bits
     const bool fine = scDeviceValid_ && fineTable_[b] > 0 && s.scWithinBody && b < (int)objSingle_.size() &&
-                      objSingle_[b] && scFineOn_;
+                      objSingle_[b] && scFineOn_ && nPad_ <= (1 << 27);
     d.scFine = fine ? 1 : 0;
     d.scFineBase = fine ? fineBase_[b] : 0;
     d.scFineMask = fine ? fineTable_[b] - 1 : 0;

@@ -423,8 +423,9 @@ private:
     int fineTotal_ = 0, fineBits_ = 0;
     bool scFineOn_ = true;
     DevBuf<unsigned int> fkeys_, fkeysAlt_;
-    DevBuf<int> fidx_, fidxAlt_, fcellStart_, fcellEnd_, fineRank_, coarseRank_, accCount_, accOff_, accUsed_;
+    DevBuf<int> fidx_, fidxAlt_, fcellStart_, fcellEnd_, coarseRank_, accCount_, accOff_, accUsed_;
     DevBuf<float4> fineP_, fineR_;
+    DevBuf<int4> fineC_;
     DevBuf<int2> accList_;
     std::vector<float> scSpacing_, scCellOverride_;
     std::vector<int> scTable_, scTableBase_, scTableOverride_;

@@ -250,9 +250,9 @@ bool Engine::autotune(int substeps, int flags, std::string& report, std::string&
         {"jacobi_tiles_per_block", &tune_.jacobiTiles, {4, 2, 1}, true},
         {"volume_mode", &tune_.volMode, {0, 1}, anyVol},
         {"shape_cluster_waves_per_block", &tune_.shapeClusterWaves, {4, 2, 1}, anyClusters},
-        {"selfcol_nodes_per_wave", &tune_.scBuildNodes, {8, 4, 16}, anySc},
+        {"selfcol_nodes_per_wave", &tune_.scBuildNodes, {32, 16, 64, 8, 4}, anySc},
         {"selfcol_waves_per_block", &tune_.scBuildWaves, {4, 2, 1}, anySc},
-        {"selfcol_xcd_runs", &tune_.scBuildXcd, {1, 0}, anySc},
+        {"selfcol_xcd_runs", &tune_.scBuildXcd, {16, 1, 0, 64}, anySc},
     };
     std::string rep = "{";
     float base = 0.0f;

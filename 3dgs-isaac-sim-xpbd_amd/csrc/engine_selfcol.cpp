@@ -94,7 +94,7 @@ SelfColArgs Engine::scArgs() const
     a.fcellEnd = fcellEnd_.p;
     a.fineP = fineP_.p;
     a.fineR = fineR_.p;
-    a.fineRank = fineRank_.p;
+    a.fineC = fineC_.p;
     a.accList = accList_.p;
     a.accOff = accOff_.p;
     a.accCount = fine ? accCount_.p : nullptr;
@@ -294,13 +294,13 @@ bool Engine::ensureSelfCol(std::string& err)
     while ((1LL << fineBits_) <= fineTotal) ++fineBits_;
     fkeys_.alloc(n); fkeysAlt_.alloc(n); fidx_.alloc(n); fidxAlt_.alloc(n);
     fcellStart_.alloc(fineTotal_); fcellEnd_.alloc(fineTotal_);
-    fineP_.alloc(n); fineR_.alloc(n); fineRank_.alloc(n); coarseRank_.alloc(n);
+    fineP_.alloc(n); fineR_.alloc(n); fineC_.alloc(n); coarseRank_.alloc(n);
     // sorted candidate lists handed to the one-thread-per-node slot kernel:
     // room for 64 per node on average (a node that finds the pool full fills
     // its slots in the build kernel)
     accList_.alloc(n * 64); accOff_.alloc(n); accCount_.alloc(n); accUsed_.alloc(1);
     if (!fkeys_.p || !fkeysAlt_.p || !fidx_.p || !fidxAlt_.p || !fcellStart_.p || !fcellEnd_.p || !fineP_.p ||
-        !fineR_.p || !fineRank_.p || !coarseRank_.p || !accList_.p || !accOff_.p || !accCount_.p || !accUsed_.p) {
+        !fineR_.p || !fineC_.p || !coarseRank_.p || !accList_.p || !accOff_.p || !accCount_.p || !accUsed_.p) {
         err = "self-collision: hipMalloc failed";
         return false;
     }

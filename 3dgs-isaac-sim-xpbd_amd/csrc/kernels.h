@@ -17,8 +17,9 @@ struct Tuning {
     int volMode = -1;           // volume cluster solve: -1 by mean member count, 0 thread, 1 wave per cluster
     int shapeClusterWaves = 4;  // clusters (one wave each) per cluster shape-matching block: 1, 2, 4
     int scBuildWaves = 4;       // waves per self-collision candidate-build block: 1, 2, 4
-    int scBuildNodes = 8;       // nodes per wave of the candidate build: 4, 8, 16
-    int scBuildXcd = 1;         // candidate build: contiguous node runs per XCD (1) or round-robin blocks (0)
+    int scBuildNodes = 32;      // nodes per wave of the candidate build: 4, 8, 16, 32, 64
+    int scBuildXcd = 16;        // candidate build: runs of that many blocks dealt round-robin over the XCDs
+                                // (2-256), one contiguous run per XCD (1), plain block order (0)
 };
 
 // Node-space launch description shared by every per-node kernel.
@@ -160,7 +161,7 @@ struct SelfColArgs {
     int* fcellEnd;
     float4* fineP;             // [nPad] fine order: p, node id bits
     float4* fineR;             // [nPad] fine order: rest, coarse key bits
-    int* fineRank;             // [nPad] fine order: coarse sorted position
+    int4* fineC;               // [nPad] fine order: coarse cell of x, coarse sorted position
     int2* accList;             // [accCap] pool of sorted accepted-candidate lists (code, priority bits)
     int* accOff;               // [nPad] start of a node's list in the pool
     int* accCount;             // [nPad] list length, -1 when the build kernel filled the slots itself

@@ -193,6 +193,14 @@ __device__ __forceinline__ int sc_cell(float v, float invCell)
     return (int)floorf(fminf(fmaxf(v * invCell, -1.0e9f), 1.0e9f));
 }
 
+// Key of a cell of the predicted-position grid: a hash of the row (cy, cz)
+// plus cx, so the three x-neighbours of a row have consecutive keys and sit
+// next to each other in the sorted order.
+__device__ __forceinline__ unsigned int sc_fine_key(int cx, int cy, int cz, unsigned int mask)
+{
+    return ((((unsigned int)cy * 19349663u) ^ ((unsigned int)cz * 83492791u)) + (unsigned int)cx) & mask;
+}
+
 // Per body: clear the build counters and the max displacement, and decide
 // whether the active restriction applies (boxes valid, finite body motion).
 __global__ void sc_prepare_kernel(int B, SelfColArgs a)
@@ -365,7 +373,7 @@ __global__ void __launch_bounds__(256) sc_fine_keys_kernel(int nPad, SelfColArgs
         if (isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p.x) && isfinite(p.y) && isfinite(p.z)) {
             const float g = B.scFineInv;
             key = (unsigned int)B.scFineBase +
-                  sc_hash(sc_cell(p.x, g), sc_cell(p.y, g), sc_cell(p.z, g), (unsigned int)B.scFineMask);
+                  sc_fine_key(sc_cell(p.x, g), sc_cell(p.y, g), sc_cell(p.z, g), (unsigned int)B.scFineMask);
         }
     }
     a.fkeys[i] = key;
@@ -384,8 +392,10 @@ __global__ void __launch_bounds__(256) sc_fine_range_kernel(int nPad, SelfColArg
 }
 
 // Fine-order copies for the fine scan: (p, node id), (rest, coarse key) and
-// the coarse sorted position (the candidate's place in the coarse scan).
-__global__ void __launch_bounds__(256) sc_fine_stage_kernel(int nPad, SelfColArgs a, const float4* __restrict__ P,
+// (coarse cell of x, coarse sorted position): the candidate's place in the
+// coarse scan.
+__global__ void __launch_bounds__(256) sc_fine_stage_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
+                                                           const float4* __restrict__ P,
                                                            const float3* __restrict__ rest,
                                                            const unsigned int* __restrict__ keysSorted,
                                                            const int* __restrict__ sortedIdx,
@@ -395,11 +405,12 @@ __global__ void __launch_bounds__(256) sc_fine_stage_kernel(int nPad, SelfColArg
     if (t >= nPad || keysSorted[t] >= (unsigned int)a.fineTotal) return;
     const int j = sortedIdx[t];
     const float4 p = P[j];
-    const float3 r = rest[j];
+    const float3 r = rest[j], x = X[j];
     const int rk = a.coarseRank[j];
+    const float inv = a.scInvCell[a.tileBody[j / kTile]];
     a.fineP[t] = make_float4(p.x, p.y, p.z, __int_as_float(j));
     a.fineR[t] = make_float4(r.x, r.y, r.z, __uint_as_float(coarseSorted[rk]));   // j's coarse key
-    a.fineRank[t] = rk;
+    a.fineC[t] = make_int4(sc_cell(x.x, inv), sc_cell(x.y, inv), sc_cell(x.z, inv), rk);
 }
 
 // Distance from x to the stencil cell at offset o along one axis, given the
@@ -519,23 +530,33 @@ constexpr int kAccChunk = 1024;  // list-pool entries a wave reserves at once (>
 // Fine scan (bodies whose nodes all belong to one object, with self-contact
 // inside it on): the only test is |p_i - p_j| < search, so every candidate
 // lies in the 27 cells of edge >= search around p_i in a second grid built on
-// the predicted positions.  Those few cells are tested instead of the coarse
+// the predicted positions.  Those few cells (nine rows of three cells, each
+// row one stretch of the sorted order) are tested instead of the coarse
 // stencil; a passing j is a reference candidate when its coarse bucket is one
 // of the stencil's buckets, and it takes the scan position (first stencil
-// offset of that bucket, sorted position inside it).  The accepted ones are
-// sorted by that position in LDS and enter the slots in that order, so the
+// offset of that bucket, sorted position inside it), read off its coarse cell
+// when that is one of the 27 around x_i.  The accepted ones are sorted by that
+// position (a 32-bit key) in LDS and enter the slots in that order, so the
 // lists and counters equal the coarse scan's.  A node with more than
 // kFineList accepted candidates takes the coarse scan.
 template <int NPW>
-__global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) sc_build_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
                                                       const float4* __restrict__ P, const float3* __restrict__ rest,
                                                       int xcd)
 {
-    __shared__ int4 lacc[4][kFineList];
-    __shared__ int2 lsort[4][kFineList];
-    // xcd: blocks of one XCD take one contiguous run of nodes (Morton order,
-    // so neighbouring cells), which keeps their buckets in that XCD's L2
-    const int blk = xcd ? xcd_block() : (int)blockIdx.x;
+    __shared__ __align__(16) unsigned int lkey[4][kFineList];   // accepted: scan-order key
+    __shared__ int2 lpay[4][kFineList];                           // accepted: node id, squared distance
+    // Blocks go to the XCDs round-robin.  xcd > 1: each XCD takes runs of xcd
+    // consecutive blocks (nodes in Morton order, so neighbouring cells stay in
+    // that XCD's L2 while crowded regions spread over every XCD); xcd == 1:
+    // one contiguous run per XCD
+    int blk = (int)blockIdx.x;
+    if (xcd == 1) {
+        blk = xcd_block();
+    } else if (xcd > 1) {
+        const int x = blk & 7, r = blk >> 3;
+        blk = ((r / xcd) * 8 + x) * xcd + r % xcd;
+    }
     const int wave = (blk * (int)blockDim.x + (int)threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int wl = threadIdx.x >> 6;
@@ -571,82 +592,153 @@ __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, 
                 // ---- fine scan ----
                 const float ginv = B.scFineInv;
                 const unsigned int fmask = (unsigned int)B.scFineMask;
-                const unsigned int fh = sc_hash(sc_cell(p4.x, ginv) + ox, sc_cell(p4.y, ginv) + oy,
-                                                sc_cell(p4.z, ginv) + oz, fmask);
-                bool ffirst = lane < 27;
-                for (int c2 = 0; c2 < 27; ++c2)
-                    if ((unsigned int)lane_i((int)fh, c2) == fh && c2 < c) ffirst = false;
-                int rs = 0, rn = 0;
-                if (ffirst) {
-                    rs = a.fcellStart[B.scFineBase + fh];
-                    rn = rs >= 0 ? a.fcellEnd[B.scFineBase + fh] - rs : 0;
+                const int fbase = B.scFineBase;
+                const float excl2 = B.scExcl2;
+                const int gx = sc_cell(p4.x, ginv), gy = sc_cell(p4.y, ginv), gz = sc_cell(p4.z, ginv);
+                // lane c < 27: fine cell (gx + ox, gy + oy, gz + oz) and its range
+                const unsigned int fk = sc_fine_key(gx + ox, gy + oy, gz + oz, fmask);
+                int cs = -1, ce = 0;
+                if (lane < 27) {
+                    cs = a.fcellStart[fbase + fk];
+                    if (cs >= 0) ce = a.fcellEnd[fbase + fk];
                 }
-                int pre;
-                const int total = sc_run_prefix(rn, lane, pre);
+                // first stencil offset holding each coarse bucket (two offsets
+                // share one when their cells collide in the hash)
+                int firstC = c;
+                for (int c2 = 26; c2 >= 0; --c2)
+                    if ((unsigned int)lane_i((int)hg, c2) == hg) firstC = c2;
+                const bool dup = __ballot(lane < 27 && firstC != lane) != 0ull;
+                // Row r = c / 3 (one (oy, oz)) covers keys k_r - 1 .. k_r + 1 (k_r
+                // the key of its middle cell), one contiguous stretch of the
+                // sorted order.  Rows whose stretches wrap around the table or
+                // come within two keys of each other would visit a key twice:
+                // then each distinct cell key is walked once instead.
+                bool clash = false;
+                for (int r2 = 0; r2 < 9; ++r2) {
+                    const unsigned int k2 = (unsigned int)lane_i((int)fk, 3 * r2 + 1);
+                    const unsigned int dk = fk > k2 ? fk - k2 : k2 - fk;
+                    if (lane != 3 * r2 + 1 && dk <= 2u) clash = true;
+                }
+                const bool perCell = __ballot(lane < 27 && ox == 0 && (clash || fk == 0u || fk == fmask)) != 0ull;
+                // row r: first run position rowP (past every run position when
+                // the row is empty) and sorted index minus run position rowD
+                int rowP[9], rowD[9];
+                int rs = 0, pre = 0, total = 0;
+                if (!perCell) {
+                    // the non-empty cells of a row are adjacent in the sorted order
+                    const int lo = cs >= 0 ? cs : 0x7fffffff, hi = cs >= 0 ? ce : 0;
+                    const int lo1 = __shfl_down(lo, 1, 64), lo2 = __shfl_down(lo, 2, 64);
+                    const int hi1 = __shfl_down(hi, 1, 64), hi2 = __shfl_down(hi, 2, 64);
+                    const int rlo = min(lo, min(lo1, lo2)), rhi = max(hi, max(hi1, hi2));
+#pragma unroll
+                    for (int r = 0; r < 9; ++r) {
+                        const int l = lane_i(rlo, 3 * r), h2 = lane_i(rhi, 3 * r);
+                        const int n = h2 > l ? h2 - l : 0;
+                        rowP[r] = n > 0 ? total : 0x7fffffff;
+                        rowD[r] = l - total;
+                        total += n;
+                    }
+                } else {
+                    bool ffirst = lane < 27;
+                    for (int c2 = 0; c2 < 27; ++c2)
+                        if ((unsigned int)lane_i((int)fk, c2) == fk && c2 < c) ffirst = false;
+                    int rn = 0;
+                    if (ffirst && cs >= 0) {
+                        rs = cs;
+                        rn = ce - cs;
+                    }
+                    total = sc_run_prefix(rn, lane, pre);
+                }
                 int nacc = 0;
                 for (int base = 0; base < total; base += 64) {
                     const int v = base + lane;
-                    const int t = sc_run_pos(v, rs, pre);
-                    bool acc = false;
-                    float pr = 0.0f;
-                    int k1 = 27, j = 0, rk = 0;
-                    unsigned int ck = 0xffffffffu;
+                    int t;
+                    if (!perCell) {
+                        int d = 0;
+#pragma unroll
+                        for (int r = 0; r < 9; ++r)
+                            if (v >= rowP[r]) d = rowD[r];
+                        t = v + d;
+                    } else {
+                        t = sc_run_pos(v, rs, pre);
+                    }
+                    bool acc = false, inside = true;
+                    unsigned int key = 0u, ck = 0u;
+                    int j = 0, rk = 0;
+                    float d2 = 0.0f;
                     if (v < total) {
                         const float4 pj = a.fineP[t];
                         const float4 qj = a.fineR[t];
-                        rk = a.fineRank[t];
-                        ck = __float_as_uint(qj.w);
+                        const int4 cj = a.fineC[t];
                         j = __float_as_int(pj.w);
-                        if (j != i) {
-                            const float ex = p4.x - pj.x, ey = p4.y - pj.y, ez = p4.z - pj.z;
-                            const float d2 = ex * ex + ey * ey + ez * ez;
-                            if (d2 < search2) {
-                                const float qx = q.x - qj.x, qy = q.y - qj.y, qz = q.z - qj.z;
-                                const float q2 = qx * qx + qy * qy + qz * qz;
-                                if (q2 >= B.scExcl2) {
-                                    pr = sqrtf(d2);
-                                    acc = true;
-                                }
-                            }
-                        }
+                        ck = __float_as_uint(qj.w);
+                        rk = cj.w;
+                        const float ex = p4.x - pj.x, ey = p4.y - pj.y, ez = p4.z - pj.z;
+                        d2 = ex * ex + ey * ey + ez * ez;
+                        const float qx = q.x - qj.x, qy = q.y - qj.y, qz = q.z - qj.z;
+                        const float q2 = qx * qx + qy * qy + qz * qz;
+                        acc = (j != i) & (d2 < search2) & (q2 >= excl2);
+                        // scan position: a candidate whose coarse cell is one of
+                        // the 27 around x_i sits in that offset's bucket
+                        const int dx = cj.x - cx, dy = cj.y - cy, dz = cj.z - cz;
+                        inside = ((unsigned int)(dx + 1) <= 2u) & ((unsigned int)(dy + 1) <= 2u) &
+                                 ((unsigned int)(dz + 1) <= 2u);
+                        key = ((unsigned int)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) << 27) | (unsigned int)rk;
                     }
-                    // scan position: first stencil offset of the candidate's coarse
-                    // bucket; lanes 0-26 hold the stencil buckets, so each
-                    // accepted candidate is matched against all of them at once
-                    unsigned long long pend = __ballot(acc);
-                    while (pend) {
-                        const int l = __builtin_ctzll(pend);
-                        pend &= pend - 1ull;
+                    if (dup) {
+                        const int f = __shfl(firstC, inside ? (int)(key >> 27) : 0, 64);
+                        key = ((unsigned int)f << 27) | (unsigned int)rk;
+                    }
+                    // any other candidate is one only when its bucket collides
+                    // with a stencil bucket: lanes 0-26 hold those, so each is
+                    // matched against all of them at once
+                    unsigned long long om = __ballot(acc && !inside);
+                    while (om) {
+                        const int l = __builtin_ctzll(om);
+                        om &= om - 1ull;
                         const unsigned int ckl = (unsigned int)lane_i((int)ck, l);
                         const unsigned long long hit = __ballot(lane < 27 && hg == ckl);
-                        if (lane == l) k1 = hit ? __builtin_ctzll(hit) : 27;
+                        if (lane == l) {
+                            if (hit) key = ((unsigned int)__builtin_ctzll(hit) << 27) | (unsigned int)rk;
+                            else acc = false;
+                        }
                     }
-                    acc = acc && k1 < 27;
                     const unsigned long long am = __ballot(acc);
                     const int slot = nacc + (int)__builtin_amdgcn_mbcnt_hi((unsigned int)(am >> 32),
                                                                           __builtin_amdgcn_mbcnt_lo((unsigned int)am, 0u));
-                    if (acc && slot < kFineList) lacc[wl][slot] = make_int4(j, __float_as_int(pr), k1, rk);
+                    if (acc && slot < kFineList) {
+                        lkey[wl][slot] = key;
+                        lpay[wl][slot] = make_int2(j, __float_as_int(d2));
+                    }
                     nacc += __popcll(am);
                 }
                 if (nacc <= kFineList) {
                     coarse = false;
+                    // pad the key list to whole uint4 reads with keys above all
+                    const int n4 = (nacc + 3) & ~3;
+                    if (lane < n4 - nacc) lkey[wl][nacc + lane] = 0xffffffffu;
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                    // rank of each accepted entry by (first offset, sorted position)
+                    // rank of each accepted entry by its scan-order key (first
+                    // offset, coarse sorted position: all distinct)
                     constexpr int R = kFineList / 64;
-                    int4 e[R];
-                    int rk[R];
+                    unsigned int ek[R];
+                    int2 ep[R];
+                    int er[R];
 #pragma unroll
                     for (int k = 0; k < R; ++k) {
-                        e[k] = (k * 64 + lane < nacc) ? lacc[wl][k * 64 + lane] : make_int4(0, 0, 27, 0);
-                        rk[k] = 0;
+                        const int s = k * 64 + lane;
+                        ek[k] = s < nacc ? lkey[wl][s] : 0xffffffffu;
+                        ep[k] = s < nacc ? lpay[wl][s] : make_int2(0, 0);
+                        er[k] = 0;
                     }
-                    for (int m = 0; m < nacc; ++m) {
-                        const int4 em = lacc[wl][m];
+                    for (int m = 0; m < n4; m += 4) {
+                        const uint4 km = *reinterpret_cast<const uint4*>(&lkey[wl][m]);
 #pragma unroll
                         for (int k = 0; k < R; ++k)
-                            rk[k] += (em.z < e[k].z || (em.z == e[k].z && em.w < e[k].w)) ? 1 : 0;
+                            er[k] += (km.x < ek[k] ? 1 : 0) + (km.y < ek[k] ? 1 : 0) + (km.z < ek[k] ? 1 : 0) +
+                                     (km.w < ek[k] ? 1 : 0);
                     }
                     // the sorted list goes to the slot kernel (one thread per node)
                     // when the shared list pool has room; the wave takes pool
@@ -662,20 +754,21 @@ __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, 
                         chunkPos += nacc;
 #pragma unroll
                         for (int k = 0; k < R; ++k)
-                            if (k * 64 + lane < nacc) a.accList[(size_t)off + rk[k]] = make_int2(e[k].x, e[k].y);
+                            if (k * 64 + lane < nacc) a.accList[(size_t)off + er[k]] = ep[k];
                         if (lane == 0) a.accOff[i] = off;
                         deferred = nacc;
                     } else {
+                        __builtin_amdgcn_wave_barrier();
 #pragma unroll
                         for (int k = 0; k < R; ++k)
-                            if (k * 64 + lane < nacc) lsort[wl][rk[k]] = make_int2(e[k].x, e[k].y);
+                            if (k * 64 + lane < nacc) lpay[wl][er[k]] = ep[k];
                         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
                         __builtin_amdgcn_wave_barrier();
                         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                         for (int r = 0; r < nacc; ++r) {
-                            const int2 e = lsort[wl][r];
+                            const int2 e = lpay[wl][r];
                             S.insert(__builtin_amdgcn_readfirstlane(e.x), 1,
-                                     __int_as_float(__builtin_amdgcn_readfirstlane(e.y)), lane);
+                                     sqrtf(__int_as_float(__builtin_amdgcn_readfirstlane(e.y))), lane);
                         }
                     }
                     __builtin_amdgcn_wave_barrier();   // the next node reuses the wave's LDS rows
@@ -801,7 +894,7 @@ __global__ void __launch_bounds__(256) sc_slots_kernel(int nPad, SelfColArgs a)
     int cnt = 0, over = 0;
     for (int r = 0; r < n; ++r) {
         const int2 e = list[r];
-        const float p = __int_as_float(e.y);
+        const float p = sqrtf(__int_as_float(e.y));   // the list holds squared distances
         if (cnt < CAP) {
 #pragma unroll
             for (int s = 0; s < CAP; ++s)
@@ -1100,19 +1193,27 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
         const int* fvs = fAlt ? a.fidxAlt : a.fidx;
         launch_fill_int(s, a.fineTotal, a.fcellStart, -1);
         sc_fine_range_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, fks);
-        sc_fine_stage_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, P, rest, fks, fvs, ks);
+        sc_fine_stage_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P, rest, fks, fvs, ks);
     }
     const int wpb = (tu.scBuildWaves == 1 || tu.scBuildWaves == 2) ? tu.scBuildWaves : 4;
-    const int npw = (tu.scBuildNodes == 4 || tu.scBuildNodes == 16) ? tu.scBuildNodes : 8;
+    const int npw = (tu.scBuildNodes == 4 || tu.scBuildNodes == 8 || tu.scBuildNodes == 16 || tu.scBuildNodes == 64)
+                        ? tu.scBuildNodes
+                        : 32;
     const int blocks = nbs(nbs(na.nPad, npw), wpb);
-    const int xcd = tu.scBuildXcd ? 1 : 0;
-    const int grid = xcd ? xcd_grid(blocks) : blocks;   // the XCD remap needs a multiple of 8
+    // the XCD remaps need whole rounds of 8 blocks (of 8 runs of xcd blocks);
+    // the blocks past the node space return at once
+    const int xcd = tu.scBuildXcd < 0 ? 0 : (tu.scBuildXcd > 256 ? 256 : tu.scBuildXcd);
+    const int grid = xcd == 0 ? blocks : (xcd == 1 ? xcd_grid(blocks) : (blocks + 8 * xcd - 1) / (8 * xcd) * (8 * xcd));
     if (npw == 4)
         sc_build_kernel<4><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);
+    else if (npw == 8)
+        sc_build_kernel<8><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);
     else if (npw == 16)
         sc_build_kernel<16><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);
+    else if (npw == 64)
+        sc_build_kernel<64><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);
     else
-        sc_build_kernel<8><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);
+        sc_build_kernel<32><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);
     if (a.fineTotal > 0) sc_slots_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a);
     sc_fill_accum_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a.accum);
     *sortedIdxOut = vs;

@@ -80,10 +80,20 @@ def run_frame(sim, script, frame, first, shapes):
         sim.shapes_device(1e-2)
 
 
-def cpu_baseline(scene, script, workload, recipe, seconds_budget=20.0):
+def cpu_baseline(scene, script, workload, recipe, seconds_budget=20.0, full_sample=100_000):
     """Oracle (serial restatement) on a bounded sample of the same workload:
-    substeps of the first frame(s) until the time budget is spent."""
+    substeps of the first frame(s) until the time budget is spent.  The full
+    recipe is sampled on a smaller scene from the same generator, recipe and
+    gripper script: at 1M the oracle's serial self-collision walk over the
+    capped cell (search + 16 d_c, reached while the grippers bite) takes
+    minutes per substep."""
     import multiprocessing
+    if workload == "full":
+        scene = scenes.make_ellipsoid_scene(full_sample, 12, seed=3)
+        script = scenes.GripperScript(scene, seed=7)
+        recipe = scenes.full_recipe(scene, seed=3)
+    n = scene["pos"].shape[0]
+    log(f"cpu baseline: oracle on {n} Gaussians ({workload})")
     sim = XPBD(ORACLE_LIB)
     sim.create(scene)
     configure(sim, workload, recipe)
@@ -104,6 +114,7 @@ def cpu_baseline(scene, script, workload, recipe, seconds_budget=20.0):
                 sim.set_attached(att[0], att[1], 1.0)
             sim.step()
             substeps += 1
+            log(f"cpu baseline: substep {substeps} at {time.perf_counter() - t0:.1f}s")
             if time.perf_counter() - t0 > seconds_budget:
                 done = True
                 break
@@ -112,10 +123,10 @@ def cpu_baseline(scene, script, workload, recipe, seconds_budget=20.0):
             break
     dt = time.perf_counter() - t0
     sim.destroy()
-    n = scene["pos"].shape[0]
     return {"value": n * substeps / dt, "unit": "Gaussian-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle, same {n}-Gaussian scene, recipe and script, first {substeps} substep(s), "
-                      f"single thread, {dt:.1f} s; host has {multiprocessing.cpu_count()} logical CPUs"}
+            "sample": f"oracle, {n}-Gaussian scene from the same generator, recipe and gripper script, "
+                      f"first {substeps} substep(s) after one untimed substep, single thread, {dt:.1f} s; "
+                      f"host has {multiprocessing.cpu_count()} logical CPUs"}
 
 
 def kernel_bytes(workload, n, m, sc_table, sc_passes, vol):
@@ -315,7 +326,10 @@ def main():
     ap.add_argument("--max-segments", type=int, default=6, help="served: batches before merging")
     ap.add_argument("--cold", action="store_true", help="served: start empty instead of at the steady population")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--tune", action="store_true",
+                    help="full/distance: autotune the launch shapes after the warmup frames (the defaults are "
+                         "the ones measured best at the compressed frames of the gripper script, which the "
+                         "light warmup state does not represent)")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run a streaming kernel of known traffic (256 MiB read + 256 MiB written, x3) "
                          "to scale the FETCH_SIZE / WRITE_SIZE counters")
@@ -358,7 +372,7 @@ def main():
         run_frame(sim, script, frame, frame == 0, shapes)
         frame += 1
     log(f"warmup {args.warmup} frames done")
-    tune = sim.ext_autotune(2) if not args.no_tune else None
+    tune = sim.ext_autotune(2) if args.tune else None
     if tune is not None:
         log("autotune: " + json.dumps(tune))
     t0 = time.perf_counter()

@@ -327,8 +327,10 @@ int xpbd_batch_copy_object(xpbd_batch* dst, int dstObj, xpbd_batch* src, int src
  * substeps from the current state (restored afterwards) and keeps the
  * fastest; writes a JSON report.  Results do not depend on these choices.
  * tuning[6]: Jacobi tiles per block, volume mode (-1 auto, 0 thread, 1 wave),
- * cluster shape waves per block, self-collision build waves per block, nodes
- * per wave and XCD node runs (1) or round-robin blocks (0).  flags bit 0 also picks the self-collision hash-table size
+ * cluster shape waves per block, self-collision build waves per block (1, 2,
+ * 4), nodes per wave (4, 8, 16, 32, 64) and XCD mapping (0 plain block order,
+ * 1 one contiguous node run per XCD, 2-256 runs of that many blocks dealt
+ * round-robin over the XCDs).  flags bit 0 also picks the self-collision hash-table size
  * (1, 4 or 16 x nextpow2(2N) cells per object), which can reorder candidate
  * lists that overflow their 32 slots. */
 int xpbd_batch_autotune(xpbd_batch* b, int substeps, int flags, char* json, int len);
