@@ -56,19 +56,21 @@ __device__ __forceinline__ float lane_f(float v, int lane)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// Cluster shape matching, solve: one wave64 per cluster.  Members are spread
-// over the lanes (64 at a time); the centroid and A = sum (p - c)(q - c0)^T
-// are summed serially in member order by reading lanes one after the other,
-// which keeps the float result of a serial loop.  A is normalised by its
-// Frobenius norm and its rotation (fp64 polar when robust) stored with the
-// centroid for the per-node projection.
-__global__ void __launch_bounds__(256) shape_cluster_kernel(int K, ShapeClusterArgs a, const float4* __restrict__ P,
-                                                           const float3* __restrict__ rest,
-                                                           const int* __restrict__ tileBody,
-                                                           const BodyDev* __restrict__ bodies)
+// Cluster shape matching, sums: one wave64 per cluster.  Members are spread
+// over the lanes (64 at a time) and their values staged in LDS; each sum of
+// the centroid and of A = sum (p - c)(q - c0)^T is then taken by one lane
+// (three, then nine lanes) serially in member order, which keeps the float
+// result of a serial loop.  The centroid and A go to the cluster's parameter
+// row; the rotation follows in shape_cluster_polar_kernel.
+constexpr int kClusterStage = 65;   // LDS row stride (odd: the summing lanes hit distinct banks)
+
+__global__ void __launch_bounds__(256) shape_cluster_sum_kernel(int K, ShapeClusterArgs a, const float4* __restrict__ P,
+                                                               const float3* __restrict__ rest)
 {
-    const int k = blockIdx.x * (int)(blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (k >= K) return;   // whole waves
+    __shared__ float st[4][9 * kClusterStage];
+    const int wl = threadIdx.x >> 6;
+    const int k = blockIdx.x * (int)(blockDim.x >> 6) + wl;
+    if (k >= K) return;   // whole waves; only wave-level barriers below
     const int lane = threadIdx.x & 63;
     const int b0 = a.off[k], cnt = a.off[k + 1] - b0;
     float* prm = &a.params[16 * (size_t)k];
@@ -76,22 +78,28 @@ __global__ void __launch_bounds__(256) shape_cluster_kernel(int K, ShapeClusterA
         if (lane == 0) prm[15] = 0.0f;
         return;
     }
-    const BodyDev& B = bodies[tileBody[a.mem[b0] / kTile]];
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    float* row = st[wl];
+    float s = 0.0f;   // lane q < 3: centroid sum of component q
     for (int base = 0; base < cnt; base += 64) {
         const int t = base + lane;
         const float4 p = (t < cnt) ? P[a.mem[b0 + t]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        row[lane] = p.x;
+        row[kClusterStage + lane] = p.y;
+        row[2 * kClusterStage + lane] = p.z;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         const int m = min(64, cnt - base);
-        for (int u = 0; u < m; ++u) {
-            sx = sx + lane_f(p.x, u);
-            sy = sy + lane_f(p.y, u);
-            sz = sz + lane_f(p.z, u);
-        }
+        if (lane < 3)
+            for (int u = 0; u < m; ++u) s = s + row[lane * kClusterStage + u];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();   // the next chunk overwrites the rows
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
     const float invC = 1.0f / (float)cnt;
-    const float cx = sx * invC, cy = sy * invC, cz = sz * invC;
+    const float cx = lane_f(s, 0) * invC, cy = lane_f(s, 1) * invC, cz = lane_f(s, 2) * invC;
     const float3 cr = a.comRest[k];
-    M3 A = m3_zero();
+    float acc = 0.0f;   // lane q < 9: entry (c, r) = (q / 3, q % 3) of A
     for (int base = 0; base < cnt; base += 64) {
         const int t = base + lane;
         float pv[3] = {0.0f, 0.0f, 0.0f}, qv[3] = {0.0f, 0.0f, 0.0f};
@@ -102,19 +110,45 @@ __global__ void __launch_bounds__(256) shape_cluster_kernel(int K, ShapeClusterA
             pv[0] = p.x - cx; pv[1] = p.y - cy; pv[2] = p.z - cz;
             qv[0] = q.x - cr.x; qv[1] = q.y - cr.y; qv[2] = q.z - cr.z;
         }
-        float prod[3][3];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int r = 0; r < 3; ++r) prod[c][r] = pv[r] * qv[c];
+            for (int r = 0; r < 3; ++r) row[(3 * c + r) * kClusterStage + lane] = pv[r] * qv[c];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         const int m = min(64, cnt - base);
-        for (int u = 0; u < m; ++u) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int r = 0; r < 3; ++r) A.m[c][r] += lane_f(prod[c][r], u);
-        }
+        if (lane < 9)
+            for (int u = 0; u < m; ++u) acc += row[lane * kClusterStage + u];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
+    if (lane < 9) prm[3 + lane] = acc;
+    if (lane == 0) {
+        prm[0] = cx; prm[1] = cy; prm[2] = cz;
+        prm[15] = 1.0f;
+    }
+}
+
+// Cluster shape matching, rotation: one thread per cluster.  A (from the sum
+// kernel) is normalised by its Frobenius norm and replaced by its rotation
+// (fp64 polar when robust) for the per-node projection.
+__global__ void __launch_bounds__(64) shape_cluster_polar_kernel(int K, ShapeClusterArgs a,
+                                                                 const int* __restrict__ tileBody,
+                                                                 const BodyDev* __restrict__ bodies)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int b0 = a.off[k];
+    if (a.off[k + 1] - b0 < 4) return;
+    const BodyDev& B = bodies[tileBody[a.mem[b0] / kTile]];
+    float* prm = &a.params[16 * (size_t)k];
+    M3 A;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) A.m[c][r] = prm[3 + 3 * c + r];
     float nsq = 0.0f;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -130,14 +164,11 @@ __global__ void __launch_bounds__(256) shape_cluster_kernel(int K, ShapeClusterA
             for (int r = 0; r < 3; ++r) A.m[c][r] /= an;
         ok = B.shapeRobust ? polar_rotation_f64(A, R) : polar_rotation_f32(A, R);
     }
-    if (lane == 0) {
-        prm[0] = cx; prm[1] = cy; prm[2] = cz;
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int r = 0; r < 3; ++r) prm[3 + 3 * c + r] = R.m[c][r];
-        prm[15] = ok ? 1.0f : 0.0f;
-    }
+        for (int r = 0; r < 3; ++r) prm[3 + 3 * c + r] = R.m[c][r];
+    prm[15] = ok ? 1.0f : 0.0f;
 }
 
 // Per-node shape matching over the graph 1-ring (or the node's cluster),
@@ -468,8 +499,11 @@ void launch_shape_match(hipStream_t s, const NodeArgs& na, const float4* Pin, fl
                         const EpiArgs& ea, const Tuning& tu)
 {
     const int per = (tu.shapeClusterWaves == 1 || tu.shapeClusterWaves == 2) ? tu.shapeClusterWaves : 4;
-    if (sca.K > 0)
-        shape_cluster_kernel<<<nbk(sca.K, per), 64 * per, 0, s>>>(sca.K, sca, Pin, rest, na.tileBody, na.bodies);
+    if (sca.K > 0) {
+        shape_cluster_sum_kernel<<<nbk(sca.K, per), 64 * per, 0, s>>>(sca.K, sca, Pin, rest);
+        // one wave per block: the few waves spread over the CUs
+        shape_cluster_polar_kernel<<<nbk(sca.K, 64), 64, 0, s>>>(sca.K, sca, na.tileBody, na.bodies);
+    }
     shape_match_kernel<<<nbk(na.nPad, 256), 256, 0, s>>>(na.nPad, na.tileBody, na.bodies, Pin, Pout, rest, rowOff,
                                                         nbr, sca, epi, ea);
 }
