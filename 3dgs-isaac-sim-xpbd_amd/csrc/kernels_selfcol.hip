@@ -513,43 +513,14 @@ __device__ __forceinline__ int sc_run_pos(int v, int rs, int pre)
 
 constexpr int kFineList = 128;   // accepted candidates a wave sorts in LDS (a multiple of 64)
 constexpr int kAccChunk = 1024;  // list-pool entries a wave reserves at once (>= kFineList)
+constexpr int kScNeedCoarse = -2;  // list length: the node takes the coarse scan
 
-// Candidate lists (forward.cu:7079-7180), one wave64 per node at a time.  Each
-// wave takes NPW consecutive nodes (a divisor of the 64-node tile, so a wave
-// never spans two bodies).
-//
-// Coarse scan (the reference's own walk): the 27 stencil cells of the
-// frame-start position, each hash bucket once at its first stencil offset, in
-// sorted order, tested 64 candidates at a time; accepted candidates enter the
-// slots in lane order, which is the scan order.  Stencil cells that cannot
-// hold a candidate are skipped: a node j passes either test only if
-// |x_i - x_j| < search + |p_i - x_i| + max |p - x|, so a cell farther than that
-// bound (with a relative margin for rounding) holds none; a bucket is still
-// scanned at its first offset whenever any offset mapping to it is needed.
-//
-// Fine scan (bodies whose nodes all belong to one object, with self-contact
-// inside it on): the only test is |p_i - p_j| < search, so every candidate
-// lies in the 27 cells of edge >= search around p_i in a second grid built on
-// the predicted positions.  Those few cells (nine rows of three cells, each
-// row one stretch of the sorted order) are tested instead of the coarse
-// stencil; a passing j is a reference candidate when its coarse bucket is one
-// of the stencil's buckets, and it takes the scan position (first stencil
-// offset of that bucket, sorted position inside it), read off its coarse cell
-// when that is one of the 27 around x_i.  The accepted ones are sorted by that
-// position (a 32-bit key) in LDS and enter the slots in that order, so the
-// lists and counters equal the coarse scan's.  A node with more than
-// kFineList accepted candidates takes the coarse scan.
-template <int NPW>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) sc_build_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
-                                                      const float4* __restrict__ P, const float3* __restrict__ rest,
-                                                      int xcd)
+// Block order for the candidate kernels.  Blocks go to the XCDs round-robin.
+// xcd > 1: each XCD takes runs of xcd consecutive blocks (nodes in Morton
+// order, so neighbouring cells stay in that XCD's L2 while crowded regions
+// spread over every XCD); xcd == 1: one contiguous run per XCD.
+__device__ __forceinline__ int sc_block(int xcd)
 {
-    __shared__ __align__(16) unsigned int lkey[4][kFineList];   // accepted: scan-order key
-    __shared__ int2 lpay[4][kFineList];                           // accepted: node id, squared distance
-    // Blocks go to the XCDs round-robin.  xcd > 1: each XCD takes runs of xcd
-    // consecutive blocks (nodes in Morton order, so neighbouring cells stay in
-    // that XCD's L2 while crowded regions spread over every XCD); xcd == 1:
-    // one contiguous run per XCD
     int blk = (int)blockIdx.x;
     if (xcd == 1) {
         blk = xcd_block();
@@ -557,22 +528,267 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
         const int x = blk & 7, r = blk >> 3;
         blk = ((r / xcd) * 8 + x) * xcd + r % xcd;
     }
-    const int wave = (blk * (int)blockDim.x + (int)threadIdx.x) >> 6;
+    return blk;
+}
+
+// Fine scan, one wave64 per node at a time (NPW consecutive nodes per wave),
+// for bodies whose nodes all belong to one object with self-contact inside it
+// on.  There the only test is |p_i - p_j| < search, so every candidate lies in
+// the 27 cells of edge >= search around p_i in a second grid built on the
+// predicted positions.  Those few cells (nine rows of three cells, each row
+// one stretch of the sorted order) are tested instead of the coarse stencil;
+// a passing j is a reference candidate when its coarse bucket is one of the
+// stencil's buckets, and it takes the scan position (first stencil offset of
+// that bucket, sorted position inside it), read off its coarse cell when that
+// is one of the 27 around x_i.  The accepted ones are ranked by that position
+// (a 32-bit key) in LDS and written in that order to the list pool; the slot
+// kernel fills the 32 slots from there, so the lists and counters equal the
+// coarse scan's.  A node with more than kFineList accepted candidates, or
+// that finds the pool full, is handed to the coarse kernel.
+template <int NPW>
+__global__ void __launch_bounds__(256) sc_fine_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
+                                                     const float4* __restrict__ P, const float3* __restrict__ rest,
+                                                     int xcd)
+{
+    __shared__ __align__(16) unsigned int lkey[4][kFineList];   // accepted: scan-order key
+    __shared__ int2 lpay[4][kFineList];                           // accepted: node id, squared distance
+    const int wave = (sc_block(xcd) * (int)blockDim.x + (int)threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int wl = threadIdx.x >> 6;
     const int i0 = wave * NPW;
     if (i0 >= nPad) return;   // whole waves
     const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i0 / kTile]);
     const BodyDev& B = a.bodies[b];
-    bool anyCand = false;
-    int chunkPos = 0, chunkEnd = 0;   // the wave's share of the deferred-list pool
+    if (!B.scOn || !B.scFine) return;
+    int chunkPos = 0, chunkEnd = 0;   // the wave's share of the list pool
     for (int ii = 0; ii < NPW; ++ii) {
         const int i = i0 + ii;
-        if (!B.scOn || i >= B.nodeEnd) break;
+        if (i >= B.nodeEnd) break;
+        const float3 x = X[i];
+        const float4 p4 = P[i];
+        int deferred = 0;   // list length in the pool, or kScNeedCoarse
+        if (a.scActive[i] && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p4.x) && isfinite(p4.y) &&
+            isfinite(p4.z)) {
+            const float3 q = rest[i];
+            const float search = B.scSearch, search2 = search * search;
+            const float inv = a.scInvCell[b];
+            const int cx = sc_cell(x.x, inv), cy = sc_cell(x.y, inv), cz = sc_cell(x.z, inv);
+            // lane c < 27: stencil offset c and its (global) coarse bucket
+            const int c = lane < 27 ? lane : 26;
+            const int ox = c % 3 - 1, oy = (c / 3) % 3 - 1, oz = c / 9 - 1;
+            const unsigned int hg =
+                (unsigned int)B.scTableBase + sc_hash(cx + ox, cy + oy, cz + oz, (unsigned int)B.scTableMask);
+            const float ginv = B.scFineInv;
+            const unsigned int fmask = (unsigned int)B.scFineMask;
+            const int fbase = B.scFineBase;
+            const float excl2 = B.scExcl2;
+            const int gx = sc_cell(p4.x, ginv), gy = sc_cell(p4.y, ginv), gz = sc_cell(p4.z, ginv);
+            // lane c < 27: fine cell (gx + ox, gy + oy, gz + oz) and its range
+            const unsigned int fk = sc_fine_key(gx + ox, gy + oy, gz + oz, fmask);
+            int cs = -1, ce = 0;
+            if (lane < 27) {
+                cs = a.fcellStart[fbase + fk];
+                if (cs >= 0) ce = a.fcellEnd[fbase + fk];
+            }
+            // first stencil offset holding each coarse bucket (two offsets
+            // share one when their cells collide in the hash)
+            int firstC = c;
+            for (int c2 = 26; c2 >= 0; --c2)
+                if ((unsigned int)lane_i((int)hg, c2) == hg) firstC = c2;
+            const bool dup = __ballot(lane < 27 && firstC != lane) != 0ull;
+            // Row r = c / 3 (one (oy, oz)) covers keys k_r - 1 .. k_r + 1 (k_r
+            // the key of its middle cell), one contiguous stretch of the
+            // sorted order.  Rows whose stretches wrap around the table or
+            // come within two keys of each other would visit a key twice:
+            // then each distinct cell key is walked once instead.
+            bool clash = false;
+            for (int r2 = 0; r2 < 9; ++r2) {
+                const unsigned int k2 = (unsigned int)lane_i((int)fk, 3 * r2 + 1);
+                const unsigned int dk = fk > k2 ? fk - k2 : k2 - fk;
+                if (lane != 3 * r2 + 1 && dk <= 2u) clash = true;
+            }
+            const bool perCell = __ballot(lane < 27 && ox == 0 && (clash || fk == 0u || fk == fmask)) != 0ull;
+            // row r: first run position rowP (past every run position when
+            // the row is empty) and sorted index minus run position rowD
+            int rowP[9], rowD[9];
+            int rs = 0, pre = 0, total = 0;
+            if (!perCell) {
+                // the non-empty cells of a row are adjacent in the sorted order
+                const int lo = cs >= 0 ? cs : 0x7fffffff, hi = cs >= 0 ? ce : 0;
+                const int lo1 = __shfl_down(lo, 1, 64), lo2 = __shfl_down(lo, 2, 64);
+                const int hi1 = __shfl_down(hi, 1, 64), hi2 = __shfl_down(hi, 2, 64);
+                const int rlo = min(lo, min(lo1, lo2)), rhi = max(hi, max(hi1, hi2));
+#pragma unroll
+                for (int r = 0; r < 9; ++r) {
+                    const int l = lane_i(rlo, 3 * r), h2 = lane_i(rhi, 3 * r);
+                    const int n = h2 > l ? h2 - l : 0;
+                    rowP[r] = n > 0 ? total : 0x7fffffff;
+                    rowD[r] = l - total;
+                    total += n;
+                }
+            } else {
+                bool ffirst = lane < 27;
+                for (int c2 = 0; c2 < 27; ++c2)
+                    if ((unsigned int)lane_i((int)fk, c2) == fk && c2 < c) ffirst = false;
+                int rn = 0;
+                if (ffirst && cs >= 0) {
+                    rs = cs;
+                    rn = ce - cs;
+                }
+                total = sc_run_prefix(rn, lane, pre);
+            }
+            int nacc = 0;
+            for (int base = 0; base < total; base += 64) {
+                const int v = base + lane;
+                int t;
+                if (!perCell) {
+                    int d = 0;
+#pragma unroll
+                    for (int r = 0; r < 9; ++r)
+                        if (v >= rowP[r]) d = rowD[r];
+                    t = v + d;
+                } else {
+                    t = sc_run_pos(v, rs, pre);
+                }
+                bool acc = false, inside = true;
+                unsigned int key = 0u, ck = 0u;
+                int j = 0, rk = 0;
+                float d2 = 0.0f;
+                if (v < total) {
+                    const float4 pj = a.fineP[t];
+                    const float4 qj = a.fineR[t];
+                    const int4 cj = a.fineC[t];
+                    j = __float_as_int(pj.w);
+                    ck = __float_as_uint(qj.w);
+                    rk = cj.w;
+                    const float ex = p4.x - pj.x, ey = p4.y - pj.y, ez = p4.z - pj.z;
+                    d2 = ex * ex + ey * ey + ez * ez;
+                    const float qx = q.x - qj.x, qy = q.y - qj.y, qz = q.z - qj.z;
+                    const float q2 = qx * qx + qy * qy + qz * qz;
+                    acc = (j != i) & (d2 < search2) & (q2 >= excl2);
+                    // scan position: a candidate whose coarse cell is one of
+                    // the 27 around x_i sits in that offset's bucket
+                    const int dx = cj.x - cx, dy = cj.y - cy, dz = cj.z - cz;
+                    inside = ((unsigned int)(dx + 1) <= 2u) & ((unsigned int)(dy + 1) <= 2u) &
+                             ((unsigned int)(dz + 1) <= 2u);
+                    key = ((unsigned int)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) << 27) | (unsigned int)rk;
+                }
+                if (dup) {
+                    const int f = __shfl(firstC, inside ? (int)(key >> 27) : 0, 64);
+                    key = ((unsigned int)f << 27) | (unsigned int)rk;
+                }
+                // any other candidate is one only when its bucket collides
+                // with a stencil bucket: lanes 0-26 hold those, so each is
+                // matched against all of them at once
+                unsigned long long om = __ballot(acc && !inside);
+                while (om) {
+                    const int l = __builtin_ctzll(om);
+                    om &= om - 1ull;
+                    const unsigned int ckl = (unsigned int)lane_i((int)ck, l);
+                    const unsigned long long hit = __ballot(lane < 27 && hg == ckl);
+                    if (lane == l) {
+                        if (hit) key = ((unsigned int)__builtin_ctzll(hit) << 27) | (unsigned int)rk;
+                        else acc = false;
+                    }
+                }
+                const unsigned long long am = __ballot(acc);
+                const int slot = nacc + (int)__builtin_amdgcn_mbcnt_hi((unsigned int)(am >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo((unsigned int)am, 0u));
+                if (acc && slot < kFineList) {
+                    lkey[wl][slot] = key;
+                    lpay[wl][slot] = make_int2(j, __float_as_int(d2));
+                }
+                nacc += __popcll(am);
+            }
+            deferred = kScNeedCoarse;   // unless the ranked list goes to the pool below
+            if (nacc <= kFineList) {
+                // pad the key list to whole uint4 reads with keys above all
+                const int n4 = (nacc + 3) & ~3;
+                if (lane < n4 - nacc) lkey[wl][nacc + lane] = 0xffffffffu;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                // rank of each accepted entry by its scan-order key (first
+                // offset, coarse sorted position: all distinct)
+                constexpr int R = kFineList / 64;
+                unsigned int ek[R];
+                int2 ep[R];
+                int er[R];
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const int s = k * 64 + lane;
+                    ek[k] = s < nacc ? lkey[wl][s] : 0xffffffffu;
+                    ep[k] = s < nacc ? lpay[wl][s] : make_int2(0, 0);
+                    er[k] = 0;
+                }
+                for (int m = 0; m < n4; m += 4) {
+                    const uint4 km = *reinterpret_cast<const uint4*>(&lkey[wl][m]);
+#pragma unroll
+                    for (int k = 0; k < R; ++k)
+                        er[k] += (km.x < ek[k] ? 1 : 0) + (km.y < ek[k] ? 1 : 0) + (km.z < ek[k] ? 1 : 0) +
+                                 (km.w < ek[k] ? 1 : 0);
+                }
+                // the sorted list goes to the slot kernel (one thread per node)
+                // when the shared list pool has room; the wave takes pool
+                // space kAccChunk entries at a time
+                if (chunkPos + nacc > chunkEnd) {
+                    int off = 0;
+                    if (lane == 0) off = atomicAdd(a.accUsed, kAccChunk);
+                    chunkPos = __builtin_amdgcn_readfirstlane(off);
+                    chunkEnd = chunkPos + kAccChunk;
+                }
+                const int off = chunkPos;
+                if ((long long)chunkEnd <= a.accCap) {
+                    chunkPos += nacc;
+#pragma unroll
+                    for (int k = 0; k < R; ++k)
+                        if (k * 64 + lane < nacc) a.accList[(size_t)off + er[k]] = ep[k];
+                    if (lane == 0) a.accOff[i] = off;
+                    deferred = nacc;
+                }
+                __builtin_amdgcn_wave_barrier();   // the next node reuses the wave's LDS rows
+            }
+        }
+        if (lane == 0) a.accCount[i] = deferred;
+    }
+}
+
+// Candidate lists (forward.cu:7079-7180), coarse scan (the reference's own
+// walk), one wave64 per node at a time; each wave takes NPW consecutive nodes
+// (a divisor of the 64-node tile, so a wave never spans two bodies).  The 27
+// stencil cells of the frame-start position, each hash bucket once at its
+// first stencil offset, in sorted order, are tested 64 candidates at a time;
+// accepted candidates enter the slots in lane order, which is the scan order.
+// Stencil cells that cannot hold a candidate are skipped: a node j passes
+// either test only if |x_i - x_j| < search + |p_i - x_i| + max |p - x|, so a
+// cell farther than that bound (with a relative margin for rounding) holds
+// none; a bucket is still scanned at its first offset whenever any offset
+// mapping to it is needed.  In fine-scan bodies only the nodes the fine
+// kernel handed over are walked here.
+template <int NPW>
+__global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
+                                                      const float4* __restrict__ P, const float3* __restrict__ rest,
+                                                      int xcd)
+{
+    const int wave = (sc_block(xcd) * (int)blockDim.x + (int)threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    const int i0 = wave * NPW;
+    if (i0 >= nPad) return;   // whole waves
+    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i0 / kTile]);
+    const BodyDev& B = a.bodies[b];
+    if (!B.scOn) return;
+    // fine-scan bodies: only the nodes the fine kernel handed over
+    const bool fine = B.scFine != 0;
+    if (fine && __ballot(lane < NPW && i0 + lane < B.nodeEnd && a.accCount[i0 + lane] == kScNeedCoarse) == 0ull)
+        return;
+    bool anyCand = false;
+    for (int ii = 0; ii < NPW; ++ii) {
+        const int i = i0 + ii;
+        if (i >= B.nodeEnd) break;
+        if (fine && a.accCount[i] != kScNeedCoarse) continue;
         const float3 x = X[i];
         const float4 p4 = P[i];
         ScSlots S;
-        int deferred = -1;   // >= 0: sorted list of that length handed to sc_slots_kernel
         if (a.scActive[i] && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p4.x) && isfinite(p4.y) &&
             isfinite(p4.z)) {
             const float3 q = rest[i];
@@ -587,195 +803,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
             const int ox = c % 3 - 1, oy = (c / 3) % 3 - 1, oz = c / 9 - 1;
             const unsigned int h = sc_hash(cx + ox, cy + oy, cz + oz, mask);
             const unsigned int hg = (unsigned int)B.scTableBase + h;
-            bool coarse = true;
-            if (B.scFine) {
-                // ---- fine scan ----
-                const float ginv = B.scFineInv;
-                const unsigned int fmask = (unsigned int)B.scFineMask;
-                const int fbase = B.scFineBase;
-                const float excl2 = B.scExcl2;
-                const int gx = sc_cell(p4.x, ginv), gy = sc_cell(p4.y, ginv), gz = sc_cell(p4.z, ginv);
-                // lane c < 27: fine cell (gx + ox, gy + oy, gz + oz) and its range
-                const unsigned int fk = sc_fine_key(gx + ox, gy + oy, gz + oz, fmask);
-                int cs = -1, ce = 0;
-                if (lane < 27) {
-                    cs = a.fcellStart[fbase + fk];
-                    if (cs >= 0) ce = a.fcellEnd[fbase + fk];
-                }
-                // first stencil offset holding each coarse bucket (two offsets
-                // share one when their cells collide in the hash)
-                int firstC = c;
-                for (int c2 = 26; c2 >= 0; --c2)
-                    if ((unsigned int)lane_i((int)hg, c2) == hg) firstC = c2;
-                const bool dup = __ballot(lane < 27 && firstC != lane) != 0ull;
-                // Row r = c / 3 (one (oy, oz)) covers keys k_r - 1 .. k_r + 1 (k_r
-                // the key of its middle cell), one contiguous stretch of the
-                // sorted order.  Rows whose stretches wrap around the table or
-                // come within two keys of each other would visit a key twice:
-                // then each distinct cell key is walked once instead.
-                bool clash = false;
-                for (int r2 = 0; r2 < 9; ++r2) {
-                    const unsigned int k2 = (unsigned int)lane_i((int)fk, 3 * r2 + 1);
-                    const unsigned int dk = fk > k2 ? fk - k2 : k2 - fk;
-                    if (lane != 3 * r2 + 1 && dk <= 2u) clash = true;
-                }
-                const bool perCell = __ballot(lane < 27 && ox == 0 && (clash || fk == 0u || fk == fmask)) != 0ull;
-                // row r: first run position rowP (past every run position when
-                // the row is empty) and sorted index minus run position rowD
-                int rowP[9], rowD[9];
-                int rs = 0, pre = 0, total = 0;
-                if (!perCell) {
-                    // the non-empty cells of a row are adjacent in the sorted order
-                    const int lo = cs >= 0 ? cs : 0x7fffffff, hi = cs >= 0 ? ce : 0;
-                    const int lo1 = __shfl_down(lo, 1, 64), lo2 = __shfl_down(lo, 2, 64);
-                    const int hi1 = __shfl_down(hi, 1, 64), hi2 = __shfl_down(hi, 2, 64);
-                    const int rlo = min(lo, min(lo1, lo2)), rhi = max(hi, max(hi1, hi2));
-#pragma unroll
-                    for (int r = 0; r < 9; ++r) {
-                        const int l = lane_i(rlo, 3 * r), h2 = lane_i(rhi, 3 * r);
-                        const int n = h2 > l ? h2 - l : 0;
-                        rowP[r] = n > 0 ? total : 0x7fffffff;
-                        rowD[r] = l - total;
-                        total += n;
-                    }
-                } else {
-                    bool ffirst = lane < 27;
-                    for (int c2 = 0; c2 < 27; ++c2)
-                        if ((unsigned int)lane_i((int)fk, c2) == fk && c2 < c) ffirst = false;
-                    int rn = 0;
-                    if (ffirst && cs >= 0) {
-                        rs = cs;
-                        rn = ce - cs;
-                    }
-                    total = sc_run_prefix(rn, lane, pre);
-                }
-                int nacc = 0;
-                for (int base = 0; base < total; base += 64) {
-                    const int v = base + lane;
-                    int t;
-                    if (!perCell) {
-                        int d = 0;
-#pragma unroll
-                        for (int r = 0; r < 9; ++r)
-                            if (v >= rowP[r]) d = rowD[r];
-                        t = v + d;
-                    } else {
-                        t = sc_run_pos(v, rs, pre);
-                    }
-                    bool acc = false, inside = true;
-                    unsigned int key = 0u, ck = 0u;
-                    int j = 0, rk = 0;
-                    float d2 = 0.0f;
-                    if (v < total) {
-                        const float4 pj = a.fineP[t];
-                        const float4 qj = a.fineR[t];
-                        const int4 cj = a.fineC[t];
-                        j = __float_as_int(pj.w);
-                        ck = __float_as_uint(qj.w);
-                        rk = cj.w;
-                        const float ex = p4.x - pj.x, ey = p4.y - pj.y, ez = p4.z - pj.z;
-                        d2 = ex * ex + ey * ey + ez * ez;
-                        const float qx = q.x - qj.x, qy = q.y - qj.y, qz = q.z - qj.z;
-                        const float q2 = qx * qx + qy * qy + qz * qz;
-                        acc = (j != i) & (d2 < search2) & (q2 >= excl2);
-                        // scan position: a candidate whose coarse cell is one of
-                        // the 27 around x_i sits in that offset's bucket
-                        const int dx = cj.x - cx, dy = cj.y - cy, dz = cj.z - cz;
-                        inside = ((unsigned int)(dx + 1) <= 2u) & ((unsigned int)(dy + 1) <= 2u) &
-                                 ((unsigned int)(dz + 1) <= 2u);
-                        key = ((unsigned int)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) << 27) | (unsigned int)rk;
-                    }
-                    if (dup) {
-                        const int f = __shfl(firstC, inside ? (int)(key >> 27) : 0, 64);
-                        key = ((unsigned int)f << 27) | (unsigned int)rk;
-                    }
-                    // any other candidate is one only when its bucket collides
-                    // with a stencil bucket: lanes 0-26 hold those, so each is
-                    // matched against all of them at once
-                    unsigned long long om = __ballot(acc && !inside);
-                    while (om) {
-                        const int l = __builtin_ctzll(om);
-                        om &= om - 1ull;
-                        const unsigned int ckl = (unsigned int)lane_i((int)ck, l);
-                        const unsigned long long hit = __ballot(lane < 27 && hg == ckl);
-                        if (lane == l) {
-                            if (hit) key = ((unsigned int)__builtin_ctzll(hit) << 27) | (unsigned int)rk;
-                            else acc = false;
-                        }
-                    }
-                    const unsigned long long am = __ballot(acc);
-                    const int slot = nacc + (int)__builtin_amdgcn_mbcnt_hi((unsigned int)(am >> 32),
-                                                                          __builtin_amdgcn_mbcnt_lo((unsigned int)am, 0u));
-                    if (acc && slot < kFineList) {
-                        lkey[wl][slot] = key;
-                        lpay[wl][slot] = make_int2(j, __float_as_int(d2));
-                    }
-                    nacc += __popcll(am);
-                }
-                if (nacc <= kFineList) {
-                    coarse = false;
-                    // pad the key list to whole uint4 reads with keys above all
-                    const int n4 = (nacc + 3) & ~3;
-                    if (lane < n4 - nacc) lkey[wl][nacc + lane] = 0xffffffffu;
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                    // rank of each accepted entry by its scan-order key (first
-                    // offset, coarse sorted position: all distinct)
-                    constexpr int R = kFineList / 64;
-                    unsigned int ek[R];
-                    int2 ep[R];
-                    int er[R];
-#pragma unroll
-                    for (int k = 0; k < R; ++k) {
-                        const int s = k * 64 + lane;
-                        ek[k] = s < nacc ? lkey[wl][s] : 0xffffffffu;
-                        ep[k] = s < nacc ? lpay[wl][s] : make_int2(0, 0);
-                        er[k] = 0;
-                    }
-                    for (int m = 0; m < n4; m += 4) {
-                        const uint4 km = *reinterpret_cast<const uint4*>(&lkey[wl][m]);
-#pragma unroll
-                        for (int k = 0; k < R; ++k)
-                            er[k] += (km.x < ek[k] ? 1 : 0) + (km.y < ek[k] ? 1 : 0) + (km.z < ek[k] ? 1 : 0) +
-                                     (km.w < ek[k] ? 1 : 0);
-                    }
-                    // the sorted list goes to the slot kernel (one thread per node)
-                    // when the shared list pool has room; the wave takes pool
-                    // space kAccChunk entries at a time
-                    if (chunkPos + nacc > chunkEnd) {
-                        int off = 0;
-                        if (lane == 0) off = atomicAdd(a.accUsed, kAccChunk);
-                        chunkPos = __builtin_amdgcn_readfirstlane(off);
-                        chunkEnd = chunkPos + kAccChunk;
-                    }
-                    const int off = chunkPos;
-                    if ((long long)chunkEnd <= a.accCap) {
-                        chunkPos += nacc;
-#pragma unroll
-                        for (int k = 0; k < R; ++k)
-                            if (k * 64 + lane < nacc) a.accList[(size_t)off + er[k]] = ep[k];
-                        if (lane == 0) a.accOff[i] = off;
-                        deferred = nacc;
-                    } else {
-                        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-                        for (int k = 0; k < R; ++k)
-                            if (k * 64 + lane < nacc) lpay[wl][er[k]] = ep[k];
-                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                        __builtin_amdgcn_wave_barrier();
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                        for (int r = 0; r < nacc; ++r) {
-                            const int2 e = lpay[wl][r];
-                            S.insert(__builtin_amdgcn_readfirstlane(e.x), 1,
-                                     sqrtf(__int_as_float(__builtin_amdgcn_readfirstlane(e.y))), lane);
-                        }
-                    }
-                    __builtin_amdgcn_wave_barrier();   // the next node reuses the wave's LDS rows
-                }
-            }
-            if (coarse) {
-                // ---- coarse scan ----
+            {
                 const float edge = 1.0f / inv;
                 const float tx = fminf(fmaxf(x.x * inv, -1.0e9f), 1.0e9f), ty = fminf(fmaxf(x.y * inv, -1.0e9f), 1.0e9f),
                             tz = fminf(fmaxf(x.z * inv, -1.0e9f), 1.0e9f);
@@ -859,15 +887,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
                 }
             }
         }
-        if (a.accCount && lane == 0) a.accCount[i] = deferred;
-        if (deferred < 0) {
-            if (lane < S.cnt) a.contacts[sc_slot(i, lane)] = S.code;
-            if (lane == 0) {
-                a.counts[i] = S.cnt;
-                a.overflow[i] = S.overflow;
-            }
-            anyCand |= S.cnt > 0;
+        if (lane < S.cnt) a.contacts[sc_slot(i, lane)] = S.code;
+        if (lane == 0) {
+            a.counts[i] = S.cnt;
+            a.overflow[i] = S.overflow;
         }
+        anyCand |= S.cnt > 0;
     }
     // the contact push of object shape matching needs to know whether the body
     // has any candidate; the totals are summed only when asked for
@@ -1204,16 +1229,18 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
     // the blocks past the node space return at once
     const int xcd = tu.scBuildXcd < 0 ? 0 : (tu.scBuildXcd > 256 ? 256 : tu.scBuildXcd);
     const int grid = xcd == 0 ? blocks : (xcd == 1 ? xcd_grid(blocks) : (blocks + 8 * xcd - 1) / (8 * xcd) * (8 * xcd));
-    if (npw == 4)
-        sc_build_kernel<4><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);
-    else if (npw == 8)
-        sc_build_kernel<8><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);
-    else if (npw == 16)
-        sc_build_kernel<16><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);
-    else if (npw == 64)
-        sc_build_kernel<64><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);
-    else
-        sc_build_kernel<32><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);
+#define XPBD_SC_LAUNCH(K)                                                                               \
+    do {                                                                                                \
+        if (npw == 4) K<4><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);                       \
+        else if (npw == 8) K<8><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);                  \
+        else if (npw == 16) K<16><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);                \
+        else if (npw == 64) K<64><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);                \
+        else K<32><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);                               \
+    } while (0)
+    // fine scan first: the coarse kernel walks the nodes it hands over
+    if (a.fineTotal > 0) XPBD_SC_LAUNCH(sc_fine_kernel);
+    XPBD_SC_LAUNCH(sc_build_kernel);
+#undef XPBD_SC_LAUNCH
     if (a.fineTotal > 0) sc_slots_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a);
     sc_fill_accum_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a.accum);
     *sortedIdxOut = vs;
