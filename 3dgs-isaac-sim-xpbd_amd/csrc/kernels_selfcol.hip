@@ -914,37 +914,45 @@ __global__ void __launch_bounds__(256) sc_slots_kernel(int nPad, SelfColArgs a)
     const int n = i < B.nodeEnd ? a.accCount[i] : -1;
     const int2* list = a.accList + (n > 0 ? a.accOff[i] : 0);
     constexpr int CAP = kSelfColCap;
-    int code[CAP];
-    float prio[CAP];
-    int cnt = 0, over = 0;
-    for (int r = 0; r < n; ++r) {
-        const int2 e = list[r];
-        const float p = sqrtf(__int_as_float(e.y));   // the list holds squared distances
-        if (cnt < CAP) {
+    const int cnt = n < CAP ? n : CAP;
+    if (n >= 0 && n <= CAP) {
+        // the whole list fits: it is the slot order
+        for (int r = 0; r < n; ++r) a.contacts[sc_slot(i, r)] = list[r].x;
+    } else if (n > CAP) {
+        int code[CAP];
+        float prio[CAP];
 #pragma unroll
-            for (int s = 0; s < CAP; ++s)
-                if (s == cnt) { code[s] = e.x; prio[s] = p; }
-            ++cnt;
-        } else {
-            int w = 0;
-            float wp = prio[0];
+        for (int s = 0; s < CAP; ++s) {
+            const int2 e = list[s];
+            code[s] = e.x;
+            prio[s] = sqrtf(__int_as_float(e.y));   // the list holds squared distances
+        }
+        // worst slot, kept until a slot changes
+        int w = 0;
+        float wp = prio[0];
 #pragma unroll
-            for (int s = 1; s < CAP; ++s)
-                if (prio[s] > wp) { wp = prio[s]; w = s; }
+        for (int s = 1; s < CAP; ++s)
+            if (prio[s] > wp) { wp = prio[s]; w = s; }
+        for (int r = CAP; r < n; ++r) {
+            const int2 e = list[r];
+            const float p = sqrtf(__int_as_float(e.y));
             if (p < wp) {
 #pragma unroll
                 for (int s = 0; s < CAP; ++s)
                     if (s == w) { code[s] = e.x; prio[s] = p; }
+                w = 0;
+                wp = prio[0];
+#pragma unroll
+                for (int s = 1; s < CAP; ++s)
+                    if (prio[s] > wp) { wp = prio[s]; w = s; }
             }
-            ++over;
         }
+#pragma unroll
+        for (int s = 0; s < CAP; ++s) a.contacts[sc_slot(i, s)] = code[s];
     }
     if (n >= 0) {
-#pragma unroll
-        for (int s = 0; s < CAP; ++s)
-            if (s < cnt) a.contacts[sc_slot(i, s)] = code[s];
         a.counts[i] = cnt;
-        a.overflow[i] = over;
+        a.overflow[i] = n - cnt;
     }
     const bool any = __ballot(n > 0 && cnt > 0) != 0ull;
     if ((threadIdx.x & 63) == 0 && any && B.objOn) a.scCounters[3 * b] = 1;
@@ -1006,6 +1014,8 @@ __device__ __forceinline__ bool sc_start_normal(float3 xi, float3 xj, float3 qi,
 }
 
 // Read pass (forward.cu:7205-7307).
+constexpr int kScSolveGroup = 8;
+
 __global__ void __launch_bounds__(256) sc_solve_kernel(int nPad, SelfColArgs a, const float4* __restrict__ P,
                                                       const float3* __restrict__ X, const float3* __restrict__ rest)
 {
@@ -1026,43 +1036,58 @@ __global__ void __launch_bounds__(256) sc_solve_kernel(int nPad, SelfColArgs a, 
     const float3 xi = X[i], qi = rest[i];
     if (wi > 0.0f) {
         const float dc2 = dc * dc;
-        for (int k = 0; k < cnt; ++k) {
-            const int code = list[k * kTile];
-            if (code < 0) {
-                const int j = -code - 1;
-                float nx, ny, nz;
-                if (!sc_start_normal(xi, X[j], qi, rest[j], nx, ny, nz)) continue;
-                const float4 pj = P[j];
-                const float proj = (pi4.x - pj.x) * nx + (pi4.y - pj.y) * ny + (pi4.z - pj.z) * nz;
-                if (!(proj < dc)) continue;
-                const float pen = dc - proj;
-                sx += nx * pen; sy += ny * pen; sz += nz * pen;
+        // contacts in groups of kScSolveGroup: the group's codes and partner
+        // positions are fetched back to back (several gathers in flight per
+        // lane), then the group is applied in slot order
+        for (int k0 = 0; k0 < cnt; k0 += kScSolveGroup) {
+            int codes[kScSolveGroup];
+            float4 pjs[kScSolveGroup];
+#pragma unroll
+            for (int u = 0; u < kScSolveGroup; ++u) codes[u] = (k0 + u < cnt) ? list[(k0 + u) * kTile] : -1;
+#pragma unroll
+            for (int u = 0; u < kScSolveGroup; ++u) {
+                const int jj = (k0 + u < cnt) ? (codes[u] >= 0 ? codes[u] : -codes[u] - 1) : i;
+                pjs[u] = P[jj];
+            }
+#pragma unroll
+            for (int u = 0; u < kScSolveGroup; ++u) {
+                if (k0 + u >= cnt) break;
+                const int code = codes[u];
+                const float4 pj = pjs[u];
+                if (code < 0) {
+                    const int j = -code - 1;
+                    float nx, ny, nz;
+                    if (!sc_start_normal(xi, X[j], qi, rest[j], nx, ny, nz)) continue;
+                    const float proj = (pi4.x - pj.x) * nx + (pi4.y - pj.y) * ny + (pi4.z - pj.z) * nz;
+                    if (!(proj < dc)) continue;
+                    const float pen = dc - proj;
+                    sx += nx * pen; sy += ny * pen; sz += nz * pen;
+                    const float wj = pj.w * B.invMassScale;
+                    wFast += wi / (wi + wj);
+                    ++nFast;
+                    continue;
+                }
+                const int j = code;
+                const float dx = pi4.x - pj.x, dy = pi4.y - pj.y, dz = pi4.z - pj.z;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                if (!(d2 < dc2)) continue;
                 const float wj = pj.w * B.invMassScale;
-                wFast += wi / (wi + wj);
-                ++nFast;
-                continue;
-            }
-            const int j = code;
-            const float4 pj = P[j];
-            const float dx = pi4.x - pj.x, dy = pi4.y - pj.y, dz = pi4.z - pj.z;
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            if (!(d2 < dc2)) continue;
-            const float wj = pj.w * B.invMassScale;
-            const float wr = wi / (wi + wj);
-            if (d2 < 1e-24f) {
-                const float3 qj = rest[j];
-                const float rx = qi.x - qj.x, ry = qi.y - qj.y, rz = qi.z - qj.z;
-                const float r2 = rx * rx + ry * ry + rz * rz;
-                if (!(r2 > 1e-24f)) continue;
-                const float s = dc * wr / sqrtf(r2);
-                ax += rx * s; ay += ry * s; az += rz * s;
+                const float wr = wi / (wi + wj);
+                if (d2 < 1e-24f) {
+                    const float3 qj = rest[j];
+                    const float rx = qi.x - qj.x, ry = qi.y - qj.y, rz = qi.z - qj.z;
+                    const float r2 = rx * rx + ry * ry + rz * rz;
+                    if (!(r2 > 1e-24f)) continue;
+                    const float s = dc * wr / sqrtf(r2);
+                    ax += rx * s; ay += ry * s; az += rz * s;
+                    ++act;
+                    continue;
+                }
+                const float d = sqrtf(d2);
+                const float s = (dc - d) / d * wr;
+                ax += dx * s; ay += dy * s; az += dz * s;
                 ++act;
-                continue;
             }
-            const float d = sqrtf(d2);
-            const float s = (dc - d) / d * wr;
-            ax += dx * s; ay += dy * s; az += dz * s;
-            ++act;
         }
     }
     float3 out = (act > 0) ? make_float3(ax / act, ay / act, az / act) : make_float3(0.0f, 0.0f, 0.0f);
