@@ -298,7 +298,7 @@ class XPBDBatch:
         return json.loads(buf.value.decode())
 
     def tuning(self):
-        t = np.zeros(6, np.int32)
+        t = np.zeros(7, np.int32)
         self._check(self.lib.xpbd_batch_get_tuning(self.h, _p(t)), "xpbd_batch_get_tuning")
         return [int(x) for x in t]
 

@@ -372,7 +372,7 @@ int xpbd_batch_get_tuning(xpbd_batch* b, int* t)
     if (!b || !t) return -1;
     const Tuning& u = b->engine.tuning();
     t[0] = u.jacobiTiles; t[1] = u.volMode; t[2] = u.shapeClusterWaves; t[3] = u.scBuildWaves; t[4] = u.scBuildNodes;
-    t[5] = u.scBuildXcd;
+    t[5] = u.scBuildXcd; t[6] = u.scGroupLanes;
     return 0;
 }
 
@@ -385,11 +385,11 @@ int xpbd_batch_set_tuning(xpbd_batch* b, const int* t)
         return false;
     };
     if (!in(t[0], {1, 2, 4}) || !in(t[1], {-1, 0, 1}) || !in(t[2], {1, 2, 4}) || !in(t[3], {1, 2, 4}) ||
-        !in(t[4], {4, 8, 16, 32, 64}) || !(t[5] >= 0 && t[5] <= 256))
+        !in(t[4], {4, 8, 16, 32, 64}) || !(t[5] >= 0 && t[5] <= 256) || !in(t[6], {0, 8, 16, 32}))
         return bfail(b, "xpbd_batch_set_tuning: value outside the tuned range");
     Tuning& u = b->engine.tuning();
     u.jacobiTiles = t[0]; u.volMode = t[1]; u.shapeClusterWaves = t[2]; u.scBuildWaves = t[3]; u.scBuildNodes = t[4];
-    u.scBuildXcd = t[5];
+    u.scBuildXcd = t[5]; u.scGroupLanes = t[6];
     return 0;
 }
 

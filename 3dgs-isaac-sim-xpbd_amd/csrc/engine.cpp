@@ -743,7 +743,7 @@ std::vector<int> Engine::planSignature() const
             anyCGround, anyObj, anyBoxes, anySc, anyObj ? objTotal_ : 0, anySc ? scTableTotal_ : 0,
             anySc ? scSortBits_ : 0, nDistGS, nVolGS, device_alloc_epoch(), (anyShape && shcDevValid_) ? shcK_ : 0,
             allocEpoch_, tune_.jacobiTiles, tune_.volMode, tune_.shapeClusterWaves, tune_.scBuildWaves,
-            tune_.scBuildNodes, tune_.scBuildXcd, anyFine, anyAngle};
+            tune_.scBuildNodes, tune_.scBuildXcd, anyFine, anyAngle, tune_.scGroupLanes};
 }
 
 // Event pairs around kernel launches (profile mode only).
@@ -805,8 +805,11 @@ void Engine::recordSubstep(Timer* t)
     if (scPass) {
         XPBD_T_BEGIN("selfcol_build");
         const int* sorted = nullptr;
+        // profile mode: the build's phases get labels of their own
+        void (*mark)(void*, const char*) = nullptr;
+        if (t) mark = [](void* c, const char* l) { static_cast<Timer*>(c)->end(); static_cast<Timer*>(c)->begin(l); };
         const bool inAlt = launch_selfcol_build(stream_, na, numBodies(), sa, x_.p, p0_.p, rest_.p, scSortBits_,
-                                                scKeysAlt_.p, scIdxAlt_.p, scHist_.p, &sorted, tune_);
+                                                scKeysAlt_.p, scIdxAlt_.p, scHist_.p, &sorted, tune_, mark, t);
         XPBD_T_END();
         scSortedIdx_ = sorted;
         scSortedKeys_ = inAlt ? scKeysAlt_.p : scKeys_.p;

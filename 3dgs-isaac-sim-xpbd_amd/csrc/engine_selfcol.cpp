@@ -100,6 +100,7 @@ SelfColArgs Engine::scArgs() const
     a.accCount = fine ? accCount_.p : nullptr;
     a.accUsed = fine ? accUsed_.p : nullptr;
     a.accCap = (long long)accList_.n;
+    a.fineWaveAll = 0;
     return a;
 }
 

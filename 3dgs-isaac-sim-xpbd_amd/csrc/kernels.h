@@ -20,6 +20,8 @@ struct Tuning {
     int scBuildNodes = 32;      // nodes per wave of the candidate build: 4, 8, 16, 32, 64
     int scBuildXcd = 16;        // candidate build: runs of that many blocks dealt round-robin over the XCDs
                                 // (2-256), one contiguous run per XCD (1), plain block order (0)
+    int scGroupLanes = 0;       // lanes per node of the grouped fine candidate scan: 8, 16, 32; 0: every
+                                // node takes the wave fine scan
 };
 
 // Node-space launch description shared by every per-node kernel.
@@ -160,13 +162,14 @@ struct SelfColArgs {
     int* fcellStart;           // [fineTotal]
     int* fcellEnd;
     float4* fineP;             // [nPad] fine order: p, node id bits
-    float4* fineR;             // [nPad] fine order: rest, coarse key bits
-    int4* fineC;               // [nPad] fine order: coarse cell of x, coarse sorted position
+    float4* fineR;             // [nPad] fine order: rest, coarse sorted position bits
+    int4* fineC;               // [nPad] fine order: coarse cell of x, coarse key
     int2* accList;             // [accCap] pool of sorted accepted-candidate lists (code, priority bits)
     int* accOff;               // [nPad] start of a node's list in the pool
     int* accCount;             // [nPad] list length, -1 when the build kernel filled the slots itself
     int* accUsed;              // [1] pool entries taken this build
     long long accCap;
+    int fineWaveAll;           // the wave fine scan takes every node (no grouped scan ran before it)
 };
 
 void launch_object_shape(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, int K, float4* P, const float3* rest,
@@ -174,7 +177,8 @@ void launch_object_shape(hipStream_t s, const NodeArgs& na, const SelfColArgs& a
 // Returns true when the sorted (keys, idx) ended in the alternate buffers.
 bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, const SelfColArgs& a, const float3* X,
                           const float4* P, const float3* rest, int sortBits, unsigned int* keysAlt, int* idxAlt,
-                          unsigned int* hist, const int** sortedIdxOut, const Tuning& tu);
+                          unsigned int* hist, const int** sortedIdxOut, const Tuning& tu,
+                          void (*mark)(void*, const char*) = nullptr, void* markCtx = nullptr);
 void launch_selfcol_iteration(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, const float4* Pin,
                               float4* Pout, const float3* X, const float3* rest, unsigned epi, const EpiArgs& ea);
 void launch_contact_push(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, int K, float4* P, const EpiArgs& ea);

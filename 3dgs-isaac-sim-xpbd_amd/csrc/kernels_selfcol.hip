@@ -357,6 +357,33 @@ __global__ void __launch_bounds__(256) sc_stage_kernel(int nPad, SelfColArgs a, 
     if (a.coarseRank) a.coarseRank[j] = t;
 }
 
+// Whether two of the 27 coarse cells around x hash to one bucket.  The hash
+// is X ^ Y ^ Z of per-axis products, so two cells collide exactly when
+// (dX ^ dY ^ dZ) & mask == 0, where each d is 0 (same coordinate) or the xor
+// of two of the three products of that axis: 63 cases instead of 351 pairs.
+__device__ __forceinline__ bool sc_stencil_shares_bucket(const float3& x, float inv, unsigned int mask)
+{
+    const int cx = sc_cell(x.x, inv), cy = sc_cell(x.y, inv), cz = sc_cell(x.z, inv);
+    const unsigned int x0 = (unsigned int)(cx - 1) * 73856093u, x1 = (unsigned int)cx * 73856093u,
+                       x2 = (unsigned int)(cx + 1) * 73856093u;
+    const unsigned int y0 = (unsigned int)(cy - 1) * 19349663u, y1 = (unsigned int)cy * 19349663u,
+                       y2 = (unsigned int)(cy + 1) * 19349663u;
+    const unsigned int z0 = (unsigned int)(cz - 1) * 83492791u, z1 = (unsigned int)cz * 83492791u,
+                       z2 = (unsigned int)(cz + 1) * 83492791u;
+    const unsigned int dx[4] = {0u, x0 ^ x1, x0 ^ x2, x1 ^ x2};
+    const unsigned int dy[4] = {0u, y0 ^ y1, y0 ^ y2, y1 ^ y2};
+    const unsigned int dz[4] = {0u, z0 ^ z1, z0 ^ z2, z1 ^ z2};
+    bool dup = false;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int w = 0; w < 4; ++w)
+                if (u | v | w) dup |= ((dx[u] ^ dy[v] ^ dz[w]) & mask) == 0u;
+    return dup;
+}
+
 // Keys of the predicted positions in the fine grid (bodies with scFine): the
 // nodes that have a coarse key (active, finite x) and a finite p.
 __global__ void __launch_bounds__(256) sc_fine_keys_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
@@ -374,6 +401,10 @@ __global__ void __launch_bounds__(256) sc_fine_keys_kernel(int nPad, SelfColArgs
             const float g = B.scFineInv;
             key = (unsigned int)B.scFineBase +
                   sc_fine_key(sc_cell(p.x, g), sc_cell(p.y, g), sc_cell(p.z, g), (unsigned int)B.scFineMask);
+            // bit 1 of the active flag: two of the 27 coarse stencil cells
+            // around x share a bucket (the group fine scan then looks up the
+            // first offset of each candidate's bucket)
+            if (sc_stencil_shares_bucket(x, a.scInvCell[b], (unsigned int)B.scTableMask)) a.scActive[i] = 3;
         }
     }
     a.fkeys[i] = key;
@@ -391,9 +422,9 @@ __global__ void __launch_bounds__(256) sc_fine_range_kernel(int nPad, SelfColArg
     if (i == nPad - 1 || keysSorted[i + 1] != k) a.fcellEnd[k] = i + 1;
 }
 
-// Fine-order copies for the fine scan: (p, node id), (rest, coarse key) and
-// (coarse cell of x, coarse sorted position): the candidate's place in the
-// coarse scan.
+// Fine-order copies for the fine scans: (p, node id), (rest, coarse sorted
+// position: the candidate's place in the coarse scan) and (coarse cell of x,
+// coarse key).
 __global__ void __launch_bounds__(256) sc_fine_stage_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
                                                            const float4* __restrict__ P,
                                                            const float3* __restrict__ rest,
@@ -409,8 +440,8 @@ __global__ void __launch_bounds__(256) sc_fine_stage_kernel(int nPad, SelfColArg
     const int rk = a.coarseRank[j];
     const float inv = a.scInvCell[a.tileBody[j / kTile]];
     a.fineP[t] = make_float4(p.x, p.y, p.z, __int_as_float(j));
-    a.fineR[t] = make_float4(r.x, r.y, r.z, __uint_as_float(coarseSorted[rk]));   // j's coarse key
-    a.fineC[t] = make_int4(sc_cell(x.x, inv), sc_cell(x.y, inv), sc_cell(x.z, inv), rk);
+    a.fineR[t] = make_float4(r.x, r.y, r.z, __int_as_float(rk));
+    a.fineC[t] = make_int4(sc_cell(x.x, inv), sc_cell(x.y, inv), sc_cell(x.z, inv), (int)coarseSorted[rk]);   // j's coarse key
 }
 
 // Distance from x to the stencil cell at offset o along one axis, given the
@@ -514,6 +545,8 @@ __device__ __forceinline__ int sc_run_pos(int v, int rs, int pre)
 constexpr int kFineList = 128;   // accepted candidates a wave sorts in LDS (a multiple of 64)
 constexpr int kAccChunk = 1024;  // list-pool entries a wave reserves at once (>= kFineList)
 constexpr int kScNeedCoarse = -2;  // list length: the node takes the coarse scan
+constexpr int kScNeedWave = -3;    // list length: the node takes the wave fine scan
+constexpr int kScDone = -4;        // list length: the group fine scan wrote the slots
 
 // Block order for the candidate kernels.  Blocks go to the XCDs round-robin.
 // xcd > 1: each XCD takes runs of xcd consecutive blocks (nodes in Morton
@@ -531,9 +564,231 @@ __device__ __forceinline__ int sc_block(int xcd)
     return blk;
 }
 
+// Fine scan by lane groups: G lanes per node, 64 / G consecutive nodes per
+// wave, for bodies whose nodes all belong to one object with self-contact
+// inside it on (the only test there is |p_i - p_j| < search).  A group walks
+// the 27 cells (edge >= search) around p_i in the predicted-position grid as
+// nine row stretches of the sorted order, G consecutive entries per step (so
+// one load of a group touches one or two cache lines), and appends the
+// scan-order keys of its accepted candidates (first stencil offset of the
+// coarse bucket << 27 | coarse sorted position, as in the wave scan below) to
+// the node's LDS list.  The keys are then ranked: with at most 32 the ranks
+// are the reference's slot order and the group writes the slots itself; with
+// up to kGroupList the ranked list (code, squared distance) goes to the list
+// pool for the slot kernel, which replays the reference's replacement order.
+// Longer lists, crowded cells and rows that wrap around the table or overlap
+// go to the wave fine scan.
+constexpr int kGroupList = 64;               // accepted candidates a group keeps
+constexpr int kGroupStride = kGroupList + 4;  // LDS row of one node: rows of the 8 groups of a wave start on distinct banks
+constexpr int kGroupScan = 2048;             // most grid entries a group walks
+
+template <int G>
+__global__ void __launch_bounds__(256) sc_fine_group_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
+                                                           const float4* __restrict__ P,
+                                                           const float3* __restrict__ rest,
+                                                           const int* __restrict__ sortedIdx)
+{
+    constexpr int NPW = 64 / G;
+    __shared__ __align__(16) unsigned int gkey[4 * NPW][kGroupStride];
+    const int lane = threadIdx.x & 63;
+    const int sub = lane % G, grp = lane / G;
+    const int wave = (xcd_block() * (int)blockDim.x + (int)threadIdx.x) >> 6;
+    const int i0 = wave * NPW;
+    if (i0 >= nPad) return;   // whole waves
+    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i0 / kTile]);
+    const BodyDev& B = a.bodies[b];
+    if (!B.scOn || !B.scFine) return;
+    const int i = i0 + grp;
+    const bool node = i < B.nodeEnd;
+    unsigned int* __restrict__ list = gkey[(threadIdx.x >> 6) * NPW + grp];
+    int n = 0;
+    bool over = false, scanned = false;
+    float3 x = make_float3(0.0f, 0.0f, 0.0f);
+    float4 p4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (node) {
+        x = X[i];
+        p4 = P[i];
+        scanned = a.scActive[i] && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p4.x) &&
+                  isfinite(p4.y) && isfinite(p4.z);
+    }
+    if (scanned) {
+        const float3 q = rest[i];
+        const float search = B.scSearch, search2 = search * search;
+        const float excl2 = B.scExcl2;
+        const float inv = a.scInvCell[b];
+        const unsigned int mask = (unsigned int)B.scTableMask;
+        const int cx = sc_cell(x.x, inv), cy = sc_cell(x.y, inv), cz = sc_cell(x.z, inv);
+        // two stencil offsets whose coarse cells share a bucket: the bucket is
+        // scanned at the first of them (flag set by the fine-keys pass)
+        const bool anyDup = (a.scActive[i] & 2) != 0;
+        const float ginv = B.scFineInv;
+        const unsigned int fmask = (unsigned int)B.scFineMask;
+        const int* __restrict__ fs = a.fcellStart + B.scFineBase;
+        const int* __restrict__ fe = a.fcellEnd + B.scFineBase;
+        const int gx = sc_cell(p4.x, ginv), gy = sc_cell(p4.y, ginv), gz = sc_cell(p4.z, ginv);
+        // middle-cell key of row r = (oy + 1) + 3 (oz + 1); its cells are
+        // keys kr - 1, kr, kr + 1, adjacent in the sorted order unless kr is
+        // at either end of the table
+        unsigned int kr[9];
+        bool perCell = false;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            kr[r] = sc_fine_key(gx, gy + r % 3 - 1, gz + r / 3 - 1, fmask);
+            perCell |= kr[r] == 0u || kr[r] == fmask;
+        }
+#pragma unroll
+        for (int r = 1; r < 9; ++r)
+#pragma unroll
+            for (int r2 = 0; r2 < r; ++r2) perCell |= (kr[r] > kr[r2] ? kr[r] - kr[r2] : kr[r2] - kr[r]) <= 2u;
+        // the nine stretches as one run: run entry v sits at v + rowD[r] for
+        // the last row r with rowP[r] <= v (empty rows are overridden)
+        int rowP[9], rowD[9];
+        int total = 0;
+        if (!perCell) {
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                const unsigned int k = kr[r];
+                const int s0 = fs[k - 1], s1 = fs[k], s2 = fs[k + 1];
+                const int e0 = fe[k - 1], e1 = fe[k], e2 = fe[k + 1];
+                int lo = s0 >= 0 ? s0 : (s1 >= 0 ? s1 : s2);
+                int hi = s2 >= 0 ? e2 : (s1 >= 0 ? e1 : (s0 >= 0 ? e0 : -1));
+                if (lo < 0) { lo = 0; hi = 0; }
+                rowP[r] = total;
+                rowD[r] = lo - total;
+                total += hi - lo;
+            }
+        }
+        over = perCell || total > kGroupScan;
+        // run entry v -> sorted position
+        auto runPos = [&](int v) {
+            int d = rowD[0];
+#pragma unroll
+            for (int r = 1; r < 9; ++r)
+                if (v >= rowP[r]) d = rowD[r];
+            return v + d;
+        };
+        // scan-order key of an entry that passed the distance and rest tests,
+        // or false when its bucket is not one of the stencil's
+        auto keyOf = [&](int t, const float4& qj, unsigned int& key) {
+            const int4 cj = a.fineC[t];   // only for accepted entries
+            const int dx = cj.x - cx, dy = cj.y - cy, dz = cj.z - cz;
+            int off = -1;
+            if (((unsigned int)(dx + 1) <= 2u) & ((unsigned int)(dy + 1) <= 2u) & ((unsigned int)(dz + 1) <= 2u)) {
+                off = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                if (anyDup) {
+                    const unsigned int hc = sc_hash(cx + dx, cy + dy, cz + dz, mask);
+                    for (int c2 = 0; c2 < off; ++c2)
+                        if (sc_hash(cx + c2 % 3 - 1, cy + (c2 / 3) % 3 - 1, cz + c2 / 9 - 1, mask) == hc) {
+                            off = c2;
+                            break;
+                        }
+                }
+            } else {
+                // outside the 27 cells (capped or fixed cell): a candidate
+                // only through a bucket shared with a stencil cell
+                const unsigned int ck = (unsigned int)cj.w - (unsigned int)B.scTableBase;
+                for (int c2 = 0; c2 < 27; ++c2)
+                    if (sc_hash(cx + c2 % 3 - 1, cy + (c2 / 3) % 3 - 1, cz + c2 / 9 - 1, mask) == ck) {
+                        off = c2;
+                        break;
+                    }
+            }
+            key = ((unsigned int)(off < 0 ? 0 : off) << 27) | (unsigned int)__float_as_int(qj.w);
+            return off >= 0;
+        };
+        auto passes = [&](const float4& pj, const float4& qj) {
+            const float ex = p4.x - pj.x, ey = p4.y - pj.y, ez = p4.z - pj.z;
+            const float d2 = ex * ex + ey * ey + ez * ez;
+            const float qx = q.x - qj.x, qy = q.y - qj.y, qz = q.z - qj.z;
+            const float q2 = qx * qx + qy * qy + qz * qz;
+            // bitwise: no branch for the compiler to sink the loads into
+            return (d2 < search2) & (__float_as_int(pj.w) != i) & (q2 >= excl2);
+        };
+        // append in lane order (the order is restored by the ranking)
+        const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+        auto append = [&](bool acc, unsigned int key) {
+            const unsigned long long gm = (__ballot(acc) >> (grp * G)) & gmask;
+            const int slot = n + __popcll(gm & ((1ull << sub) - 1ull));
+            if (acc && slot < kGroupList) list[slot] = key;
+            n += __popcll(gm);
+        };
+        // 2 G entries per step, both loads of both issued together
+        for (int base = 0; base < total && !over; base += 2 * G) {
+            const int v0 = base + sub, v1 = v0 + G;
+            const bool in0 = v0 < total, in1 = v1 < total;
+            const int t0 = runPos(in0 ? v0 : 0), t1 = runPos(in1 ? v1 : 0);
+            const float4 pa = a.fineP[t0], ra = a.fineR[t0];
+            const float4 pb = a.fineP[t1], rb = a.fineR[t1];
+            unsigned int k0 = 0u, k1 = 0u;
+            bool acc0 = in0 & passes(pa, ra);
+            bool acc1 = in1 & passes(pb, rb);
+            if (acc0) acc0 = keyOf(t0, ra, k0);
+            if (acc1) acc1 = keyOf(t1, rb, k1);
+            append(acc0, k0);
+            append(acc1, k1);
+            over = n > kGroupList;
+        }
+    }
+    // pad the node's keys to whole uint4 reads with keys above all
+    const int n4 = (n + 3) & ~3;
+    if (scanned && !over && sub < n4 - n) list[n + sub] = 0xffffffffu;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    // the contact push of object shape matching needs to know whether the body
+    // has any candidate (every lane of the wave is active here)
+    const bool anyCand = __ballot(node && !over && n > 0) != 0ull;
+    if (lane == 0 && anyCand && B.objOn) a.scCounters[3 * b] = 1;
+    if (!node) return;
+    if (over) {
+        if (sub == 0) a.accCount[i] = kScNeedWave;
+        return;
+    }
+    // rank of each key (all distinct): its place in the scan order
+    int off = 0;
+    const bool pool = n > kSelfColCap;
+    if (pool) {
+        if (sub == 0) off = atomicAdd(a.accUsed, n);
+        off = __shfl(off, grp * G, 64);
+        if ((long long)off + n > a.accCap) {
+            if (sub == 0) a.accCount[i] = kScNeedCoarse;   // the pool is full
+            return;
+        }
+    }
+    for (int e = sub; e < n; e += G) {
+        const unsigned int ke = list[e];
+        int rank = 0;
+        for (int m = 0; m < n4; m += 4) {
+            const uint4 km = *reinterpret_cast<const uint4*>(&list[m]);
+            rank += (km.x < ke ? 1 : 0) + (km.y < ke ? 1 : 0) + (km.z < ke ? 1 : 0) + (km.w < ke ? 1 : 0);
+        }
+        const int j = sortedIdx[ke & 0x07ffffffu];
+        if (!pool) {
+            a.contacts[sc_slot(i, rank)] = j;
+        } else {
+            // the slot kernel's priority: the distance the test used
+            const float4 pj = P[j];
+            const float ex = p4.x - pj.x, ey = p4.y - pj.y, ez = p4.z - pj.z;
+            const float d2 = ex * ex + ey * ey + ez * ez;
+            a.accList[(size_t)off + rank] = make_int2(j, __float_as_int(d2));
+        }
+    }
+    if (sub == 0) {
+        if (pool) {
+            a.accOff[i] = off;
+            a.accCount[i] = n;
+        } else {
+            a.counts[i] = n;
+            a.overflow[i] = 0;
+            a.accCount[i] = kScDone;
+        }
+    }
+}
+
 // Fine scan, one wave64 per node at a time (NPW consecutive nodes per wave),
 // for bodies whose nodes all belong to one object with self-contact inside it
-// on.  There the only test is |p_i - p_j| < search, so every candidate lies in
+// on, walking only the nodes the group fine scan handed over.  There the only
+// test is |p_i - p_j| < search, so every candidate lies in
 // the 27 cells of edge >= search around p_i in a second grid built on the
 // predicted positions.  Those few cells (nine rows of three cells, each row
 // one stretch of the sorted order) are tested instead of the coarse stencil;
@@ -560,10 +815,14 @@ __global__ void __launch_bounds__(256) sc_fine_kernel(int nPad, SelfColArgs a, c
     const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i0 / kTile]);
     const BodyDev& B = a.bodies[b];
     if (!B.scOn || !B.scFine) return;
+    if (!a.fineWaveAll &&
+        __ballot(lane < NPW && i0 + lane < B.nodeEnd && a.accCount[i0 + lane] == kScNeedWave) == 0ull)
+        return;
     int chunkPos = 0, chunkEnd = 0;   // the wave's share of the list pool
     for (int ii = 0; ii < NPW; ++ii) {
         const int i = i0 + ii;
         if (i >= B.nodeEnd) break;
+        if (!a.fineWaveAll && a.accCount[i] != kScNeedWave) continue;
         const float3 x = X[i];
         const float4 p4 = P[i];
         int deferred = 0;   // list length in the pool, or kScNeedCoarse
@@ -659,8 +918,8 @@ __global__ void __launch_bounds__(256) sc_fine_kernel(int nPad, SelfColArgs a, c
                     const float4 qj = a.fineR[t];
                     const int4 cj = a.fineC[t];
                     j = __float_as_int(pj.w);
-                    ck = __float_as_uint(qj.w);
-                    rk = cj.w;
+                    ck = (unsigned int)cj.w;
+                    rk = __float_as_int(qj.w);
                     const float ex = p4.x - pj.x, ey = p4.y - pj.y, ez = p4.z - pj.z;
                     d2 = ex * ex + ey * ey + ez * ez;
                     const float qx = q.x - qj.x, qy = q.y - qj.y, qz = q.z - qj.z;
@@ -1224,7 +1483,8 @@ void launch_object_shape(hipStream_t s, const NodeArgs& na, const SelfColArgs& a
 
 bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, const SelfColArgs& a, const float3* X,
                           const float4* P, const float3* rest, int sortBits, unsigned int* keysAlt, int* idxAlt,
-                          unsigned int* hist, const int** sortedIdxOut, const Tuning& tu)
+                          unsigned int* hist, const int** sortedIdxOut, const Tuning& tu,
+                          void (*mark)(void*, const char*), void* markCtx)
 {
     sc_prepare_kernel<<<nbs(numBodies, 64), 64, 0, s>>>(numBodies, a);
     sc_active_disp_kernel<<<nbs(na.nPad, kDispThreads), kDispThreads, 0, s>>>(na.nPad, a, X, P);
@@ -1254,18 +1514,38 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
     // the blocks past the node space return at once
     const int xcd = tu.scBuildXcd < 0 ? 0 : (tu.scBuildXcd > 256 ? 256 : tu.scBuildXcd);
     const int grid = xcd == 0 ? blocks : (xcd == 1 ? xcd_grid(blocks) : (blocks + 8 * xcd - 1) / (8 * xcd) * (8 * xcd));
-#define XPBD_SC_LAUNCH(K)                                                                               \
+#define XPBD_SC_LAUNCH_A(K, A)                                                                          \
     do {                                                                                                \
-        if (npw == 4) K<4><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);                       \
-        else if (npw == 8) K<8><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);                  \
-        else if (npw == 16) K<16><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);                \
-        else if (npw == 64) K<64><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);                \
-        else K<32><<<grid, 64 * wpb, 0, s>>>(na.nPad, a, X, P, rest, xcd);                               \
+        if (npw == 4) K<4><<<grid, 64 * wpb, 0, s>>>(na.nPad, A, X, P, rest, xcd);                       \
+        else if (npw == 8) K<8><<<grid, 64 * wpb, 0, s>>>(na.nPad, A, X, P, rest, xcd);                  \
+        else if (npw == 16) K<16><<<grid, 64 * wpb, 0, s>>>(na.nPad, A, X, P, rest, xcd);                \
+        else if (npw == 64) K<64><<<grid, 64 * wpb, 0, s>>>(na.nPad, A, X, P, rest, xcd);                \
+        else K<32><<<grid, 64 * wpb, 0, s>>>(na.nPad, A, X, P, rest, xcd);                               \
     } while (0)
-    // fine scan first: the coarse kernel walks the nodes it hands over
-    if (a.fineTotal > 0) XPBD_SC_LAUNCH(sc_fine_kernel);
+#define XPBD_SC_LAUNCH(K) XPBD_SC_LAUNCH_A(K, a)
+    // fine scans first (thread, then wave for the nodes with more than 32
+    // candidates): the coarse kernel walks the nodes they hand over
+    if (a.fineTotal > 0) {
+        if (mark) mark(markCtx, "selfcol_build_fine_group");
+        // G lanes per node, 64 / G nodes per wave, 4 waves per block; with
+        // G = 0 the wave scan takes every node
+        const int g = tu.scGroupLanes;
+        if (g == 8 || g == 16 || g == 32) {
+            const int gridG = xcd_grid(nbs(na.nPad, 4 * (64 / g)));
+            if (g == 16) sc_fine_group_kernel<16><<<gridG, 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
+            else if (g == 32) sc_fine_group_kernel<32><<<gridG, 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
+            else sc_fine_group_kernel<8><<<gridG, 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
+        }
+        if (mark) mark(markCtx, "selfcol_build_fine_wave");
+        SelfColArgs aw = a;
+        aw.fineWaveAll = (g == 8 || g == 16 || g == 32) ? 0 : 1;
+        XPBD_SC_LAUNCH_A(sc_fine_kernel, aw);
+    }
+    if (mark) mark(markCtx, "selfcol_build_coarse");
     XPBD_SC_LAUNCH(sc_build_kernel);
 #undef XPBD_SC_LAUNCH
+#undef XPBD_SC_LAUNCH_A
+    if (mark) mark(markCtx, "selfcol_build_slots");
     if (a.fineTotal > 0) sc_slots_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a);
     sc_fill_accum_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a.accum);
     *sortedIdxOut = vs;

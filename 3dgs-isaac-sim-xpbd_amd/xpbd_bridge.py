@@ -351,6 +351,14 @@ class XPBD:
         fn.restype = ctypes.c_int
         self._check(fn(int(bool(enabled))), "xpbd_ext_set_self_collision_fine")
 
+    def ext_set_group_lanes(self, lanes):
+        """Lanes per node of the grouped fine candidate scan (8, 16, 32), or 0
+        for the one-wave-per-node scan; same lists."""
+        fn = self.lib.xpbd_ext_set_group_lanes
+        fn.argtypes = [ctypes.c_int]
+        fn.restype = ctypes.c_int
+        self._check(fn(int(lanes)), "xpbd_ext_set_group_lanes")
+
     def ext_autotune(self, substeps=2, hash_table=False):
         """Pick the fastest launch shapes on the current state (restored
         afterwards; results unchanged).  Returns the tuner's report."""

@@ -388,8 +388,17 @@ def main():
     log(f"self-collision: candidates/active/overflow {stats}, cell/d_c/search {grid}")
 
     # per-kernel HIP-event timing on the library stream, same state and script
-    prof = sim.ext_profile(SUBSTEPS)
-    log("kernel ms (total, launches): " + json.dumps(prof))
+    phases = sim.ext_profile(SUBSTEPS)
+    log("kernel ms (total, launches): " + json.dumps(phases))
+    # the self-collision build's phases (selfcol_build_*) form one launch set
+    # priced by one formula
+    prof = {}
+    for k, (tot, cnt) in phases.items():
+        base = "selfcol_build" if k.startswith("selfcol_build") else k
+        if base in prof:
+            prof[base] = (prof[base][0] + tot, prof[base][1])
+        else:
+            prof[base] = (tot, cnt)
     vol = (n, 3 * n) if shapes else (0, 0)
     table = recipe["table"] if shapes else 0
     passes = (int(np.ceil(np.log2(table + 1))) + 7) // 8 if shapes else 0

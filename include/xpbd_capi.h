@@ -150,6 +150,10 @@ int xpbd_ext_profile(int substeps, char* json, int len);
  * xpbd_batch_autotune: the state is restored and results do not depend on
  * the launch shapes it picks; flags bit 0 also sizes the hash table. */
 int xpbd_ext_autotune(int substeps, int flags, char* json, int len);
+/* Lanes per node of the grouped fine candidate scan of the instance (8, 16 or
+ * 32; 0 sends every node to the one-wave-per-node scan).  Candidate lists do
+ * not depend on it. */
+int xpbd_ext_set_group_lanes(int lanes);
 
 /* Volume clusters given explicitly instead of by the k-ring rule: node i leads
  * the cluster {i} + members[off[i] .. off[i+1]) (an empty range = no cluster).
@@ -326,11 +330,12 @@ int xpbd_batch_copy_object(xpbd_batch* dst, int dstObj, xpbd_batch* src, int src
 /* Launch-shape autotuner: times candidates of each tunable over `substeps`
  * substeps from the current state (restored afterwards) and keeps the
  * fastest; writes a JSON report.  Results do not depend on these choices.
- * tuning[6]: Jacobi tiles per block, volume mode (-1 auto, 0 thread, 1 wave),
+ * tuning[7]: Jacobi tiles per block, volume mode (-1 auto, 0 thread, 1 wave),
  * cluster shape waves per block, self-collision build waves per block (1, 2,
- * 4), nodes per wave (4, 8, 16, 32, 64) and XCD mapping (0 plain block order,
+ * 4), nodes per wave (4, 8, 16, 32, 64), XCD mapping (0 plain block order,
  * 1 one contiguous node run per XCD, 2-256 runs of that many blocks dealt
- * round-robin over the XCDs).  flags bit 0 also picks the self-collision hash-table size
+ * round-robin over the XCDs) and lanes per node of the grouped fine scan (8,
+ * 16, 32; 0 sends every node to the one-wave-per-node scan).  flags bit 0 also picks the self-collision hash-table size
  * (1, 4 or 16 x nextpow2(2N) cells per object), which can reorder candidate
  * lists that overflow their 32 slots. */
 int xpbd_batch_autotune(xpbd_batch* b, int substeps, int flags, char* json, int len);
