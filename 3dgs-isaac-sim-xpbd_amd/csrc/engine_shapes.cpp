@@ -1,6 +1,7 @@
 // engine_shapes.cpp — deformed Gaussian shapes and contact ellipsoids of the
 // Engine (forward.cu:13230-13247 computeDeformedShapes; 6733-6757
 // updateContactShapes; native/xpbd_capi.cu:179-191, 248-260).
+#include <cstdint>
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -151,7 +152,17 @@ bool Engine::ensureShapeClusters(std::string& err)
         for (int i = 0; i < n; ++i)
             if (of[i] >= 0) mem[fill[of[i]]++] = i;
         const int g0 = (int)hOff.size() - 1;
-        for (int k = 0; k < K; ++k) {
+        // clusters numbered by their first member in the device order, so
+        // consecutive clusters gather from nearby memory (each cluster's
+        // result does not depend on its number)
+        std::vector<int> first(K, INT32_MAX), order(K), rank(K);
+        for (int i = 0; i < n; ++i)
+            if (of[i] >= 0) first[of[i]] = std::min(first[of[i]], inv_[b][i]);
+        for (int k = 0; k < K; ++k) order[k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](int u, int v) { return first[u] < first[v]; });
+        for (int r = 0; r < K; ++r) rank[order[r]] = r;
+        for (int r = 0; r < K; ++r) {
+            const int k = order[r];
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
             for (int t = off[k]; t < off[k + 1]; ++t) {
                 const int i = mem[t];
@@ -164,7 +175,7 @@ bool Engine::ensureShapeClusters(std::string& err)
             hOff.push_back((int)hMem.size());
         }
         for (int i = 0; i < n; ++i)
-            if (of[i] >= 0) hOf[nb + inv_[b][i]] = g0 + of[i];
+            if (of[i] >= 0) hOf[nb + inv_[b][i]] = g0 + rank[of[i]];
     }
     shcK_ = (int)hOff.size() - 1;
     const size_t K = std::max(shcK_, 1);

@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(256) shape_cluster_sum_kernel(int K, ShapeClus
 {
     __shared__ float st[4][9 * kClusterStage];
     const int wl = threadIdx.x >> 6;
-    const int k = blockIdx.x * (int)(blockDim.x >> 6) + wl;
+    const int k = xcd_block() * (int)(blockDim.x >> 6) + wl;   // clusters in node order: neighbours share an XCD
     if (k >= K) return;   // whole waves; only wave-level barriers below
     const int lane = threadIdx.x & 63;
     const int b0 = a.off[k], cnt = a.off[k + 1] - b0;
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(256) shape_match_kernel(
     float4* __restrict__ Pout, const float3* __restrict__ rest, const int* __restrict__ rowOff,
     const int* __restrict__ nbr, ShapeClusterArgs sca, unsigned epi, EpiArgs ea)
 {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int idx = xcd_block() * blockDim.x + threadIdx.x;
     if (idx >= nPad) return;
     const BodyDev& B = bodies[__builtin_amdgcn_readfirstlane(tileBody[idx / kTile])];
     if (idx >= B.nodeEnd) return;
@@ -500,11 +500,11 @@ void launch_shape_match(hipStream_t s, const NodeArgs& na, const float4* Pin, fl
 {
     const int per = (tu.shapeClusterWaves == 1 || tu.shapeClusterWaves == 2) ? tu.shapeClusterWaves : 4;
     if (sca.K > 0) {
-        shape_cluster_sum_kernel<<<nbk(sca.K, per), 64 * per, 0, s>>>(sca.K, sca, Pin, rest);
+        shape_cluster_sum_kernel<<<xcd_grid(nbk(sca.K, per)), 64 * per, 0, s>>>(sca.K, sca, Pin, rest);
         // one wave per block: the few waves spread over the CUs
         shape_cluster_polar_kernel<<<nbk(sca.K, 64), 64, 0, s>>>(sca.K, sca, na.tileBody, na.bodies);
     }
-    shape_match_kernel<<<nbk(na.nPad, 256), 256, 0, s>>>(na.nPad, na.tileBody, na.bodies, Pin, Pout, rest, rowOff,
+    shape_match_kernel<<<xcd_grid(nbk(na.nPad, 256)), 256, 0, s>>>(na.nPad, na.tileBody, na.bodies, Pin, Pout, rest, rowOff,
                                                         nbr, sca, epi, ea);
 }
 

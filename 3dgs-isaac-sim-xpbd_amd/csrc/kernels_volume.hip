@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(256) vol_solve_thread_kernel(
     const float* __restrict__ detRest, float* __restrict__ lamVol, float4* __restrict__ bbC,
     float4* __restrict__ bbS, float2* __restrict__ bbS2)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block() * blockDim.x + threadIdx.x;   // leaders in node order: neighbours share an XCD
     if (t >= nActive) return;
     const int idx = active[t];
     const BodyDev& B = body_of(tileBody, bodies, idx);
@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(256) vol_solve_wave_kernel(
     const float* __restrict__ detRest, float* __restrict__ lamVol, float4* __restrict__ bbC,
     float4* __restrict__ bbS, float2* __restrict__ bbS2)
 {
-    const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const int wave = __builtin_amdgcn_readfirstlane((int)((xcd_block() * blockDim.x + threadIdx.x) >> 6));
     const int lane = threadIdx.x & 63;
     if (wave >= nActive) return;
     const int idx = __builtin_amdgcn_readfirstlane(active[wave]);
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(256) vol_apply_kernel(
     const float4* __restrict__ bbC, const float4* __restrict__ bbS, const float2* __restrict__ bbS2,
     const float* __restrict__ restLen, unsigned epi, EpiArgs ea)
 {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int idx = xcd_block() * blockDim.x + threadIdx.x;
     if (idx >= nPad) return;
     const BodyDev& B = bodies[__builtin_amdgcn_readfirstlane(tileBody[idx / kTile])];
     if (idx >= B.nodeEnd) return;
@@ -385,11 +385,11 @@ void launch_vol_solve(hipStream_t s, const NodeArgs& na, bool wave, bool first, 
 {
     if (va.nActive <= 0) return;
 #define XPBD_VOL_T(F, L)                                                                                         \
-    vol_solve_thread_kernel<F, L><<<nb(va.nActive, 256), 256, 0, s>>>(va.nActive, va.active, na.tileBody, na.bodies, \
+    vol_solve_thread_kernel<F, L><<<xcd_grid(nb(va.nActive, 256)), 256, 0, s>>>(va.nActive, va.active, na.tileBody, na.bodies, \
                                                                       P, va.volOff, va.volIdx, va.detRest,        \
                                                                       va.lam, va.bbC, va.bbS, va.bbS2)
 #define XPBD_VOL_W(F, L)                                                                                        \
-    vol_solve_wave_kernel<F, L, 4><<<nb(va.nActive, 4), 256, 0, s>>>(va.nActive, va.active, na.tileBody, na.bodies, \
+    vol_solve_wave_kernel<F, L, 4><<<xcd_grid(nb(va.nActive, 4)), 256, 0, s>>>(va.nActive, va.active, na.tileBody, na.bodies, \
                                                                      P, va.volOff, va.volIdx, va.detRest,        \
                                                                      va.lam, va.bbC, va.bbS, va.bbS2)
     if (!wave) {
@@ -410,7 +410,7 @@ void launch_vol_solve(hipStream_t s, const NodeArgs& na, bool wave, bool first, 
 void launch_vol_apply(hipStream_t s, const NodeArgs& na, const float4* Pin, float4* Pout, const VolumeArgs& va,
                       unsigned epi, const EpiArgs& ea)
 {
-    vol_apply_kernel<<<nb(na.nPad, 256), 256, 0, s>>>(na.nPad, na.tileBody, na.bodies, Pin, Pout, va.revOff,
+    vol_apply_kernel<<<xcd_grid(nb(na.nPad, 256)), 256, 0, s>>>(na.nPad, na.tileBody, na.bodies, Pin, Pout, va.revOff,
                                                      va.revIdx, va.bbC, va.bbS, va.bbS2, va.restLen, epi, ea);
 }
 

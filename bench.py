@@ -32,7 +32,8 @@ from xpbd_bridge import XPBD  # noqa: E402
 
 LIB = os.path.join(PKG, "libxpbd_isaac.so")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle_xpbd.so")
-PROFILE = os.path.join(ROOT, "profiles", "distance_jacobi_1M.md")
+PROFILES = {"full": os.path.join(ROOT, "profiles", "full_recipe_1M.md"),
+            "distance": os.path.join(ROOT, "profiles", "distance_jacobi_1M.md")}
 HBM_PEAK_GBS = 8000.0
 SUBSTEPS = 10
 ITERS = 4
@@ -42,11 +43,12 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (scaled by the streaming calibration)."""
+    FETCH_SIZE / WRITE_SIZE summary of the workload (scaled by the streaming
+    calibration)."""
     try:
-        for line in open(PROFILE):
+        for line in open(PROFILES[workload]):
             if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
                 return int(line.split(":")[1])
     except OSError:
@@ -439,8 +441,8 @@ def main():
                    "shape_clusters": int(recipe["clusters"].max() + 1) if shapes else 0,
                    "selfcol_stats_last": list(stats), "selfcol_cell_dc_search": list(grid), "autotune": tune},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant),
-                     "traffic_source": os.path.relpath(PROFILE, ROOT),
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant, args.workload),
+                     "traffic_source": os.path.relpath(PROFILES[args.workload], ROOT),
                      "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_us": dk["avg_us"]},
         "cpu_baseline": cpu,
         "kernels": kernels,
