@@ -292,6 +292,8 @@ void selfcol_build(const std::vector<V3>& p)
     sc.counts.assign(n, 0);
     int overflow = 0;
     const float excludeR2 = excl * excl, crossR2 = dc * dc, fastRel = s.scFastRel * dc;
+    long long contactsTotal = 0;
+#pragma omp parallel for num_threads(xo::g_threads) schedule(static) reduction(+ : overflow, contactsTotal)
     for (int i = 0; i < n; ++i) {
         const V3 x = g.x[i], pp = p[i];
         int cnt = 0;
@@ -364,8 +366,9 @@ void selfcol_build(const std::vector<V3>& p)
                     }
         }
         sc.counts[i] = cnt;
-        sc.contactsTotal += cnt;
+        contactsTotal += cnt;
     }
+    sc.contactsTotal += contactsTotal;
     sc.overflow = overflow;
     sc.activeCount = useActive ? activeCount : n;
     sc.dc = dc;
@@ -393,6 +396,7 @@ void selfcol_iteration(std::vector<V3>& pos)
     const int n = g.n, CAP = 32;
     const float dc = sc.dc;
     std::vector<V3> dp(n, V3{0, 0, 0}), dpc(n, V3{0, 0, 0});
+    XO_PARALLEL_FOR
     for (int i = 0; i < n; ++i) {
         const int cnt = sc.counts[i];
         if (cnt <= 0) continue;

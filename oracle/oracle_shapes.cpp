@@ -214,6 +214,7 @@ static bool polar_float(const M3& A, M3& R)
 void shape_matching(const std::vector<V3>& pin, std::vector<V3>& pout)
 {
     const int n = g.n;
+    XO_PARALLEL_FOR
     for (int idx = 0; idx < n; ++idx) {
         const V3 pi = pin[idx];
         const float wi = g.invMass[idx] * s.invMassScale;
@@ -320,6 +321,7 @@ void shape_cluster_matching(const std::vector<V3>& pin, std::vector<V3>& pout)
     std::vector<V3> comCurr(K);
     std::vector<M3> rot(K);
     std::vector<char> ok(K, 0);
+    XO_PARALLEL_FOR
     for (int k = 0; k < K; ++k) {
         const int b0 = shc.off[k], cnt = shc.off[k + 1] - b0;
         if (cnt < 4) continue;
@@ -346,6 +348,7 @@ void shape_cluster_matching(const std::vector<V3>& pin, std::vector<V3>& pout)
         rot[k] = R;
         ok[k] = 1;
     }
+    XO_PARALLEL_FOR
     for (int idx = 0; idx < n; ++idx) {
         const V3 pi = pin[idx];
         pout[idx] = pi;
@@ -410,6 +413,7 @@ int deformed_shapes(float deformEps, std::vector<float>& outScale, std::vector<f
     outScale.assign((size_t)3 * n, 0.0f);
     outQuat.assign((size_t)4 * n, 0.0f);
     int deformedCount = 0;
+#pragma omp parallel for num_threads(xo::g_threads) schedule(static) reduction(+ : deformedCount)
     for (int i = 0; i < n; ++i) {
         const float s0[3] = {g.scales[3 * i], g.scales[3 * i + 1], g.scales[3 * i + 2]};
         const float q0[4] = {g.rots[4 * i], g.rots[4 * i + 1], g.rots[4 * i + 2], g.rots[4 * i + 3]};

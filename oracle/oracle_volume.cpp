@@ -326,6 +326,7 @@ void volume_precompute()
 static void volume_solve(const std::vector<V3>& pin, float dt)
 {
     const int n = g.n;
+    XO_PARALLEL_FOR
     for (int idx = 0; idx < n; ++idx) {
         vol.coef[idx] = 0.0f;
         if (vol.matType[idx] != 0) continue;
@@ -390,6 +391,7 @@ static void volume_solve(const std::vector<V3>& pin, float dt)
 static void volume_apply(const std::vector<V3>& pin, std::vector<V3>& pout, float underRelax)
 {
     const int n = g.n;
+    XO_PARALLEL_FOR
     for (int idx = 0; idx < n; ++idx) {
         const V3 pi = pin[idx];
         const float wi = g.invMass[idx] * s.invMassScale;

@@ -30,6 +30,7 @@
 namespace xo {
 
 State g;   // the one global instance (native/xpbd_capi.cu:6, 17-29)
+int g_threads = 1;
 Settings s;
 
 int fail(const std::string& msg)
@@ -98,6 +99,7 @@ static void ensure_init()
 // xpbdPredictKernel (forward.cu:5954-5979)
 static void predict(float dt, V3 grav)
 {
+    XO_PARALLEL_FOR
     for (int i = 0; i < g.n; ++i) {
         V3 x = g.x[i], v = g.v[i];
         if (g.invMass[i] > 0.0f) {
@@ -118,6 +120,7 @@ static void distance_jacobi(const std::vector<V3>& pin, std::vector<V3>& pout,
 {
     const int n = g.n;
     const float dt2 = std::fmax(dt * dt, 1e-8f);
+    XO_PARALLEL_FOR
     for (int i = 0; i < n; ++i) {
         const V3 pi = pin[i];
         const float wi = g.invMass[i] * s.invMassScale;
@@ -238,6 +241,7 @@ static void ground_project(std::vector<V3>& pos)
 {
     const float* n = s.groundN;
     const float planeD = s.groundHeight + s.groundRadius;
+    XO_PARALLEL_FOR
     for (int i = 0; i < g.n; ++i) {
         if (g.invMass[i] <= 0.0f) continue;
         const V3 p = pos[i];
@@ -249,6 +253,7 @@ static void ground_project(std::vector<V3>& pos)
 // xpbdUpdateVelocityKernel (forward.cu:6623-6661; lines 6662-6671 are dead)
 static void velocity_plain(const std::vector<V3>& pfin, float dt, float velDamping)
 {
+    XO_PARALLEL_FOR
     for (int i = 0; i < g.n; ++i) {
         const V3 x = g.x[i], p = pfin[i];
         if (g.invMass[i] > 0.0f) {
@@ -275,6 +280,7 @@ static void velocity_ground(const std::vector<V3>& pfin, float dt, float velDamp
     const float planeD = s.groundHeight + s.groundRadius;
     const float restingSpeed = 2.0f * s.groundGravity * dt;
     const float restitution = s.groundRestitution;   // object body contact is never on through the C ABI
+    XO_PARALLEL_FOR
     for (int i = 0; i < g.n; ++i) {
         const V3 x = g.x[i], p = pfin[i];
         if (g.invMass[i] <= 0.0f) { g.v[i] = V3{0, 0, 0}; g.x[i] = p; continue; }
@@ -452,6 +458,14 @@ using namespace xo;
 extern "C" {
 
 const char* xpbd_last_error(void) { return g.err.c_str(); }
+
+// Oracle only (no counterpart in the product or the reference): threads of
+// the order-independent loops, for the timed multi-core CPU baseline.
+int xpbd_oracle_set_threads(int threads)
+{
+    xo::g_threads = threads > 0 ? threads : 1;
+    return xo::g_threads;
+}
 const char* xpbd_ext_backend(void) { return "oracle-cpu"; }
 
 static void free_all()

@@ -9,6 +9,12 @@
 
 namespace xo {
 
+// Threads of the order-independent per-node / per-cluster loops (every
+// iteration writes only its own entries, so the result does not depend on the
+// count); 1 unless the CPU-baseline timing asks for more.
+extern int g_threads;
+#define XO_PARALLEL_FOR _Pragma("omp parallel for num_threads(xo::g_threads) schedule(static)")
+
 struct V3 { float x, y, z; };
 inline V3 add(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
 inline V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
