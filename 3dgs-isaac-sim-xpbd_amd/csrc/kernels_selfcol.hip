@@ -825,10 +825,11 @@ __global__ void __launch_bounds__(256) sc_fine_kernel(int nPad, SelfColArgs a, c
         if (!a.fineWaveAll && a.accCount[i] != kScNeedWave) continue;
         const float3 x = X[i];
         const float4 p4 = P[i];
+        const int act = a.scActive[i];
+        const float3 q = rest[i];
         int deferred = 0;   // list length in the pool, or kScNeedCoarse
-        if (a.scActive[i] && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p4.x) && isfinite(p4.y) &&
+        if (act && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p4.x) && isfinite(p4.y) &&
             isfinite(p4.z)) {
-            const float3 q = rest[i];
             const float search = B.scSearch, search2 = search * search;
             const float inv = a.scInvCell[b];
             const int cx = sc_cell(x.x, inv), cy = sc_cell(x.y, inv), cz = sc_cell(x.z, inv);
@@ -844,10 +845,13 @@ __global__ void __launch_bounds__(256) sc_fine_kernel(int nPad, SelfColArgs a, c
             const int gx = sc_cell(p4.x, ginv), gy = sc_cell(p4.y, ginv), gz = sc_cell(p4.z, ginv);
             // lane c < 27: fine cell (gx + ox, gy + oy, gz + oz) and its range
             const unsigned int fk = sc_fine_key(gx + ox, gy + oy, gz + oz, fmask);
+            // start and end fetched together (an end is only used with its
+            // start, which marks the cells that are occupied this build)
             int cs = -1, ce = 0;
             if (lane < 27) {
                 cs = a.fcellStart[fbase + fk];
-                if (cs >= 0) ce = a.fcellEnd[fbase + fk];
+                ce = a.fcellEnd[fbase + fk];
+                if (cs < 0) ce = 0;
             }
             // first stencil offset holding each coarse bucket (two offsets
             // share one when their cells collide in the hash)
