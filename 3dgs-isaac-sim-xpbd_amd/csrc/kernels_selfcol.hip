@@ -739,22 +739,34 @@ __global__ void __launch_bounds__(256) sc_fine_group_kernel(int nPad, SelfColArg
     // has any candidate (every lane of the wave is active here)
     const bool anyCand = __ballot(node && !over && n > 0) != 0ull;
     if (lane == 0 && anyCand && B.objOn) a.scCounters[3 * b] = 1;
+    // pool space for the lists longer than 32: one reservation per wave, each
+    // group's share from a prefix sum of the groups' lengths
+    const bool pool = node && !over && n > kSelfColCap;
+    const int need = (pool && sub == 0) ? n : 0;
+    int incl = need;
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        const int o = __shfl_up(incl, m, 64);
+        if (lane >= m) incl += o;
+    }
+    const int waveNeed = __shfl(incl, 63, 64);
+    int off = 0;
+    if (waveNeed > 0) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(a.accUsed, waveNeed);
+        base = __shfl(base, 0, 64);
+        off = __shfl(base + incl - need, grp * G, 64);   // the group leader's exclusive prefix
+    }
     if (!node) return;
     if (over) {
         if (sub == 0) a.accCount[i] = kScNeedWave;
         return;
     }
-    // rank of each key (all distinct): its place in the scan order
-    int off = 0;
-    const bool pool = n > kSelfColCap;
-    if (pool) {
-        if (sub == 0) off = atomicAdd(a.accUsed, n);
-        off = __shfl(off, grp * G, 64);
-        if ((long long)off + n > a.accCap) {
-            if (sub == 0) a.accCount[i] = kScNeedCoarse;   // the pool is full
-            return;
-        }
+    if (pool && (long long)off + n > a.accCap) {
+        if (sub == 0) a.accCount[i] = kScNeedCoarse;   // the pool is full
+        return;
     }
+    // rank of each key (all distinct): its place in the scan order
     for (int e = sub; e < n; e += G) {
         const unsigned int ke = list[e];
         int rank = 0;
