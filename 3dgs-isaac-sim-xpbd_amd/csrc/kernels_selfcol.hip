@@ -831,6 +831,7 @@ __global__ void __launch_bounds__(256) sc_fine_kernel(int nPad, SelfColArgs a, c
         __ballot(lane < NPW && i0 + lane < B.nodeEnd && a.accCount[i0 + lane] == kScNeedWave) == 0ull)
         return;
     int chunkPos = 0, chunkEnd = 0;   // the wave's share of the list pool
+    bool anyDirect = false;           // a list written to the slots here
     for (int ii = 0; ii < NPW; ++ii) {
         const int i = i0 + ii;
         if (i >= B.nodeEnd) break;
@@ -975,7 +976,7 @@ __global__ void __launch_bounds__(256) sc_fine_kernel(int nPad, SelfColArgs a, c
                 }
                 nacc += __popcll(am);
             }
-            deferred = kScNeedCoarse;   // unless the ranked list goes to the pool below
+            deferred = kScNeedCoarse;   // unless the ranked list is written below
             if (nacc <= kFineList) {
                 // pad the key list to whole uint4 reads with keys above all
                 const int n4 = (nacc + 3) & ~3;
@@ -1003,29 +1004,45 @@ __global__ void __launch_bounds__(256) sc_fine_kernel(int nPad, SelfColArgs a, c
                         er[k] += (km.x < ek[k] ? 1 : 0) + (km.y < ek[k] ? 1 : 0) + (km.z < ek[k] ? 1 : 0) +
                                  (km.w < ek[k] ? 1 : 0);
                 }
-                // the sorted list goes to the slot kernel (one thread per node)
-                // when the shared list pool has room; the wave takes pool
-                // space kAccChunk entries at a time
-                if (chunkPos + nacc > chunkEnd) {
-                    int off = 0;
-                    if (lane == 0) off = atomicAdd(a.accUsed, kAccChunk);
-                    chunkPos = __builtin_amdgcn_readfirstlane(off);
-                    chunkEnd = chunkPos + kAccChunk;
-                }
-                const int off = chunkPos;
-                if ((long long)chunkEnd <= a.accCap) {
-                    chunkPos += nacc;
+                if (nacc <= kSelfColCap) {
+                    // the whole list fits: the ranks are the slots
 #pragma unroll
                     for (int k = 0; k < R; ++k)
-                        if (k * 64 + lane < nacc) a.accList[(size_t)off + er[k]] = ep[k];
-                    if (lane == 0) a.accOff[i] = off;
-                    deferred = nacc;
+                        if (k * 64 + lane < nacc) a.contacts[sc_slot(i, er[k])] = ep[k].x;
+                    if (lane == 0) {
+                        a.counts[i] = nacc;
+                        a.overflow[i] = 0;
+                    }
+                    anyDirect |= nacc > 0;
+                    deferred = kScDone;
+                } else {
+                    // the sorted list goes to the slot kernel (one thread per
+                    // node) when the shared list pool has room; the wave takes
+                    // pool space kAccChunk entries at a time
+                    if (chunkPos + nacc > chunkEnd) {
+                        int off = 0;
+                        if (lane == 0) off = atomicAdd(a.accUsed, kAccChunk);
+                        chunkPos = __builtin_amdgcn_readfirstlane(off);
+                        chunkEnd = chunkPos + kAccChunk;
+                    }
+                    const int off = chunkPos;
+                    if ((long long)chunkEnd <= a.accCap) {
+                        chunkPos += nacc;
+#pragma unroll
+                        for (int k = 0; k < R; ++k)
+                            if (k * 64 + lane < nacc) a.accList[(size_t)off + er[k]] = ep[k];
+                        if (lane == 0) a.accOff[i] = off;
+                        deferred = nacc;
+                    }
                 }
                 __builtin_amdgcn_wave_barrier();   // the next node reuses the wave's LDS rows
             }
         }
         if (lane == 0) a.accCount[i] = deferred;
     }
+    // the contact push of object shape matching needs to know whether the body
+    // has any candidate (the slot kernel marks the pooled lists)
+    if (lane == 0 && anyDirect && B.objOn) a.scCounters[3 * b] = 1;
 }
 
 // Candidate lists (forward.cu:7079-7180), coarse scan (the reference's own
