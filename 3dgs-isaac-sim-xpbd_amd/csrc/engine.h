@@ -438,7 +438,7 @@ private:
     DevBuf<int> scOver_, scStats_;   // per-node overflow; statistics read-back
     DevBuf<float> scBodyDisp_, scInvCell_;
     DevBuf<unsigned char> scActive_;
-    DevBuf<float3> scDp_, scDpCross_, scAccum_;
+    DevBuf<float3> scAccum_;
     DevBuf<float4> scSortP_, scSortX_, scSortR_;
 
     // graph

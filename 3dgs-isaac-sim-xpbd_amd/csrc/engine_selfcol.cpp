@@ -72,8 +72,6 @@ SelfColArgs Engine::scArgs() const
     a.contacts = contacts_.p;
     a.counts = counts_.p;
     a.overflow = scOver_.p;
-    a.dp = scDp_.p;
-    a.dpCross = scDpCross_.p;
     a.accum = scAccum_.p;
     a.pushSums = pushSums_.p;
     a.pushT = pushT_.p;
@@ -269,11 +267,11 @@ bool Engine::ensureSelfCol(std::string& err)
     scHist_.alloc(radix_sort_hist_size(nPad_));
     cellStart_.alloc(scTableTotal_); cellEnd_.alloc(scTableTotal_);
     contacts_.alloc(n * kSelfColCap); counts_.alloc(n); scOver_.alloc(n);
-    scActive_.alloc(n); scDp_.alloc(n); scDpCross_.alloc(n); scAccum_.alloc(n);
+    scActive_.alloc(n); scAccum_.alloc(n);
     scSortP_.alloc(n); scSortX_.alloc(n); scSortR_.alloc(n);
     if (!scMax2_.p || !scCounters_.p || !scUseActive_.p || !scBodyDisp_.p || !scInvCell_.p || !scKeys_.p ||
         !scKeysAlt_.p || !scIdx_.p || !scIdxAlt_.p || !scHist_.p || !cellStart_.p || !cellEnd_.p || !contacts_.p ||
-        !counts_.p || !scOver_.p || !scActive_.p || !scDp_.p || !scDpCross_.p || !scAccum_.p || !scSortP_.p ||
+        !counts_.p || !scOver_.p || !scActive_.p || !scAccum_.p || !scSortP_.p ||
         !scSortX_.p || !scSortR_.p) {
         err = "self-collision: hipMalloc failed";
         return false;

@@ -142,8 +142,6 @@ struct SelfColArgs {
     int* contacts;             // [nPad * kSelfColCap]
     int* counts;               // [nPad]
     int* overflow;             // [nPad] accepted candidates beyond the 32 slots
-    float3* dp;
-    float3* dpCross;
     float3* accum;
     float* pushSums;           // [4K]
     float* pushT;              // [4K]

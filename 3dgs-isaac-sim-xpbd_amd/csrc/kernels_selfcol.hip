@@ -1305,18 +1305,20 @@ __device__ __forceinline__ bool sc_start_normal(float3 xi, float3 xj, float3 qi,
     return true;
 }
 
-// Read pass (forward.cu:7205-7307).
+// Contacts of one node fetched in groups (several gathers in flight per lane).
 constexpr int kScSolveGroup = 8;
 
-__global__ void __launch_bounds__(256) sc_solve_kernel(int nPad, SelfColArgs a, const float4* __restrict__ P,
-                                                      const float3* __restrict__ X, const float3* __restrict__ rest)
+// Contact solve and apply of one iteration (forward.cu:7205-7307 read pass,
+// 7311-7322 write pass), one thread per node: the node's push from its
+// contact list is computed from the input positions (partners included) and
+// added to its own output position, so the two passes of the reference fuse
+// into one (input and output are different buffers); the iteration epilogue
+// follows.
+__device__ __forceinline__ void sc_node_push(int i, const SelfColArgs& a, const BodyDev& B,
+                                             const float4* __restrict__ P, const float3* __restrict__ X,
+                                             const float3* __restrict__ rest, float3& dpOut, float3& crossOut)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nPad) return;
-    const BodyDev& B = body_at(a, i);
-    if (!B.scOn || i >= B.nodeEnd) return;
     const int cnt = a.counts[i];
-    if (cnt <= 0) return;
     constexpr int CAP = kSelfColCap;
     const float4 pi4 = P[i];
     const float wi = pi4.w * B.invMassScale;
@@ -1407,13 +1409,13 @@ __global__ void __launch_bounds__(256) sc_solve_kernel(int nPad, SelfColArgs a, 
             }
         }
     }
-    a.dp[i] = out;
-    a.dpCross[i] = make_float3(cpx, cpy, cpz);
+    dpOut = out;
+    crossOut = make_float3(cpx, cpy, cpz);
 }
 
-// Write pass (forward.cu:7311-7322) plus the iteration epilogue.
-__global__ void __launch_bounds__(256) sc_apply_kernel(int nPad, SelfColArgs a, const float4* __restrict__ Pin,
-                                                      float4* __restrict__ Pout, unsigned epi, EpiArgs ea)
+__global__ void __launch_bounds__(256) sc_contact_kernel(int nPad, SelfColArgs a, const float4* __restrict__ Pin,
+                                                        float4* __restrict__ Pout, const float3* __restrict__ X,
+                                                        const float3* __restrict__ rest, unsigned epi, EpiArgs ea)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nPad) return;
@@ -1422,9 +1424,9 @@ __global__ void __launch_bounds__(256) sc_apply_kernel(int nPad, SelfColArgs a, 
     const float4 p4 = Pin[i];
     float3 p = make_float3(p4.x, p4.y, p4.z);
     if (B.scOn && a.counts[i] > 0) {
-        const float3 d = a.dp[i];
+        float3 d, c;
+        sc_node_push(i, a, B, Pin, X, rest, d, c);
         p = make_float3(p.x + d.x, p.y + d.y, p.z + d.z);
-        const float3 c = a.dpCross[i];
         const float3 acc = a.accum[i];
         a.accum[i] = make_float3(acc.x + c.x, acc.y + c.y, acc.z + c.z);
     }
@@ -1588,8 +1590,7 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
 void launch_selfcol_iteration(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, const float4* Pin,
                               float4* Pout, const float3* X, const float3* rest, unsigned epi, const EpiArgs& ea)
 {
-    sc_solve_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, Pin, X, rest);
-    sc_apply_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, Pin, Pout, epi, ea);
+    sc_contact_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, Pin, Pout, X, rest, epi, ea);
 }
 
 void launch_contact_push(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, int K, float4* P, const EpiArgs& ea)
