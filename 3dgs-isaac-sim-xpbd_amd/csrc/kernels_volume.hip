@@ -300,18 +300,34 @@ __global__ void __launch_bounds__(256) vol_apply_kernel(
     if (B.volumeOn && wi > 0.0f) {
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         int received = 0;
-        for (int k = revOff[idx]; k < revOff[idx + 1]; ++k) {
-            const int i = revIdx[k];
-            const float4 c = bbC[i];
-            if (c.w == 0.0f) continue;
-            const float4 sa = bbS[i];
-            const float2 sb = bbS2[i];
-            const M3 Si = m3_load_sym(sa.x, sa.y, sa.z, sa.w, sb.x, sb.y);
-            const float3 g = m3_mul_vec(Si, p4.x - c.x, p4.y - c.y, p4.z - c.z);
-            const float dx = g.x * wi * c.w, dy = g.y * wi * c.w, dz = g.z * wi * c.w;
-            if (!isfinite(dx) || !isfinite(dy) || !isfinite(dz)) continue;
-            sx = sx + dx; sy = sy + dy; sz = sz + dz;
-            ++received;
+        // the node's clusters four at a time: indices, then every cluster's
+        // blackboard fetched together; summed in entry order as one at a time
+        constexpr int U = 4;
+        const int k0 = revOff[idx], k1 = revOff[idx + 1];
+        for (int k = k0; k < k1; k += U) {
+            int ci[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) ci[u] = revIdx[k + u < k1 ? k + u : k];
+            float4 cc[U], sa[U];
+            float2 sb[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                cc[u] = bbC[ci[u]];
+                sa[u] = bbS[ci[u]];
+                sb[u] = bbS2[ci[u]];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (k + u >= k1) break;
+                const float4 c = cc[u];
+                if (c.w == 0.0f) continue;
+                const M3 Si = m3_load_sym(sa[u].x, sa[u].y, sa[u].z, sa[u].w, sb[u].x, sb[u].y);
+                const float3 g = m3_mul_vec(Si, p4.x - c.x, p4.y - c.y, p4.z - c.z);
+                const float dx = g.x * wi * c.w, dy = g.y * wi * c.w, dz = g.z * wi * c.w;
+                if (!isfinite(dx) || !isfinite(dy) || !isfinite(dz)) continue;
+                sx = sx + dx; sy = sy + dy; sz = sz + dz;
+                ++received;
+            }
         }
         if (received > 0) {
             const float inv = 1.0f / (float)received;
