@@ -125,9 +125,21 @@ __global__ void __launch_bounds__(256) vol_solve_thread_kernel(
     if (!B.volumeOn) return;
     const float4 pi = P[idx];
     const int off = volOff[idx], cnt = volOff[idx + 1] - off;
+    // the first kMemCache members' positions, fetched together and kept for
+    // the three passes; every sum still runs in member order
+    constexpr int kMemCache = 4;
+    float4 qc[kMemCache];
+    {
+        int jc[kMemCache];
+#pragma unroll
+        for (int u = 0; u < kMemCache; ++u) jc[u] = u < cnt ? volIdx[off + u] : idx;
+#pragma unroll
+        for (int u = 0; u < kMemCache; ++u) qc[u] = P[jc[u]];
+    }
+    auto member = [&](int k) { return k < kMemCache ? qc[k] : P[volIdx[off + k]]; };
     float cx = pi.x, cy = pi.y, cz = pi.z;
     for (int k = 0; k < cnt; ++k) {
-        const float4 q = P[volIdx[off + k]];
+        const float4 q = member(k);
         cx = cx + q.x; cy = cy + q.y; cz = cz + q.z;
     }
     const float invN = 1.0f / (float)(cnt + 1);
@@ -135,7 +147,7 @@ __global__ void __launch_bounds__(256) vol_solve_thread_kernel(
     M3 S = m3_zero();
     m3_add_outer(S, pi.x - cx, pi.y - cy, pi.z - cz);
     for (int k = 0; k < cnt; ++k) {
-        const float4 q = P[volIdx[off + k]];
+        const float4 q = member(k);
         m3_add_outer(S, q.x - cx, q.y - cy, q.z - cz);
     }
     m3_scale(S, invN);
@@ -151,7 +163,7 @@ __global__ void __launch_bounds__(256) vol_solve_thread_kernel(
             denom += (pi.w * B.invMassScale) * (gx * gx + gy * gy + gz * gz);
         }
         for (int k = 0; k < cnt; ++k) {
-            const float4 q = P[volIdx[off + k]];
+            const float4 q = member(k);
             const float3 gv = m3_mul_vec(Si, q.x - cx, q.y - cy, q.z - cz);
             const float gx = gs * gv.x, gy = gs * gv.y, gz = gs * gv.z;
             denom += (q.w * B.invMassScale) * (gx * gx + gy * gy + gz * gz);
