@@ -79,6 +79,7 @@ SelfColArgs Engine::scArgs() const
     a.sortP = scSortP_.p;
     a.sortX = scSortX_.p;
     a.sortR = scSortR_.p;
+    a.sortedIdx = nullptr;
     bool fine = false;
     for (int b = 0; b < numBodies() && !fine; ++b) fine = hostBodies_[b].scFine != 0;
     a.coarseRank = fine ? coarseRank_.p : nullptr;

@@ -150,6 +150,7 @@ struct SelfColArgs {
     float4* sortX;             // [nPad] sorted order: x, node id bits
     float4* sortR;             // [nPad] sorted order: rest
     int* coarseRank;           // [nPad] sorted position of each node
+    const int* sortedIdx;      // [nPad] node at each sorted position (set for the coarse scan)
     // second grid on the predicted positions (cell edge >= search), used for
     // bodies whose nodes all belong to one object
     int fineTotal, fineBits;   // cells of all fine tables; key bits of the sort

@@ -340,8 +340,11 @@ __global__ void __launch_bounds__(256) sc_range_kernel(int nPad, SelfColArgs a, 
     if (i == nPad - 1 || keysSorted[i + 1] != k) a.cellEnd[k] = i + 1;
 }
 
-// Sorted-order copies of what the candidate scan reads, so that each cell of
-// the stencil is one contiguous run: (p, object id), (x, node id), rest.
+// Sorted-order copies of what the coarse candidate scan reads, so that each
+// cell of the stencil is one contiguous run: (p, object id), (x, node id),
+// rest.  Bodies with the predicted-position grid get only the sorted rank:
+// the coarse scan walks few of their nodes and reads those through the
+// permutation.
 __global__ void __launch_bounds__(256) sc_stage_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
                                                       const float4* __restrict__ P, const float3* __restrict__ rest,
                                                       const int* __restrict__ sortedIdx)
@@ -349,12 +352,15 @@ __global__ void __launch_bounds__(256) sc_stage_kernel(int nPad, SelfColArgs a, 
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nPad) return;
     const int j = sortedIdx[t];
+    if (a.coarseRank) {
+        a.coarseRank[j] = t;
+        if (a.bodies[a.tileBody[j / kTile]].scFine) return;
+    }
     const float4 p = P[j];
     const float3 x = X[j], r = rest[j];
     a.sortP[t] = make_float4(p.x, p.y, p.z, __int_as_float(a.objId[j]));
     a.sortX[t] = make_float4(x.x, x.y, x.z, 0.0f);
     a.sortR[t] = make_float4(r.x, r.y, r.z, __int_as_float(j));
-    if (a.coarseRank) a.coarseRank[j] = t;
 }
 
 // Whether two of the 27 coarse cells around x hash to one bucket.  The hash
@@ -1071,6 +1077,7 @@ __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, 
     if (!B.scOn) return;
     // fine-scan bodies: only the nodes the fine kernel handed over
     const bool fine = B.scFine != 0;
+    const bool fineBody = fine && a.sortedIdx != nullptr && a.coarseRank != nullptr;
     if (fine && __ballot(lane < NPW && i0 + lane < B.nodeEnd && a.accCount[i0 + lane] == kScNeedCoarse) == 0ull)
         return;
     bool anyCand = false;
@@ -1132,8 +1139,18 @@ __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, 
                     int code = 0, tr = 0;
                     float pr = 0.0f;
                     if (v < total) {
-                        const float4 pj = a.sortP[t];
-                        const float4 qj = a.sortR[t];   // issued with pj: no dependent second trip
+                        float4 pj, qj;
+                        if (fineBody) {
+                            // not staged (see sc_stage_kernel): through the permutation
+                            const int j = a.sortedIdx[t];
+                            const float4 pp = P[j];
+                            const float3 rr = rest[j];
+                            pj = make_float4(pp.x, pp.y, pp.z, __int_as_float(a.objId[j]));
+                            qj = make_float4(rr.x, rr.y, rr.z, __int_as_float(j));
+                        } else {
+                            pj = a.sortP[t];
+                            qj = a.sortR[t];   // issued with pj: no dependent second trip
+                        }
                         const bool cross = __float_as_int(pj.w) != oi;
                         if (cross || B.scWithin) {
                             float d2 = 0.0f, distX = 0.0f, rel = 0.0f;
@@ -1577,7 +1594,9 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
         XPBD_SC_LAUNCH_A(sc_fine_kernel, aw);
     }
     if (mark) mark(markCtx, "selfcol_build_coarse");
-    XPBD_SC_LAUNCH(sc_build_kernel);
+    SelfColArgs ac = a;
+    ac.sortedIdx = vs;
+    XPBD_SC_LAUNCH_A(sc_build_kernel, ac);
 #undef XPBD_SC_LAUNCH
 #undef XPBD_SC_LAUNCH_A
     if (mark) mark(markCtx, "selfcol_build_slots");
