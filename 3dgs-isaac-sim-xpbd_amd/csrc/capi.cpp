@@ -465,8 +465,8 @@ int xpbd_ext_set_group_lanes(int lanes)
 {
     Global& g = G();
     if (!g.engine) return fail("xpbd_ext_set_group_lanes: no instance");
-    if (lanes != 0 && lanes != 8 && lanes != 16 && lanes != 32)
-        return fail("xpbd_ext_set_group_lanes: lanes must be 0, 8, 16 or 32");
+    if (lanes != -1 && lanes != 0 && lanes != 8 && lanes != 16 && lanes != 32)
+        return fail("xpbd_ext_set_group_lanes: lanes must be -1, 0, 8, 16 or 32");
     g.engine->tuning().scGroupLanes = lanes;
     return 0;
 }

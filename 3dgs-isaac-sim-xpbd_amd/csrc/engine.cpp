@@ -808,8 +808,18 @@ void Engine::recordSubstep(Timer* t)
         // profile mode: the build's phases get labels of their own
         void (*mark)(void*, const char*) = nullptr;
         if (t) mark = [](void* c, const char* l) { static_cast<Timer*>(c)->end(); static_cast<Timer*>(c)->begin(l); };
+        // automatic group width: the lane-group scan wins on bodies up to a
+        // few 100k nodes, the one-wave-per-node scan on larger ones
+        Tuning tu = tune_;
+        if (tu.scGroupLanes < 0) {
+            long long nodes = 0;
+            int bodies = 0;
+            for (int b = 0; b < numBodies(); ++b)
+                if (set_[b].scEnabled) { nodes += bodyCount_[b]; ++bodies; }
+            tu.scGroupLanes = (bodies > 0 && nodes <= 300000LL * bodies) ? 8 : 0;
+        }
         const bool inAlt = launch_selfcol_build(stream_, na, numBodies(), sa, x_.p, p0_.p, rest_.p, scSortBits_,
-                                                scKeysAlt_.p, scIdxAlt_.p, scHist_.p, &sorted, tune_, mark, t);
+                                                scKeysAlt_.p, scIdxAlt_.p, scHist_.p, &sorted, tu, mark, t);
         XPBD_T_END();
         scSortedIdx_ = sorted;
         scSortedKeys_ = inAlt ? scKeysAlt_.p : scKeys_.p;

@@ -20,8 +20,9 @@ struct Tuning {
     int scBuildNodes = 32;      // nodes per wave of the candidate build: 4, 8, 16, 32, 64
     int scBuildXcd = 16;        // candidate build: runs of that many blocks dealt round-robin over the XCDs
                                 // (2-256), one contiguous run per XCD (1), plain block order (0)
-    int scGroupLanes = 0;       // lanes per node of the grouped fine candidate scan: 8, 16, 32; 0: every
-                                // node takes the wave fine scan
+    int scGroupLanes = -1;      // lanes per node of the grouped fine candidate scan: 8, 16, 32; 0: every
+                                // node takes the wave fine scan; -1: 8 when the self-colliding bodies
+                                // average at most 300k nodes (measured faster there), else 0
 };
 
 // Node-space launch description shared by every per-node kernel.

@@ -352,8 +352,8 @@ class XPBD:
         self._check(fn(int(bool(enabled))), "xpbd_ext_set_self_collision_fine")
 
     def ext_set_group_lanes(self, lanes):
-        """Lanes per node of the grouped fine candidate scan (8, 16, 32), or 0
-        for the one-wave-per-node scan; same lists."""
+        """Lanes per node of the grouped fine candidate scan (8, 16, 32), 0 for
+        the one-wave-per-node scan, -1 (default) by mean body size; same lists."""
         fn = self.lib.xpbd_ext_set_group_lanes
         fn.argtypes = [ctypes.c_int]
         fn.restype = ctypes.c_int

@@ -151,8 +151,9 @@ int xpbd_ext_profile(int substeps, char* json, int len);
  * the launch shapes it picks; flags bit 0 also sizes the hash table. */
 int xpbd_ext_autotune(int substeps, int flags, char* json, int len);
 /* Lanes per node of the grouped fine candidate scan of the instance (8, 16 or
- * 32; 0 sends every node to the one-wave-per-node scan).  Candidate lists do
- * not depend on it. */
+ * 32; 0 sends every node to the one-wave-per-node scan; -1, the default, picks
+ * 8 when the self-colliding bodies average at most 300k nodes, else 0).
+ * Candidate lists do not depend on it. */
 int xpbd_ext_set_group_lanes(int lanes);
 
 /* Volume clusters given explicitly instead of by the k-ring rule: node i leads
@@ -335,7 +336,9 @@ int xpbd_batch_copy_object(xpbd_batch* dst, int dstObj, xpbd_batch* src, int src
  * 4), nodes per wave (4, 8, 16, 32, 64), XCD mapping (0 plain block order,
  * 1 one contiguous node run per XCD, 2-256 runs of that many blocks dealt
  * round-robin over the XCDs) and lanes per node of the grouped fine scan (8,
- * 16, 32; 0 sends every node to the one-wave-per-node scan).  flags bit 0 also picks the self-collision hash-table size
+ * 16, 32; 0 sends every node to the one-wave-per-node scan; -1, the default,
+ * picks 8 when the self-colliding bodies average at most 300k nodes, else 0).
+ * flags bit 0 also picks the self-collision hash-table size
  * (1, 4 or 16 x nextpow2(2N) cells per object), which can reorder candidate
  * lists that overflow their 32 slots. */
 int xpbd_batch_autotune(xpbd_batch* b, int substeps, int flags, char* json, int len);
