@@ -44,6 +44,9 @@ struct Global {
     bool stepped = false;
     bool contactOn = false;
     int contactEnabled = 0;
+    // setSelfCollisionParams(false) zeroes the candidate and overflow
+    // statistics (forward.cu:4539); the next build recounts them
+    bool scStatsZero = false;
     std::string err;
 };
 
@@ -205,6 +208,7 @@ int xpbd_set_attached(int count, const int* idx, const float* pos, float weight)
 void xpbd_set_self_collision(int enabled, float radiusScale, float excludeScale, int withinBody)
 {
     settings_self_collision(G().set, enabled, radiusScale, excludeScale, withinBody);
+    if (!enabled) G().scStatsZero = true;
 }
 
 void xpbd_self_collision_stats(int* out)
@@ -215,6 +219,10 @@ void xpbd_self_collision_stats(int* out)
     if (!g.engine) return;
     std::string err;
     if (!g.engine->selfCollisionStats(0, out, err)) fail("xpbd_self_collision_stats: " + err);
+    if (g.scStatsZero) {
+        out[0] = 0;
+        out[2] = 0;
+    }
 }
 
 void xpbd_set_shape_robust(int enabled) { G().set.shapeRobust = enabled != 0; }
@@ -254,6 +262,7 @@ int xpbd_step(void)
     E.press(0).autoLog = E.press(0).active ? E.press(0).autoLog : g.pressAutoLog;
     if (!E.step(1, err) || !E.sync(err)) return fail("xpbd_step: " + err);
     g.stepped = true;
+    if (g.set.scEnabled && !E.press(0).active) g.scStatsZero = false;
     return 0;
 }
 
@@ -587,6 +596,24 @@ int xpbd_ext_angle_pairs(int* pairNext, float* restCos, int m)
     if (pairNext && m) std::memcpy(pairNext, t.pairNext.data(), sizeof(int) * m);
     if (restCos && m) std::memcpy(restCos, t.restCos.data(), sizeof(float) * m);
     return 0;
+}
+
+int xpbd_ext_get_settings(float* out, int len)
+{
+    const Global& g = G();
+    const Settings& s = g.set;
+    const float v[42] = {
+        (float)s.iters, s.dt, s.underRelax, s.velDamping, s.stiffnessScale, s.invMassScale, s.distCompliance,
+        s.shapeCompliance, s.shapeBlend, s.angleCompliance, s.angleBlend, (float)s.useDistance, (float)s.useShape,
+        (float)s.useAngle, (float)s.useVolume, s.volCompliance, (float)s.volRingK, (float)s.volMaxMembers,
+        (float)s.volLeaderMinHop, (float)s.groundEnabled, s.groundN[0], s.groundN[1], s.groundN[2], s.groundHeight,
+        s.groundFriction, s.groundRestitution, s.groundRadius, s.groundSlop, s.groundGravity, s.objShapeStiffness,
+        (float)s.objMinComponent, (float)s.objShapeGPU, (float)s.scEnabled, s.scRadiusScale, s.scExcludeScale,
+        (float)s.scWithinBody, (float)s.shapeRobust, (float)g.colCount, g.colMargin, g.colFriction,
+        (float)g.pressAutoLog, (float)g.volJStats};
+    if (!out || len < 42) return fail("xpbd_ext_get_settings: need 42 floats");
+    std::memcpy(out, v, sizeof(v));
+    return 42;
 }
 
 int xpbd_ext_set_self_collision_fine(int enabled)

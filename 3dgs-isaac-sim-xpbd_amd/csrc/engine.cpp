@@ -183,11 +183,13 @@ Engine::~Engine()
 {
     if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
     if (graph_) (void)hipGraphDestroy(graph_);
-    if (hostBodies_) (void)hipHostFree(hostBodies_);
-    if (hostCols_) (void)hipHostFree(hostCols_);
     if (stream_) {
         (void)hipStreamSynchronize(stream_);
         (void)hipStreamDestroy(stream_);
+    }
+    for (StageSlot& sl : stage_) {
+        if (sl.pin) (void)hipHostFree(sl.pin);
+        if (sl.done) (void)hipEventDestroy(sl.done);
     }
 }
 
@@ -401,11 +403,7 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     launch_fill_float(stream_, nPad_, objW_.p, 1.0f);
     launch_fill_int(stream_, B * kMaxColliders, colHits_.p, 0);
     launch_fill_float(stream_, B * kMaxColliders * kColAcc, colPush_.p, 0.0f);
-    if (hostBodies_) { (void)hipHostFree(hostBodies_); hostBodies_ = nullptr; }
-    if (hostCols_) { (void)hipHostFree(hostCols_); hostCols_ = nullptr; }
-    XPBD_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostBodies_), sizeof(BodyDev) * B, 0));
-    XPBD_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostCols_), sizeof(ColliderDev) * B * kMaxColliders, 0));
-    std::memset(hostBodies_, 0, sizeof(BodyDev) * B);
+    bodyHost_.assign(B, BodyDev{});
     for (int b = 0; b < B; ++b) refreshCallerPositions(b);
     lastBodies_.clear();
     alphaKey_.clear();
@@ -644,7 +642,10 @@ void Engine::fillBodyDev(int b, BodyDev& d) const
     d.objOn = (s.objShapeStiffness > 0.0f && !ps.active && K > 0) ? 1 : 0;
     d.objBase = objDeviceValid_ ? objBase_[b] : 0;
     d.objK = K;
-    d.objWeighted = attActive_[b] ? 1 : 0;
+    // the attachment-weighted fit is the device path's (forward.cu:11671-11702);
+    // with the device path off, or past 64 objects, the reference fits on the
+    // host without weights (11573-11668, chosen at 12164): same sums, unweighted
+    d.objWeighted = (attActive_[b] && s.objShapeGPU && K <= 64) ? 1 : 0;
     d.objStiffness = s.objShapeStiffness;
     d.objBoxes = (d.objOn && s.scEnabled) ? 1 : 0;
     // active set only when the previous shift is valid (forward.cu:6959-7030)
@@ -672,19 +673,44 @@ void Engine::fillBodyDev(int b, BodyDev& d) const
     d.scFineInv = 1.0f / (d.scSearch * 1.01f + 1e-6f);
 }
 
+// Queue an upload of `bytes` from host memory through the next pinned slot.
+// The slot's previous copy must have left it before it is overwritten: the
+// launches of several substeps may still be queued.
+bool Engine::stageUpload(void* dst, const void* src, size_t bytes, std::string& err)
+{
+    StageSlot& sl = stage_[stageNext_];
+    stageNext_ = (stageNext_ + 1) % kStageSlots;
+    if (sl.pending) {
+        XPBD_HIP(hipEventSynchronize(sl.done));
+        sl.pending = false;
+    }
+    if (sl.cap < bytes) {
+        if (sl.pin) (void)hipHostFree(sl.pin);
+        sl.pin = nullptr;
+        sl.cap = 0;
+        XPBD_HIP(hipHostMalloc(&sl.pin, bytes, 0));
+        sl.cap = bytes;
+    }
+    if (!sl.done) XPBD_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    std::memcpy(sl.pin, src, bytes);
+    XPBD_HIP(hipMemcpyAsync(dst, sl.pin, bytes, hipMemcpyHostToDevice, stream_));
+    XPBD_HIP(hipEventRecord(sl.done, stream_));
+    sl.pending = true;
+    return true;
+}
+
 bool Engine::uploadParams(std::string& err)
 {
     const int B = numBodies();
-    for (int b = 0; b < B; ++b) fillBodyDev(b, hostBodies_[b]);
+    for (int b = 0; b < B; ++b) fillBodyDev(b, bodyHost_[b]);
     bool changed = (int)lastBodies_.size() != B ||
-                   std::memcmp(lastBodies_.data(), hostBodies_, sizeof(BodyDev) * B) != 0;
+                   std::memcmp(lastBodies_.data(), bodyHost_.data(), sizeof(BodyDev) * B) != 0;
     if (changed) {
-        XPBD_HIP(hipMemcpyAsync(bodies_.p, hostBodies_, sizeof(BodyDev) * B, hipMemcpyHostToDevice, stream_));
-        lastBodies_.assign(hostBodies_, hostBodies_ + B);
+        if (!stageUpload(bodies_.p, bodyHost_.data(), sizeof(BodyDev) * B, err)) return false;
+        lastBodies_ = bodyHost_;
     }
     if (colDirty_) {
-        std::memcpy(hostCols_, colHost_.data(), sizeof(ColliderDev) * colHost_.size());
-        XPBD_HIP(hipMemcpyAsync(cols_.p, hostCols_, sizeof(ColliderDev) * colHost_.size(), hipMemcpyHostToDevice, stream_));
+        if (!stageUpload(cols_.p, colHost_.data(), sizeof(ColliderDev) * colHost_.size(), err)) return false;
         colDirty_ = false;
     }
     // alpha tilde depends on (dt, compliance, stiffness scale) of each body
@@ -719,7 +745,7 @@ std::vector<int> Engine::planSignature() const
     int anyDist = 0, anyCol = 0, anyGround = 0, anyPress = 0, anyVol = 0, anyShape = 0, anyCGround = 0;
     int anyObj = 0, anyBoxes = 0, anySc = 0, anyDistGS = 0, anyVolGS = 0, anyFine = 0, anyAngle = 0;
     for (int b = 0; b < B; ++b) {
-        const BodyDev& d = hostBodies_[b];
+        const BodyDev& d = bodyHost_[b];
         anyObj |= d.objOn;
         anyBoxes |= d.objBoxes;
         anySc |= d.scOn;
@@ -737,13 +763,24 @@ std::vector<int> Engine::planSignature() const
     }
     const int volWave = anyVol ? (tune_.volMode >= 0 ? (tune_.volMode ? 1 : 0) : (volAvgMembers_ >= kVolWaveMembers ? 1 : 0))
                                : 0;
+    // automatic fine-scan group width: the lane-group scan wins on bodies up to
+    // a few 100k nodes, the one-wave-per-node scan on larger ones; only bodies
+    // that take the fine scan count
+    int groupLanes = tune_.scGroupLanes;
+    if (groupLanes < 0) {
+        long long nodes = 0;
+        int bodies = 0;
+        for (int b = 0; b < B; ++b)
+            if (bodyHost_[b].scOn && bodyHost_[b].scFine) { nodes += bodyCount_[b]; ++bodies; }
+        groupLanes = (bodies > 0 && nodes <= 300000LL * bodies) ? 8 : 0;
+    }
     const int nDistGS = (anyDistGS && dgsDevValid_) ? (int)dgsColorOff_.size() - 1 : 0;
     const int nVolGS = (anyVolGS && vgsDevValid_) ? (int)vgsColorOff_.size() - 1 : 0;
     return {iters, anyDist, anyCol, anyGround, anyPress, nPad_, anyVol, volWave, anyVol ? volActive_ : 0, anyShape,
             anyCGround, anyObj, anyBoxes, anySc, anyObj ? objTotal_ : 0, anySc ? scTableTotal_ : 0,
             anySc ? scSortBits_ : 0, nDistGS, nVolGS, device_alloc_epoch(), (anyShape && shcDevValid_) ? shcK_ : 0,
             allocEpoch_, tune_.jacobiTiles, tune_.volMode, tune_.shapeClusterWaves, tune_.scBuildWaves,
-            tune_.scBuildNodes, tune_.scBuildXcd, anyFine, anyAngle, tune_.scGroupLanes};
+            tune_.scBuildNodes, tune_.scBuildXcd, anyFine, anyAngle, groupLanes};
 }
 
 // Event pairs around kernel launches (profile mode only).
@@ -808,16 +845,8 @@ void Engine::recordSubstep(Timer* t)
         // profile mode: the build's phases get labels of their own
         void (*mark)(void*, const char*) = nullptr;
         if (t) mark = [](void* c, const char* l) { static_cast<Timer*>(c)->end(); static_cast<Timer*>(c)->begin(l); };
-        // automatic group width: the lane-group scan wins on bodies up to a
-        // few 100k nodes, the one-wave-per-node scan on larger ones
         Tuning tu = tune_;
-        if (tu.scGroupLanes < 0) {
-            long long nodes = 0;
-            int bodies = 0;
-            for (int b = 0; b < numBodies(); ++b)
-                if (set_[b].scEnabled) { nodes += bodyCount_[b]; ++bodies; }
-            tu.scGroupLanes = (bodies > 0 && nodes <= 300000LL * bodies) ? 8 : 0;
-        }
+        tu.scGroupLanes = sig[30];
         const bool inAlt = launch_selfcol_build(stream_, na, numBodies(), sa, x_.p, p0_.p, rest_.p, scSortBits_,
                                                 scKeysAlt_.p, scIdxAlt_.p, scHist_.p, &sorted, tu, mark, t);
         XPBD_T_END();
@@ -983,8 +1012,10 @@ bool Engine::prepare(std::string& err, bool advance)
 // reference for the next substep's active set.
 void Engine::afterSubstep()
 {
-    for (int b = 0; b < numBodies(); ++b)
-        if (hostBodies_[b].objOn) objShiftValid_[b] = 1;
+    for (int b = 0; b < numBodies(); ++b) {
+        if (bodyHost_[b].objOn) objShiftValid_[b] = 1;
+        if (bodyHost_[b].scOn && b < (int)scBuilt_.size()) scBuilt_[b] = 1;
+    }
 }
 
 bool Engine::warm(std::string& err)

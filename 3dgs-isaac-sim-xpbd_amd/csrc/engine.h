@@ -372,9 +372,21 @@ private:
     DevBuf<int> colHits_, attIdx_;
     DevBuf<float> colPush_, attPos_;
 
-    // parameter staging
-    BodyDev* hostBodies_ = nullptr;   // pinned
-    ColliderDev* hostCols_ = nullptr; // pinned
+    // parameter staging: bodyHost_ holds the parameter blocks of the next
+    // substep; uploads go through pinned slots, and a slot is written again
+    // only after the copy that last read it has completed (its event), so
+    // substeps queued back to back each see their own parameters
+    struct StageSlot {
+        void* pin = nullptr;
+        size_t cap = 0;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    };
+    static constexpr int kStageSlots = 4;
+    StageSlot stage_[kStageSlots];
+    int stageNext_ = 0;
+    bool stageUpload(void* dst, const void* src, size_t bytes, std::string& err);
+    std::vector<BodyDev> bodyHost_;
     std::vector<BodyDev> lastBodies_;
     std::vector<float> alphaKey_;
 
@@ -431,6 +443,7 @@ private:
     std::vector<int> scTable_, scTableBase_, scTableOverride_;
     int scTableTotal_ = 0, scSortBits_ = 0;
     bool scDeviceValid_ = false;
+    std::vector<char> scBuilt_;             // a build has filled the body's lists since the arrays were made
     const int* scSortedIdx_ = nullptr;
     const unsigned int* scSortedKeys_ = nullptr;
     DevBuf<unsigned int> scMax2_, scKeys_, scKeysAlt_, scHist_;

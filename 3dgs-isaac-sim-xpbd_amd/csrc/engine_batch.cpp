@@ -240,6 +240,11 @@ bool Engine::autotune(int substeps, int flags, std::string& report, std::string&
         anySc |= set_[b].scEnabled;
         anyClusters |= set_[b].useShape && !shapeClusterOf_[b].empty();
     }
+    // A knob whose first value is automatic (-1) keeps it unless a fixed
+    // value is faster by more than kFixedMargin: the automatic rule follows
+    // later changes of the batch (merges, bodies that stop self-colliding)
+    // that a fixed value cannot.
+    const float kFixedMargin = 0.03f;
     struct Knob {
         const char* name;
         int* field;
@@ -253,7 +258,7 @@ bool Engine::autotune(int substeps, int flags, std::string& report, std::string&
         {"selfcol_nodes_per_wave", &tune_.scBuildNodes, {32, 16, 64, 8, 4}, anySc},
         {"selfcol_waves_per_block", &tune_.scBuildWaves, {4, 2, 1}, anySc},
         {"selfcol_xcd_runs", &tune_.scBuildXcd, {16, 1, 0, 64}, anySc},
-        {"selfcol_group_lanes", &tune_.scGroupLanes, {0, 8, 16, 32}, anySc},
+        {"selfcol_group_lanes", &tune_.scGroupLanes, {-1, 0, 8, 16, 32}, anySc},
     };
     std::string rep = "{";
     float base = 0.0f;
@@ -271,7 +276,9 @@ bool Engine::autotune(int substeps, int flags, std::string& report, std::string&
             *k.field = k.values[j];
             float ms = 0.0f;
             ok = trial(ms);
-            if (ok && ms < bestMs) { bestMs = ms; bestV = k.values[j]; }
+            const bool autoFirst = k.values[0] == -1;
+            const float need = (autoFirst && j > 0 && bestV == -1) ? bestMs * (1.0f - kFixedMargin) : bestMs;
+            if (ok && ms < need) { bestMs = ms; bestV = k.values[j]; }
             std::snprintf(buf, sizeof(buf), "%s\"%d\": %.4f", j ? ", " : "", k.values[j], ms);
             rep += buf;
         }

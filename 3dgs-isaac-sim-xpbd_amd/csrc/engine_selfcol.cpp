@@ -81,7 +81,7 @@ SelfColArgs Engine::scArgs() const
     a.sortR = scSortR_.p;
     a.sortedIdx = nullptr;
     bool fine = false;
-    for (int b = 0; b < numBodies() && !fine; ++b) fine = hostBodies_[b].scFine != 0;
+    for (int b = 0; b < numBodies() && !fine; ++b) fine = bodyHost_[b].scFine != 0;
     a.coarseRank = fine ? coarseRank_.p : nullptr;
     a.fineTotal = fine ? fineTotal_ : 0;
     a.fineBits = fineBits_;
@@ -314,6 +314,7 @@ bool Engine::ensureSelfCol(std::string& err)
     scSortedIdx_ = nullptr;
     scSortedKeys_ = nullptr;
     scDeviceValid_ = true;
+    scBuilt_.assign(numBodies(), 0);
     return true;
 }
 
@@ -337,7 +338,7 @@ bool Engine::selfCollisionStats(int b, int out[3], std::string& err)
 {
     XPBD_ALLOC_SCOPE;
     out[0] = out[1] = out[2] = 0;
-    if (!scDeviceValid_ || !set_[b].scEnabled) return true;
+    if (!scDeviceValid_ || b >= (int)scBuilt_.size() || !scBuilt_[b]) return true;
     if (!scStats_.p) {
         scStats_.alloc(3);
         if (!scStats_.p) { err = "self-collision: hipMalloc failed"; return false; }

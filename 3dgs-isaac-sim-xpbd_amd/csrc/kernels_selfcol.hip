@@ -1140,13 +1140,14 @@ __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, 
                     float pr = 0.0f;
                     if (v < total) {
                         float4 pj, qj;
+                        int jp = -1;   // fine-scan body: the candidate's node
                         if (fineBody) {
                             // not staged (see sc_stage_kernel): through the permutation
-                            const int j = a.sortedIdx[t];
-                            const float4 pp = P[j];
-                            const float3 rr = rest[j];
-                            pj = make_float4(pp.x, pp.y, pp.z, __int_as_float(a.objId[j]));
-                            qj = make_float4(rr.x, rr.y, rr.z, __int_as_float(j));
+                            jp = a.sortedIdx[t];
+                            const float4 pp = P[jp];
+                            const float3 rr = rest[jp];
+                            pj = make_float4(pp.x, pp.y, pp.z, __int_as_float(a.objId[jp]));
+                            qj = make_float4(rr.x, rr.y, rr.z, __int_as_float(jp));
                         } else {
                             pj = a.sortP[t];
                             qj = a.sortR[t];   // issued with pj: no dependent second trip
@@ -1156,7 +1157,13 @@ __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, 
                             float d2 = 0.0f, distX = 0.0f, rel = 0.0f;
                             bool pass;
                             if (cross) {
-                                const float4 xj = a.sortX[t];
+                                float4 xj;
+                                if (jp >= 0) {
+                                    const float3 xx = X[jp];
+                                    xj = make_float4(xx.x, xx.y, xx.z, 0.0f);
+                                } else {
+                                    xj = a.sortX[t];
+                                }
                                 const float ex = x.x - xj.x, ey = x.y - xj.y, ez = x.z - xj.z;
                                 distX = sqrtf(ex * ex + ey * ey + ez * ez);
                                 const float rx = dix - (pj.x - xj.x), ry = diy - (pj.y - xj.y),

@@ -237,6 +237,20 @@ int xpbd_ext_volume_gs_order(int* order, int* colorOff);
 /* Library identity: "hip-gfx950" for the GPU library, "oracle-cpu" for the oracle. */
 const char* xpbd_ext_backend(void);
 
+/* The persistent solver settings after the setters' clamps, as floats in this
+ * order (booleans 0/1): 0 iters, 1 dt, 2 underRelax, 3 velDamping,
+ * 4 stiffnessScale, 5 invMassScale, 6 distCompliance, 7 shapeCompliance,
+ * 8 shapeBlend, 9 angleCompliance, 10 angleBlend, 11-14 distance / shape /
+ * angle / volume enabled, 15 volume compliance, 16 ring k, 17 member cap,
+ * 18 leader min hop, 19 ground enabled, 20-22 ground normal, 23 height,
+ * 24 friction, 25 restitution, 26 contact radius, 27 contact slop,
+ * 28 gravity, 29 object-shape stiffness, 30 minimum component,
+ * 31 object shape on the device, 32 self-collision enabled, 33 radius scale,
+ * 34 exclude scale, 35 within body, 36 robust polar, 37 collider count,
+ * 38 collider margin, 39 collider friction, 40 press autolog, 41 J statistics
+ * enabled.  Returns the count written (42), or -1 when len is smaller. */
+int xpbd_ext_get_settings(float* out, int len);
+
 /* ---------------------------------------------------------------------------
  * Part 3: batched objects (no counterpart in the reference, whose state is one
  * global instance, native/xpbd_capi.cu:6, 17-29).  A batch holds many

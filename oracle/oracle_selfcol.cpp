@@ -98,7 +98,9 @@ bool object_ensure_components()
 void object_shape_match(std::vector<V3>& p)
 {
     const int n = g.n, K = obj.K;
-    const bool wOn = g.attActive && !g.objW.empty();
+    // weights only on the device path (12164: objShapeGPU and K <= 64); the
+    // host path (11573-11668) fits without them
+    const bool wOn = g.attActive && !g.objW.empty() && s.objShapeGPU && K <= 64;
     const int stride = wOn ? 16 : 12;
     std::vector<double> sums((size_t)stride * K, 0.0);
     for (int i = 0; i < n; ++i) {

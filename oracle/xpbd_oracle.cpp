@@ -782,6 +782,22 @@ int xpbd_ext_profile(int substeps, char* json, int len)
     return 0;
 }
 
+int xpbd_ext_get_settings(float* out, int len)
+{
+    const float v[42] = {
+        (float)s.iters, s.dt, s.underRelax, s.velDamping, s.stiffnessScale, s.invMassScale, s.distCompliance,
+        s.shapeCompliance, s.shapeBlend, s.angleCompliance, s.angleBlend, (float)s.useDistance, (float)s.useShape,
+        (float)s.useAngle, (float)s.useVolume, s.volCompliance, (float)s.volRingK, (float)s.volMaxMembers,
+        (float)s.volLeaderMinHop, (float)s.groundEnabled, s.groundN[0], s.groundN[1], s.groundN[2], s.groundHeight,
+        s.groundFriction, s.groundRestitution, s.groundRadius, s.groundSlop, s.groundGravity, s.objShapeStiffness,
+        (float)s.objMinComponent, (float)s.objShapeGPU, (float)s.scEnabled, s.scRadiusScale, s.scExcludeScale,
+        (float)s.scWithinBody, (float)s.shapeRobust, (float)s.colCount, s.colMargin, s.colFriction,
+        (float)s.sqAutoLog, (float)s.volJStats};
+    if (!out || len < 42) return fail("xpbd_ext_get_settings: need 42 floats");
+    std::memcpy(out, v, sizeof(v));
+    return 42;
+}
+
 int xpbd_ext_get_inv_mass(float* invMass, int n)
 {
     if (!g.stepped) return fail("xpbd_ext_get_inv_mass: no step yet");

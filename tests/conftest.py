@@ -57,3 +57,6 @@ def reset_extension_settings():
         lib.xpbd_set_self_collision(0, ctypes.c_float(1.5), ctypes.c_float(2.0), 0)
         lib.xpbd_set_object_shape(ctypes.c_float(0.0))
         lib.xpbd_set_colliders(0, None, None, None, ctypes.c_float(0.0), ctypes.c_float(0.0))
+        lib.xpbd_press_stop()
+        lib.xpbd_press_autolog(1)
+        lib.xpbd_set_object_shape_gpu(1)
