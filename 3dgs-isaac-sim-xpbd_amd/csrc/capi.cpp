@@ -616,6 +616,30 @@ int xpbd_ext_get_settings(float* out, int len)
     return 42;
 }
 
+int xpbd_ext_compute_shapes_noalloc(int n, const float* restDev, const float* posDev, const int* rowOffDev,
+                                    const int* nbrDev, const float* scalesDev, const float* rotsDev, float deformEps,
+                                    float* outScalesDev, float* outQuatsDev)
+{
+    if (n <= 0 || !restDev || !posDev || !rowOffDev || !nbrDev || !scalesDev || !rotsDev || !outScalesDev ||
+        !outQuatsDev)
+        return fail("xpbd_ext_compute_shapes_noalloc: invalid arguments");
+    if (n > deformed_shape_scratch_max())
+        return fail("xpbd_ext_compute_shapes_noalloc: n exceeds the limit of " +
+                    std::to_string(deformed_shape_scratch_max()) + " Gaussians per call");
+    unsigned int* counts = nullptr;
+    std::string err;
+    if (!deformed_shape_scratch(&counts, err)) return fail("xpbd_ext_compute_shapes_noalloc: " + err);
+    launch_deformed_shapes(nullptr, 0, n, reinterpret_cast<const float3*>(restDev),
+                           reinterpret_cast<const float3*>(posDev), rowOffDev, nbrDev, scalesDev, rotsDev, deformEps,
+                           outScalesDev, outQuatsDev, counts);
+    hipError_t e = hipGetLastError();
+    unsigned int total = 0;
+    if (e == hipSuccess)
+        e = hipMemcpy(&total, counts + deformed_shape_blocks(n), sizeof(unsigned int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(std::string("xpbd_ext_compute_shapes_noalloc: ") + hipGetErrorString(e));
+    return (int)total;
+}
+
 int xpbd_ext_set_self_collision_fine(int enabled)
 {
     Global& g = G();

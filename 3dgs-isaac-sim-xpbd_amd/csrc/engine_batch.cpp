@@ -152,7 +152,6 @@ bool Engine::computeShapesAll(float deformEps, std::string& err)
 {
     XPBD_ALLOC_SCOPE;
     if (!ensureShapeBuffers(err)) return false;
-    XPBD_HIP(hipMemsetAsync(shapeCtr_.p, 0, sizeof(unsigned int), stream_));
     // padding nodes have no neighbours and are never counted as deformed
     launch_deformed_shapes(stream_, 0, nPad_, rest_.p, x_.p, rowOff_.p, nbr_.p, gsScale_.p, gsRot_.p, deformEps,
                            outScale_.p, outQuat_.p, shapeCtr_.p);

@@ -28,7 +28,7 @@ bool Engine::ensureShapeBuffers(std::string& err)
     gsOpacity_.alloc(nPad_);
     outScale_.alloc((size_t)3 * nPad_);
     outQuat_.alloc((size_t)4 * nPad_);
-    shapeCtr_.alloc(1);
+    shapeCtr_.alloc(deformed_shape_blocks(nPad_) + 1);
     if (!gsScale_.p || !gsRot_.p || !gsOpacity_.p || !outScale_.p || !outQuat_.p || !shapeCtr_.p) {
         err = "shapes: hipMalloc failed";
         return false;
@@ -84,7 +84,6 @@ bool Engine::computeShapes(int b, float deformEps, float* hostScales, float* hos
     if (!ensureShapeBuffers(err)) return false;
     const size_t nb = bodyBegin_[b];
     const int n = bodyCount_[b];
-    XPBD_HIP(hipMemsetAsync(shapeCtr_.p, 0, sizeof(unsigned int), stream_));
     launch_deformed_shapes(stream_, (int)nb, n, rest_.p, x_.p, rowOff_.p, nbr_.p, gsScale_.p + 3 * nb, gsRot_.p + 4 * nb,
                            deformEps, outScale_.p + 3 * nb, outQuat_.p + 4 * nb, shapeCtr_.p);
     if (contactOn_[b] && cM_.p)
@@ -92,7 +91,8 @@ bool Engine::computeShapes(int b, float deformEps, float* hostScales, float* hos
                             contactTau_[b], cap_or_zero(contactCap_[b]), cM_.p, cRmax_.p);
     unsigned int c = 0;
     std::vector<float> sc(hostScales ? 3 * (size_t)n : 0), qt(hostQuats ? 4 * (size_t)n : 0);
-    XPBD_HIP(hipMemcpyAsync(&c, shapeCtr_.p, sizeof(unsigned int), hipMemcpyDeviceToHost, stream_));
+    XPBD_HIP(hipMemcpyAsync(&c, shapeCtr_.p + deformed_shape_blocks(n), sizeof(unsigned int), hipMemcpyDeviceToHost,
+                            stream_));
     if (hostScales)
         XPBD_HIP(hipMemcpyAsync(sc.data(), outScale_.p + 3 * nb, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, stream_));
     if (hostQuats)

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "xpbd_device.h"
 
 namespace xpbd {
@@ -205,10 +207,16 @@ void launch_shape_match(hipStream_t s, const NodeArgs& na, const float4* Pin, fl
                         const int* rowOff, const int* nbr, const ShapeClusterArgs& sca, unsigned epi,
                         const EpiArgs& ea, const Tuning& tu);
 // scales / rots / outScale / outQuat are the body's own arrays (local index);
-// node arrays are indexed from nodeBegin.
+// node arrays are indexed from nodeBegin.  counts holds deformed_shape_blocks(n)
+// per-block counts followed by their total.
+int deformed_shape_blocks(int n);
 void launch_deformed_shapes(hipStream_t s, int nodeBegin, int n, const float3* rest, const float3* cur,
                             const int* rowOff, const int* nbr, const float* scales, const float* rots, float deformEps,
-                            float* outScale, float* outQuat, unsigned int* ctr);
+                            float* outScale, float* outQuat, unsigned int* counts);
+// The file-scope count array of the allocation-free deformed-shape call on
+// the current device, and the most Gaussians one call may take.
+bool deformed_shape_scratch(unsigned int** out, std::string& err);
+int deformed_shape_scratch_max();
 void launch_contact_pack(hipStream_t s, int nodeBegin, int n, const float* sc, const float* qt, const float* op,
                          float tau, float rcap, float* M, float* rmax);
 void launch_contact_ground(hipStream_t s, const NodeArgs& na, const float* M, float* rg);

@@ -13,6 +13,9 @@
 // from the 1-ring, singular values clamped to [1e-3, 1.5], M = S0 R0 S_f R^T,
 // Sigma' = M^T M, then a cyclic fp64 Jacobi eigen-decomposition (13022-13061)
 // of Sigma' / (tr / 3) gives the scales and the quaternion.
+#include <mutex>
+#include <string>
+
 #include "kernels.h"
 #include "mat3.h"
 #include "node_ops.h"
@@ -445,9 +448,27 @@ __global__ void __launch_bounds__(256) deformed_shape_kernel(
             outQuat[4 * li + 3] = (float)(qz / qn);
         }
     }
-    // one atomic per wave for the deformed count
+    // deformed count of the block: wave ballots summed in LDS, one partial per
+    // block (every partial is written, so nothing needs clearing first)
+    __shared__ unsigned int waveCount[4];
     const unsigned long long ballot = __ballot(counted);
-    if ((threadIdx.x & 63) == 0 && ballot) atomicAdd(ctr, (unsigned int)__popcll(ballot));
+    if ((threadIdx.x & 63) == 0) waveCount[threadIdx.x >> 6] = (unsigned int)__popcll(ballot);
+    __syncthreads();
+    if (threadIdx.x == 0) ctr[blockIdx.x] = waveCount[0] + waveCount[1] + waveCount[2] + waveCount[3];
+}
+
+// Sum of the per-block deformed counts into partials[blocks] (one block,
+// fixed order: the total does not depend on timing).
+__global__ void __launch_bounds__(256) shape_count_total_kernel(int blocks, unsigned int* __restrict__ partials)
+{
+    __shared__ unsigned int part[4];
+    unsigned int s = 0;
+    for (int k = threadIdx.x; k < blocks; k += 256) s += partials[k];
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) s += (unsigned int)__shfl_xor((int)s, m, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blocks] = part[0] + part[1] + part[2] + part[3];
 }
 
 // contactShapePackKernel (forward.cu:7358-7384)
@@ -508,13 +529,52 @@ void launch_shape_match(hipStream_t s, const NodeArgs& na, const float4* Pin, fl
                                                         nbr, sca, epi, ea);
 }
 
+int deformed_shape_blocks(int n) { return nbk(n, 256); }
+
 void launch_deformed_shapes(hipStream_t s, int nodeBegin, int n, const float3* rest, const float3* cur,
                             const int* rowOff, const int* nbr, const float* scales, const float* rots, float deformEps,
-                            float* outScale, float* outQuat, unsigned int* ctr)
+                            float* outScale, float* outQuat, unsigned int* counts)
 {
+    const int blocks = nbk(n, 256);
     if (n > 0)
-        deformed_shape_kernel<<<nbk(n, 256), 256, 0, s>>>(nodeBegin, n, rest, cur, rowOff, nbr, scales, rots, deformEps,
-                                                         outScale, outQuat, ctr);
+        deformed_shape_kernel<<<blocks, 256, 0, s>>>(nodeBegin, n, rest, cur, rowOff, nbr, scales, rots, deformEps,
+                                                    outScale, outQuat, counts);
+    shape_count_total_kernel<<<1, 256, 0, s>>>(blocks, counts);
+}
+
+// ---------------------------------------------------------------------------
+// Deformed shapes without allocation: the per-block counts go to a
+// file-scope device array instead of an engine buffer.  Each device holds its
+// own copy of the array; its address on the current device is looked up once
+// per device (hipGetSymbolAddress) and kept.  Calls on one device share the
+// array, so calls that overlap on one device race.
+constexpr int kShapeScratchBlocks = 1 << 16;   // blocks of 256 Gaussians: up to 16,777,216 per call
+static __device__ __align__(256) unsigned int scratch[kShapeScratchBlocks + 1];
+
+int deformed_shape_scratch_max() { return kShapeScratchBlocks * 256; }
+
+bool deformed_shape_scratch(unsigned int** out, std::string& err)
+{
+    static std::mutex mu;
+    static unsigned int* perDevice[256] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= 256) {
+        err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+        return false;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!perDevice[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return false;
+        }
+        perDevice[dev] = static_cast<unsigned int*>(p);
+    }
+    *out = perDevice[dev];
+    return true;
 }
 
 void launch_contact_pack(hipStream_t s, int nodeBegin, int n, const float* sc, const float* qt, const float* op,

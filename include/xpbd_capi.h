@@ -251,6 +251,24 @@ const char* xpbd_ext_backend(void);
  * enabled.  Returns the count written (42), or -1 when len is smaller. */
 int xpbd_ext_get_settings(float* out, int len);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the deformed Gaussian shapes of xpbd_compute_shapes
+ * (forward.cu:13230-13247) from caller-owned device arrays, with no device
+ * allocation.  n Gaussians; restDev and posDev float[n*3] (for example the
+ * array of xpbd_positions_device), the symmetric CSR rowOffDev int[n+1] and
+ * nbrDev (each edge (i,j) as i->j and j->i, in input edge order, as
+ * xpbd_create builds it), scalesDev float[n*3] and rotsDev float[n*4]
+ * (w,x,y,z) of the rest shapes; writes outScalesDev float[n*3] and
+ * outQuatsDev float[n*4].  Runs on the null stream and returns the deformed
+ * count (or -1) after the work has completed.  The per-block counts go to a
+ * file-scope device array sized for n <= 16,777,216; larger n is refused.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * race.  The CSR must index [0, n). */
+int xpbd_ext_compute_shapes_noalloc(int n, const float* restDev, const float* posDev, const int* rowOffDev,
+                                    const int* nbrDev, const float* scalesDev, const float* rotsDev, float deformEps,
+                                    float* outScalesDev, float* outQuatsDev);
+
 /* ---------------------------------------------------------------------------
  * Part 3: batched objects (no counterpart in the reference, whose state is one
  * global instance, native/xpbd_capi.cu:6, 17-29).  A batch holds many
