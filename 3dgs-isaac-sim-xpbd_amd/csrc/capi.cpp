@@ -437,6 +437,17 @@ int xpbd_ext_volume_topology(int* leaders, int* memberCount, int* revCount, int*
     return 0;
 }
 
+int xpbd_ext_volume_members(int* members, int len)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_ext_volume_members: no step yet");
+    std::string err;
+    int total = 0;
+    g.engine->settings(0) = g.set;
+    if (!g.engine->volumeMembers(0, members, len, total, err)) return fail("xpbd_ext_volume_members: " + err);
+    return total;
+}
+
 int xpbd_ext_get_state(float* pos, float* vel, int n)
 {
     Global& g = G();

@@ -232,6 +232,8 @@ public:
     bool volumeTopology(int b, int* leaders, int* memberCount, int* revCount, int* matType, float* detRest,
                         std::string& err);
     double volumeAvgMembers() const { return volAvgMembers_; }
+    // Members of every cluster of body b (local ids) in leader order.
+    bool volumeMembers(int b, int* members, int len, int& total, std::string& err);
 
     // Deformed Gaussian shapes of body b from the committed positions
     // (forward.cu:13230-13247); host outputs may be null.  Returns the

@@ -212,4 +212,16 @@ bool Engine::volumeTopology(int b, int* leaders, int* memberCount, int* revCount
     return true;
 }
 
+bool Engine::volumeMembers(int b, int* members, int len, int& total, std::string& err)
+{
+    XPBD_ALLOC_SCOPE;
+    if (!ensureVolume(true, err)) return false;
+    const VolumeTopo& t = volTopo_[b];
+    const int n = bodyCount_[b];
+    total = t.memOff.empty() ? 0 : t.memOff[n];
+    if (len < total || (total > 0 && !members)) { err = "buffer too small"; return false; }
+    for (int k = 0; k < total; ++k) members[k] = t.memIdx[k];
+    return true;
+}
+
 }  // namespace xpbd

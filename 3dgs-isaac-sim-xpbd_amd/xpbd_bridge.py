@@ -392,6 +392,19 @@ class XPBD:
                     "xpbd_ext_volume_topology")
         return lead, mc, rc, mt, dr
 
+    def ext_volume_members(self):
+        """Members of every volume cluster, one int32 array per node (empty for
+        non-leaders), in the builder's order."""
+        fn = self.lib.xpbd_ext_volume_members
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        fn.restype = ctypes.c_int
+        counts = self.ext_volume_topology()[1]
+        flat = np.zeros(max(int(counts.sum()), 1), np.int32)
+        total = fn(_ptr(flat), flat.size)
+        if total < 0:
+            self._check(-1, "xpbd_ext_volume_members")
+        return np.split(flat[:total], np.cumsum(counts)[:-1])
+
     def ext_inv_mass(self):
         out = np.empty(self.n, np.float32)
         self._check(self.lib.xpbd_ext_get_inv_mass(out, self.n), "xpbd_ext_get_inv_mass")

@@ -170,6 +170,11 @@ int xpbd_ext_set_volume_clusters(const int* off, const int* members, int minMemb
  * determinant.  Any pointer may be null. */
 int xpbd_ext_volume_topology(int* leaders, int* memberCount, int* revCount, int* matType, float* detRest, int n);
 
+/* Members of every volume cluster, concatenated in leader order (caller ids,
+ * each cluster in the builder's order; its size is memberCount of
+ * xpbd_ext_volume_topology).  Returns the total, or -1 when len is smaller. */
+int xpbd_ext_volume_members(int* members, int len);
+
 /* Self-collision grid: cellSize > 0 fixes the hash-cell edge (instead of
  * search radius + 2 x max displacement, capped at search + 16 d_c); tableSize
  * 0 keeps nextpow2(max(1024, 2n)) cells, else a power of two >= 1024.  Applies

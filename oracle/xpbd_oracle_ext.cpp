@@ -387,6 +387,16 @@ int xpbd_ext_set_volume_clusters(const int* off, const int* members, int minMemb
     return 0;
 }
 
+int xpbd_ext_volume_members(int* members, int len)
+{
+    if (!g.stepped) return fail("xpbd_ext_volume_members: no step yet");
+    volume_ensure();
+    const int total = vol.memOff.empty() ? 0 : vol.memOff[g.n];
+    if (len < total || (total > 0 && !members)) return fail("xpbd_ext_volume_members: buffer too small");
+    for (int k = 0; k < total; ++k) members[k] = vol.memIdx[k];
+    return total;
+}
+
 int xpbd_ext_volume_topology(int* leaders, int* memberCount, int* revCount, int* matType, float* detRest, int n)
 {
     if (!g.stepped) return fail("xpbd_ext_volume_topology: no step yet");

@@ -51,6 +51,9 @@ def test_volume_parity(scene, product_path, oracle_path, case):
     for name, a, b in zip(("leader", "members", "rev", "matType"), tg[:4], tc[:4]):
         np.testing.assert_array_equal(a, b, err_msg=name)
     np.testing.assert_array_equal(tg[4].view(np.int32), tc[4].view(np.int32), err_msg="detRest bits")
+    mg, mc = gpu.ext_volume_members(), cpu.ext_volume_members()
+    for i, (a, b) in enumerate(zip(mg, mc)):
+        np.testing.assert_array_equal(a, b, err_msg=f"members of cluster {i}")
     extent = scenes.scene_extent(scene)
     worst = 0.0
     for f in range(30):
