@@ -17,7 +17,7 @@ inline int xcd_grid(int blocks) { return (blocks + 7) & ~7; }
 struct Tuning {
     int jacobiTiles = 4;        // 64-node tiles (one wave each) per distance Jacobi block: 1, 2, 4
     int volMode = -1;           // volume cluster solve: -1 by mean member count, 0 thread, 1 wave per cluster
-    int shapeClusterWaves = 4;  // clusters (one wave each) per cluster shape-matching block: 1, 2, 4
+    int shapeClusterWaves = 4;  // waves (4 clusters each) per cluster shape-matching block: 1, 2, 4
     int scBuildWaves = 4;       // waves per self-collision candidate-build block: 1, 2, 4
     int scBuildNodes = 32;      // nodes per wave of the candidate build: 4, 8, 16, 32, 64
     int scBuildXcd = 16;        // candidate build: runs of that many blocks dealt round-robin over the XCDs

@@ -54,104 +54,120 @@ __device__ bool polar_rotation_f32(const M3& A, M3& R)
     return true;
 }
 
-__device__ __forceinline__ float lane_f(float v, int lane)
+// Butterfly sum over the 64 lanes of a wave: every lane ends with the same
+// total, added in a fixed tree (lane partials l and l ^ m, m = 32 .. 1).
+__device__ __forceinline__ float wave_tree_sum(float v)
 {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = v + __shfl_xor(v, m, 64);
+    return v;
 }
 
-// Cluster shape matching, sums: one wave64 per cluster.  Members are spread
-// over the lanes (64 at a time) and their values staged in LDS; each sum of
-// the centroid and of A = sum (p - c)(q - c0)^T is then taken by one lane
-// (three, then nine lanes) serially in member order, which keeps the float
-// result of a serial loop.  The centroid and A go to the cluster's parameter
-// row; the rotation follows in shape_cluster_polar_kernel.
-constexpr int kClusterStage = 65;   // LDS row stride (odd: the summing lanes hit distinct banks)
+// Cluster shape matching.  A block takes 4 clusters per wave.  A wave first
+// loads the member indices of its clusters' first 64 members (one per lane),
+// then their current and rest positions, all clusters at once so the gathers
+// overlap.  Lane l sums members l, l + 64, ... (member order, caller order
+// inside the cluster) and the 64 partials are combined by wave_tree_sum,
+// first for the centroid, then for the nine entries of
+// A = sum (p - c)(q - c0)^T; lane 0 leaves centroid and A in LDS.  After the
+// barrier the first wave takes one cluster per lane: A is normalised by its
+// Frobenius norm and replaced by its rotation (fp64 polar when robust), and
+// centroid, rotation and the valid flag go to the cluster's parameter row for
+// the per-node projection.  The polar chain thus runs once per cluster, on
+// as many lanes as the block has clusters.
+constexpr int kClustersPerWave = 4;
+constexpr int kClusterRounds = 2;   // members per lane loaded up front
 
-__global__ void __launch_bounds__(256) shape_cluster_sum_kernel(int K, ShapeClusterArgs a, const float4* __restrict__ P,
-                                                               const float3* __restrict__ rest)
+__global__ void __launch_bounds__(256) shape_cluster_kernel(int K, ShapeClusterArgs a, const float4* __restrict__ P,
+                                                           const float3* __restrict__ rest,
+                                                           const int* __restrict__ tileBody,
+                                                           const BodyDev* __restrict__ bodies)
 {
-    __shared__ float st[4][9 * kClusterStage];
-    const int wl = threadIdx.x >> 6;
-    const int k = xcd_block() * (int)(blockDim.x >> 6) + wl;   // clusters in node order: neighbours share an XCD
-    if (k >= K) return;   // whole waves; only wave-level barriers below
-    const int lane = threadIdx.x & 63;
-    const int b0 = a.off[k], cnt = a.off[k + 1] - b0;
-    float* prm = &a.params[16 * (size_t)k];
-    if (cnt < 4) {
-        if (lane == 0) prm[15] = 0.0f;
-        return;
+    constexpr int C = kClustersPerWave, NR = kClusterRounds;
+    __shared__ float sums[4 * C][12];   // centroid, then A entry (c, r) at 3 + 3 c + r
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int perBlock = (int)(blockDim.x >> 6) * C;
+    const int k0 = xcd_block() * perBlock;   // neighbouring clusters share an XCD
+    int b0[C], cnt[C], j[C][NR];
+    float4 p[C][NR];
+    float3 q[C][NR];
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+        const int k = k0 + wave * C + i;
+        b0[i] = k < K ? a.off[k] : 0;
+        cnt[i] = k < K ? a.off[k + 1] - b0[i] : 0;
+        if (cnt[i] < 4) cnt[i] = 0;   // too small: no rotation
     }
-    float* row = st[wl];
-    float s = 0.0f;   // lane q < 3: centroid sum of component q
-    for (int base = 0; base < cnt; base += 64) {
-        const int t = base + lane;
-        const float4 p = (t < cnt) ? P[a.mem[b0 + t]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        row[lane] = p.x;
-        row[kClusterStage + lane] = p.y;
-        row[2 * kClusterStage + lane] = p.z;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        const int m = min(64, cnt - base);
-        if (lane < 3)
-            for (int u = 0; u < m; ++u) s = s + row[lane * kClusterStage + u];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();   // the next chunk overwrites the rows
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    }
-    const float invC = 1.0f / (float)cnt;
-    const float cx = lane_f(s, 0) * invC, cy = lane_f(s, 1) * invC, cz = lane_f(s, 2) * invC;
-    const float3 cr = a.comRest[k];
-    float acc = 0.0f;   // lane q < 9: entry (c, r) = (q / 3, q % 3) of A
-    for (int base = 0; base < cnt; base += 64) {
-        const int t = base + lane;
-        float pv[3] = {0.0f, 0.0f, 0.0f}, qv[3] = {0.0f, 0.0f, 0.0f};
-        if (t < cnt) {
-            const int j = a.mem[b0 + t];
-            const float4 p = P[j];
-            const float3 q = rest[j];
-            pv[0] = p.x - cx; pv[1] = p.y - cy; pv[2] = p.z - cz;
-            qv[0] = q.x - cr.x; qv[1] = q.y - cr.y; qv[2] = q.z - cr.z;
+#pragma unroll
+    for (int i = 0; i < C; ++i)
+#pragma unroll
+        for (int u = 0; u < NR; ++u) j[i][u] = lane + 64 * u < cnt[i] ? a.mem[b0[i] + lane + 64 * u] : -1;
+#pragma unroll
+    for (int i = 0; i < C; ++i)
+#pragma unroll
+        for (int u = 0; u < NR; ++u) {
+            p[i][u] = j[i][u] >= 0 ? P[j[i][u]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            q[i][u] = j[i][u] >= 0 ? rest[j[i][u]] : make_float3(0.0f, 0.0f, 0.0f);
         }
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+    for (int i = 0; i < C; ++i) {
+        if (cnt[i] == 0) continue;   // wave-uniform
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
 #pragma unroll
-            for (int r = 0; r < 3; ++r) row[(3 * c + r) * kClusterStage + lane] = pv[r] * qv[c];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        const int m = min(64, cnt - base);
-        if (lane < 9)
-            for (int u = 0; u < m; ++u) acc += row[lane * kClusterStage + u];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (int u = 0; u < NR; ++u)
+            if (j[i][u] >= 0) { sx = sx + p[i][u].x; sy = sy + p[i][u].y; sz = sz + p[i][u].z; }
+        for (int t = lane + 64 * NR; t < cnt[i]; t += 64) {
+            const float4 pt = P[a.mem[b0[i] + t]];
+            sx = sx + pt.x; sy = sy + pt.y; sz = sz + pt.z;
+        }
+        const float invC = 1.0f / (float)cnt[i];
+        const float cx = wave_tree_sum(sx) * invC, cy = wave_tree_sum(sy) * invC, cz = wave_tree_sum(sz) * invC;
+        const float3 cr = a.comRest[k0 + wave * C + i];
+        float acc[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) acc[e] = 0.0f;
+        for (int t = lane, u = 0; t < cnt[i]; t += 64, ++u) {
+            float4 pt;
+            float3 qt;
+            if (u < NR) {
+                pt = p[i][0]; qt = q[i][0];
+#pragma unroll
+                for (int v = 1; v < NR; ++v)
+                    if (u == v) { pt = p[i][v]; qt = q[i][v]; }
+            } else {
+                const int jt = a.mem[b0[i] + t];
+                pt = P[jt];
+                qt = rest[jt];
+            }
+            const float pv[3] = {pt.x - cx, pt.y - cy, pt.z - cz}, qv[3] = {qt.x - cr.x, qt.y - cr.y, qt.z - cr.z};
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) acc[3 * c + r] = acc[3 * c + r] + pv[r] * qv[c];
+        }
+#pragma unroll
+        for (int e = 0; e < 9; ++e) acc[e] = wave_tree_sum(acc[e]);
+        if (lane == 0) {
+            const int kl = wave * C + i;
+            sums[kl][0] = cx; sums[kl][1] = cy; sums[kl][2] = cz;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) sums[kl][3 + e] = acc[e];
+        }
     }
-    if (lane < 9) prm[3 + lane] = acc;
-    if (lane == 0) {
-        prm[0] = cx; prm[1] = cy; prm[2] = cz;
-        prm[15] = 1.0f;
-    }
-}
-
-// Cluster shape matching, rotation: one thread per cluster.  A (from the sum
-// kernel) is normalised by its Frobenius norm and replaced by its rotation
-// (fp64 polar when robust) for the per-node projection.
-__global__ void __launch_bounds__(64) shape_cluster_polar_kernel(int K, ShapeClusterArgs a,
-                                                                 const int* __restrict__ tileBody,
-                                                                 const BodyDev* __restrict__ bodies)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
-    const int b0 = a.off[k];
-    if (a.off[k + 1] - b0 < 4) return;
-    const BodyDev& B = bodies[tileBody[a.mem[b0] / kTile]];
+    __syncthreads();
+    const int k = k0 + lane;
+    if (wave != 0 || lane >= perBlock || k >= K) return;
+    const int first = a.off[k];
     float* prm = &a.params[16 * (size_t)k];
+    if (a.off[k + 1] - first < 4) {
+        prm[15] = 0.0f;
+        return;
+    }
     M3 A;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int r = 0; r < 3; ++r) A.m[c][r] = prm[3 + 3 * c + r];
+        for (int r = 0; r < 3; ++r) A.m[c][r] = sums[lane][3 + 3 * c + r];
     float nsq = 0.0f;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -161,12 +177,14 @@ __global__ void __launch_bounds__(64) shape_cluster_polar_kernel(int K, ShapeClu
     bool ok = isfinite(an) && !(an < 1e-8f);
     M3 R = m3_diag(1.0f, 1.0f, 1.0f);
     if (ok) {
+        const BodyDev& B = bodies[tileBody[a.mem[first] / kTile]];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int r = 0; r < 3; ++r) A.m[c][r] /= an;
         ok = B.shapeRobust ? polar_rotation_f64(A, R) : polar_rotation_f32(A, R);
     }
+    prm[0] = sums[lane][0]; prm[1] = sums[lane][1]; prm[2] = sums[lane][2];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -520,11 +538,9 @@ void launch_shape_match(hipStream_t s, const NodeArgs& na, const float4* Pin, fl
                         const EpiArgs& ea, const Tuning& tu)
 {
     const int per = (tu.shapeClusterWaves == 1 || tu.shapeClusterWaves == 2) ? tu.shapeClusterWaves : 4;
-    if (sca.K > 0) {
-        shape_cluster_sum_kernel<<<xcd_grid(nbk(sca.K, per)), 64 * per, 0, s>>>(sca.K, sca, Pin, rest);
-        // one wave per block: the few waves spread over the CUs
-        shape_cluster_polar_kernel<<<nbk(sca.K, 64), 64, 0, s>>>(sca.K, sca, na.tileBody, na.bodies);
-    }
+    if (sca.K > 0)
+        shape_cluster_kernel<<<xcd_grid(nbk(sca.K, per * kClustersPerWave)), 64 * per, 0, s>>>(sca.K, sca, Pin, rest, na.tileBody,
+                                                                         na.bodies);
     shape_match_kernel<<<xcd_grid(nbk(na.nPad, 256)), 256, 0, s>>>(na.nPad, na.tileBody, na.bodies, Pin, Pout, rest, rowOff,
                                                         nbr, sca, epi, ea);
 }

@@ -284,7 +284,8 @@ void shape_matching(const std::vector<V3>& pin, std::vector<V3>& pout)
 // current centroid, A = sum (p - c)(q - c0)^T normalised by its Frobenius norm,
 // polar rotation (fp64 when robust), then every member is projected toward
 // c + R (q - c0) with xpbdProjectTowardGoal (forward.cu:4972-4988), like
-// the per-node pass (6088-6383).  Sums run in member order.
+// the per-node pass (6088-6383).  The sums follow the GPU's wave layout: 64
+// lane partials in member order, combined by a fixed butterfly tree.
 // ---------------------------------------------------------------------------
 ShapeClusters shc;
 
@@ -315,6 +316,19 @@ void shape_clusters_set(const int* clusterOf, int n)
     shc.on = true;
 }
 
+// Butterfly combination of 64 partials (entries [e0, e1) of each): round m
+// (32, 16, ..., 1) replaces partial l by partial l + partial (l xor m).
+static void tree_sum64(float (&part)[64][12], int e0, int e1)
+{
+    for (int m = 32; m >= 1; m >>= 1) {
+        float next[64][12];
+        for (int l = 0; l < 64; ++l)
+            for (int e = e0; e < e1; ++e) next[l][e] = part[l][e] + part[l ^ m][e];
+        for (int l = 0; l < 64; ++l)
+            for (int e = e0; e < e1; ++e) part[l][e] = next[l][e];
+    }
+}
+
 void shape_cluster_matching(const std::vector<V3>& pin, std::vector<V3>& pout)
 {
     const int n = g.n, K = (int)shc.comRest.size();
@@ -325,17 +339,31 @@ void shape_cluster_matching(const std::vector<V3>& pin, std::vector<V3>& pout)
     for (int k = 0; k < K; ++k) {
         const int b0 = shc.off[k], cnt = shc.off[k + 1] - b0;
         if (cnt < 4) continue;
-        V3 sc{0, 0, 0};
-        for (int t = 0; t < cnt; ++t) sc = add(sc, pin[shc.mem[b0 + t]]);
-        const V3 cc = mul(sc, 1.0f / (float)cnt);
+        // sums in the GPU's fixed order: 64 lane partials (members l, l + 64,
+        // ... in member order), combined by a butterfly tree
+        float part[64][12];
+        for (int l = 0; l < 64; ++l)
+            for (int e = 0; e < 12; ++e) part[l][e] = 0.0f;
+        for (int l = 0; l < 64; ++l)
+            for (int t = l; t < cnt; t += 64) {
+                const V3 p = pin[shc.mem[b0 + t]];
+                part[l][0] = part[l][0] + p.x; part[l][1] = part[l][1] + p.y; part[l][2] = part[l][2] + p.z;
+            }
+        tree_sum64(part, 0, 3);
+        const float invC = 1.0f / (float)cnt;
+        const V3 cc{part[0][0] * invC, part[0][1] * invC, part[0][2] * invC};
         const V3 cr = shc.comRest[k];
-        M3 A = mzero();
-        for (int t = 0; t < cnt; ++t) {
-            const int j = shc.mem[b0 + t];
-            const V3 p = sub(pin[j], cc), q = sub(g.rest[j], cr);
-            const float pv[3] = {p.x, p.y, p.z}, qv[3] = {q.x, q.y, q.z};
-            for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) A.m[c][r] += pv[r] * qv[c];
-        }
+        for (int l = 0; l < 64; ++l)
+            for (int t = l; t < cnt; t += 64) {
+                const int j = shc.mem[b0 + t];
+                const V3 p = pin[j], q = g.rest[j];
+                const float pv[3] = {p.x - cc.x, p.y - cc.y, p.z - cc.z}, qv[3] = {q.x - cr.x, q.y - cr.y, q.z - cr.z};
+                for (int c = 0; c < 3; ++c)
+                    for (int r = 0; r < 3; ++r) part[l][3 + 3 * c + r] = part[l][3 + 3 * c + r] + pv[r] * qv[c];
+            }
+        tree_sum64(part, 3, 12);
+        M3 A;
+        for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) A.m[c][r] = part[0][3 + 3 * c + r];
         float nsq = 0.0f;
         for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) nsq += A.m[c][r] * A.m[c][r];
         const float an = std::sqrt(nsq);

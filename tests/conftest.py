@@ -60,3 +60,6 @@ def reset_extension_settings():
         lib.xpbd_press_stop()
         lib.xpbd_press_autolog(1)
         lib.xpbd_set_object_shape_gpu(1)
+        lib.xpbd_set_shape_robust(1)
+        lib.xpbd_set_contact_shape(0, ctypes.c_float(0.2), ctypes.c_float(0.0))
+        lib.xpbd_set_volume(ctypes.c_float(1e-6), 1, 64, 1)

@@ -133,3 +133,34 @@ def test_cluster_shape_matching_parity(scene, product_path, oracle_path, robust)
     assert worst <= TOL_REL
     gpu.destroy()
     cpu.destroy()
+
+
+@pytest.mark.parametrize("robust", [True, False])
+@pytest.mark.parametrize("size", [64, 190])
+def test_cluster_sums_bit_exact(scene, product_path, oracle_path, robust, size):
+    """Cluster shape matching alone, one substep from an identical perturbed
+    state: the wave's lane partials and butterfly tree are restated in the
+    oracle, so positions agree bit for bit (190-member clusters take three
+    lane rounds)."""
+    lab = scenes.kmeans_clusters(scene["pos"], size=size, seed=5)
+    rng = np.random.default_rng(1)
+    pos = (scene["pos"] + rng.normal(0, 0.003, scene["pos"].shape)).astype(np.float32)
+    out = []
+    for path in (product_path, oracle_path):
+        sim = XPBD(path)
+        sim.create(scene)
+        sim.set_solver(iters=1, dt=1 / 600, under_relax=0.6, vel_damping=0.01, dist_compliance=1e-6,
+                       shape_compliance=1e-7, shape_blend=0.3)
+        sim.set_constraints(distance=False, shape=True, angle=False, volume=False)
+        sim.set_shape_robust(robust)
+        sim.set_ground(False, (0, 0, 1), 0.0, gravity=0.0)
+        sim.ext_set_shape_clusters(lab)
+        sim.step()
+        sim.ext_set_state(pos, np.zeros_like(pos))
+        sim.step()
+        out.append(sim.positions())
+        sim.destroy()
+    moved = np.abs(out[1] - pos).max()
+    print(f"cluster sums size={size} robust={robust}: {lab.max() + 1} clusters, moved {moved:.3e}")
+    assert moved > 1e-5
+    np.testing.assert_array_equal(out[0], out[1])
