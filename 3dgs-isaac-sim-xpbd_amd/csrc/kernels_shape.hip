@@ -9,7 +9,7 @@
 // S0 < 0.08 Smax, reflection fix) or the float path, goal = c + R q_i and an
 // XPBD pull toward the goal (4972-4988).  One thread per node; Jacobi.
 //
-// Deformed shapes (forward.cu:13063-13225): A = (Q P^T + aI)(P P^T + aI)^-1
+// Deformed shapes (forward.cu:13063-13225): G = (Q P^T + eI)(P P^T + eI)^-1
 // from the 1-ring, singular values clamped to [1e-3, 1.5], M = S0 R0 S_f R^T,
 // Sigma' = M^T M, then a cyclic fp64 Jacobi eigen-decomposition (13022-13061)
 // of Sigma' / (tr / 3) gives the scales and the quaternion.
@@ -19,6 +19,7 @@
 #include "kernels.h"
 #include "mat3.h"
 #include "node_ops.h"
+#include "jacobi3.h"
 #include "polar.h"
 
 namespace xpbd {
@@ -307,44 +308,145 @@ __global__ void __launch_bounds__(256) shape_match_kernel(
 }
 
 // ---------------------------------------------------------------------------
-__device__ void sym_eig3_f64(double a[3][3], double ev[3], double V[3][3])
+// Deformed shapes, in four steps per Gaussian (forward.cu:13063-13225).
+
+// 1. Local deformation gradient from the 1-ring: with P the rest offsets and
+// Q the current offsets of the neighbours, G = (Q P^T + e I)(P P^T + e I)^-1,
+// e = 1e-4 tr(P P^T) (1e-6 when the trace is 0).  The inverse is taken of
+// (P P^T + e I) / m, m its mean diagonal, and rescaled by 1/m.  False when
+// that normalised matrix is singular (|det| < 1e-9).
+__device__ __forceinline__ bool ring_gradient(float3 x0, float3 x, int first, int count,
+                                              const float3* __restrict__ rest, const float3* __restrict__ cur,
+                                              const int* __restrict__ nbr, M3& G)
 {
-    for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) V[r][c] = (r == c) ? 1.0 : 0.0;
-    for (int sweep = 0; sweep < 32; ++sweep) {
-        const double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
-        const double diag = fabs(a[0][0]) + fabs(a[1][1]) + fabs(a[2][2]);
-        if (off <= 1e-15 * diag + 1e-300) break;
-        for (int k = 0; k < 3; ++k) {
-            const int p = (k == 2) ? 1 : 0, q = (k == 0) ? 1 : 2;
-            const double apq = a[p][q];
-            if (fabs(apq) <= 1e-300) continue;
-            const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
-            const double t = ((theta >= 0.0) ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-            const double c = 1.0 / sqrt(t * t + 1.0);
-            const double s = t * c;
-            for (int m = 0; m < 3; ++m) {
-                const double amp = a[m][p], amq = a[m][q];
-                a[m][p] = c * amp - s * amq;
-                a[m][q] = s * amp + c * amq;
+    M3 rr = m3_zero(), cr = m3_zero();   // sum p p^T, sum q p^T
+    for (int k = 0; k < count; ++k) {
+        const int j = nbr[first + k];
+        const float3 a = rest[j], b = cur[j];
+        const float pr[3] = {a.x - x0.x, a.y - x0.y, a.z - x0.z};
+        const float qc[3] = {b.x - x.x, b.y - x.y, b.z - x.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                rr.m[c][r] += pr[r] * pr[c];
+                cr.m[c][r] += qc[r] * pr[c];
             }
-            for (int m = 0; m < 3; ++m) {
-                const double apm = a[p][m], aqm = a[q][m];
-                a[p][m] = c * apm - s * aqm;
-                a[q][m] = s * apm + c * aqm;
-            }
-            for (int m = 0; m < 3; ++m) {
-                const double vmp = V[m][p], vmq = V[m][q];
-                V[m][p] = c * vmp - s * vmq;
-                V[m][q] = s * vmp + c * vmq;
-            }
-        }
     }
-    ev[0] = a[0][0];
-    ev[1] = a[1][1];
-    ev[2] = a[2][2];
+    const float trace = rr.m[0][0] + rr.m[1][1] + rr.m[2][2];
+    const float ridge = trace > 0.0f ? trace * 1e-4f : 1e-6f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {   // every entry takes the sum (so -0 off the diagonal becomes +0)
+            const float d = c == r ? ridge : 0.0f;
+            rr.m[c][r] += d;
+            cr.m[c][r] += d;
+        }
+    const float mean = fmaxf((rr.m[0][0] + rr.m[1][1] + rr.m[2][2]) * (1.0f / 3.0f), 1e-30f);
+    const float invMean = 1.0f / mean;
+    M3 unit = rr, inv;
+    m3_scale(unit, invMean);
+    if (!m3_inverse_safe(unit, inv, 1e-9f)) return false;
+    m3_scale(inv, invMean);
+    G = m3_mul(cr, inv);
+    return true;
 }
 
+// 2. Stretch and rotation of G with its singular values clamped to
+// [1e-3, 1.5]: G^T G = V diag(s^2) V^T (float Jacobi), U = G V diag(1/s)
+// (1/s = 0 below 1e-7), R = U V^T with U's first column negated when R is a
+// reflection, stretch = V diag(s) V^T.
+__device__ __forceinline__ void clamped_stretch(const M3& G, M3& stretch, M3& R)
+{
+    float s2[3];
+    M3 V;
+    m3_eig_sym(m3_mul(m3_transpose(G), G), s2, V);
+    float sv[3], inv[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        sv[k] = fminf(fmaxf(sqrtf(fmaxf(0.0f, s2[k])), 1e-3f), 1.5f);
+        inv[k] = sv[k] > 1.0e-7f ? 1.0f / sv[k] : 0.0f;
+    }
+    M3 U = m3_mul(m3_mul(G, V), m3_diag(inv[0], inv[1], inv[2]));
+    const M3 Vt = m3_transpose(V);
+    R = m3_mul(U, Vt);
+    if (m3_det(R) < 0.0f) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) U.m[0][r] *= -1.0f;
+        R = m3_mul(U, Vt);
+    }
+    stretch = m3_mul(m3_mul(V, m3_diag(sv[0], sv[1], sv[2])), Vt);
+}
+
+// 3. Deformed covariance Sigma' = M^T M, M = diag(s0) R0 stretch R^T, R0 the
+// rotation of the unit quaternion (w, x, y, z).
+__device__ __forceinline__ M3 deformed_covariance(float sx, float sy, float sz, float w, float x, float y, float z,
+                                                  const M3& stretch, const M3& R)
+{
+    M3 R0;
+    R0.m[0][0] = 1.f - 2.f * (y * y + z * z); R0.m[0][1] = 2.f * (x * y - w * z); R0.m[0][2] = 2.f * (x * z + w * y);
+    R0.m[1][0] = 2.f * (x * y + w * z); R0.m[1][1] = 1.f - 2.f * (x * x + z * z); R0.m[1][2] = 2.f * (y * z - w * x);
+    R0.m[2][0] = 2.f * (x * z - w * y); R0.m[2][1] = 2.f * (y * z + w * x); R0.m[2][2] = 1.f - 2.f * (x * x + y * y);
+    const M3 M = m3_mul(m3_mul(m3_mul(m3_diag(sx, sy, sz), R0), stretch), m3_transpose(R));
+    return m3_mul(m3_transpose(M), M);
+}
+
+// Unit quaternion (w, x, y, z) of a proper rotation matrix F (rows r, columns
+// c), by the largest of w^2, x^2, y^2, z^2 (Shepperd, J. Guidance and Control
+// 1(3), 1978), normalised at the end.
+__device__ __forceinline__ void rotation_to_quat(const double (&F)[3][3], double (&q)[4])
+{
+    const double tr = F[0][0] + F[1][1] + F[2][2];
+    if (tr > 0.0) {
+        const double d = 2.0 * sqrt(tr + 1.0);
+        q[0] = 0.25 * d; q[1] = (F[2][1] - F[1][2]) / d; q[2] = (F[0][2] - F[2][0]) / d; q[3] = (F[1][0] - F[0][1]) / d;
+    } else if (F[0][0] > F[1][1] && F[0][0] > F[2][2]) {
+        const double d = 2.0 * sqrt(1.0 + F[0][0] - F[1][1] - F[2][2]);
+        q[0] = (F[2][1] - F[1][2]) / d; q[1] = 0.25 * d; q[2] = (F[0][1] + F[1][0]) / d; q[3] = (F[0][2] + F[2][0]) / d;
+    } else if (F[1][1] > F[2][2]) {
+        const double d = 2.0 * sqrt(1.0 + F[1][1] - F[0][0] - F[2][2]);
+        q[0] = (F[0][2] - F[2][0]) / d; q[1] = (F[0][1] + F[1][0]) / d; q[2] = 0.25 * d; q[3] = (F[1][2] + F[2][1]) / d;
+    } else {
+        const double d = 2.0 * sqrt(1.0 + F[2][2] - F[0][0] - F[1][1]);
+        q[0] = (F[1][0] - F[0][1]) / d; q[1] = (F[0][2] + F[2][0]) / d; q[2] = (F[1][2] + F[2][1]) / d; q[3] = 0.25 * d;
+    }
+    const double n = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = q[k] / n;
+}
+
+// 4. Scales and orientation of a covariance with trace tr > 0: the
+// symmetrised Sigma / (tr / 3) is diagonalised in fp64 by cyclic Jacobi
+// (jacobi3.h; at most 32 sweeps, relative stop 1e-15); the eigenvector frame
+// is made proper by negating its third axis; scale k = sqrt(lambda_k tr / 3),
+// at least floorScale.
+__device__ __forceinline__ void covariance_shape(const M3& Sigma, double tr, double floorScale, float* scale,
+                                                 float* quat)
+{
+    const double norm = 3.0 / tr;
+    double a[3][3], F[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a[r][c] = 0.5 * ((double)Sigma.m[c][r] + (double)Sigma.m[r][c]) * norm;
+    set_identity(F);
+    jacobi_cyclic(a, F, 32, 1e-15, 1e-300);
+    if (det_rows(F) < 0.0)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) F[r][2] = -F[r][2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) scale[k] = (float)fmax(sqrt(fmax(a[k][k], 0.0) / norm), floorScale);
+    double q[4];
+    rotation_to_quat(F, q);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) quat[k] = (float)q[k];
+}
+
+// One thread per Gaussian of the body: Gaussians that moved more than
+// deformEps from rest and have at least 3 neighbours get the deformed scale
+// and orientation; the others keep their own.  ctr[block] receives the
+// block's count of deformed Gaussians.
 __global__ void __launch_bounds__(256) deformed_shape_kernel(
     int nodeBegin, int n, const float3* __restrict__ rest, const float3* __restrict__ cur,
     const int* __restrict__ rowOff, const int* __restrict__ nbr, const float* __restrict__ scales,
@@ -355,115 +457,29 @@ __global__ void __launch_bounds__(256) deformed_shape_kernel(
     bool counted = false;
     if (li < n) {
         const int i = nodeBegin + li;
-        const float s0x = scales[3 * li], s0y = scales[3 * li + 1], s0z = scales[3 * li + 2];
-        const float q0w = rots[4 * li], q0x = rots[4 * li + 1], q0y = rots[4 * li + 2], q0z = rots[4 * li + 3];
+        const float* s0 = scales + 3 * li;
+        const float* q0 = rots + 4 * li;
         const float3 x = cur[i], x0 = rest[i];
-        const float dxv = x.x - x0.x, dyv = x.y - x0.y, dzv = x.z - x0.z;
-        const float diff = sqrtf(dxv * dxv + dyv * dyv + dzv * dzv);
-        const int off = rowOff[i], kn = rowOff[i + 1] - off;
-        bool deformed = false;
-        M3 Sigma = m3_zero();
-        if (diff > deformEps && kn >= 3) {
-            M3 PPt = m3_zero(), QPt = m3_zero();
-            for (int k = 0; k < kn; ++k) {
-                const int j = nbr[off + k];
-                const float3 rj = rest[j], cj = cur[j];
-                const float p[3] = {rj.x - x0.x, rj.y - x0.y, rj.z - x0.z};
-                const float q[3] = {cj.x - x.x, cj.y - x.y, cj.z - x.z};
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        PPt.m[c][r] += p[r] * p[c];
-                        QPt.m[c][r] += q[r] * p[c];
-                    }
-            }
-            const float tr = PPt.m[0][0] + PPt.m[1][1] + PPt.m[2][2];
-            const float alpha = (tr > 0.0f) ? (tr * 1e-4f) : 1e-6f;
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    const float d = (c == r) ? alpha : 0.0f;
-                    PPt.m[c][r] += d;
-                    QPt.m[c][r] += d;
-                }
-            const float sP = fmaxf((PPt.m[0][0] + PPt.m[1][1] + PPt.m[2][2]) * (1.0f / 3.0f), 1e-30f);
-            const float invSP = 1.0f / sP;
-            M3 Ph = PPt;
-            m3_scale(Ph, invSP);
-            M3 invHat;
-            if (m3_inverse_safe(Ph, invHat, 1e-9f)) {
-                M3 invPPt = invHat;
-                m3_scale(invPPt, invSP);
-                const M3 A = m3_mul(QPt, invPPt);
-                const M3 AtA = m3_mul(m3_transpose(A), A);
-                float S2[3];
-                M3 V;
-                m3_eig_sym(AtA, S2, V);
-                float sv[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) sv[k] = fminf(fmaxf(sqrtf(fmaxf(0.0f, S2[k])), 1e-3f), 1.5f);
-                const float e = 1.0e-7f;
-                const M3 Sinv = m3_diag(sv[0] > e ? 1.0f / sv[0] : 0.0f, sv[1] > e ? 1.0f / sv[1] : 0.0f,
-                                        sv[2] > e ? 1.0f / sv[2] : 0.0f);
-                M3 U = m3_mul(m3_mul(A, V), Sinv);
-                M3 R = m3_mul(U, m3_transpose(V));
-                if (m3_det(R) < 0.0f) {
-                    U.m[0][0] *= -1.0f; U.m[0][1] *= -1.0f; U.m[0][2] *= -1.0f;
-                    R = m3_mul(U, m3_transpose(V));
-                }
-                const M3 Sf = m3_mul(m3_mul(V, m3_diag(sv[0], sv[1], sv[2])), m3_transpose(V));
-                const M3 S = m3_diag(s0x, s0y, s0z);
-                const float r = q0w, qx = q0x, qy = q0y, qz = q0z;
-                M3 R0;
-                R0.m[0][0] = 1.f - 2.f * (qy * qy + qz * qz); R0.m[0][1] = 2.f * (qx * qy - r * qz); R0.m[0][2] = 2.f * (qx * qz + r * qy);
-                R0.m[1][0] = 2.f * (qx * qy + r * qz); R0.m[1][1] = 1.f - 2.f * (qx * qx + qz * qz); R0.m[1][2] = 2.f * (qy * qz - r * qx);
-                R0.m[2][0] = 2.f * (qx * qz - r * qy); R0.m[2][1] = 2.f * (qy * qz + r * qx); R0.m[2][2] = 1.f - 2.f * (qx * qx + qy * qy);
-                const M3 M = m3_mul(m3_mul(m3_mul(S, R0), Sf), m3_transpose(R));
-                Sigma = m3_mul(m3_transpose(M), M);
-                deformed = m3_finite(Sigma);
-            }
+        const float mx = x.x - x0.x, my = x.y - x0.y, mz = x.z - x0.z;
+        const float moved = sqrtf(mx * mx + my * my + mz * mz);
+        const int first = rowOff[i], count = rowOff[i + 1] - first;
+        M3 Sigma = m3_zero(), G, stretch, R;
+        bool ok = moved > deformEps && count >= 3 && ring_gradient(x0, x, first, count, rest, cur, nbr, G);
+        if (ok) {
+            clamped_stretch(G, stretch, R);
+            Sigma = deformed_covariance(s0[0], s0[1], s0[2], q0[0], q0[1], q0[2], q0[3], stretch, R);
+            ok = m3_finite(Sigma);
         }
-        const double tr = deformed ? ((double)Sigma.m[0][0] + Sigma.m[1][1] + Sigma.m[2][2]) : 0.0;
-        if (!deformed || !(tr > 0.0)) {
-            outScale[3 * li] = s0x; outScale[3 * li + 1] = s0y; outScale[3 * li + 2] = s0z;
-            outQuat[4 * li] = q0w; outQuat[4 * li + 1] = q0x; outQuat[4 * li + 2] = q0y; outQuat[4 * li + 3] = q0z;
-        } else {
+        const double tr = ok ? ((double)Sigma.m[0][0] + Sigma.m[1][1] + Sigma.m[2][2]) : 0.0;
+        if (ok && tr > 0.0) {
             counted = true;
-            const double inv = 3.0 / tr;
-            double a[3][3];
-            for (int rr = 0; rr < 3; ++rr)
-                for (int c = 0; c < 3; ++c) a[rr][c] = 0.5 * ((double)Sigma.m[c][rr] + (double)Sigma.m[rr][c]) * inv;
-            double ev[3], E[3][3];
-            sym_eig3_f64(a, ev, E);
-            const double det = E[0][0] * (E[1][1] * E[2][2] - E[1][2] * E[2][1]) -
-                               E[0][1] * (E[1][0] * E[2][2] - E[1][2] * E[2][0]) +
-                               E[0][2] * (E[1][0] * E[2][1] - E[1][1] * E[2][0]);
-            if (det < 0.0)
-                for (int rr = 0; rr < 3; ++rr) E[rr][2] = -E[rr][2];
-            const double sFloor = 1e-3 * (double)fmaxf(s0x, fmaxf(s0y, s0z));
-            for (int k = 0; k < 3; ++k) outScale[3 * li + k] = (float)fmax(sqrt(fmax(ev[k], 0.0) / inv), sFloor);
-            double w, qx, qy, qz;
-            const double t = E[0][0] + E[1][1] + E[2][2];
-            if (t > 0.0) {
-                const double sq = 2.0 * sqrt(t + 1.0);
-                w = 0.25 * sq; qx = (E[2][1] - E[1][2]) / sq; qy = (E[0][2] - E[2][0]) / sq; qz = (E[1][0] - E[0][1]) / sq;
-            } else if (E[0][0] > E[1][1] && E[0][0] > E[2][2]) {
-                const double sq = 2.0 * sqrt(1.0 + E[0][0] - E[1][1] - E[2][2]);
-                w = (E[2][1] - E[1][2]) / sq; qx = 0.25 * sq; qy = (E[0][1] + E[1][0]) / sq; qz = (E[0][2] + E[2][0]) / sq;
-            } else if (E[1][1] > E[2][2]) {
-                const double sq = 2.0 * sqrt(1.0 + E[1][1] - E[0][0] - E[2][2]);
-                w = (E[0][2] - E[2][0]) / sq; qx = (E[0][1] + E[1][0]) / sq; qy = 0.25 * sq; qz = (E[1][2] + E[2][1]) / sq;
-            } else {
-                const double sq = 2.0 * sqrt(1.0 + E[2][2] - E[0][0] - E[1][1]);
-                w = (E[1][0] - E[0][1]) / sq; qx = (E[0][2] + E[2][0]) / sq; qy = (E[1][2] + E[2][1]) / sq; qz = 0.25 * sq;
-            }
-            const double qn = sqrt(w * w + qx * qx + qy * qy + qz * qz);
-            outQuat[4 * li] = (float)(w / qn);
-            outQuat[4 * li + 1] = (float)(qx / qn);
-            outQuat[4 * li + 2] = (float)(qy / qn);
-            outQuat[4 * li + 3] = (float)(qz / qn);
+            covariance_shape(Sigma, tr, 1e-3 * (double)fmaxf(s0[0], fmaxf(s0[1], s0[2])), outScale + 3 * li,
+                             outQuat + 4 * li);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) outScale[3 * li + k] = s0[k];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) outQuat[4 * li + k] = q0[k];
         }
     }
     // deformed count of the block: wave ballots summed in LDS, one partial per

@@ -8,6 +8,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "jacobi3.h"
+
 namespace xpbd {
 
 struct M3 {
@@ -94,43 +96,6 @@ __host__ __device__ __forceinline__ float3 m3_mul_vec(const M3& M, float x, floa
                        M.m[0][2] * x + M.m[1][2] * y + M.m[2][2] * z);
 }
 
-// Symmetric 3x3 eigenvalues by cyclic-max Jacobi rotations, ascending
-// (the scheme of forward.cu:782-897: largest off-diagonal first, stop below
-// 1e-6, at most 50 rotations).
-__host__ __device__ __forceinline__ void m3_eig_sym_values(const M3& S, float ev[3])
-{
-    float A[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) A[r][c] = S.m[c][r];
-    for (int iter = 0; iter < 50; ++iter) {
-        int p = 0, q = 1;
-        float maxOff = fabsf(A[0][1]);
-        if (fabsf(A[0][2]) > maxOff) { p = 0; q = 2; maxOff = fabsf(A[0][2]); }
-        if (fabsf(A[1][2]) > maxOff) { p = 1; q = 2; maxOff = fabsf(A[1][2]); }
-        if (maxOff < 1e-6f) break;
-        const float app = A[p][p], aqq = A[q][q], apq = A[p][q];
-        const float phi = 0.5f * (aqq - app) / (apq + 1e-20f);
-        const float t = (phi >= 0.0f) ? (1.0f / (phi + sqrtf(1.0f + phi * phi))) : (1.0f / (phi - sqrtf(1.0f + phi * phi)));
-        const float c = 1.0f / sqrtf(1.0f + t * t);
-        const float s = t * c;
-        A[p][p] = app - t * apq;
-        A[q][q] = aqq + t * apq;
-        A[p][q] = A[q][p] = 0.0f;
-        const int r = 3 - p - q;
-        const float arp = A[r][p], arq = A[r][q];
-        A[r][p] = A[p][r] = c * arp - s * arq;
-        A[r][q] = A[q][r] = c * arq + s * arp;
-    }
-    ev[0] = A[0][0];
-    ev[1] = A[1][1];
-    ev[2] = A[2][2];
-    if (ev[0] > ev[1]) { const float t = ev[0]; ev[0] = ev[1]; ev[1] = t; }
-    if (ev[0] > ev[2]) { const float t = ev[0]; ev[0] = ev[2]; ev[2] = t; }
-    if (ev[1] > ev[2]) { const float t = ev[1]; ev[1] = ev[2]; ev[2] = t; }
-}
-
 __host__ __device__ __forceinline__ M3 m3_diag(float a, float b, float c)
 {
     M3 m = m3_zero();
@@ -169,57 +134,43 @@ __host__ __device__ __forceinline__ bool m3_finite(const M3& A)
     return ok;
 }
 
-// Same Jacobi scheme as m3_eig_sym_values, also returning the eigenvectors as
-// columns of V, sorted with the values (ascending).
-__host__ __device__ __forceinline__ void m3_eig_sym(const M3& S, float ev[3], M3& Vout)
+// Eigen-decomposition of a symmetric matrix in float: largest-pair Jacobi
+// (jacobi3.h) until every off-diagonal entry is below 1e-6, at most 50
+// rotations, values ascending (the float solver of forward.cu:782-869).
+// m3_eig_sym also returns the eigenvectors as the columns of V.
+struct SettledBelow {
+    float eps;
+    __host__ __device__ bool operator()(const float (&)[3][3], float largest) const { return largest < eps; }
+};
+
+template <bool kVectors>
+__host__ __device__ __forceinline__ void m3_eig_sym_f32(const M3& S, float ev[3], M3* V)
 {
-    float A[3][3], V[3][3];
+    float a[3][3], v[3][3];
+    set_identity(v);
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            A[r][c] = S.m[c][r];
-            V[r][c] = (r == c) ? 1.0f : 0.0f;
-        }
-    for (int iter = 0; iter < 50; ++iter) {
-        int p = 0, q = 1;
-        float maxOff = fabsf(A[0][1]);
-        if (fabsf(A[0][2]) > maxOff) { p = 0; q = 2; maxOff = fabsf(A[0][2]); }
-        if (fabsf(A[1][2]) > maxOff) { p = 1; q = 2; maxOff = fabsf(A[1][2]); }
-        if (maxOff < 1e-6f) break;
-        const float app = A[p][p], aqq = A[q][q], apq = A[p][q];
-        const float phi = 0.5f * (aqq - app) / (apq + 1e-20f);
-        const float t = (phi >= 0.0f) ? (1.0f / (phi + sqrtf(1.0f + phi * phi))) : (1.0f / (phi - sqrtf(1.0f + phi * phi)));
-        const float c = 1.0f / sqrtf(1.0f + t * t);
-        const float s = t * c;
-        A[p][p] = app - t * apq;
-        A[q][q] = aqq + t * apq;
-        A[p][q] = A[q][p] = 0.0f;
-        const int r3 = 3 - p - q;
-        const float arp = A[r3][p], arq = A[r3][q];
-        A[r3][p] = A[p][r3] = c * arp - s * arq;
-        A[r3][q] = A[q][r3] = c * arq + s * arp;
+        for (int c = 0; c < 3; ++c) a[r][c] = S.m[c][r];
+    jacobi_largest_pair<float, kVectors>(a, v, 1e-20f, SettledBelow{1e-6f});
+    float e[3] = {a[0][0], a[1][1], a[2][2]};
+    sort_ascending<float, kVectors>(e, v);
+    ev[0] = e[0]; ev[1] = e[1]; ev[2] = e[2];
+    if (kVectors)
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            const float vrp = V[r][p], vrq = V[r][q];
-            V[r][p] = c * vrp - s * vrq;
-            V[r][q] = s * vrp + c * vrq;
-        }
-    }
-    ev[0] = A[0][0];
-    ev[1] = A[1][1];
-    ev[2] = A[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = i + 1; j < 3; ++j)
-            if (ev[i] > ev[j]) {
-                const float te = ev[i]; ev[i] = ev[j]; ev[j] = te;
+        for (int c = 0; c < 3; ++c)
 #pragma unroll
-                for (int r = 0; r < 3; ++r) { const float tv = V[r][i]; V[r][i] = V[r][j]; V[r][j] = tv; }
-            }
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int r = 0; r < 3; ++r) Vout.m[c][r] = V[r][c];
+            for (int r = 0; r < 3; ++r) V->m[c][r] = v[r][c];
+}
+
+__host__ __device__ __forceinline__ void m3_eig_sym_values(const M3& S, float ev[3])
+{
+    m3_eig_sym_f32<false>(S, ev, nullptr);
+}
+
+__host__ __device__ __forceinline__ void m3_eig_sym(const M3& S, float ev[3], M3& V)
+{
+    m3_eig_sym_f32<true>(S, ev, &V);
 }
 
 // symmetric storage [00, 11, 22, 10, 20, 21] (forward.cu:7632-7652)
