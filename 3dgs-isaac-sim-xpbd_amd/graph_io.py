@@ -8,10 +8,11 @@
   dist, stiff, pos_rest, sibr_to_usd), nodes in the order of the cropped PLY
   (tools/import_graph.py:124-126).
 
-``import_graph`` converts the first into the second the way the reference's
-tools/import_graph.py does: every viewer node is matched to the PLY splat with
-the same float32 position bit pattern, each USD node takes its viewer node's
-neighbour list in the same order, and neighbour ids are renumbered.
+``import_graph`` converts the first into the second with the result of the
+reference's tools/import_graph.py: every viewer node is matched to the PLY
+splat with the same float32 position bit pattern, each USD node takes its
+viewer node's neighbour list in the same order, and neighbour ids are
+renumbered.
 ``csr_from_edges`` builds the symmetric CSR the solver derives from an edge
 list (ChainMail::loadGraph, forward.cu:2702-2779), for writing graphs.
 """
@@ -79,47 +80,61 @@ def csr_from_edges(n, edges, rest, stiff):
     return offset, count, dst[order].astype(np.int32), d2[order], s2[order]
 
 
-def _position_keys(p):
-    return np.ascontiguousarray(p, np.float32).view(np.dtype((np.void, 12))).ravel()
+def _bit_rows(p):
+    """(n, 3) uint32 bit patterns of float32 positions."""
+    return np.ascontiguousarray(p, np.float32).reshape(-1, 3).view(np.uint32)
+
+
+def match_positions(viewer_pos, splat_pos):
+    """For every viewer node, the splat with bit-identical float32 position.
+    Both sets are sorted lexicographically by bit pattern and must then be
+    equal row for row, which makes the match one-to-one.  Raises ValueError
+    when the sizes differ, a splat position repeats, or a viewer node has no
+    splat."""
+    a, b = _bit_rows(viewer_pos), _bit_rows(splat_pos)
+    if a.shape[0] != b.shape[0]:
+        raise ValueError(f"graph has {a.shape[0]} nodes, the PLY {b.shape[0]} splats")
+    oa, ob = np.lexsort(a.T[::-1]), np.lexsort(b.T[::-1])
+    sa, sb = a[oa], b[ob]
+    repeated = int((sb[1:] == sb[:-1]).all(1).sum())
+    if repeated:
+        raise ValueError(f"{repeated} splat positions occur more than once; the match is ambiguous")
+    differ = (sa != sb).any(1)
+    if differ.any():
+        # viewer rows absent from the splats (the sorted runs no longer line up)
+        keys = lambda r: r.view(np.dtype((np.void, 12))).ravel()
+        missing = int((~np.isin(keys(sa), keys(sb))).sum())
+        raise ValueError(f"{missing} graph nodes have no splat with the same position bits")
+    viewer_to_splat = np.empty(a.shape[0], np.int64)
+    viewer_to_splat[oa] = ob
+    return viewer_to_splat
 
 
 def import_graph(graph, usd_pos):
-    """Viewer-order graph (read_apggrph1's dict) -> NPZ fields in the order
-    of the PLY positions ``usd_pos`` (N,3).  Every viewer node must match
-    exactly one splat by its float32 bit pattern."""
-    pos_s = np.asarray(graph["pos"], np.float32)
+    """Viewer-order graph (read_apggrph1's dict) -> NPZ fields in the order of
+    the PLY positions ``usd_pos`` (N, 3) (what tools/import_graph.py:73-126
+    writes).  Splat u takes the neighbour list of its viewer node, entries in
+    the same order, neighbours renumbered to splats (-1 where the viewer's id
+    is out of range); dist and stiff follow their entries."""
     usd = np.asarray(usd_pos, np.float32).reshape(-1, 3)
-    n = pos_s.shape[0]
-    if usd.shape[0] != n:
-        raise ValueError(f"graph has {n} nodes, the PLY {usd.shape[0]} splats")
-    ku, ks = _position_keys(usd), _position_keys(pos_s)
-    uniq, first, cnt = np.unique(ku, return_index=True, return_counts=True)
-    at = np.clip(np.searchsorted(uniq, ks), 0, max(len(uniq) - 1, 0))
-    if n and not (uniq[at] == ks).all():
-        raise ValueError(f"{int((uniq[at] != ks).sum())} graph nodes have no splat with the same position bits")
-    if (cnt > 1).any():
-        raise ValueError(f"{int((cnt > 1).sum())} splat positions occur more than once; the match is ambiguous")
-    sibr_to_usd = first[at].astype(np.int64)
-    usd_to_sibr = np.empty(n, np.int64)
-    usd_to_sibr[sibr_to_usd] = np.arange(n)
-    off_s = np.asarray(graph["offset"], np.int64)
-    cnt_s = np.asarray(graph["count"], np.int64)
-    count = cnt_s[usd_to_sibr]
-    offset = np.zeros(n, np.int64)
-    if n:
-        offset[1:] = np.cumsum(count)[:-1]
-    total = int(count.sum())
-    # entry t of USD node u comes from entry t of viewer node usd_to_sibr[u]
-    src_start = np.repeat(off_s[usd_to_sibr], count)
-    within = np.arange(total) - np.repeat(offset, count)
-    gather = src_start + within
-    idx_s = np.asarray(graph["idx"], np.int64)[gather]
-    valid = (idx_s >= 0) & (idx_s < n)
-    idx = np.where(valid, sibr_to_usd[np.clip(idx_s, 0, max(n - 1, 0))], -1).astype(np.int32)
+    viewer_to_splat = match_positions(graph["pos"], usd)
+    n = usd.shape[0]
+    splat_to_viewer = np.empty(n, np.int64)
+    splat_to_viewer[viewer_to_splat] = np.arange(n)
+    count = np.asarray(graph["count"], np.int64)[splat_to_viewer]
+    offset = np.concatenate([[0], np.cumsum(count)[:-1]]).astype(np.int64) if n else np.zeros(0, np.int64)
+    # entry e of the new CSR (row u, slot e - offset[u]) reads viewer entry
+    # viewer_offset[row] + slot
+    shift = np.asarray(graph["offset"], np.int64)[splat_to_viewer] - offset
+    src = np.repeat(shift, count) + np.arange(int(count.sum()), dtype=np.int64)
+    old = np.asarray(graph["idx"], np.int64)[src]
+    inside = (old >= 0) & (old < n)
+    idx = np.full(old.shape, -1, np.int32)
+    idx[inside] = viewer_to_splat[old[inside]]
     return {
         "offset": offset.astype(np.int32), "count": count.astype(np.int32), "idx": idx,
-        "dist": np.asarray(graph["dist"], np.float32)[gather], "stiff": np.asarray(graph["stiff"], np.float32)[gather],
-        "pos_rest": usd, "sibr_to_usd": sibr_to_usd.astype(np.int32),
+        "dist": np.asarray(graph["dist"], np.float32)[src], "stiff": np.asarray(graph["stiff"], np.float32)[src],
+        "pos_rest": usd, "sibr_to_usd": viewer_to_splat.astype(np.int32),
     }
 
 

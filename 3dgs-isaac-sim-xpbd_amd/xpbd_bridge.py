@@ -11,6 +11,7 @@ Extensions beyond the reference class are prefixed ``ext_``.
 """
 import ctypes
 import os
+from pathlib import Path
 
 import numpy as np
 
@@ -20,130 +21,146 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_LIB = os.path.join(_HERE, "libxpbd_isaac.so")
 
 
+def _columns(table, *names):
+    """float32 (N, len(names)) array of the named fields of a structured array."""
+    return np.column_stack([np.asarray(table[f], np.float32) for f in names])
+
+
 def load_inputs(d, name):
-    """Inputs for xpbd_create from <d>/<name>_crop.ply and <d>/<name>_graph.npz
-    (src/xpbd_bridge.py:17-41): activated scales, normalised quaternions,
-    activated opacity, and the CSR graph reduced to undirected i<j edges."""
-    _, data = read_ply(os.path.join(d, f"{name}_crop.ply"))
-    g = np.load(os.path.join(d, f"{name}_graph.npz"), allow_pickle=False)
-    pos = np.stack([data["x"], data["y"], data["z"]], axis=1).astype(np.float32)
-    if not np.array_equal(pos, g["pos_rest"]):
+    """xpbd_create inputs from <d>/<name>_crop.ply and <d>/<name>_graph.npz,
+    both in USD order (the loader of src/xpbd_bridge.py:17-41).
+
+    PLY fields: x/y/z, scale_0..2 (log scales), rot_0..3 (w, x, y, z, not
+    normalised) and opacity (logit).  NPZ: the bidirectional CSR (count, idx,
+    dist, stiff) and the positions it was built on (pos_rest), which must
+    equal the PLY's.  Scales are exponentiated, quaternions normalised,
+    opacity passed through the logistic function (in double), and each
+    undirected edge kept once, as its (i, j) entry with i < j, with that
+    entry's rest length and stiffness."""
+    folder = Path(d)
+    _, verts = read_ply(folder / f"{name}_crop.ply")
+    with np.load(folder / f"{name}_graph.npz", allow_pickle=False) as graph:
+        count, nbr = graph["count"], graph["idx"]
+        dist, stiff, anchor = graph["dist"], graph["stiff"], graph["pos_rest"]
+    pos = _columns(verts, "x", "y", "z")
+    if not np.array_equal(pos, anchor):
         raise ValueError("graph pos_rest and the cropped PLY positions differ; re-run the graph import")
-    scales = np.exp(np.stack([data["scale_0"], data["scale_1"], data["scale_2"]], axis=1)).astype(np.float32)
-    rots = np.stack([data["rot_0"], data["rot_1"], data["rot_2"], data["rot_3"]], axis=1).astype(np.float32)
-    rots /= np.linalg.norm(rots, axis=1, keepdims=True)
-    opacity = (1.0 / (1.0 + np.exp(-data["opacity"].astype(np.float64)))).astype(np.float32)
-    owner = np.repeat(np.arange(len(pos), dtype=np.int32), g["count"])
-    idx = g["idx"]
-    keep = idx > owner
-    edges = np.stack([owner[keep], idx[keep]], axis=1).astype(np.int32)
-    return {
-        "pos": np.ascontiguousarray(pos),
-        "scales": np.ascontiguousarray(scales),
-        "rots": np.ascontiguousarray(rots),
-        "opacity": np.ascontiguousarray(opacity),
-        "edges": np.ascontiguousarray(edges),
-        "rest": np.ascontiguousarray(g["dist"][keep].astype(np.float32)),
-        "stiff": np.ascontiguousarray(g["stiff"][keep].astype(np.float32)),
+    quat = _columns(verts, "rot_0", "rot_1", "rot_2", "rot_3")
+    quat /= np.linalg.norm(quat, axis=1, keepdims=True)
+    logit = np.asarray(verts["opacity"], np.float64)
+    row_of = np.repeat(np.arange(pos.shape[0], dtype=np.int32), count)
+    upper = np.flatnonzero(nbr > row_of)
+    inp = {
+        "pos": pos,
+        "scales": np.exp(_columns(verts, "scale_0", "scale_1", "scale_2")).astype(np.float32),
+        "rots": quat,
+        "opacity": (1.0 / (1.0 + np.exp(-logit))).astype(np.float32),
+        "edges": np.column_stack([row_of[upper], nbr[upper]]).astype(np.int32),
+        "rest": dist[upper].astype(np.float32),
+        "stiff": stiff[upper].astype(np.float32),
     }
+    return {k: np.ascontiguousarray(v) for k, v in inp.items()}
+
+
+_F32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_I32 = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_INT, _FLT, _PTR = ctypes.c_int, ctypes.c_float, ctypes.c_void_p
+_FOUT = ctypes.POINTER(ctypes.c_float)
+
+# symbol: (result type, argument types), as declared in include/xpbd_capi.h.
+# The first 29 are the reference's ABI (xpbd_capi.cu:66-310); the others are
+# this library's extensions, present in both the HIP library and the oracle.
+ABI = {
+    "xpbd_last_error": (ctypes.c_char_p, []),
+    "xpbd_create": (_INT, [_INT, _F32, _F32, _F32, _F32, _INT, _I32, _F32, _F32]),
+    "xpbd_destroy": (None, []),
+    "xpbd_count": (_INT, []),
+    "xpbd_set_solver": (None, [_INT] + [_FLT] * 10),
+    "xpbd_set_constraints": (None, [_INT] * 4),
+    "xpbd_set_volume": (None, [_FLT, _INT, _INT, _INT]),
+    "xpbd_set_ground": (None, [_INT] + [_FLT] * 9),
+    "xpbd_set_object_shape": (None, [_FLT]),
+    "xpbd_set_shape_robust": (None, [_INT]),
+    "xpbd_set_object_shape_gpu": (None, [_INT]),
+    "xpbd_set_self_collision": (None, [_INT, _FLT, _FLT, _INT]),
+    "xpbd_self_collision_stats": (None, [_I32]),
+    "xpbd_set_colliders": (None, [_INT, _I32, _F32, _F32, _FLT, _FLT]),
+    "xpbd_collider_stats": (_INT, [_I32, _F32, _F32, _INT]),
+    "xpbd_add_rigid_velocity": (_INT, [_F32, _F32, _F32]),
+    "xpbd_set_attached": (_INT, [_INT, _I32, _F32, _FLT]),
+    "xpbd_set_contact_shape": (_INT, [_INT, _FLT, _FLT]),
+    "xpbd_step": (_INT, []),
+    "xpbd_positions_device": (ctypes.c_ulonglong, []),
+    "xpbd_positions_host": (_INT, [_F32, _INT]),
+    "xpbd_reset": (_INT, []),
+    "xpbd_launch": (_INT, [_F32, _F32, _F32]),
+    "xpbd_volume_stats": (None, [_INT, _FOUT, _FOUT]),
+    "xpbd_compute_shapes": (_INT, [_FLT, _F32, _F32, _INT]),
+    "xpbd_press_start": (_INT, [_INT, _FLT, _FLT, _INT]),
+    "xpbd_press_stop": (None, []),
+    "xpbd_press_autolog": (None, [_INT]),
+    "xpbd_press_state": (_INT, [_FOUT] * 4),
+    "xpbd_ext_get_state": (_INT, [_PTR, _PTR, _INT]),
+    "xpbd_ext_set_state": (_INT, [_PTR, _PTR, _INT]),
+    "xpbd_ext_get_inv_mass": (_INT, [_F32, _INT]),
+    "xpbd_ext_backend": (ctypes.c_char_p, []),
+    "xpbd_ext_profile": (_INT, [_INT, ctypes.c_char_p, _INT]),
+    "xpbd_ext_set_volume_clusters": (_INT, [_PTR, _PTR, _INT]),
+    "xpbd_ext_volume_topology": (_INT, [_PTR] * 5 + [_INT]),
+    "xpbd_ext_set_node_order": (_INT, [_INT]),
+    "xpbd_ext_self_collision_grid_info": (_INT, [_PTR]),
+    "xpbd_ext_set_shape_clusters": (_INT, [_PTR, _INT]),
+    "xpbd_ext_set_gauss_seidel": (_INT, [_INT, _INT]),
+    "xpbd_ext_gs_info": (_INT, [_PTR]),
+    "xpbd_ext_distance_gs_edges": (_INT, [_PTR] * 4),
+    "xpbd_ext_volume_gs_order": (_INT, [_PTR] * 2),
+    "xpbd_ext_set_self_collision_grid": (_INT, [_FLT, _INT]),
+    "xpbd_ext_self_collision_lists": (_INT, [_PTR] * 4 + [_INT]),
+}
+
+# extensions of the HIP library only (looked up when used)
+ABI_OPTIONAL = {
+    "xpbd_ext_angle_pairs": (_INT, [_PTR, _PTR, _INT]),
+    "xpbd_ext_set_self_collision_fine": (_INT, [_INT]),
+    "xpbd_ext_set_group_lanes": (_INT, [_INT]),
+    "xpbd_ext_autotune": (_INT, [_INT, _INT, ctypes.c_char_p, _INT]),
+    "xpbd_ext_volume_members": (_INT, [_PTR, _INT]),
+}
 
 
 def bind(lib):
-    """Declare argtypes / restypes of every exported symbol on a CDLL."""
-    L = lib
-    f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
-    i32 = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
-    c_int, c_float = ctypes.c_int, ctypes.c_float
-    L.xpbd_last_error.restype = ctypes.c_char_p
-    L.xpbd_create.argtypes = [c_int, f32, f32, f32, f32, c_int, i32, f32, f32]
-    L.xpbd_create.restype = c_int
-    L.xpbd_destroy.restype = None
-    L.xpbd_count.restype = c_int
-    L.xpbd_set_solver.argtypes = [c_int] + [c_float] * 10
-    L.xpbd_set_solver.restype = None
-    L.xpbd_set_constraints.argtypes = [c_int] * 4
-    L.xpbd_set_constraints.restype = None
-    L.xpbd_set_volume.argtypes = [c_float, c_int, c_int, c_int]
-    L.xpbd_set_volume.restype = None
-    L.xpbd_set_ground.argtypes = [c_int] + [c_float] * 9
-    L.xpbd_set_ground.restype = None
-    L.xpbd_set_object_shape.argtypes = [c_float]
-    L.xpbd_set_object_shape.restype = None
-    L.xpbd_set_shape_robust.argtypes = [c_int]
-    L.xpbd_set_shape_robust.restype = None
-    L.xpbd_set_object_shape_gpu.argtypes = [c_int]
-    L.xpbd_set_object_shape_gpu.restype = None
-    L.xpbd_set_self_collision.argtypes = [c_int, c_float, c_float, c_int]
-    L.xpbd_set_self_collision.restype = None
-    L.xpbd_self_collision_stats.argtypes = [i32]
-    L.xpbd_self_collision_stats.restype = None
-    L.xpbd_set_colliders.argtypes = [c_int, i32, f32, f32, c_float, c_float]
-    L.xpbd_set_colliders.restype = None
-    L.xpbd_collider_stats.argtypes = [i32, f32, f32, c_int]
-    L.xpbd_collider_stats.restype = c_int
-    L.xpbd_add_rigid_velocity.argtypes = [f32, f32, f32]
-    L.xpbd_add_rigid_velocity.restype = c_int
-    L.xpbd_set_attached.argtypes = [c_int, i32, f32, c_float]
-    L.xpbd_set_attached.restype = c_int
-    L.xpbd_set_contact_shape.argtypes = [c_int, c_float, c_float]
-    L.xpbd_set_contact_shape.restype = c_int
-    L.xpbd_step.restype = c_int
-    L.xpbd_positions_device.restype = ctypes.c_ulonglong
-    L.xpbd_positions_host.argtypes = [f32, c_int]
-    L.xpbd_positions_host.restype = c_int
-    L.xpbd_reset.restype = c_int
-    L.xpbd_launch.argtypes = [f32, f32, f32]
-    L.xpbd_launch.restype = c_int
-    L.xpbd_volume_stats.argtypes = [c_int, ctypes.POINTER(c_float), ctypes.POINTER(c_float)]
-    L.xpbd_volume_stats.restype = None
-    L.xpbd_compute_shapes.argtypes = [c_float, f32, f32, c_int]
-    L.xpbd_compute_shapes.restype = c_int
-    L.xpbd_press_start.argtypes = [c_int, c_float, c_float, c_int]
-    L.xpbd_press_start.restype = c_int
-    L.xpbd_press_stop.restype = None
-    L.xpbd_press_autolog.argtypes = [c_int]
-    L.xpbd_press_autolog.restype = None
-    L.xpbd_press_state.argtypes = [ctypes.POINTER(c_float)] * 4
-    L.xpbd_press_state.restype = c_int
-    L.xpbd_ext_get_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p, c_int]
-    L.xpbd_ext_get_state.restype = c_int
-    L.xpbd_ext_set_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p, c_int]
-    L.xpbd_ext_set_state.restype = c_int
-    L.xpbd_ext_get_inv_mass.argtypes = [f32, c_int]
-    L.xpbd_ext_get_inv_mass.restype = c_int
-    L.xpbd_ext_backend.restype = ctypes.c_char_p
-    L.xpbd_ext_profile.argtypes = [c_int, ctypes.c_char_p, c_int]
-    L.xpbd_ext_profile.restype = c_int
-    L.xpbd_ext_set_volume_clusters.argtypes = [ctypes.c_void_p, ctypes.c_void_p, c_int]
-    L.xpbd_ext_set_volume_clusters.restype = c_int
-    L.xpbd_ext_volume_topology.argtypes = [ctypes.c_void_p] * 5 + [c_int]
-    L.xpbd_ext_volume_topology.restype = c_int
-    L.xpbd_ext_set_node_order.argtypes = [c_int]
-    L.xpbd_ext_set_node_order.restype = c_int
-    L.xpbd_ext_self_collision_grid_info.argtypes = [ctypes.c_void_p]
-    L.xpbd_ext_self_collision_grid_info.restype = c_int
-    L.xpbd_ext_set_shape_clusters.argtypes = [ctypes.c_void_p, c_int]
-    L.xpbd_ext_set_shape_clusters.restype = c_int
-    L.xpbd_ext_set_gauss_seidel.argtypes = [c_int, c_int]
-    L.xpbd_ext_set_gauss_seidel.restype = c_int
-    L.xpbd_ext_gs_info.argtypes = [ctypes.c_void_p]
-    L.xpbd_ext_gs_info.restype = c_int
-    L.xpbd_ext_distance_gs_edges.argtypes = [ctypes.c_void_p] * 4
-    L.xpbd_ext_distance_gs_edges.restype = c_int
-    L.xpbd_ext_volume_gs_order.argtypes = [ctypes.c_void_p] * 2
-    L.xpbd_ext_volume_gs_order.restype = c_int
-    L.xpbd_ext_set_self_collision_grid.argtypes = [c_float, c_int]
-    L.xpbd_ext_set_self_collision_grid.restype = c_int
-    L.xpbd_ext_self_collision_lists.argtypes = [ctypes.c_void_p] * 4 + [c_int]
-    L.xpbd_ext_self_collision_lists.restype = c_int
-    return L
+    """Declare result and argument types of every symbol in ABI (and of
+    those in ABI_OPTIONAL the library exports) on a CDLL; returns it."""
+    for table, required in ((ABI, True), (ABI_OPTIONAL, False)):
+        for name, (res, args) in table.items():
+            if not required and not hasattr(lib, name):
+                continue
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+    return lib
 
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
+def _f32(v, shape=None):
+    a = np.ascontiguousarray(v, dtype=np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+def _or_dummy(a, rows, width):
+    """`a` itself, or one zero row of `width` when there are no rows: the
+    library takes a valid pointer even for an empty list."""
+    return a if rows else np.zeros(width, a.dtype)
+
+
 class XPBD:
+    """The solver singleton behind the C ABI, with the reference bridge's
+    methods (src/xpbd_bridge.py:45-271): same names, defaults, argument
+    meaning; failing calls raise RuntimeError("<symbol>: <xpbd_last_error>")."""
+
     SPHERE, BOX, CAPSULE = 0, 1, 2
 
     def __init__(self, dll_path=None):
@@ -166,17 +183,26 @@ class XPBD:
         if rc != 0:
             raise RuntimeError(f"{what}: {self.lib.xpbd_last_error().decode(errors='replace')}")
 
-    def create(self, inp):
-        e = inp["edges"].shape[0]
-        edges = inp["edges"].ravel() if e else np.zeros(2, np.int32)
-        rest = inp["rest"] if e else np.zeros(1, np.float32)
-        stiff = inp["stiff"] if e else np.zeros(1, np.float32)
-        self._check(self.lib.xpbd_create(inp["pos"].shape[0], inp["pos"].ravel(), inp["scales"].ravel(),
-                                         inp["rots"].ravel(), inp["opacity"], e, edges, rest, stiff), "xpbd_create")
-        self.n = inp["pos"].shape[0]
-        ed = np.asarray(inp["edges"]).reshape(-1, 2)
-        self._edges_valid = int(np.count_nonzero((ed[:, 0] != ed[:, 1]) & (ed >= 0).all(1) & (ed < self.n).all(1)))
+    def _call(self, symbol, *args):
+        self._check(getattr(self.lib, symbol)(*args), symbol)
 
+    # ---- instance ---------------------------------------------------------
+    def create(self, inp):
+        pos = inp["pos"]
+        edges = np.asarray(inp["edges"]).reshape(-1, 2)
+        m = edges.shape[0]
+        self._call("xpbd_create", pos.shape[0], pos.ravel(), inp["scales"].ravel(), inp["rots"].ravel(),
+                   inp["opacity"], m, _or_dummy(inp["edges"].ravel(), m, 2), _or_dummy(inp["rest"], m, 1),
+                   _or_dummy(inp["stiff"], m, 1))
+        self.n = pos.shape[0]
+        in_range = (edges >= 0).all(1) & (edges < self.n).all(1)
+        self._edges_valid = int(np.count_nonzero(in_range & (edges[:, 0] != edges[:, 1])))
+
+    def destroy(self):
+        self.lib.xpbd_destroy()
+        self.n = 0
+
+    # ---- settings ---------------------------------------------------------
     def set_solver(self, iters=20, dt=1 / 60, under_relax=0.6, vel_damping=0.01, stiffness_scale=1.0,
                    inv_mass_scale=1.0, dist_compliance=0.0, shape_compliance=0.0, shape_blend=0.1,
                    angle_compliance=1e-2, angle_blend=0.2):
@@ -184,15 +210,16 @@ class XPBD:
                                  dist_compliance, shape_compliance, shape_blend, angle_compliance, angle_blend)
 
     def set_constraints(self, distance=True, shape=True, angle=False, volume=True):
-        self.lib.xpbd_set_constraints(int(distance), int(shape), int(angle), int(volume))
+        self.lib.xpbd_set_constraints(*(int(bool(f)) for f in (distance, shape, angle, volume)))
 
     def set_volume(self, compliance=1e-6, ring_k=3, max_members=2048, leader_min_hop=2):
         self.lib.xpbd_set_volume(compliance, ring_k, max_members, leader_min_hop)
 
     def set_ground(self, enabled, up, height, friction=0.5, restitution=0.0, contact_radius=0.0,
                    contact_slop=0.0, gravity=0.0):
-        self.lib.xpbd_set_ground(int(enabled), float(up[0]), float(up[1]), float(up[2]), height, friction,
-                                 restitution, contact_radius, contact_slop, gravity)
+        ux, uy, uz = (float(c) for c in up[:3])
+        self.lib.xpbd_set_ground(int(enabled), ux, uy, uz, height, friction, restitution, contact_radius,
+                                 contact_slop, gravity)
 
     def set_object_shape(self, stiffness):
         self.lib.xpbd_set_object_shape(stiffness)
@@ -200,72 +227,79 @@ class XPBD:
     def set_object_shape_gpu(self, enabled):
         self.lib.xpbd_set_object_shape_gpu(int(enabled))
 
-    def set_colliders(self, types, poses, dims, margin=0.0, friction=0.0):
-        """Kinematic colliders: types (K,) 0 sphere / 1 box / 2 capsule; poses (K,12) R row-major + t;
-        dims (K,3). An empty list switches colliders off."""
-        types = np.ascontiguousarray(np.asarray(types, dtype=np.int32).reshape(-1))
-        k = len(types)
-        poses = np.ascontiguousarray(np.asarray(poses, dtype=np.float32).reshape(k, 12) if k else np.zeros((1, 12), np.float32))
-        dims = np.ascontiguousarray(np.asarray(dims, dtype=np.float32).reshape(k, 3) if k else np.zeros((1, 3), np.float32))
-        if not k:
-            types = np.zeros(1, dtype=np.int32)
-        self.lib.xpbd_set_colliders(k, types, poses.ravel(), dims.ravel(), float(margin), float(friction))
-
-    def collider_stats(self, count):
-        hits = np.zeros(max(count, 1), dtype=np.int32)
-        push = np.zeros(max(count, 1) * 3, dtype=np.float32)
-        cen = np.zeros(max(count, 1) * 3, dtype=np.float32)
-        n = self.lib.xpbd_collider_stats(hits, push, cen, count)
-        return hits[:n], push[:3 * n].reshape(n, 3), cen[:3 * n].reshape(n, 3)
-
-    def add_rigid_velocity(self, dv, dw=(0.0, 0.0, 0.0), center=(0.0, 0.0, 0.0)):
-        a = [np.ascontiguousarray(v, dtype=np.float32) for v in (dv, dw, center)]
-        self._check(self.lib.xpbd_add_rigid_velocity(*a), "xpbd_add_rigid_velocity")
-
-    def set_attached(self, idx, pos, weight=1.0):
-        idx = np.ascontiguousarray(idx, dtype=np.int32).ravel()
-        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
-        if len(idx) == 0:
-            idx, pos = np.zeros(1, np.int32), np.zeros((1, 3), np.float32)
-            self._check(self.lib.xpbd_set_attached(0, idx, pos.ravel(), float(weight)), "xpbd_set_attached")
-            return
-        self._check(self.lib.xpbd_set_attached(len(idx), idx, pos.ravel(), float(weight)), "xpbd_set_attached")
-
-    def clear_attached(self):
-        self._check(self.lib.xpbd_set_attached(0, np.zeros(1, np.int32), np.zeros(3, np.float32), 1.0),
-                    "xpbd_set_attached")
-
-    def set_contact_shape(self, enabled, tau=0.2, radius_cap=0.0):
-        self._check(self.lib.xpbd_set_contact_shape(int(enabled), float(tau), float(radius_cap)),
-                    "xpbd_set_contact_shape")
-        return True
+    def set_shape_robust(self, enabled):
+        self.lib.xpbd_set_shape_robust(int(enabled))
 
     def set_self_collision(self, enabled, radius_scale=1.5, exclude_scale=2.0, within_body=False):
         self.lib.xpbd_set_self_collision(int(enabled), radius_scale, exclude_scale, int(within_body))
 
-    def self_collision_stats(self):
-        out = np.zeros(3, dtype=np.int32)
-        self.lib.xpbd_self_collision_stats(out)
-        return int(out[0]), int(out[1]), int(out[2])
+    def set_contact_shape(self, enabled, tau=0.2, radius_cap=0.0):
+        self._call("xpbd_set_contact_shape", int(enabled), float(tau), float(radius_cap))
+        return True
 
-    def set_shape_robust(self, enabled):
-        self.lib.xpbd_set_shape_robust(int(enabled))
+    # ---- colliders, grippers, impulses ----------------------------------
+    def set_colliders(self, types, poses, dims, margin=0.0, friction=0.0):
+        """Kinematic colliders, one-way (collider -> Gaussians), in input
+        coordinates; may be called again before any step.  types (K,): 0 sphere,
+        1 box, 2 capsule; poses (K, 12): local-to-input rotation (row-major 9)
+        then centre; dims (K, 3): box half extents, sphere (radius, 0, 0),
+        capsule (radius, half length along local z, 0).  An empty list switches
+        colliders off."""
+        kinds = np.ascontiguousarray(np.asarray(types, dtype=np.int32).ravel())
+        k = kinds.size
+        frames = _f32(poses, (k, 12)) if k else np.zeros((1, 12), np.float32)
+        sizes = _f32(dims, (k, 3)) if k else np.zeros((1, 3), np.float32)
+        self.lib.xpbd_set_colliders(k, _or_dummy(kinds, k, 1), frames.ravel(), sizes.ravel(), float(margin),
+                                    float(friction))
 
+    def collider_stats(self, count):
+        """Per collider of the last step: (Gaussians touched (K,), summed push
+        (K, 3), contact centre (K, 3)), input units."""
+        cap = max(int(count), 1)
+        hits = np.zeros(cap, np.int32)
+        push = np.zeros((cap, 3), np.float32)
+        centre = np.zeros((cap, 3), np.float32)
+        k = self.lib.xpbd_collider_stats(hits, push.ravel(), centre.ravel(), count)
+        return hits[:k], push[:k], centre[:k]
+
+    def add_rigid_velocity(self, dv, dw=(0.0, 0.0, 0.0), center=(0.0, 0.0, 0.0)):
+        """Adds dv + dw x (x - center) to every Gaussian's velocity (between steps)."""
+        self._call("xpbd_add_rigid_velocity", _f32(dv), _f32(dw), _f32(center))
+
+    def set_attached(self, idx, pos, weight=1.0):
+        """Held Gaussians (gripper fingers): idx pinned at pos (N, 3, input
+        coordinates), called every step with the new positions; weight of one
+        held point in the object-shape rigid fit."""
+        held = np.ascontiguousarray(idx, dtype=np.int32).ravel()
+        where = _f32(pos, (-1, 3))
+        m = held.size
+        self._call("xpbd_set_attached", m, _or_dummy(held, m, 1), _or_dummy(where.ravel(), m, 3), float(weight))
+
+    def clear_attached(self):
+        self.set_attached(np.zeros(0, np.int32), np.zeros((0, 3), np.float32))
+
+    # ---- stepping and results -------------------------------------------
     def step(self):
-        self._check(self.lib.xpbd_step(), "xpbd_step")
+        self._call("xpbd_step")
 
     def positions(self):
         out = np.empty((self.n, 3), dtype=np.float32)
         self._check(self.lib.xpbd_positions_host(out.ravel(), self.n), "xpbd_positions_host")
         return out
 
+    def positions_device_ptr(self):
+        return int(self.lib.xpbd_positions_device())
+
     def shapes(self, deform_eps=1e-2):
-        sc = np.empty((self.n, 3), dtype=np.float32)
-        q = np.empty((self.n, 4), dtype=np.float32)
-        k = self.lib.xpbd_compute_shapes(float(deform_eps), sc.ravel(), q.ravel(), self.n)
+        """Deformed Gaussian shapes: (activated scales (N, 3), quaternions
+        (N, 4) w x y z, number deformed); Gaussians that moved at most
+        deform_eps keep their own."""
+        scales = np.empty((self.n, 3), dtype=np.float32)
+        quats = np.empty((self.n, 4), dtype=np.float32)
+        k = self.lib.xpbd_compute_shapes(float(deform_eps), scales.ravel(), quats.ravel(), self.n)
         if k < 0:
-            self._check(-1, "xpbd_compute_shapes")
-        return sc, q, k
+            self._check(k, "xpbd_compute_shapes")
+        return scales, quats, k
 
     def shapes_device(self, deform_eps=1e-2):
         """compute_shapes with the outputs left in device memory (null host
@@ -274,27 +308,31 @@ class XPBD:
             ("xpbd_compute_shapes", self.lib))
         k = fn(float(deform_eps), None, None, self.n)
         if k < 0:
-            self._check(-1, "xpbd_compute_shapes")
+            self._check(k, "xpbd_compute_shapes")
         return k
 
-    def positions_device_ptr(self):
-        return int(self.lib.xpbd_positions_device())
-
-    def reset(self):
-        self._check(self.lib.xpbd_reset(), "xpbd_reset")
-
-    def launch(self, lin=(0, 0, 0), ang=(0, 0, 0), tilt=(0, 0, 0)):
-        a = [np.ascontiguousarray(v, dtype=np.float32) for v in (lin, ang, tilt)]
-        self._check(self.lib.xpbd_launch(*a), "xpbd_launch")
+    def self_collision_stats(self):
+        """(candidates, active Gaussians, over-capacity replacements) of the last step."""
+        out = np.zeros(3, dtype=np.int32)
+        self.lib.xpbd_self_collision_stats(out)
+        return tuple(int(v) for v in out)
 
     def volume_stats(self, enable=True):
-        m, s = ctypes.c_float(0.0), ctypes.c_float(0.0)
-        self.lib.xpbd_volume_stats(int(enable), ctypes.byref(m), ctypes.byref(s))
-        return m.value, s.value
+        mean, spread = ctypes.c_float(), ctypes.c_float()
+        self.lib.xpbd_volume_stats(int(enable), ctypes.byref(mean), ctypes.byref(spread))
+        return mean.value, spread.value
+
+    # ---- scripted motion ------------------------------------------------
+    def reset(self):
+        self._call("xpbd_reset")
+
+    def launch(self, lin=(0, 0, 0), ang=(0, 0, 0), tilt=(0, 0, 0)):
+        self._call("xpbd_launch", _f32(lin), _f32(ang), _f32(tilt))
 
     def press_start(self, axis, ramp_per_sec, max_disp_pct, from_lo):
-        self._check(self.lib.xpbd_press_start(int(axis), float(ramp_per_sec), float(max_disp_pct), int(from_lo)),
-                    "xpbd_press_start")
+        """Plate press along axis 0/1/2; the low plate moves when from_lo;
+        ramp in input units per second."""
+        self._call("xpbd_press_start", int(axis), float(ramp_per_sec), float(max_disp_pct), int(from_lo))
 
     def press_autolog(self, enabled):
         self.lib.xpbd_press_autolog(int(enabled))
@@ -303,13 +341,10 @@ class XPBD:
         self.lib.xpbd_press_stop()
 
     def press_state(self):
-        v = [ctypes.c_float(0.0) for _ in range(4)]
-        active = self.lib.xpbd_press_state(*[ctypes.byref(x) for x in v])
-        return bool(active), v[0].value, v[1].value, v[2].value, v[3].value
-
-    def destroy(self):
-        self.lib.xpbd_destroy()
-        self.n = 0
+        """(active, lo plate, hi plate, current displacement, maximum displacement)."""
+        vals = (ctypes.c_float * 4)()
+        active = self.lib.xpbd_press_state(*(ctypes.cast(ctypes.byref(vals, 4 * i), _FOUT) for i in range(4)))
+        return (bool(active),) + tuple(float(v) for v in vals)
 
     # ---- extensions -----------------------------------------------------
     def ext_get_state(self):
