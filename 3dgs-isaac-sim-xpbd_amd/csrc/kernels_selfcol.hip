@@ -408,9 +408,30 @@ __global__ void __launch_bounds__(256) sc_fine_keys_kernel(int nPad, SelfColArgs
             key = (unsigned int)B.scFineBase +
                   sc_fine_key(sc_cell(p.x, g), sc_cell(p.y, g), sc_cell(p.z, g), (unsigned int)B.scFineMask);
             // bit 1 of the active flag: two of the 27 coarse stencil cells
-            // around x share a bucket (the group fine scan then looks up the
-            // first offset of each candidate's bucket)
-            if (sc_stencil_shares_bucket(x, a.scInvCell[b], (unsigned int)B.scTableMask)) a.scActive[i] = 3;
+            // around x share a bucket (the fine scans then look up the first
+            // offset of each candidate's bucket)
+            int flags = 1;
+            if (sc_stencil_shares_bucket(x, a.scInvCell[b], (unsigned int)B.scTableMask)) flags |= 2;
+            // bit 2: the nine row stretches of the 27 fine cells around p
+            // (keys k_r - 1 .. k_r + 1 around the middle-cell key k_r) cannot
+            // be walked as stretches, because one wraps around the table or two
+            // come within two keys of each other; the fine scans then walk each
+            // distinct cell once
+            const int gx = sc_cell(p.x, g), gy = sc_cell(p.y, g), gz = sc_cell(p.z, g);
+            const unsigned int fmask = (unsigned int)B.scFineMask;
+            unsigned int kr[9];
+            bool perCell = false;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                kr[r] = sc_fine_key(gx, gy + r % 3 - 1, gz + r / 3 - 1, fmask);
+                perCell |= kr[r] == 0u || kr[r] == fmask;
+            }
+#pragma unroll
+            for (int r = 1; r < 9; ++r)
+#pragma unroll
+                for (int r2 = 0; r2 < r; ++r2) perCell |= (kr[r] > kr[r2] ? kr[r] - kr[r2] : kr[r2] - kr[r]) <= 2u;
+            if (perCell) flags |= 4;
+            if (flags != 1) a.scActive[i] = flags;
         }
     }
     a.fkeys[i] = key;
@@ -635,17 +656,11 @@ __global__ void __launch_bounds__(256) sc_fine_group_kernel(int nPad, SelfColArg
         // middle-cell key of row r = (oy + 1) + 3 (oz + 1); its cells are
         // keys kr - 1, kr, kr + 1, adjacent in the sorted order unless kr is
         // at either end of the table
+        // (stretches that wrap or overlap: flagged by the fine-keys pass)
         unsigned int kr[9];
-        bool perCell = false;
+        const bool perCell = (a.scActive[i] & 4) != 0;
 #pragma unroll
-        for (int r = 0; r < 9; ++r) {
-            kr[r] = sc_fine_key(gx, gy + r % 3 - 1, gz + r / 3 - 1, fmask);
-            perCell |= kr[r] == 0u || kr[r] == fmask;
-        }
-#pragma unroll
-        for (int r = 1; r < 9; ++r)
-#pragma unroll
-            for (int r2 = 0; r2 < r; ++r2) perCell |= (kr[r] > kr[r2] ? kr[r] - kr[r2] : kr[r2] - kr[r]) <= 2u;
+        for (int r = 0; r < 9; ++r) kr[r] = sc_fine_key(gx, gy + r % 3 - 1, gz + r / 3 - 1, fmask);
         // the nine stretches as one run: run entry v sits at v + rowD[r] for
         // the last row r with rowP[r] <= v (empty rows are overridden)
         int rowP[9], rowD[9];
@@ -874,22 +889,19 @@ __global__ void __launch_bounds__(256) sc_fine_kernel(int nPad, SelfColArgs a, c
             }
             // first stencil offset holding each coarse bucket (two offsets
             // share one when their cells collide in the hash)
+            // (flagged per node by the fine-keys pass; rare)
+            const bool dup = (act & 2) != 0;
             int firstC = c;
-            for (int c2 = 26; c2 >= 0; --c2)
-                if ((unsigned int)lane_i((int)hg, c2) == hg) firstC = c2;
-            const bool dup = __ballot(lane < 27 && firstC != lane) != 0ull;
+            if (dup)
+                for (int c2 = 26; c2 >= 0; --c2)
+                    if ((unsigned int)lane_i((int)hg, c2) == hg) firstC = c2;
             // Row r = c / 3 (one (oy, oz)) covers keys k_r - 1 .. k_r + 1 (k_r
             // the key of its middle cell), one contiguous stretch of the
             // sorted order.  Rows whose stretches wrap around the table or
             // come within two keys of each other would visit a key twice:
-            // then each distinct cell key is walked once instead.
-            bool clash = false;
-            for (int r2 = 0; r2 < 9; ++r2) {
-                const unsigned int k2 = (unsigned int)lane_i((int)fk, 3 * r2 + 1);
-                const unsigned int dk = fk > k2 ? fk - k2 : k2 - fk;
-                if (lane != 3 * r2 + 1 && dk <= 2u) clash = true;
-            }
-            const bool perCell = __ballot(lane < 27 && ox == 0 && (clash || fk == 0u || fk == fmask)) != 0ull;
+            // then each distinct cell key is walked once instead (flagged by
+            // the fine-keys pass).
+            const bool perCell = (act & 4) != 0;
             // row r: first run position rowP (past every run position when
             // the row is empty) and sorted index minus run position rowD
             int rowP[9], rowD[9];
@@ -920,26 +932,22 @@ __global__ void __launch_bounds__(256) sc_fine_kernel(int nPad, SelfColArgs a, c
                 total = sc_run_prefix(rn, lane, pre);
             }
             int nacc = 0;
-            for (int base = 0; base < total; base += 64) {
-                const int v = base + lane;
-                int t;
-                if (!perCell) {
-                    int d = 0;
+            auto runPos = [&](int v) {
+                if (perCell) return sc_run_pos(v, rs, pre);
+                int d = 0;
 #pragma unroll
-                    for (int r = 0; r < 9; ++r)
-                        if (v >= rowP[r]) d = rowD[r];
-                    t = v + d;
-                } else {
-                    t = sc_run_pos(v, rs, pre);
-                }
+                for (int r = 0; r < 9; ++r)
+                    if (v >= rowP[r]) d = rowD[r];
+                return v + d;
+            };
+            // test one run entry per lane and append the accepted ones in
+            // lane order
+            auto scanEntry = [&](int v, const float4& pj, const float4& qj, const int4& cj) {
                 bool acc = false, inside = true;
                 unsigned int key = 0u, ck = 0u;
                 int j = 0, rk = 0;
                 float d2 = 0.0f;
                 if (v < total) {
-                    const float4 pj = a.fineP[t];
-                    const float4 qj = a.fineR[t];
-                    const int4 cj = a.fineC[t];
                     j = __float_as_int(pj.w);
                     ck = (unsigned int)cj.w;
                     rk = __float_as_int(qj.w);
@@ -981,6 +989,18 @@ __global__ void __launch_bounds__(256) sc_fine_kernel(int nPad, SelfColArgs a, c
                     lpay[wl][slot] = make_int2(j, __float_as_int(d2));
                 }
                 nacc += __popcll(am);
+            };
+            for (int base = 0; base < total; base += 64) {
+                const int v = base + lane;
+                const int t = runPos(v);   // every lane: the per-cell walk shuffles across the wave
+                float4 pj = make_float4(0.0f, 0.0f, 0.0f, 0.0f), qj = pj;
+                int4 cj = make_int4(0, 0, 0, 0);
+                if (v < total) {
+                    pj = a.fineP[t];
+                    qj = a.fineR[t];
+                    cj = a.fineC[t];
+                }
+                scanEntry(v, pj, qj, cj);
             }
             deferred = kScNeedCoarse;   // unless the ranked list is written below
             if (nacc <= kFineList) {

@@ -346,6 +346,9 @@ def main():
                     help="full/distance: autotune the launch shapes after the warmup frames (the defaults are "
                          "the ones measured best at the compressed frames of the gripper script, which the "
                          "light warmup state does not represent)")
+    ap.add_argument("--group-lanes", type=int, default=None,
+                    help="full: lanes per node of the grouped fine candidate scan (8, 16, 32; 0 = wave scan; "
+                         "default: the library's automatic choice)")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run a streaming kernel of known traffic (256 MiB read + 256 MiB written, x3) "
                          "to scale the FETCH_SIZE / WRITE_SIZE counters")
@@ -386,6 +389,8 @@ def main():
     frame = 0
     for _ in range(args.warmup):
         run_frame(sim, script, frame, frame == 0, shapes)
+        if frame == 0 and args.group_lanes is not None:
+            sim.ext_set_group_lanes(args.group_lanes)
         frame += 1
     log(f"warmup {args.warmup} frames done")
     tune = sim.ext_autotune(2) if args.tune else None

@@ -153,6 +153,7 @@ def test_full_recipe_1m(product_path, oracle_path):
     assert err0 <= TOL_REL, f"1M frame 0: {err0:.3e}"
     compare_shapes(gpu, cpu, "1M frame 0")
     t1 = time.perf_counter()
+    print(f"1M full recipe: frame 0 compared ({t1 - t0:.1f}s)", flush=True)
     # GPU alone to frame 12, then one substep from that state on both
     for f in range(1, 12):
         frame_script(gpu, script, f)
@@ -170,6 +171,7 @@ def test_full_recipe_1m(product_path, oracle_path):
     err1 = max_abs(gpu.positions(), cpu.positions()) / extent
     assert err1 <= TOL_REL, f"1M frame 12 substep: {err1:.3e}"
     t2 = time.perf_counter()
+    print(f"1M full recipe: frame-12 substep compared ({t2 - t1:.1f}s)", flush=True)
     cpu.destroy()
     # 20 more frames: properties
     jmin = []
