@@ -12,6 +12,11 @@ once per frame (outputs left in device memory).  dt = 1/60 s per frame as 10
 substeps x 4 iterations; one bench step = one frame.  "--workload distance"
 keeps only distance + ground + grippers.
 
+The default run also measures the batched 64-object mix ("--workload
+batched": its Gaussian-substeps/s over one object per call, both sides
+autotuned) and records it under "batched_mix" in the same line
+(--no-batched skips it).
+
 Prints one JSON line (see the repository contract).  --gpus N>1 is refused:
 this build runs on one GPU.
 """
@@ -145,10 +150,11 @@ def cpu_baseline(scene, script, workload, recipe, seconds_budget=12.0, full_samp
             "single_thread": {"value": one, "cores": 1}}
 
 
-def kernel_bytes(workload, n, m, sc_table, sc_passes, vol):
+def kernel_bytes(workload, n, m, sc_table, sc_passes, vol, sc_cnt=0, sc_touched=0):
     """Algorithmic bytes per launch (compulsory traffic) of each profiled kernel
     label: N nodes, M directed distance entries, L leaders, S members,
-    R = S + L reverse entries, T hash cells."""
+    R = S + L reverse entries, T hash cells, cnt self-collision candidates,
+    touched nodes with at least one candidate."""
     L, S = vol
     R = S + L
     out = {
@@ -163,7 +169,8 @@ def kernel_bytes(workload, n, m, sc_table, sc_passes, vol):
             "volume_solve_wave": 100 * L + 4 * S,
             "volume_apply": 56 * n + 4 * R,
             "shape_match": 60 * n,
-            "selfcol_build": 150 * n + 4 * sc_table + 16 * n * sc_passes,
+            "selfcol_build": 150 * n + 4 * sc_table + 4 * sc_cnt + 16 * n * sc_passes,
+            "selfcol_iteration": 4 * n + 4 * sc_cnt + 60 * sc_touched,
             "contact_push_commit": 64 * n,
         })
     return out
@@ -227,6 +234,12 @@ def bench_batched(args):
         for s, r in zip(singles, reqs):
             plain_frame(s, r.script, frame)
         frame += 1
+    # the same autotuner on every single-object batch (launch shapes and hash
+    # table sizes as on the batched side), so that both sides are tuned
+    t = time.perf_counter()
+    plain_tunes = [s.autotune(substeps=2, hash_tables=args.tune_hash) for s in singles] if not args.no_tune_plain else []
+    if plain_tunes:
+        log(f"plain side autotuned ({len(singles)} objects) in {time.perf_counter() - t:.1f}s")
     plain_steps = max(1, args.steps // 2)
     t0 = time.perf_counter()
     for _ in range(plain_steps):
@@ -259,9 +272,10 @@ def bench_batched(args):
         "batched": {"gaussian_substeps_per_s": rate_b, "ms_per_frame": 1e3 * dt_b / args.steps, "tuning": tuning,
                     "autotune": tune, "kernels_ms": prof},
         "plain": {"gaussian_substeps_per_s": rate_p, "ms_per_frame": 1e3 * dt_p / plain_steps, "frames": plain_steps,
-                  "path": "one single-object batch per object, one synchronous call per substep, objects in turn"},
+                  "path": "one single-object batch per object, one synchronous call per substep, objects in turn",
+                  "autotuned": bool(plain_tunes)},
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_served(args):
@@ -335,6 +349,8 @@ def main():
     ap.add_argument("--workload", choices=("full", "distance", "batched", "served"), default="full")
     ap.add_argument("--objects", type=int, default=64, help="batched: object count")
     ap.add_argument("--tune-hash", action="store_true", help="batched: let the autotuner size the hash tables")
+    ap.add_argument("--no-tune-plain", action="store_true",
+                    help="batched: leave the plain side at the default launch shapes (the batched side is autotuned)")
     ap.add_argument("--templates", type=int, default=40, help="served: prepared objects requests draw from")
     ap.add_argument("--ticks", type=int, default=150, help="served: ticks to run")
     ap.add_argument("--skip", type=int, default=10, help="served: ticks left out of the latency figures")
@@ -342,6 +358,8 @@ def main():
     ap.add_argument("--max-segments", type=int, default=6, help="served: batches before merging")
     ap.add_argument("--cold", action="store_true", help="served: start empty instead of at the steady population")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="full: skip the batched 64-object mix that the default run also measures")
     ap.add_argument("--tune", action="store_true",
                     help="full/distance: autotune the launch shapes after the warmup frames (the defaults are "
                          "the ones measured best at the compressed frames of the gripper script, which the "
@@ -366,7 +384,8 @@ def main():
         print(json.dumps({"calibration": "mul", "bytes_read": a.numel() * 4, "bytes_written": b.numel() * 4}))
         return
     if args.workload == "batched":
-        return bench_batched(args)
+        print(json.dumps(bench_batched(args)), flush=True)
+        return
     if args.workload == "served":
         return bench_served(args)
 
@@ -406,7 +425,9 @@ def main():
     log(f"timed {args.steps} frames in {elapsed:.3f}s -> {value / 1e9:.3f} G Gaussian-substeps/s")
     stats = sim.self_collision_stats() if shapes else (0, 0, 0)
     grid = sim.ext_self_collision_grid_info() if shapes else (0.0, 0.0, 0.0)
-    log(f"self-collision: candidates/active/overflow {stats}, cell/d_c/search {grid}")
+    touched = int((sim.ext_self_collision_lists()[2] > 0).sum()) if shapes else 0
+    log(f"self-collision: candidates/active/overflow {stats}, nodes with candidates {touched}, "
+        f"cell/d_c/search {grid}")
 
     # per-kernel HIP-event timing on the library stream, same state and script
     phases = sim.ext_profile(SUBSTEPS)
@@ -423,7 +444,7 @@ def main():
     vol = (n, 3 * n) if shapes else (0, 0)
     table = recipe["table"] if shapes else 0
     passes = (int(np.ceil(np.log2(table + 1))) + 7) // 8 if shapes else 0
-    formulas = kernel_bytes(args.workload, n, m, table, passes, vol)
+    formulas = kernel_bytes(args.workload, n, m, table, passes, vol, stats[0], touched)
     kernels = {}
     for k, (tot, cnt) in prof.items():
         if k in formulas and cnt > 0:
@@ -436,9 +457,19 @@ def main():
     sim.destroy()
 
     cpu = None if args.no_cpu else cpu_baseline(scene, script, args.workload, recipe)
+    # the batched many-object mix (its own speedup over one object per call),
+    # measured in the default run too so that it is on record beside the 1M line
+    batched = None
+    if args.workload == "full" and not args.no_batched:
+        b = bench_batched(args)
+        batched = {"metric": b["metric"], "value": b["value"], "unit": b["unit"], "steps": b["steps"],
+                   "config": b["config"], "batched_gaussian_substeps_per_s": b["batched"]["gaussian_substeps_per_s"],
+                   "plain_gaussian_substeps_per_s": b["plain"]["gaussian_substeps_per_s"],
+                   "both_sides_autotuned": b["plain"]["autotuned"]}
     if args.workload == "full":
         wl = (f"1M-Gaussian ellipsoid full solver: kNN k=12 distance, node+3NN volume clusters, k-means shape "
-              f"clusters of 64, self-collision within the body (2^23 cells), ground+gravity, 2 capsule grippers "
+              f"clusters of 64, self-collision within the body (2^23 cells; cell edge by the reference's rule, "
+              f"search + 2 x largest predicted move, not 2 x mean scale), ground+gravity, 2 capsule grippers "
               f"pull/twist, per-frame shape update; {SUBSTEPS} substeps x {ITERS} iters per step; plain C-ABI path")
     else:
         wl = (f"1M-Gaussian ellipsoid, kNN k=12 distance, ground+gravity, 2 capsule grippers pull/twist; "
@@ -458,13 +489,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl, "gaussians": n, "edges": e, "substeps_per_step": SUBSTEPS, "iters": ITERS,
                    "shape_clusters": int(recipe["clusters"].max() + 1) if shapes else 0,
-                   "selfcol_stats_last": list(stats), "selfcol_cell_dc_search": list(grid), "autotune": tune},
+                   "selfcol_stats_last": list(stats), "selfcol_touched": touched,
+                   "selfcol_cell_dc_search": list(grid), "autotune": tune},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dominant, args.workload),
                      "traffic_source": os.path.relpath(PROFILES[args.workload], ROOT),
                      "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_us": dk["avg_us"]},
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "batched_mix": batched,
     }
     print(json.dumps(out), flush=True)
 
