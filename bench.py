@@ -72,19 +72,54 @@ def configure(sim, workload, recipe):
         sim.set_self_collision(False)
 
 
-def run_frame(sim, script, frame, first, shapes):
+def run_frame(sim, script, frame, first, shapes, eager=False):
+    """One frame of the gripper script.  eager: the substeps run through the
+    library's event profile (ordinary launches instead of the captured graph)
+    with the same parameters, for counter collection."""
     t, p, d = script.colliders(frame)
     sim.set_colliders(t, p, d)
     att = script.attached(frame)
-    for s in range(SUBSTEPS):
+    s = 0
+    while s < SUBSTEPS:
         if s == (1 if first else 0):
             if att is not None:
                 sim.set_attached(att[0], att[1], 1.0)
             elif frame == script.ramp:
                 sim.clear_attached()
-        sim.step()
+        if eager and (s > 0 or not first):
+            sim.ext_profile(SUBSTEPS - s)   # the rest of the frame, no parameter change inside it
+            s = SUBSTEPS
+        else:
+            sim.step()
+            s += 1
     if shapes:
         sim.shapes_device(1e-2)
+
+
+def summarize_pmc(pattern, last):
+    """Per kernel, the mean of each counter over its dispatches among the last
+    `last` dispatches of rocprofv3 --pmc CSV output (counter_collection.csv)."""
+    import csv
+    import glob
+    import collections
+    for path in sorted(glob.glob(pattern, recursive=True)):
+        rows = list(csv.DictReader(open(path)))
+        if not rows:
+            continue
+        disp = sorted({int(r["Dispatch_Id"]) for r in rows})
+        keep = set(disp[-last:])
+        acc = collections.defaultdict(lambda: collections.defaultdict(float))
+        seen = collections.defaultdict(set)
+        for r in rows:
+            d = int(r["Dispatch_Id"])
+            if d in keep:
+                acc[r["Kernel_Name"]][r["Counter_Name"]] += float(r["Counter_Value"])
+                seen[r["Kernel_Name"]].add(d)
+        print(f"# {path}: last {len(keep)} dispatches")
+        for k in sorted(acc, key=lambda k: -sum(acc[k].values())):
+            n = len(seen[k])
+            vals = " ".join(f"{c}={v / n:.6g}" for c, v in sorted(acc[k].items()))
+            print(f"{k.split('(')[0]} | dispatches {n} | {vals}")
 
 
 def cpu_baseline(scene, script, workload, recipe, seconds_budget=12.0, full_sample=100_000):
@@ -367,6 +402,13 @@ def main():
     ap.add_argument("--group-lanes", type=int, default=None,
                     help="full: lanes per node of the grouped fine candidate scan (8, 16, 32; 0 = wave scan; "
                          "default: the library's automatic choice)")
+    ap.add_argument("--eager", action="store_true",
+                    help="full/distance: run the substeps as ordinary launches (the event profile) instead of the "
+                         "captured graph, for rocprofv3 --pmc runs (the timed figure is then not the headline)")
+    ap.add_argument("--pmc-summary", default=None, metavar="GLOB",
+                    help="only summarise rocprofv3 --pmc CSV files (counter_collection.csv) matching GLOB")
+    ap.add_argument("--last-dispatches", type=int, default=800,
+                    help="with --pmc-summary: dispatches at the end of the run to average over")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run a streaming kernel of known traffic (256 MiB read + 256 MiB written, x3) "
                          "to scale the FETCH_SIZE / WRITE_SIZE counters")
@@ -374,6 +416,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.pmc_summary:
+        summarize_pmc(args.pmc_summary, args.last_dispatches)
+        return
     if args.calibrate:
         import torch
         a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
@@ -407,7 +452,7 @@ def main():
     shapes = args.workload == "full"
     frame = 0
     for _ in range(args.warmup):
-        run_frame(sim, script, frame, frame == 0, shapes)
+        run_frame(sim, script, frame, frame == 0, shapes, args.eager)
         if frame == 0 and args.group_lanes is not None:
             sim.ext_set_group_lanes(args.group_lanes)
         frame += 1
@@ -417,7 +462,7 @@ def main():
         log("autotune: " + json.dumps(tune))
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        run_frame(sim, script, frame, frame == 0, shapes)
+        run_frame(sim, script, frame, frame == 0, shapes, args.eager)
         frame += 1
     t1 = time.perf_counter()
     elapsed = t1 - t0
