@@ -294,6 +294,21 @@ int xpbd_compute_shapes(float deformEps, float* hostScales, float* hostQuats, in
     return deformed;
 }
 
+int xpbd_ext_frame_outputs(float* posDev, float* scaleDev, float* quatDev, int quatXYZW, float deformEps,
+                           float* prevDev, float moveEps, double* host4, int n)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_ext_frame_outputs: no step yet");
+    if (n != g.n) return fail("xpbd_ext_frame_outputs: size mismatch");
+    if (!host4) return fail("xpbd_ext_frame_outputs: host4 is null");
+    std::string err;
+    int deformed = 0;
+    if (!g.engine->frameOutputs(0, posDev, scaleDev, quatDev, quatXYZW != 0, deformEps, prevDev, moveEps, host4,
+                                deformed, err))
+        return fail("xpbd_ext_frame_outputs: " + err);
+    return deformed;
+}
+
 int xpbd_reset(void)
 {
     Global& g = G();

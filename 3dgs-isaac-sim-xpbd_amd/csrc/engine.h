@@ -239,6 +239,11 @@ public:
     // (forward.cu:13230-13247); host outputs may be null.  Returns the
     // deformed count in `deformed`.  Repacks the contact ellipsoids when on.
     bool computeShapes(int b, float deformEps, float* hostScales, float* hostQuats, int& deformed, std::string& err);
+    // Frame outputs of body b into caller-owned device buffers (any may be
+    // null); host4 = sums of x, y, z and the largest change against prev,
+    // which takes the new positions when that change exceeds moveEps.
+    bool frameOutputs(int b, float* dPos, float* dScale, float* dQuat, bool xyzw, float deformEps, float* dPrev,
+                      float moveEps, double host4[4], int& deformed, std::string& err);
     // Contact ellipsoids (native/xpbd_capi.cu:179-191): bit 1 colliders, bit 2 ground.
     bool setContactShape(int b, int enabled, float tau, float radiusCap, std::string& err);
 
@@ -410,6 +415,7 @@ private:
     DevBuf<float> gsScale_, gsRot_, gsOpacity_, outScale_, outQuat_;
     DevBuf<float> cM_, cRmax_, cRg_;
     DevBuf<unsigned int> shapeCtr_;
+    DevBuf<double> frameScratch_;   // frame-output partials, then the four totals
     std::vector<char> contactOn_, contactColliders_, contactGround_;
     std::vector<float> contactTau_, contactCap_;
 

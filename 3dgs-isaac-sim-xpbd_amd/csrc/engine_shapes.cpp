@@ -105,6 +105,48 @@ bool Engine::computeShapes(int b, float deformEps, float* hostScales, float* hos
     return true;
 }
 
+bool Engine::frameOutputs(int b, float* dPos, float* dScale, float* dQuat, bool xyzw, float deformEps, float* dPrev,
+                          float moveEps, double host4[4], int& deformed, std::string& err)
+{
+    XPBD_ALLOC_SCOPE;
+    const int nb = (int)bodyBegin_[b];
+    const int n = bodyCount_[b];
+    // caller index of device node i: callerPos[i] - callerBase
+    int callerBase = 0;
+    for (int k = 0; k < b; ++k) callerBase += bodyCount_[k];
+    deformed = 0;
+    unsigned int c = 0;
+    const bool shapes = dScale || dQuat;
+    if (shapes) {
+        if (!ensureShapeBuffers(err)) return false;
+        launch_deformed_shapes(stream_, nb, n, rest_.p, x_.p, rowOff_.p, nbr_.p, gsScale_.p + 3 * (size_t)nb,
+                               gsRot_.p + 4 * (size_t)nb, deformEps, outScale_.p + 3 * (size_t)nb,
+                               outQuat_.p + 4 * (size_t)nb, shapeCtr_.p);
+        if (contactOn_[b] && cM_.p)
+            launch_contact_pack(stream_, nb, n, outScale_.p + 3 * (size_t)nb, outQuat_.p + 4 * (size_t)nb,
+                                gsOpacity_.p + nb, contactTau_[b], cap_or_zero(contactCap_[b]), cM_.p, cRmax_.p);
+        launch_shapes_to_caller(stream_, nb, n, callerBase, callerPos_.p, outScale_.p + 3 * (size_t)nb,
+                                outQuat_.p + 4 * (size_t)nb, dScale, dQuat, xyzw);
+        XPBD_HIP(hipMemcpyAsync(&c, shapeCtr_.p + deformed_shape_blocks(n), sizeof(unsigned int),
+                                hipMemcpyDeviceToHost, stream_));
+    }
+    const size_t need = 4 * (size_t)frame_output_blocks(n) + 4;
+    if (frameScratch_.n < need) {
+        frameScratch_.alloc(need);
+        if (!frameScratch_.p) { err = "hipMalloc failed"; return false; }
+    }
+    // the positions in caller order (the caller-order copy, or the device
+    // array itself when the nodes keep the caller's order)
+    const float3* pos = permuted_ ? xOut_.p + callerBase : x_.p + nb;
+    double* tot = frameScratch_.p + need - 4;
+    launch_frame_positions(stream_, n, pos, dPos, dPrev, moveEps, frameScratch_.p, tot);
+    XPBD_HIP(hipMemcpyAsync(host4, tot, 4 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    XPBD_HIP(hipGetLastError());
+    deformed = shapes ? (int)c : 0;
+    return true;
+}
+
 }  // namespace xpbd
 
 namespace xpbd {

@@ -220,6 +220,15 @@ int deformed_shape_scratch_max();
 void launch_contact_pack(hipStream_t s, int nodeBegin, int n, const float* sc, const float* qt, const float* op,
                          float tau, float rcap, float* M, float* rmax);
 void launch_contact_ground(hipStream_t s, const NodeArgs& na, const float* M, float* rg);
+// Frame outputs (caller-owned device buffers): positions copied, compared with
+// prev (overwritten when the largest change exceeds moveEps) and summed; part
+// holds frame_output_blocks(n) * 4 doubles, out receives sum x, y, z and the
+// largest change.
+int frame_output_blocks(int n);
+void launch_frame_positions(hipStream_t s, int n, const float3* pos, float* dPos, float* dPrev, float moveEps,
+                            double* part, double* out);
+void launch_shapes_to_caller(hipStream_t s, int nodeBegin, int n, int callerBase, const int* callerPos,
+                             const float* scale, const float* quat, float* dScale, float* dQuat, bool xyzw);
 
 void launch_predict(hipStream_t s, const NodeArgs& na, const float3* X, float3* V, const float* invMass, float4* P);
 void launch_distance_jacobi(hipStream_t s, const NodeArgs& na, bool first, bool last, const float4* Pin, float4* Pout,

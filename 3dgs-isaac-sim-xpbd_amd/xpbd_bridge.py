@@ -125,6 +125,7 @@ ABI_OPTIONAL = {
     "xpbd_ext_set_group_lanes": (_INT, [_INT]),
     "xpbd_ext_autotune": (_INT, [_INT, _INT, ctypes.c_char_p, _INT]),
     "xpbd_ext_volume_members": (_INT, [_PTR, _INT]),
+    "xpbd_ext_frame_outputs": (_INT, [_PTR, _PTR, _PTR, _INT, _FLT, _PTR, _FLT, ctypes.POINTER(ctypes.c_double), _INT]),
 }
 
 

@@ -274,6 +274,27 @@ int xpbd_ext_compute_shapes_noalloc(int n, const float* restDev, const float* po
                                     const int* nbrDev, const float* scalesDev, const float* rotsDev, float deformEps,
                                     float* outScalesDev, float* outQuatsDev);
 
+/* Frame outputs in device memory: what a scene writer consumes every frame,
+ * without the host copies of the reference's runtime (runtime_loop.py:387-402
+ * reads positions and shapes to the host; 938-946 then takes their mean and
+ * the largest change to decide whether to write).  All pointers are device
+ * memory owned by the caller, n the instance size:
+ *   posDev   float[n*3]  positions, caller order (may be null);
+ *   scaleDev float[n*3], quatDev float[n*4]: the deformed shapes of
+ *            xpbd_compute_shapes (either may be null; both null skips the
+ *            shape update); quaternions (w,x,y,z), or (x,y,z,w) when quatXYZW;
+ *   prevDev  float[n*3]  the positions of the last frame written (may be
+ *            null): host4[3] receives the largest |coordinate change| against
+ *            it, and it takes the new positions when that change exceeds
+ *            moveEps (every call for moveEps < 0), decided on the device;
+ *   host4    double[4]   host memory: the sums of x, y, z over the positions
+ *            (double, fixed order) and the largest change (0 without prevDev).
+ * The caller's own work on these buffers must have completed before the call;
+ * the library's work on them has completed when it returns.  Returns the
+ * deformed count (0 without shapes) or -1. */
+int xpbd_ext_frame_outputs(float* posDev, float* scaleDev, float* quatDev, int quatXYZW, float deformEps,
+                           float* prevDev, float moveEps, double* host4, int n);
+
 /* ---------------------------------------------------------------------------
  * Part 3: batched objects (no counterpart in the reference, whose state is one
  * global instance, native/xpbd_capi.cu:6, 17-29).  A batch holds many
