@@ -385,7 +385,7 @@ int xpbd_batch_set_tuning(xpbd_batch* b, const int* t)
         return false;
     };
     if (!in(t[0], {1, 2, 4}) || !in(t[1], {-1, 0, 1}) || !in(t[2], {1, 2, 4}) || !in(t[3], {1, 2, 4}) ||
-        !in(t[4], {4, 8, 16, 32, 64}) || !(t[5] >= 0 && t[5] <= 256) || !in(t[6], {-1, 0, 8, 16, 32}))
+        !in(t[4], {4, 8, 16, 32, 64}) || !(t[5] >= 0 && t[5] <= 256) || !in(t[6], {-1, 0, 1}))
         return bfail(b, "xpbd_batch_set_tuning: value outside the tuned range");
     Tuning& u = b->engine.tuning();
     u.jacobiTiles = t[0]; u.volMode = t[1]; u.shapeClusterWaves = t[2]; u.scBuildWaves = t[3]; u.scBuildNodes = t[4];

@@ -500,8 +500,8 @@ int xpbd_ext_set_group_lanes(int lanes)
 {
     Global& g = G();
     if (!g.engine) return fail("xpbd_ext_set_group_lanes: no instance");
-    if (lanes != -1 && lanes != 0 && lanes != 8 && lanes != 16 && lanes != 32)
-        return fail("xpbd_ext_set_group_lanes: lanes must be -1, 0, 8, 16 or 32");
+    if (lanes != -1 && lanes != 0 && lanes != 1)
+        return fail("xpbd_ext_set_group_lanes: lanes must be -1, 0 or 1");
     g.engine->tuning().scGroupLanes = lanes;
     return 0;
 }
@@ -543,6 +543,16 @@ int xpbd_ext_self_collision_grid_info(float* info)
     if (!info) return fail("xpbd_ext_self_collision_grid_info: null output");
     std::string err;
     if (!g.engine->selfCollisionGridInfo(0, info, err)) return fail("xpbd_ext_self_collision_grid_info: " + err);
+    return 0;
+}
+
+int xpbd_ext_self_collision_paths(int* out)
+{
+    Global& g = G();
+    if (!g.stepped) return fail("xpbd_ext_self_collision_paths: no step yet");
+    if (!out) return fail("xpbd_ext_self_collision_paths: null output");
+    std::string err;
+    if (!g.engine->selfCollisionPaths(0, out, err)) return fail("xpbd_ext_self_collision_paths: " + err);
     return 0;
 }
 

@@ -328,6 +328,7 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     invMass_.alloc(nPad_); invMassSaved_.alloc(nPad_); objW_.alloc(nPad_);
     rowOff_.alloc(nPad_ + 1); nbr_.alloc(std::max<long long>(m, 1));
     callerPos_.alloc(nPad_);
+    devOfCaller_.alloc(nPad_);
     if (permuted_) xOut_.alloc(nPad_);
     else xOut_.free();
     nstiff_.alloc(std::max<long long>(m, 1));
@@ -337,7 +338,7 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     colHits_.alloc((size_t)B * kMaxColliders);
     colPush_.alloc((size_t)B * kMaxColliders * kColAcc);
     if (!x_.p || !v_.p || !rest_.p || !p0_.p || !p1_.p || !invMass_.p || !invMassSaved_.p || !objW_.p || !rowOff_.p ||
-        !nbr_.p || !nstiff_.p || !tileBody_.p || !bodies_.p || !cols_.p || !callerPos_.p || (permuted_ && !xOut_.p) ||
+        !nbr_.p || !nstiff_.p || !tileBody_.p || !bodies_.p || !cols_.p || !callerPos_.p || !devOfCaller_.p || (permuted_ && !xOut_.p) ||
         !colHits_.p || !colPush_.p) {
         err = "engine: hipMalloc failed (out of device memory?)";
         return false;
@@ -349,6 +350,12 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     XPBD_HIP(hipMemsetAsync(p1_.p, 0, sizeof(float4) * nPad_, stream_));
     XPBD_HIP(hipMemcpyAsync(rowOff_.p, hrow.data(), sizeof(int) * (nPad_ + 1), hipMemcpyHostToDevice, stream_));
     XPBD_HIP(hipMemcpyAsync(callerPos_.p, hcaller.data(), sizeof(int) * nPad_, hipMemcpyHostToDevice, stream_));
+    {
+        std::vector<int> hdev(nPad_);
+        for (int i = 0; i < nPad_; ++i) hdev[hcaller[i]] = i;
+        XPBD_HIP(hipMemcpyAsync(devOfCaller_.p, hdev.data(), sizeof(int) * nPad_, hipMemcpyHostToDevice, stream_));
+        XPBD_HIP(hipStreamSynchronize(stream_));   // hdev leaves scope
+    }
     if (permuted_) XPBD_HIP(hipMemsetAsync(xOut_.p, 0, sizeof(float3) * nPad_, stream_));
     if (m > 0) {
         XPBD_HIP(hipMemcpyAsync(nbr_.p, hnbr.data(), sizeof(int) * m, hipMemcpyHostToDevice, stream_));
@@ -763,17 +770,8 @@ std::vector<int> Engine::planSignature() const
     }
     const int volWave = anyVol ? (tune_.volMode >= 0 ? (tune_.volMode ? 1 : 0) : (volAvgMembers_ >= kVolWaveMembers ? 1 : 0))
                                : 0;
-    // automatic fine-scan group width: the lane-group scan wins on bodies up to
-    // a few 100k nodes, the one-wave-per-node scan on larger ones; only bodies
-    // that take the fine scan count
-    int groupLanes = tune_.scGroupLanes;
-    if (groupLanes < 0) {
-        long long nodes = 0;
-        int bodies = 0;
-        for (int b = 0; b < B; ++b)
-            if (bodyHost_[b].scOn && bodyHost_[b].scFine) { nodes += bodyCount_[b]; ++bodies; }
-        groupLanes = (bodies > 0 && nodes <= 300000LL * bodies) ? 8 : 0;
-    }
+    // fine candidate scan: the automatic choice is the lane scan
+    const int groupLanes = tune_.scGroupLanes < 0 ? 1 : tune_.scGroupLanes;
     const int nDistGS = (anyDistGS && dgsDevValid_) ? (int)dgsColorOff_.size() - 1 : 0;
     const int nVolGS = (anyVolGS && vgsDevValid_) ? (int)vgsColorOff_.size() - 1 : 0;
     return {iters, anyDist, anyCol, anyGround, anyPress, nPad_, anyVol, volWave, anyVol ? volActive_ : 0, anyShape,

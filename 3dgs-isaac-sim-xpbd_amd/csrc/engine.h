@@ -258,6 +258,9 @@ public:
     // are the same either way.
     void setSelfCollisionFine(bool on) { scFineOn_ = on; }
     bool selfCollisionGridInfo(int b, float info[3], std::string& err);   // cell, d_c, search
+    // node counts per candidate-search path of the last build: lane scan (<= 32),
+    // lane scan + slot rule, coarse walk, outside the grid
+    bool selfCollisionPaths(int b, int out[4], std::string& err);
     bool selfCollisionLists(int b, unsigned int* keys, int* sorted, int* counts, int* contacts, std::string& err);
 
     // Gauss-Seidel colourings (built on demand, caller ids): info = distance
@@ -327,6 +330,7 @@ private:
     int nodeOrderMode_ = 1;
     bool permuted_ = false;
     DevBuf<int> callerPos_;      // [nPad] caller-order global index of each device node
+    DevBuf<int> devOfCaller_;    // [nPad] its inverse: device node of each caller-order index
     DevBuf<float3> xOut_;        // committed positions in caller order (permuted only)
     void refreshCallerPositions(int b);
     template <typename T>
@@ -440,13 +444,14 @@ private:
     // bodies whose nodes form one object (same lists as the coarse scan)
     std::vector<char> objSingle_;           // every node of the body in one object
     std::vector<int> fineTable_, fineBase_;
-    int fineTotal_ = 0, fineBits_ = 0;
+    int fineTotal_ = 0;
     bool scFineOn_ = true;
-    DevBuf<unsigned int> fkeys_, fkeysAlt_;
-    DevBuf<int> fidx_, fidxAlt_, fcellStart_, fcellEnd_, coarseRank_, accCount_, accOff_, accUsed_;
-    DevBuf<float4> fineP_, fineR_;
+    DevBuf<unsigned int> fkeys_;
+    DevBuf<int> frank_, fcount_, fstart_, fscan_, finePos_, laneN_;
+    DevBuf<int> coarseRank_, accCount_;
+    DevBuf<float4> fineE_;
     DevBuf<int4> fineC_;
-    DevBuf<int2> accList_;
+    DevBuf<int2> laneKey_;
     std::vector<float> scSpacing_, scCellOverride_;
     std::vector<int> scTable_, scTableBase_, scTableOverride_;
     int scTableTotal_ = 0, scSortBits_ = 0;
@@ -455,7 +460,7 @@ private:
     const int* scSortedIdx_ = nullptr;
     const unsigned int* scSortedKeys_ = nullptr;
     DevBuf<unsigned int> scMax2_, scKeys_, scKeysAlt_, scHist_;
-    DevBuf<int> scCounters_, scUseActive_, scIdx_, scIdxAlt_, cellStart_, cellEnd_, contacts_, counts_;
+    DevBuf<int> scCounters_, scUseActive_, scExact_, scIdx_, scIdxAlt_, cellStart_, cellEnd_, contacts_, counts_;
     DevBuf<int> scOver_, scStats_;   // per-node overflow; statistics read-back
     DevBuf<float> scBodyDisp_, scInvCell_;
     DevBuf<unsigned char> scActive_;

@@ -257,7 +257,7 @@ bool Engine::autotune(int substeps, int flags, std::string& report, std::string&
         {"selfcol_nodes_per_wave", &tune_.scBuildNodes, {32, 16, 64, 8, 4}, anySc},
         {"selfcol_waves_per_block", &tune_.scBuildWaves, {4, 2, 1}, anySc},
         {"selfcol_xcd_runs", &tune_.scBuildXcd, {16, 1, 0, 64}, anySc},
-        {"selfcol_group_lanes", &tune_.scGroupLanes, {-1, 0, 8, 16, 32}, anySc},
+        {"selfcol_fine_scan", &tune_.scGroupLanes, {-1, 0}, anySc},
     };
     std::string rep = "{";
     float base = 0.0f;

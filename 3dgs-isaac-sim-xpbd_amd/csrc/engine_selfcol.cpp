@@ -60,6 +60,8 @@ SelfColArgs Engine::scArgs() const
     a.scMax2 = scMax2_.p;
     a.scCounters = scCounters_.p;
     a.scUseActive = scUseActive_.p;
+    a.scExact = scExact_.p;
+    a.devOfCaller = devOfCaller_.p;
     a.scBodyDisp = scBodyDisp_.p;
     a.scInvCell = scInvCell_.p;
     a.scActive = scActive_.p;
@@ -84,22 +86,17 @@ SelfColArgs Engine::scArgs() const
     for (int b = 0; b < numBodies() && !fine; ++b) fine = bodyHost_[b].scFine != 0;
     a.coarseRank = fine ? coarseRank_.p : nullptr;
     a.fineTotal = fine ? fineTotal_ : 0;
-    a.fineBits = fineBits_;
     a.fkeys = fkeys_.p;
-    a.fidx = fidx_.p;
-    a.fkeysAlt = fkeysAlt_.p;
-    a.fidxAlt = fidxAlt_.p;
-    a.fcellStart = fcellStart_.p;
-    a.fcellEnd = fcellEnd_.p;
-    a.fineP = fineP_.p;
-    a.fineR = fineR_.p;
+    a.frank = frank_.p;
+    a.fcount = fcount_.p;
+    a.fstart = fstart_.p;
+    a.fscan = fscan_.p;
+    a.finePos = finePos_.p;
+    a.fineE = fineE_.p;
     a.fineC = fineC_.p;
-    a.accList = accList_.p;
-    a.accOff = accOff_.p;
+    a.laneN = laneN_.p;
+    a.laneKey = laneKey_.p;
     a.accCount = fine ? accCount_.p : nullptr;
-    a.accUsed = fine ? accUsed_.p : nullptr;
-    a.accCap = (long long)accList_.n;
-    a.fineWaveAll = 0;
     return a;
 }
 
@@ -263,14 +260,15 @@ bool Engine::ensureSelfCol(std::string& err)
     while ((1LL << bits) <= total) ++bits;   // keys in [0, total], the sentinel included
     scSortBits_ = bits;
     const size_t n = nPad_;
-    scMax2_.alloc(B); scCounters_.alloc(3 * (size_t)B); scUseActive_.alloc(B); scBodyDisp_.alloc(B); scInvCell_.alloc(B);
+    scMax2_.alloc(B); scCounters_.alloc(3 * (size_t)B); scUseActive_.alloc(B); scExact_.alloc(B); scBodyDisp_.alloc(B);
+    scInvCell_.alloc(B);
     scKeys_.alloc(n); scKeysAlt_.alloc(n); scIdx_.alloc(n); scIdxAlt_.alloc(n);
     scHist_.alloc(radix_sort_hist_size(nPad_));
     cellStart_.alloc(scTableTotal_); cellEnd_.alloc(scTableTotal_);
     contacts_.alloc(n * kSelfColCap); counts_.alloc(n); scOver_.alloc(n);
     scActive_.alloc(n); scAccum_.alloc(n);
     scSortP_.alloc(n); scSortX_.alloc(n); scSortR_.alloc(n);
-    if (!scMax2_.p || !scCounters_.p || !scUseActive_.p || !scBodyDisp_.p || !scInvCell_.p || !scKeys_.p ||
+    if (!scMax2_.p || !scCounters_.p || !scUseActive_.p || !scExact_.p || !scBodyDisp_.p || !scInvCell_.p || !scKeys_.p ||
         !scKeysAlt_.p || !scIdx_.p || !scIdxAlt_.p || !scHist_.p || !cellStart_.p || !cellEnd_.p || !contacts_.p ||
         !counts_.p || !scOver_.p || !scActive_.p || !scAccum_.p || !scSortP_.p ||
         !scSortX_.p || !scSortR_.p) {
@@ -290,17 +288,17 @@ bool Engine::ensureSelfCol(std::string& err)
     }
     if (fineTotal >= 0x7fffffffLL) { err = "self-collision: hash tables too large"; return false; }
     fineTotal_ = (int)fineTotal;
-    fineBits_ = 0;
-    while ((1LL << fineBits_) <= fineTotal) ++fineBits_;
-    fkeys_.alloc(n); fkeysAlt_.alloc(n); fidx_.alloc(n); fidxAlt_.alloc(n);
-    fcellStart_.alloc(fineTotal_); fcellEnd_.alloc(fineTotal_);
-    fineP_.alloc(n); fineR_.alloc(n); fineC_.alloc(n); coarseRank_.alloc(n);
-    // sorted candidate lists handed to the one-thread-per-node slot kernel:
-    // room for 64 per node on average (a node that finds the pool full fills
-    // its slots in the build kernel)
-    accList_.alloc(n * 64); accOff_.alloc(n); accCount_.alloc(n); accUsed_.alloc(1);
-    if (!fkeys_.p || !fkeysAlt_.p || !fidx_.p || !fidxAlt_.p || !fcellStart_.p || !fcellEnd_.p || !fineP_.p ||
-        !fineR_.p || !fineC_.p || !coarseRank_.p || !accList_.p || !accOff_.p || !accCount_.p || !accUsed_.p) {
+    fkeys_.alloc(n); frank_.alloc(n); finePos_.alloc(n);
+    fcount_.alloc(fineTotal_); fstart_.alloc((size_t)fineTotal_ + 1);
+    fscan_.alloc(exclusive_scan_tmp_size(fineTotal_));
+    fineE_.alloc(2 * n); fineC_.alloc(n); coarseRank_.alloc(n);
+    // the lane fine scan's unsorted lists: kLaneList per fine position,
+    // slot-major per wave of fine positions
+    laneN_.alloc(n); laneKey_.alloc(n * kLaneList);
+    // per node: what its candidate search still needs
+    accCount_.alloc(n);
+    if (!fkeys_.p || !frank_.p || !finePos_.p || !fcount_.p || !fstart_.p || !fscan_.p || !fineE_.p ||
+        !fineC_.p || !coarseRank_.p || !laneN_.p || !laneKey_.p || !accCount_.p) {
         err = "self-collision: hipMalloc failed";
         return false;
     }
@@ -331,6 +329,30 @@ bool Engine::selfCollisionGridInfo(int b, float info[3], std::string& err)
     info[0] = inv > 0.0f ? 1.0f / inv : 0.0f;
     info[1] = set_[b].scRadiusScale * scSpacing_[b];
     info[2] = kScSearchScaleHost * info[1];
+    return true;
+}
+
+bool Engine::selfCollisionPaths(int b, int out[4], std::string& err)
+{
+    XPBD_ALLOC_SCOPE;
+    out[0] = out[1] = out[2] = out[3] = 0;
+    if (!scDeviceValid_ || b >= (int)scBuilt_.size() || !scBuilt_[b]) return true;
+    const int nb = bodyBegin_[b], n = bodyCount_[b];
+    if (!bodyHost_[b].scFine) {
+        out[2] = n;   // every node takes the coarse walk
+        return true;
+    }
+    std::vector<int> acc(n), cnt(n);
+    XPBD_HIP(hipMemcpyAsync(acc.data(), accCount_.p + nb, sizeof(int) * n, hipMemcpyDeviceToHost, stream_));
+    XPBD_HIP(hipMemcpyAsync(cnt.data(), counts_.p + nb, sizeof(int) * n, hipMemcpyDeviceToHost, stream_));
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    for (int i = 0; i < n; ++i) {
+        const int c = acc[i];
+        if (c == kScAccLane) ++out[0];
+        else if (c == kScAccRule || c == kScAccRuleHi) ++out[1];
+        else if (c == kScAccCoarse) ++out[2];
+        else ++out[3];
+    }
     return true;
 }
 

@@ -22,9 +22,9 @@ struct Tuning {
     int scBuildNodes = 32;      // nodes per wave of the candidate build: 4, 8, 16, 32, 64
     int scBuildXcd = 16;        // candidate build: runs of that many blocks dealt round-robin over the XCDs
                                 // (2-256), one contiguous run per XCD (1), plain block order (0)
-    int scGroupLanes = -1;      // lanes per node of the grouped fine candidate scan: 8, 16, 32; 0: every
-                                // node takes the wave fine scan; -1: 8 when the self-colliding bodies
-                                // average at most 300k nodes (measured faster there), else 0
+    int scGroupLanes = -1;      // fine candidate scan: 1 one lane per node in the predicted-position grid's
+                                // own order (the nodes it hands over take the coarse walk); 0 every node
+                                // takes the coarse walk; -1 automatic (1)
 };
 
 // Node-space launch description shared by every per-node kernel.
@@ -110,6 +110,13 @@ void launch_vol_gs(hipStream_t s, bool first, bool last, int begin, int count, c
                    const VolumeArgs& va, const int* tileBody, const BodyDev* bodies);
 
 constexpr int kSelfColCap = 32;   // forward.cu:223 SELFCOL_MAX_CONTACTS
+// per-node candidate-search path codes (accCount of fine-grid bodies)
+constexpr int kScAccCoarse = -2;   // the coarse walk (the reference's own)
+constexpr int kScAccLane = -5;     // the lane fine scan
+constexpr int kScAccRule = -6;     // the lane fine scan, more than 32 candidates (slot rule)
+constexpr int kScAccRuleHi = -7;   // the same (ranked list stored in the upper half of the list)
+constexpr int kScAccLong = -8;     // the lane fine scan, list ranked by the group kernel (transient)
+constexpr int kLaneList = 160;     // candidates the lane fine scan keeps per node (more: the coarse walk)
 
 // Object-shape and self-collision buffers.  Objects are numbered globally
 // (objBase of each body + local component id).
@@ -133,6 +140,9 @@ struct SelfColArgs {
     unsigned int* scMax2;      // [B] max |p - x|^2 bits
     int* scCounters;           // [3B] candidates, overflow, active
     int* scUseActive;          // [B]
+    int* scExact;              // [B] the cell edge is below search + 2 max move (capped or fixed):
+                               //     candidates may lie outside the 27 cells around x
+    const int* devOfCaller;    // [nPad] device node of each caller-order index
     float* scBodyDisp;         // [B]
     float* scInvCell;          // [B]
     unsigned char* scActive;   // [nPad]
@@ -155,23 +165,28 @@ struct SelfColArgs {
     int* coarseRank;           // [nPad] sorted position of each node
     const int* sortedIdx;      // [nPad] node at each sorted position (set for the coarse scan)
     // second grid on the predicted positions (cell edge >= search), used for
-    // bodies whose nodes all belong to one object
-    int fineTotal, fineBits;   // cells of all fine tables; key bits of the sort
+    // bodies whose nodes all belong to one object; ordered by a counting sort
+    // (the order inside a cell does not matter: lists are ranked by scan key)
+    int fineTotal;             // cells of all fine tables
     unsigned int* fkeys;       // [nPad] fine key of each node (fineTotal = none)
-    int* fidx;                 // [nPad]
-    unsigned int* fkeysAlt;    // [nPad] radix-sort ping-pong
-    int* fidxAlt;
-    int* fcellStart;           // [fineTotal]
-    int* fcellEnd;
-    float4* fineP;             // [nPad] fine order: p, node id bits
-    float4* fineR;             // [nPad] fine order: rest, coarse sorted position bits
-    int4* fineC;               // [nPad] fine order: coarse cell of x, coarse key
-    int2* accList;             // [accCap] pool of sorted accepted-candidate lists (code, priority bits)
-    int* accOff;               // [nPad] start of a node's list in the pool
-    int* accCount;             // [nPad] list length, -1 when the build kernel filled the slots itself
-    int* accUsed;              // [1] pool entries taken this build
-    long long accCap;
-    int fineWaveAll;           // the wave fine scan takes every node (no grouped scan ran before it)
+    int* frank;                // [nPad] rank of the node inside its fine cell
+    int* fcount;               // [fineTotal] nodes per fine cell
+    int* fstart;               // [fineTotal + 1] first fine position of each cell (exclusive prefix sum)
+    int* fscan;                // block sums of the prefix sum
+    int* finePos;              // [nPad] fine position of each node, -1 when not in the grid
+    float4* fineE;             // [2 nPad] fine order, two float4 per entry: (p.x, p.y, p.z, rest.x),
+                               //   (rest.y, rest.z, caller-order position bits: the coarse sort's order
+                               //   among equal keys, coarse cell of x packed: low 11/11/10 bits of x/y/z);
+                               //   one 32 B entry per candidate test
+    int4* fineC;               // [nPad] fine order: coarse cell of x, node id
+    // lists of the lane fine scan, slot-major per wave of 64 fine positions
+    // (entry k of position t at ((t / 64) kLaneList + k) 64 + t % 64)
+    int* laneN;                // [nPad] fine order: candidates of the lane scan, -1 handed over
+    int2* laneKey;             // [nPad * kLaneList] (scan key, squared distance bits) of each candidate,
+                               //   unsorted; (node id, priority bits) in rank order for lists that take
+                               //   the slot rule
+    int* accCount;             // [nPad] per node: what its candidate search needs (kScLane, kScNeedCoarse,
+                               //        kScRule, or 0: nothing to scan)
 };
 
 void launch_object_shape(hipStream_t s, const NodeArgs& na, const SelfColArgs& a, int K, float4* P, const float3* rest,
@@ -190,6 +205,10 @@ int radix_sort_tiles(int n);
 size_t radix_sort_hist_size(int n);   // entries of the sort's histogram buffer
 bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0, unsigned int* k1, int* v1,
                        unsigned int* hist, bool iota);
+// Exclusive prefix sum of in[0, n) into out[0, n] (out[n] = total); tmp holds
+// exclusive_scan_tmp_size(n) ints.
+size_t exclusive_scan_tmp_size(int n);
+void launch_exclusive_scan(hipStream_t s, int n, const int* in, int* out, int* tmp);
 void launch_fill_double(hipStream_t s, int n, double* a);
 
 // Shape-matching node clusters (K clusters; members in increasing caller

@@ -183,6 +183,14 @@ __global__ void __launch_bounds__(256) obj_match_kernel(int nPad, SelfColArgs a,
 // ---------------------------------------------------------------------------
 // Self-collision build
 // ---------------------------------------------------------------------------
+// accCount of a node of a fine-scan body: what its candidate search needs
+// (0: nothing, the node is not in the grid)
+constexpr int kScNeedCoarse = kScAccCoarse;  // the node takes the coarse walk
+constexpr int kScLane = kScAccLane;          // the lane fine scan lists its candidates
+constexpr int kScRule = kScAccRule;          // more than 32 candidates: the slot-rule kernel fills its slots
+                                             // from the ranked list at list entry 0
+constexpr int kScRuleHi = kScAccRuleHi;      // the same, ranked list at list entry kLaneList / 2
+constexpr int kScLong = kScAccLong;          // a list too long to rank in one lane: the group ranking kernel
 __device__ __forceinline__ unsigned int sc_hash(int cx, int cy, int cz, unsigned int mask)
 {
     return (((unsigned int)cx * 73856093u) ^ ((unsigned int)cy * 19349663u) ^ ((unsigned int)cz * 83492791u)) & mask;
@@ -210,7 +218,6 @@ __global__ void sc_prepare_kernel(int B, SelfColArgs a)
     const BodyDev& D = a.bodies[b];
     a.scMax2[b] = 0u;
     a.scCounters[3 * b] = 0; a.scCounters[3 * b + 1] = 0; a.scCounters[3 * b + 2] = 0;
-    if (b == 0 && a.accUsed) *a.accUsed = 0;   // the deferred-list pool starts empty
     float maxDisp = 0.0f;
     for (int k = 0; k < D.objK; ++k) maxDisp = fmaxf(maxDisp, a.objDisp[D.objBase + k]);
     a.scUseActive[b] = (D.scOn && D.objActiveAllowed && isfinite(maxDisp)) ? 1 : 0;
@@ -300,11 +307,13 @@ __global__ void sc_reach_kernel(int B, SelfColArgs a)
     const BodyDev& D = a.bodies[b];
     const float maxDisp2 = __uint_as_float(a.scMax2[b]);
     const float maxDisp = (isfinite(maxDisp2) && maxDisp2 > 0.0f) ? sqrtf(maxDisp2) : 0.0f;
-    float reach = D.scSearch + 2.0f * maxDisp;
+    const float need = D.scSearch + 2.0f * maxDisp;
+    float reach = need;
     const float cap = D.scSearch + 16.0f * D.scDc;
     if (D.scCellOverride > 0.0f) reach = D.scCellOverride;
     else if (reach > cap) reach = cap;
     a.scInvCell[b] = 1.0f / reach;
+    a.scExact[b] = reach < need ? 1 : 0;
 }
 
 // Keys of the frame-start positions (forward.cu:7034-7044) in the body's table,
@@ -391,9 +400,14 @@ __device__ __forceinline__ bool sc_stencil_shares_bucket(const float3& x, float 
 }
 
 // Keys of the predicted positions in the fine grid (bodies with scFine): the
-// nodes that have a coarse key (active, finite x) and a finite p.
+// nodes that have a coarse key (active, finite x) and a finite p.  Each
+// node's rank inside its cell comes from the cell's counter (the order
+// inside a cell does not matter).  laneScan: the grid nodes take the lane
+// fine scan, else the coarse walk; so do nodes whose candidates may lie
+// outside the 27 coarse cells around x (cell edge capped or fixed below search
+// + 2 max move).
 __global__ void __launch_bounds__(256) sc_fine_keys_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
-                                                          const float4* __restrict__ P)
+                                                          const float4* __restrict__ P, int laneScan)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nPad) return;
@@ -405,20 +419,18 @@ __global__ void __launch_bounds__(256) sc_fine_keys_kernel(int nPad, SelfColArgs
         const float4 p = P[i];
         if (isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p.x) && isfinite(p.y) && isfinite(p.z)) {
             const float g = B.scFineInv;
-            key = (unsigned int)B.scFineBase +
-                  sc_fine_key(sc_cell(p.x, g), sc_cell(p.y, g), sc_cell(p.z, g), (unsigned int)B.scFineMask);
+            const int gx = sc_cell(p.x, g), gy = sc_cell(p.y, g), gz = sc_cell(p.z, g);
+            const unsigned int fmask = (unsigned int)B.scFineMask;
+            key = (unsigned int)B.scFineBase + sc_fine_key(gx, gy, gz, fmask);
             // bit 1 of the active flag: two of the 27 coarse stencil cells
-            // around x share a bucket (the fine scans then look up the first
+            // around x share a bucket (the scan then looks up the first
             // offset of each candidate's bucket)
             int flags = 1;
             if (sc_stencil_shares_bucket(x, a.scInvCell[b], (unsigned int)B.scTableMask)) flags |= 2;
-            // bit 2: the nine row stretches of the 27 fine cells around p
-            // (keys k_r - 1 .. k_r + 1 around the middle-cell key k_r) cannot
-            // be walked as stretches, because one wraps around the table or two
-            // come within two keys of each other; the fine scans then walk each
-            // distinct cell once
-            const int gx = sc_cell(p.x, g), gy = sc_cell(p.y, g), gz = sc_cell(p.z, g);
-            const unsigned int fmask = (unsigned int)B.scFineMask;
+            // bit 2: the nine row stretches of the 27 cells around p (keys k_r
+            // - 1 .. k_r + 1 around the middle-cell key k_r) cannot be walked
+            // as stretches when one wraps around the table or two come within
+            // two keys of each other: the lane scan then walks cell by cell
             unsigned int kr[9];
             bool perCell = false;
 #pragma unroll
@@ -431,44 +443,42 @@ __global__ void __launch_bounds__(256) sc_fine_keys_kernel(int nPad, SelfColArgs
 #pragma unroll
                 for (int r2 = 0; r2 < r; ++r2) perCell |= (kr[r] > kr[r2] ? kr[r] - kr[r2] : kr[r2] - kr[r]) <= 2u;
             if (perCell) flags |= 4;
-            if (flags != 1) a.scActive[i] = flags;
+            if (flags != 1) a.scActive[i] = (unsigned char)flags;
         }
     }
     a.fkeys[i] = key;
-    a.fidx[i] = i;
+    if (key < (unsigned int)a.fineTotal) a.frank[i] = atomicAdd(&a.fcount[key], 1);
+    if (B.scOn && B.scFine && i < B.nodeEnd)
+        a.accCount[i] = key < (unsigned int)a.fineTotal ? ((laneScan && !a.scExact[b]) ? kScLane : kScNeedCoarse) : 0;
 }
 
-__global__ void __launch_bounds__(256) sc_fine_range_kernel(int nPad, SelfColArgs a,
-                                                           const unsigned int* __restrict__ keysSorted)
+// Fine-order copies for the fine scans, written at each node's place in the
+// counting sort (cell start + rank): (p, rest, caller-order position, coarse
+// cell of x packed) and (coarse cell of x, node id); finePos maps node to
+// place.
+__global__ void __launch_bounds__(256) sc_fine_scatter_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
+                                                             const float4* __restrict__ P,
+                                                             const float3* __restrict__ rest)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nPad) return;
-    const unsigned int k = keysSorted[i];
-    if (k >= (unsigned int)a.fineTotal) return;
-    if (i == 0 || keysSorted[i - 1] != k) a.fcellStart[k] = i;
-    if (i == nPad - 1 || keysSorted[i + 1] != k) a.fcellEnd[k] = i + 1;
-}
-
-// Fine-order copies for the fine scans: (p, node id), (rest, coarse sorted
-// position: the candidate's place in the coarse scan) and (coarse cell of x,
-// coarse key).
-__global__ void __launch_bounds__(256) sc_fine_stage_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
-                                                           const float4* __restrict__ P,
-                                                           const float3* __restrict__ rest,
-                                                           const unsigned int* __restrict__ keysSorted,
-                                                           const int* __restrict__ sortedIdx,
-                                                           const unsigned int* __restrict__ coarseSorted)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nPad || keysSorted[t] >= (unsigned int)a.fineTotal) return;
-    const int j = sortedIdx[t];
-    const float4 p = P[j];
-    const float3 r = rest[j], x = X[j];
-    const int rk = a.coarseRank[j];
-    const float inv = a.scInvCell[a.tileBody[j / kTile]];
-    a.fineP[t] = make_float4(p.x, p.y, p.z, __int_as_float(j));
-    a.fineR[t] = make_float4(r.x, r.y, r.z, __int_as_float(rk));
-    a.fineC[t] = make_int4(sc_cell(x.x, inv), sc_cell(x.y, inv), sc_cell(x.z, inv), (int)coarseSorted[rk]);   // j's coarse key
+    const unsigned int key = a.fkeys[i];
+    if (key >= (unsigned int)a.fineTotal) {
+        a.finePos[i] = -1;
+        return;
+    }
+    const int t = a.fstart[key] + a.frank[i];
+    const int b = a.tileBody[i / kTile];
+    const float4 p = P[i];
+    const float3 x = X[i], r = rest[i];
+    const float inv = a.scInvCell[b];
+    const int cx = sc_cell(x.x, inv), cy = sc_cell(x.y, inv), cz = sc_cell(x.z, inv);
+    a.fineE[2 * (size_t)t] = make_float4(p.x, p.y, p.z, r.x);
+    const unsigned int packed = ((unsigned int)cx & 2047u) | (((unsigned int)cy & 2047u) << 11) |
+                                (((unsigned int)cz & 1023u) << 22);
+    a.fineE[2 * (size_t)t + 1] = make_float4(r.y, r.z, __int_as_float(a.callerPos[i]), __uint_as_float(packed));
+    a.fineC[t] = make_int4(cx, cy, cz, i);
+    a.finePos[i] = t;
 }
 
 // Distance from x to the stencil cell at offset o along one axis, given the
@@ -569,11 +579,6 @@ __device__ __forceinline__ int sc_run_pos(int v, int rs, int pre)
     return __shfl(rs, c0, 64) + (v - __shfl(pre, c0, 64));
 }
 
-constexpr int kFineList = 128;   // accepted candidates a wave sorts in LDS (a multiple of 64)
-constexpr int kAccChunk = 1024;  // list-pool entries a wave reserves at once (>= kFineList)
-constexpr int kScNeedCoarse = -2;  // list length: the node takes the coarse scan
-constexpr int kScNeedWave = -3;    // list length: the node takes the wave fine scan
-constexpr int kScDone = -4;        // list length: the group fine scan wrote the slots
 
 // Block order for the candidate kernels.  Blocks go to the XCDs round-robin.
 // xcd > 1: each XCD takes runs of xcd consecutive blocks (nodes in Morton
@@ -591,484 +596,122 @@ __device__ __forceinline__ int sc_block(int xcd)
     return blk;
 }
 
-// Fine scan by lane groups: G lanes per node, 64 / G consecutive nodes per
-// wave, for bodies whose nodes all belong to one object with self-contact
-// inside it on (the only test there is |p_i - p_j| < search).  A group walks
-// the 27 cells (edge >= search) around p_i in the predicted-position grid as
-// nine row stretches of the sorted order, G consecutive entries per step (so
-// one load of a group touches one or two cache lines), and appends the
-// scan-order keys of its accepted candidates (first stencil offset of the
-// coarse bucket << 27 | coarse sorted position, as in the wave scan below) to
-// the node's LDS list.  The keys are then ranked: with at most 32 the ranks
-// are the reference's slot order and the group writes the slots itself; with
-// up to kGroupList the ranked list (code, squared distance) goes to the list
-// pool for the slot kernel, which replays the reference's replacement order.
-// Longer lists, crowded cells and rows that wrap around the table or overlap
-// go to the wave fine scan.
-constexpr int kGroupList = 64;               // accepted candidates a group keeps
-constexpr int kGroupStride = kGroupList + 4;  // LDS row of one node: rows of the 8 groups of a wave start on distinct banks
-constexpr int kGroupScan = 2048;             // most grid entries a group walks
-
-template <int G>
-__global__ void __launch_bounds__(256) sc_fine_group_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
-                                                           const float4* __restrict__ P,
-                                                           const float3* __restrict__ rest,
-                                                           const int* __restrict__ sortedIdx)
+// Entry k of the lane fine scan's list of fine position t (slot-major per
+// wave of 64 positions: a step of the 64 lanes is one 256 B row).
+__device__ __forceinline__ size_t sc_lane_at(int t, int k)
 {
-    constexpr int NPW = 64 / G;
-    __shared__ __align__(16) unsigned int gkey[4 * NPW][kGroupStride];
-    const int lane = threadIdx.x & 63;
-    const int sub = lane % G, grp = lane / G;
-    const int wave = (xcd_block() * (int)blockDim.x + (int)threadIdx.x) >> 6;
-    const int i0 = wave * NPW;
-    if (i0 >= nPad) return;   // whole waves
-    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i0 / kTile]);
-    const BodyDev& B = a.bodies[b];
-    if (!B.scOn || !B.scFine) return;
-    const int i = i0 + grp;
-    const bool node = i < B.nodeEnd;
-    unsigned int* __restrict__ list = gkey[(threadIdx.x >> 6) * NPW + grp];
-    int n = 0;
-    bool over = false, scanned = false;
-    float3 x = make_float3(0.0f, 0.0f, 0.0f);
-    float4 p4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (node) {
-        x = X[i];
-        p4 = P[i];
-        scanned = a.scActive[i] && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p4.x) &&
-                  isfinite(p4.y) && isfinite(p4.z);
-    }
-    if (scanned) {
-        const float3 q = rest[i];
-        const float search = B.scSearch, search2 = search * search;
-        const float excl2 = B.scExcl2;
-        const float inv = a.scInvCell[b];
-        const unsigned int mask = (unsigned int)B.scTableMask;
-        const int cx = sc_cell(x.x, inv), cy = sc_cell(x.y, inv), cz = sc_cell(x.z, inv);
-        // two stencil offsets whose coarse cells share a bucket: the bucket is
-        // scanned at the first of them (flag set by the fine-keys pass)
-        const bool anyDup = (a.scActive[i] & 2) != 0;
-        const float ginv = B.scFineInv;
-        const unsigned int fmask = (unsigned int)B.scFineMask;
-        const int* __restrict__ fs = a.fcellStart + B.scFineBase;
-        const int* __restrict__ fe = a.fcellEnd + B.scFineBase;
-        const int gx = sc_cell(p4.x, ginv), gy = sc_cell(p4.y, ginv), gz = sc_cell(p4.z, ginv);
-        // middle-cell key of row r = (oy + 1) + 3 (oz + 1); its cells are
-        // keys kr - 1, kr, kr + 1, adjacent in the sorted order unless kr is
-        // at either end of the table
-        // (stretches that wrap or overlap: flagged by the fine-keys pass)
-        unsigned int kr[9];
-        const bool perCell = (a.scActive[i] & 4) != 0;
-#pragma unroll
-        for (int r = 0; r < 9; ++r) kr[r] = sc_fine_key(gx, gy + r % 3 - 1, gz + r / 3 - 1, fmask);
-        // the nine stretches as one run: run entry v sits at v + rowD[r] for
-        // the last row r with rowP[r] <= v (empty rows are overridden)
-        int rowP[9], rowD[9];
-        int total = 0;
-        if (!perCell) {
-#pragma unroll
-            for (int r = 0; r < 9; ++r) {
-                const unsigned int k = kr[r];
-                const int s0 = fs[k - 1], s1 = fs[k], s2 = fs[k + 1];
-                const int e0 = fe[k - 1], e1 = fe[k], e2 = fe[k + 1];
-                int lo = s0 >= 0 ? s0 : (s1 >= 0 ? s1 : s2);
-                int hi = s2 >= 0 ? e2 : (s1 >= 0 ? e1 : (s0 >= 0 ? e0 : -1));
-                if (lo < 0) { lo = 0; hi = 0; }
-                rowP[r] = total;
-                rowD[r] = lo - total;
-                total += hi - lo;
-            }
-        }
-        over = perCell || total > kGroupScan;
-        // run entry v -> sorted position
-        auto runPos = [&](int v) {
-            int d = rowD[0];
-#pragma unroll
-            for (int r = 1; r < 9; ++r)
-                if (v >= rowP[r]) d = rowD[r];
-            return v + d;
-        };
-        // scan-order key of an entry that passed the distance and rest tests,
-        // or false when its bucket is not one of the stencil's
-        auto keyOf = [&](int t, const float4& qj, unsigned int& key) {
-            const int4 cj = a.fineC[t];   // only for accepted entries
-            const int dx = cj.x - cx, dy = cj.y - cy, dz = cj.z - cz;
-            int off = -1;
-            if (((unsigned int)(dx + 1) <= 2u) & ((unsigned int)(dy + 1) <= 2u) & ((unsigned int)(dz + 1) <= 2u)) {
-                off = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-                if (anyDup) {
-                    const unsigned int hc = sc_hash(cx + dx, cy + dy, cz + dz, mask);
-                    for (int c2 = 0; c2 < off; ++c2)
-                        if (sc_hash(cx + c2 % 3 - 1, cy + (c2 / 3) % 3 - 1, cz + c2 / 9 - 1, mask) == hc) {
-                            off = c2;
-                            break;
-                        }
-                }
-            } else {
-                // outside the 27 cells (capped or fixed cell): a candidate
-                // only through a bucket shared with a stencil cell
-                const unsigned int ck = (unsigned int)cj.w - (unsigned int)B.scTableBase;
-                for (int c2 = 0; c2 < 27; ++c2)
-                    if (sc_hash(cx + c2 % 3 - 1, cy + (c2 / 3) % 3 - 1, cz + c2 / 9 - 1, mask) == ck) {
-                        off = c2;
-                        break;
-                    }
-            }
-            key = ((unsigned int)(off < 0 ? 0 : off) << 27) | (unsigned int)__float_as_int(qj.w);
-            return off >= 0;
-        };
-        auto passes = [&](const float4& pj, const float4& qj) {
-            const float ex = p4.x - pj.x, ey = p4.y - pj.y, ez = p4.z - pj.z;
-            const float d2 = ex * ex + ey * ey + ez * ez;
-            const float qx = q.x - qj.x, qy = q.y - qj.y, qz = q.z - qj.z;
-            const float q2 = qx * qx + qy * qy + qz * qz;
-            // bitwise: no branch for the compiler to sink the loads into
-            return (d2 < search2) & (__float_as_int(pj.w) != i) & (q2 >= excl2);
-        };
-        // append in lane order (the order is restored by the ranking)
-        const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
-        auto append = [&](bool acc, unsigned int key) {
-            const unsigned long long gm = (__ballot(acc) >> (grp * G)) & gmask;
-            const int slot = n + __popcll(gm & ((1ull << sub) - 1ull));
-            if (acc && slot < kGroupList) list[slot] = key;
-            n += __popcll(gm);
-        };
-        // 2 G entries per step, both loads of both issued together
-        for (int base = 0; base < total && !over; base += 2 * G) {
-            const int v0 = base + sub, v1 = v0 + G;
-            const bool in0 = v0 < total, in1 = v1 < total;
-            const int t0 = runPos(in0 ? v0 : 0), t1 = runPos(in1 ? v1 : 0);
-            const float4 pa = a.fineP[t0], ra = a.fineR[t0];
-            const float4 pb = a.fineP[t1], rb = a.fineR[t1];
-            unsigned int k0 = 0u, k1 = 0u;
-            bool acc0 = in0 & passes(pa, ra);
-            bool acc1 = in1 & passes(pb, rb);
-            if (acc0) acc0 = keyOf(t0, ra, k0);
-            if (acc1) acc1 = keyOf(t1, rb, k1);
-            append(acc0, k0);
-            append(acc1, k1);
-            over = n > kGroupList;
-        }
-    }
-    // pad the node's keys to whole uint4 reads with keys above all
-    const int n4 = (n + 3) & ~3;
-    if (scanned && !over && sub < n4 - n) list[n + sub] = 0xffffffffu;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    // the contact push of object shape matching needs to know whether the body
-    // has any candidate (every lane of the wave is active here)
-    const bool anyCand = __ballot(node && !over && n > 0) != 0ull;
-    if (lane == 0 && anyCand && B.objOn) a.scCounters[3 * b] = 1;
-    // pool space for the lists longer than 32: one reservation per wave, each
-    // group's share from a prefix sum of the groups' lengths
-    const bool pool = node && !over && n > kSelfColCap;
-    const int need = (pool && sub == 0) ? n : 0;
-    int incl = need;
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-        const int o = __shfl_up(incl, m, 64);
-        if (lane >= m) incl += o;
-    }
-    const int waveNeed = __shfl(incl, 63, 64);
-    int off = 0;
-    if (waveNeed > 0) {
-        int base = 0;
-        if (lane == 0) base = atomicAdd(a.accUsed, waveNeed);
-        base = __shfl(base, 0, 64);
-        off = __shfl(base + incl - need, grp * G, 64);   // the group leader's exclusive prefix
-    }
-    if (!node) return;
-    if (over) {
-        if (sub == 0) a.accCount[i] = kScNeedWave;
-        return;
-    }
-    if (pool && (long long)off + n > a.accCap) {
-        if (sub == 0) a.accCount[i] = kScNeedCoarse;   // the pool is full
-        return;
-    }
-    // rank of each key (all distinct): its place in the scan order
-    for (int e = sub; e < n; e += G) {
-        const unsigned int ke = list[e];
-        int rank = 0;
-        for (int m = 0; m < n4; m += 4) {
-            const uint4 km = *reinterpret_cast<const uint4*>(&list[m]);
-            rank += (km.x < ke ? 1 : 0) + (km.y < ke ? 1 : 0) + (km.z < ke ? 1 : 0) + (km.w < ke ? 1 : 0);
-        }
-        const int j = sortedIdx[ke & 0x07ffffffu];
-        if (!pool) {
-            a.contacts[sc_slot(i, rank)] = j;
-        } else {
-            // the slot kernel's priority: the distance the test used
-            const float4 pj = P[j];
-            const float ex = p4.x - pj.x, ey = p4.y - pj.y, ez = p4.z - pj.z;
-            const float d2 = ex * ex + ey * ey + ez * ez;
-            a.accList[(size_t)off + rank] = make_int2(j, __float_as_int(d2));
-        }
-    }
-    if (sub == 0) {
-        if (pool) {
-            a.accOff[i] = off;
-            a.accCount[i] = n;
-        } else {
-            a.counts[i] = n;
-            a.overflow[i] = 0;
-            a.accCount[i] = kScDone;
-        }
-    }
+    return ((size_t)(t >> 6) * kLaneList + (size_t)k) * 64 + (size_t)(t & 63);
 }
 
-// Fine scan, one wave64 per node at a time (NPW consecutive nodes per wave),
-// for bodies whose nodes all belong to one object with self-contact inside it
-// on, walking only the nodes the group fine scan handed over.  There the only
-// test is |p_i - p_j| < search, so every candidate lies in
-// the 27 cells of edge >= search around p_i in a second grid built on the
-// predicted positions.  Those few cells (nine rows of three cells, each row
-// one stretch of the sorted order) are tested instead of the coarse stencil;
-// a passing j is a reference candidate when its coarse bucket is one of the
-// stencil's buckets, and it takes the scan position (first stencil offset of
-// that bucket, sorted position inside it), read off its coarse cell when that
-// is one of the 27 around x_i.  The accepted ones are ranked by that position
-// (a 32-bit key) in LDS and written in that order to the list pool; the slot
-// kernel fills the 32 slots from there, so the lists and counters equal the
-// coarse scan's.  A node with more than kFineList accepted candidates, or
-// that finds the pool full, is handed to the coarse kernel.
-template <int NPW>
-__global__ void __launch_bounds__(256) sc_fine_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
-                                                     const float4* __restrict__ P, const float3* __restrict__ rest,
-                                                     int xcd)
+// Fine scan, one lane per node, in the predicted-position grid's own order,
+// for bodies whose nodes all belong to one object with self-contact inside
+// it on.  There the only tests are |p_i - p_j| < search and
+// |q_i - q_j|^2 >= excl^2, so every candidate lies in the 27 cells (edge >=
+// search) around p_i: nine rows of three cells, each row one stretch
+// [S[k - 1], S[k + 2]) of the counting sort (k the key of the row's middle
+// cell).  Consecutive lanes hold nodes of one row, so their stretches
+// overlap and their loads share cache lines; each lane keeps kLaneU entries
+// (32 B: p, rest, caller-order position, packed coarse cell) in flight.  An
+// accepted candidate is keyed at once: its coarse bucket's first stencil
+// offset << 27 | its caller-order position.  A candidate within search lies
+// within search + 2 max move of x_i in x, so its coarse cell differs from
+// x_i's by at most one per axis and the low bits of the packed cell give the
+// offset exactly (when two stencil cells share a bucket, the ranking kernel
+// first moves each key to the first offset holding its bucket); nodes
+// for which that does not hold (cell edge capped or fixed below that bound)
+// take the coarse walk, as do nodes with a candidate two coarse cells away
+// (possible within rounding of the bound).  Nodes whose row
+// stretches wrap around the table or meet in the key space walk their 27
+// cells one by one instead, counting each entry only in its own cell.  The
+// (key, squared distance) pairs go unsorted to the lane's list; lists past
+// kLaneList go to the coarse walk.
+constexpr int kLaneU = 4;   // entries in flight per lane
+
+// Scan key of an accepted candidate of node i: the first stencil offset of
+// its coarse bucket << 27 | its caller-order position cp.  The candidate's
+// coarse cell differs from x_i's by at most one per axis (the node takes
+// this scan only when the cell edge covers search + 2 max move), so the low
+// bits of the packed cells (pi, pj) give the offset.  ~0 when the cells differ by more (possible only
+// within rounding of that bound): the node then takes the coarse walk,
+// which decides such a candidate by its bucket.
+__device__ __forceinline__ unsigned int sc_lane_key(unsigned int pi, unsigned int pj, unsigned int cp)
 {
-    __shared__ __align__(16) unsigned int lkey[4][kFineList];   // accepted: scan-order key
-    __shared__ int2 lpay[4][kFineList];                           // accepted: node id, squared distance
-    const int wave = (sc_block(xcd) * (int)blockDim.x + (int)threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    const int wl = threadIdx.x >> 6;
-    const int i0 = wave * NPW;
-    if (i0 >= nPad) return;   // whole waves
-    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i0 / kTile]);
+    const unsigned int ux = (pj - pi + 1u) & 2047u, uy = ((pj >> 11) - (pi >> 11) + 1u) & 2047u,
+                       uz = ((pj >> 22) - (pi >> 22) + 1u) & 1023u;
+    return (ux <= 2u && uy <= 2u && uz <= 2u) ? (((ux + 3u * uy + 9u * uz) << 27) | cp) : 0xffffffffu;
+}
+
+__global__ void __launch_bounds__(256) sc_fine_lane_kernel(SelfColArgs a, const float4* __restrict__ E,
+                                                          const int4* __restrict__ FC, const int* __restrict__ FS,
+                                                          int2* __restrict__ LK)
+{
+    const int t = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+    if (t >= FS[a.fineTotal]) return;   // fine positions in use
+    const int i = FC[t].w;
+    const float4 m0 = E[2 * (size_t)t], m1 = E[2 * (size_t)t + 1];
+    if (a.accCount[i] != kScLane) return;   // flagged for the coarse walk
+    const int b = a.tileBody[i / kTile];
     const BodyDev& B = a.bodies[b];
-    if (!B.scOn || !B.scFine) return;
-    if (!a.fineWaveAll &&
-        __ballot(lane < NPW && i0 + lane < B.nodeEnd && a.accCount[i0 + lane] == kScNeedWave) == 0ull)
-        return;
-    int chunkPos = 0, chunkEnd = 0;   // the wave's share of the list pool
-    bool anyDirect = false;           // a list written to the slots here
-    for (int ii = 0; ii < NPW; ++ii) {
-        const int i = i0 + ii;
-        if (i >= B.nodeEnd) break;
-        if (!a.fineWaveAll && a.accCount[i] != kScNeedWave) continue;
-        const float3 x = X[i];
-        const float4 p4 = P[i];
-        const int act = a.scActive[i];
-        const float3 q = rest[i];
-        int deferred = 0;   // list length in the pool, or kScNeedCoarse
-        if (act && isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(p4.x) && isfinite(p4.y) &&
-            isfinite(p4.z)) {
-            const float search = B.scSearch, search2 = search * search;
-            const float inv = a.scInvCell[b];
-            const int cx = sc_cell(x.x, inv), cy = sc_cell(x.y, inv), cz = sc_cell(x.z, inv);
-            // lane c < 27: stencil offset c and its (global) coarse bucket
-            const int c = lane < 27 ? lane : 26;
-            const int ox = c % 3 - 1, oy = (c / 3) % 3 - 1, oz = c / 9 - 1;
-            const unsigned int hg =
-                (unsigned int)B.scTableBase + sc_hash(cx + ox, cy + oy, cz + oz, (unsigned int)B.scTableMask);
-            const float ginv = B.scFineInv;
-            const unsigned int fmask = (unsigned int)B.scFineMask;
-            const int fbase = B.scFineBase;
-            const float excl2 = B.scExcl2;
-            const int gx = sc_cell(p4.x, ginv), gy = sc_cell(p4.y, ginv), gz = sc_cell(p4.z, ginv);
-            // lane c < 27: fine cell (gx + ox, gy + oy, gz + oz) and its range
-            const unsigned int fk = sc_fine_key(gx + ox, gy + oy, gz + oz, fmask);
-            // start and end fetched together (an end is only used with its
-            // start, which marks the cells that are occupied this build)
-            int cs = -1, ce = 0;
-            if (lane < 27) {
-                cs = a.fcellStart[fbase + fk];
-                ce = a.fcellEnd[fbase + fk];
-                if (cs < 0) ce = 0;
-            }
-            // first stencil offset holding each coarse bucket (two offsets
-            // share one when their cells collide in the hash)
-            // (flagged per node by the fine-keys pass; rare)
-            const bool dup = (act & 2) != 0;
-            int firstC = c;
-            if (dup)
-                for (int c2 = 26; c2 >= 0; --c2)
-                    if ((unsigned int)lane_i((int)hg, c2) == hg) firstC = c2;
-            // Row r = c / 3 (one (oy, oz)) covers keys k_r - 1 .. k_r + 1 (k_r
-            // the key of its middle cell), one contiguous stretch of the
-            // sorted order.  Rows whose stretches wrap around the table or
-            // come within two keys of each other would visit a key twice:
-            // then each distinct cell key is walked once instead (flagged by
-            // the fine-keys pass).
-            const bool perCell = (act & 4) != 0;
-            // row r: first run position rowP (past every run position when
-            // the row is empty) and sorted index minus run position rowD
-            int rowP[9], rowD[9];
-            int rs = 0, pre = 0, total = 0;
-            if (!perCell) {
-                // the non-empty cells of a row are adjacent in the sorted order
-                const int lo = cs >= 0 ? cs : 0x7fffffff, hi = cs >= 0 ? ce : 0;
-                const int lo1 = __shfl_down(lo, 1, 64), lo2 = __shfl_down(lo, 2, 64);
-                const int hi1 = __shfl_down(hi, 1, 64), hi2 = __shfl_down(hi, 2, 64);
-                const int rlo = min(lo, min(lo1, lo2)), rhi = max(hi, max(hi1, hi2));
+    const float search2 = B.scSearch * B.scSearch, excl2 = B.scExcl2;
+    const float g = B.scFineInv;
+    const unsigned int fmask = (unsigned int)B.scFineMask;
+    const int* __restrict__ S = FS + B.scFineBase;
+    const int gx = sc_cell(m0.x, g), gy = sc_cell(m0.y, g), gz = sc_cell(m0.z, g);
+    // rows whose stretches wrap or meet in the key space: walk the 27 cells
+    // one by one, each entry counted only in its own cell
+    const bool percell = (a.scActive[i] & 4) != 0;
+    const unsigned int pi = __float_as_uint(m1.w);
+    const size_t L0 = sc_lane_at(t, 0);   // entry k at L0 + 64 k
+    int n = 0;
+    bool far = false;   // a candidate two coarse cells away (rounding): the coarse walk decides
+#pragma unroll 1
+    for (int w = 0; w < (percell ? 27 : 9); ++w) {
+        int lo, h, cx = 0, cy = 0, cz = 0;
+        if (!percell) {
+            const unsigned int k = sc_fine_key(gx, gy + w % 3 - 1, gz + w / 3 - 1, fmask);
+            lo = S[k - 1];
+            h = S[k + 2];
+        } else {
+            cx = gx + w % 3 - 1;
+            cy = gy + (w / 3) % 3 - 1;
+            cz = gz + w / 9 - 1;
+            const unsigned int k = sc_fine_key(cx, cy, cz, fmask);
+            lo = S[k];
+            h = S[k + 1];
+        }
+        for (int e0 = lo; e0 < h; e0 += kLaneU) {
+            float4 A[kLaneU], C[kLaneU];
 #pragma unroll
-                for (int r = 0; r < 9; ++r) {
-                    const int l = lane_i(rlo, 3 * r), h2 = lane_i(rhi, 3 * r);
-                    const int n = h2 > l ? h2 - l : 0;
-                    rowP[r] = n > 0 ? total : 0x7fffffff;
-                    rowD[r] = l - total;
-                    total += n;
+            for (int u = 0; u < kLaneU; ++u)
+                if (e0 + u < h) {
+                    A[u] = E[2 * (size_t)(e0 + u)];
+                    C[u] = E[2 * (size_t)(e0 + u) + 1];
                 }
-            } else {
-                bool ffirst = lane < 27;
-                for (int c2 = 0; c2 < 27; ++c2)
-                    if ((unsigned int)lane_i((int)fk, c2) == fk && c2 < c) ffirst = false;
-                int rn = 0;
-                if (ffirst && cs >= 0) {
-                    rs = cs;
-                    rn = ce - cs;
-                }
-                total = sc_run_prefix(rn, lane, pre);
-            }
-            int nacc = 0;
-            auto runPos = [&](int v) {
-                if (perCell) return sc_run_pos(v, rs, pre);
-                int d = 0;
 #pragma unroll
-                for (int r = 0; r < 9; ++r)
-                    if (v >= rowP[r]) d = rowD[r];
-                return v + d;
-            };
-            // test one run entry per lane and append the accepted ones in
-            // lane order
-            auto scanEntry = [&](int v, const float4& pj, const float4& qj, const int4& cj) {
-                bool acc = false, inside = true;
-                unsigned int key = 0u, ck = 0u;
-                int j = 0, rk = 0;
-                float d2 = 0.0f;
-                if (v < total) {
-                    j = __float_as_int(pj.w);
-                    ck = (unsigned int)cj.w;
-                    rk = __float_as_int(qj.w);
-                    const float ex = p4.x - pj.x, ey = p4.y - pj.y, ez = p4.z - pj.z;
-                    d2 = ex * ex + ey * ey + ez * ez;
-                    const float qx = q.x - qj.x, qy = q.y - qj.y, qz = q.z - qj.z;
-                    const float q2 = qx * qx + qy * qy + qz * qz;
-                    acc = (j != i) & (d2 < search2) & (q2 >= excl2);
-                    // scan position: a candidate whose coarse cell is one of
-                    // the 27 around x_i sits in that offset's bucket
-                    const int dx = cj.x - cx, dy = cj.y - cy, dz = cj.z - cz;
-                    inside = ((unsigned int)(dx + 1) <= 2u) & ((unsigned int)(dy + 1) <= 2u) &
-                             ((unsigned int)(dz + 1) <= 2u);
-                    key = ((unsigned int)((dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)) << 27) | (unsigned int)rk;
-                }
-                if (dup) {
-                    const int f = __shfl(firstC, inside ? (int)(key >> 27) : 0, 64);
-                    key = ((unsigned int)f << 27) | (unsigned int)rk;
-                }
-                // any other candidate is one only when its bucket collides
-                // with a stencil bucket: lanes 0-26 hold those, so each is
-                // matched against all of them at once
-                unsigned long long om = __ballot(acc && !inside);
-                while (om) {
-                    const int l = __builtin_ctzll(om);
-                    om &= om - 1ull;
-                    const unsigned int ckl = (unsigned int)lane_i((int)ck, l);
-                    const unsigned long long hit = __ballot(lane < 27 && hg == ckl);
-                    if (lane == l) {
-                        if (hit) key = ((unsigned int)__builtin_ctzll(hit) << 27) | (unsigned int)rk;
-                        else acc = false;
-                    }
-                }
-                const unsigned long long am = __ballot(acc);
-                const int slot = nacc + (int)__builtin_amdgcn_mbcnt_hi((unsigned int)(am >> 32),
-                                                                      __builtin_amdgcn_mbcnt_lo((unsigned int)am, 0u));
-                if (acc && slot < kFineList) {
-                    lkey[wl][slot] = key;
-                    lpay[wl][slot] = make_int2(j, __float_as_int(d2));
-                }
-                nacc += __popcll(am);
-            };
-            for (int base = 0; base < total; base += 64) {
-                const int v = base + lane;
-                const int t = runPos(v);   // every lane: the per-cell walk shuffles across the wave
-                float4 pj = make_float4(0.0f, 0.0f, 0.0f, 0.0f), qj = pj;
-                int4 cj = make_int4(0, 0, 0, 0);
-                if (v < total) {
-                    pj = a.fineP[t];
-                    qj = a.fineR[t];
-                    cj = a.fineC[t];
-                }
-                scanEntry(v, pj, qj, cj);
-            }
-            deferred = kScNeedCoarse;   // unless the ranked list is written below
-            if (nacc <= kFineList) {
-                // pad the key list to whole uint4 reads with keys above all
-                const int n4 = (nacc + 3) & ~3;
-                if (lane < n4 - nacc) lkey[wl][nacc + lane] = 0xffffffffu;
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-                // rank of each accepted entry by its scan-order key (first
-                // offset, coarse sorted position: all distinct)
-                constexpr int R = kFineList / 64;
-                unsigned int ek[R];
-                int2 ep[R];
-                int er[R];
-#pragma unroll
-                for (int k = 0; k < R; ++k) {
-                    const int s = k * 64 + lane;
-                    ek[k] = s < nacc ? lkey[wl][s] : 0xffffffffu;
-                    ep[k] = s < nacc ? lpay[wl][s] : make_int2(0, 0);
-                    er[k] = 0;
-                }
-                for (int m = 0; m < n4; m += 4) {
-                    const uint4 km = *reinterpret_cast<const uint4*>(&lkey[wl][m]);
-#pragma unroll
-                    for (int k = 0; k < R; ++k)
-                        er[k] += (km.x < ek[k] ? 1 : 0) + (km.y < ek[k] ? 1 : 0) + (km.z < ek[k] ? 1 : 0) +
-                                 (km.w < ek[k] ? 1 : 0);
-                }
-                if (nacc <= kSelfColCap) {
-                    // the whole list fits: the ranks are the slots
-#pragma unroll
-                    for (int k = 0; k < R; ++k)
-                        if (k * 64 + lane < nacc) a.contacts[sc_slot(i, er[k])] = ep[k].x;
-                    if (lane == 0) {
-                        a.counts[i] = nacc;
-                        a.overflow[i] = 0;
-                    }
-                    anyDirect |= nacc > 0;
-                    deferred = kScDone;
-                } else {
-                    // the sorted list goes to the slot kernel (one thread per
-                    // node) when the shared list pool has room; the wave takes
-                    // pool space kAccChunk entries at a time
-                    if (chunkPos + nacc > chunkEnd) {
-                        int off = 0;
-                        if (lane == 0) off = atomicAdd(a.accUsed, kAccChunk);
-                        chunkPos = __builtin_amdgcn_readfirstlane(off);
-                        chunkEnd = chunkPos + kAccChunk;
-                    }
-                    const int off = chunkPos;
-                    if ((long long)chunkEnd <= a.accCap) {
-                        chunkPos += nacc;
-#pragma unroll
-                        for (int k = 0; k < R; ++k)
-                            if (k * 64 + lane < nacc) a.accList[(size_t)off + er[k]] = ep[k];
-                        if (lane == 0) a.accOff[i] = off;
-                        deferred = nacc;
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();   // the next node reuses the wave's LDS rows
+            for (int u = 0; u < kLaneU; ++u) {
+                const int e = e0 + u;
+                if (e >= h) break;
+                const float ex = m0.x - A[u].x, ey = m0.y - A[u].y, ez = m0.z - A[u].z;
+                const float d2 = ex * ex + ey * ey + ez * ez;
+                const float qx = m0.w - A[u].w, qy = m1.x - C[u].x, qz = m1.y - C[u].y;
+                const float q2 = qx * qx + qy * qy + qz * qz;
+                if (!((int)(d2 < search2) & (int)(q2 >= excl2) & (int)(e != t))) continue;
+                if (percell && !((int)(sc_cell(A[u].x, g) == cx) & (int)(sc_cell(A[u].y, g) == cy) &
+                                 (int)(sc_cell(A[u].z, g) == cz)))
+                    continue;
+                const unsigned int key =
+                    sc_lane_key(pi, __float_as_uint(C[u].w), (unsigned int)__float_as_int(C[u].z));
+                far |= key == 0xffffffffu;
+                if (n < kLaneList) LK[L0 + (size_t)n * 64] = make_int2((int)key, __float_as_int(d2));
+                ++n;
             }
         }
-        if (lane == 0) a.accCount[i] = deferred;
     }
-    // the contact push of object shape matching needs to know whether the body
-    // has any candidate (the slot kernel marks the pooled lists)
-    if (lane == 0 && anyDirect && B.objOn) a.scCounters[3 * b] = 1;
+    if (n > kLaneList || far) {
+        a.accCount[i] = kScNeedCoarse;
+        n = -1;
+    }
+    a.laneN[t] = n;
 }
 
 // Candidate lists (forward.cu:7079-7180), coarse scan (the reference's own
@@ -1235,47 +878,43 @@ __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, 
     if (lane == 0 && anyCand && B.objOn) a.scCounters[3 * b] = 1;
 }
 
-// Slots of the nodes whose sorted candidate list the fine scan deferred, one
-// thread per node: the first 32 fill the slots in order, each later one
-// replaces the worst slot (largest priority, lowest slot among equals) when
-// strictly better, as the wave insertion does (all fine candidates are
-// same-object ones, one tier).
-__global__ void __launch_bounds__(256) sc_slots_kernel(int nPad, SelfColArgs a)
+// The reference's slot rule (forward.cu:7149-7174) over a list of n > 32
+// entries already in scan order: the first 32 fill the slots in order, each
+// later one replaces the worst slot (largest priority, lowest slot among
+// equals) when strictly better (all fine candidates are same-object ones,
+// one tier).  entry(r) gives the r-th candidate's code and priority bits;
+// entries are fetched eight at a time.
+template <typename Entry>
+__device__ __forceinline__ void sc_fill_slots(int i, int n, SelfColArgs& a, Entry entry)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nPad) return;   // nPad is a multiple of 64: whole waves
-    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
-    const BodyDev& B = a.bodies[b];
-    if (!B.scOn || !B.scFine) return;
-    const int n = i < B.nodeEnd ? a.accCount[i] : -1;
-    const int2* list = a.accList + (n > 0 ? a.accOff[i] : 0);
     constexpr int CAP = kSelfColCap;
-    const int cnt = n < CAP ? n : CAP;
-    if (n >= 0 && n <= CAP) {
-        // the whole list fits: it is the slot order
-        for (int r = 0; r < n; ++r) a.contacts[sc_slot(i, r)] = list[r].x;
-    } else if (n > CAP) {
-        int code[CAP];
-        float prio[CAP];
+    int code[CAP];
+    float prio[CAP];
 #pragma unroll
-        for (int s = 0; s < CAP; ++s) {
-            const int2 e = list[s];
-            code[s] = e.x;
-            prio[s] = sqrtf(__int_as_float(e.y));   // the list holds squared distances
-        }
-        // worst slot, kept until a slot changes
-        int w = 0;
-        float wp = prio[0];
+    for (int s = 0; s < CAP; ++s) {
+        const int2 e = entry(s);
+        code[s] = e.x;
+        prio[s] = __int_as_float(e.y);
+    }
+    // worst slot, kept until a slot changes
+    int w = 0;
+    float wp = prio[0];
 #pragma unroll
-        for (int s = 1; s < CAP; ++s)
-            if (prio[s] > wp) { wp = prio[s]; w = s; }
-        for (int r = CAP; r < n; ++r) {
-            const int2 e = list[r];
-            const float p = sqrtf(__int_as_float(e.y));
+    for (int s = 1; s < CAP; ++s)
+        if (prio[s] > wp) { wp = prio[s]; w = s; }
+    constexpr int G = 8;
+    for (int r0 = CAP; r0 < n; r0 += G) {
+        int2 ev[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) ev[g] = r0 + g < n ? entry(r0 + g) : make_int2(0, 0);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (r0 + g >= n) break;
+            const float p = __int_as_float(ev[g].y);
             if (p < wp) {
 #pragma unroll
                 for (int s = 0; s < CAP; ++s)
-                    if (s == w) { code[s] = e.x; prio[s] = p; }
+                    if (s == w) { code[s] = ev[g].x; prio[s] = p; }
                 w = 0;
                 wp = prio[0];
 #pragma unroll
@@ -1283,15 +922,237 @@ __global__ void __launch_bounds__(256) sc_slots_kernel(int nPad, SelfColArgs a)
                     if (prio[s] > wp) { wp = prio[s]; w = s; }
             }
         }
+    }
 #pragma unroll
-        for (int s = 0; s < CAP; ++s) a.contacts[sc_slot(i, s)] = code[s];
+    for (int s = 0; s < CAP; ++s) a.contacts[sc_slot(i, s)] = code[s];
+}
+
+// Two stencil cells of node i share a bucket: a candidate's bucket is scanned
+// at the first offset holding it (its cell is x_i's cell + its offset), so
+// the keys of its list move to that offset.
+__device__ __forceinline__ void sc_lane_first_offsets(int4 ci, unsigned int tmask, int m, size_t L0,
+                                                      int2* __restrict__ LK)
+{
+    // first[o] for the 27 offsets, 5 bits each, six per word
+    unsigned int hs[27];
+#pragma unroll
+    for (int c = 0; c < 27; ++c) hs[c] = sc_hash(ci.x + c % 3 - 1, ci.y + (c / 3) % 3 - 1, ci.z + c / 9 - 1, tmask);
+    unsigned int first[5] = {0u, 0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int o = 0; o < 27; ++o) {
+        unsigned int f = (unsigned int)o;
+#pragma unroll
+        for (int c = o - 1; c >= 0; --c)
+            if (hs[c] == hs[o]) f = (unsigned int)c;
+        first[o / 6] |= f << (5 * (o % 6));
     }
-    if (n >= 0) {
+    for (int k = 0; k < m; ++k) {
+        const int2 en = LK[L0 + (size_t)k * 64];
+        const unsigned int o = (unsigned int)en.x >> 27;
+        const unsigned int wsel = o / 6u;
+        const unsigned int word = wsel == 0u ? first[0] : wsel == 1u ? first[1] : wsel == 2u ? first[2]
+                                : wsel == 3u ? first[3] : first[4];
+        const unsigned int f = (word >> (5u * (o % 6u))) & 31u;
+        if (f != o) LK[L0 + (size_t)k * 64] = make_int2((int)((f << 27) | ((unsigned int)en.x & 0x07ffffffu)), en.y);
+    }
+}
+
+// Ranking of the lane fine scan's lists, one lane per list in the grid's own
+// order (the list length and entries of the 64 lanes are contiguous reads).
+// Each lane sorts its keys (all distinct; padding ~0 sorts last) with a
+// bitonic network in registers: N keys, N the wave's power of two at or above
+// its longest list (up to kRankNet), min/max steps only.  With at most 32
+// candidates the sorted keys are the slots; with more, the node ids in key
+// order go to list entries kLaneList / 2 + rank and the node is flagged for
+// the slot-rule kernel.  Longer lists are flagged for the group ranking kernel.
+constexpr int kRankNet = 64;
+static_assert(kRankNet <= kLaneList / 2, "ranked copies sit in the upper half of a list");
+
+template <int N>
+__device__ __forceinline__ void sc_bitonic(unsigned int* k)
+{
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1)
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+            for (int x = 0; x < N; ++x) {
+                const int y = x ^ stride;
+                if (y > x) {
+                    const unsigned int lo = min(k[x], k[y]), hi = max(k[x], k[y]);
+                    const bool up = (x & size) == 0;
+                    k[x] = up ? lo : hi;
+                    k[y] = up ? hi : lo;
+                }
+            }
+}
+
+// Load, sort and hand out one lane's list with an N-key network.
+template <int N>
+__device__ __forceinline__ void sc_rank_list(SelfColArgs& a, int i, int m, size_t L0, const int2* __restrict__ LK,
+                                             int2* __restrict__ LKw)
+{
+    unsigned int k[N];
+#pragma unroll
+    for (int x = 0; x < N; ++x) k[x] = x < m ? (unsigned int)LK[L0 + (size_t)x * 64].x : 0xffffffffu;
+    sc_bitonic<N>(k);
+    if (m <= kSelfColCap) {
+#pragma unroll
+        for (int x = 0; x < N && x < kSelfColCap; ++x)
+            if (x < m) a.contacts[sc_slot(i, x)] = a.devOfCaller[k[x] & 0x07ffffffu];
+    } else {
+#pragma unroll
+        for (int x = 0; x < N; ++x)
+            if (x < m) LKw[L0 + (size_t)(kLaneList / 2 + x) * 64] = make_int2(a.devOfCaller[k[x] & 0x07ffffffu], 0);
+    }
+}
+
+__global__ void __launch_bounds__(256) sc_lane_rank_kernel(SelfColArgs a, const int4* __restrict__ FC,
+                                                          const int* __restrict__ FS, int2* __restrict__ LK)
+{
+    const int t = blockIdx.x * blockDim.x + (int)threadIdx.x;
+    bool listed = false;
+    int m = 0, i = 0, b = 0;
+    if (t < FS[a.fineTotal]) {
+        i = FC[t].w;
+        b = a.tileBody[i / kTile];
+        listed = a.accCount[i] == kScLane;
+        if (listed) m = a.laneN[t];
+    }
+    const size_t L0 = sc_lane_at(t, 0);
+    if (listed && (a.scActive[i] & 2))
+        sc_lane_first_offsets(FC[t], (unsigned int)a.bodies[b].scTableMask, m, L0, LK);
+    const bool here = listed && m <= kRankNet;
+    if (listed && !here) a.accCount[i] = kScLong;
+    // the wave's network size
+    int need = here ? m : 0;
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) need = max(need, __shfl_xor(need, s, 64));
+    need = __builtin_amdgcn_readfirstlane(need);
+    if (here && m > 0) {
+        if (need <= 8) sc_rank_list<8>(a, i, m, L0, LK, LK);
+        else if (need <= 16) sc_rank_list<16>(a, i, m, L0, LK, LK);
+        else if (need <= 32) sc_rank_list<32>(a, i, m, L0, LK, LK);
+        else sc_rank_list<64>(a, i, m, L0, LK, LK);
+    }
+    if (here) {
+        a.counts[i] = m < kSelfColCap ? m : kSelfColCap;
+        a.overflow[i] = m > kSelfColCap ? m - kSelfColCap : 0;
+        if (m > kSelfColCap) a.accCount[i] = kScRuleHi;
+    }
+    // the contact push of object shape matching needs to know whether the
+    // body has any candidate
+    const unsigned long long any = __ballot(here && m > 0);
+    if ((threadIdx.x & 63) == 0 && any) {
+        const int lb = __builtin_amdgcn_readlane(b, __builtin_ctzll(any));
+        if (a.bodies[lb].objOn) a.scCounters[3 * lb] = 1;
+    }
+}
+
+// Slots of the nodes whose lane-scan lists are too long for one lane to rank
+// (more than kRankRow), 16 lanes per node (node order).  The node's (key,
+// squared distance) list is spread over the group's lanes; its keys (all
+// distinct) are ranked in LDS; the list goes back to its entries in rank
+// order as (id, priority = |p_i - p_j|) and the node is flagged for the
+// slot-rule kernel.  Nodes outside the grid get empty lists.
+constexpr int kLaneGroup = 16;
+constexpr int kLanePer = kLaneList / kLaneGroup;
+
+__global__ void __launch_bounds__(256) sc_slots_kernel(int nPad, SelfColArgs a)
+{
+    __shared__ unsigned int skey[256 / kLaneGroup][kLaneList];
+    const int sub = threadIdx.x % kLaneGroup, grp = threadIdx.x / kLaneGroup;
+    const int i = blockIdx.x * (256 / kLaneGroup) + grp;   // nPad is a multiple of 64: whole blocks
+    if (i >= nPad) return;
+    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
+    const BodyDev& B = a.bodies[b];
+    if (!B.scOn || !B.scFine) return;
+    const int n = i < B.nodeEnd ? a.accCount[i] : -1;
+    const bool lane = n == kScLong;
+    int m = 0;
+    size_t L0 = 0;
+    if (lane) {
+        const int t = a.finePos[i];
+        m = a.laneN[t];
+        L0 = sc_lane_at(t, 0);
+    }
+    // keys per lane in use: the wave's largest
+    int kn = (m + kLaneGroup - 1) / kLaneGroup;
+#pragma unroll
+    for (int s = kLaneGroup; s < 64; s <<= 1) kn = max(kn, __shfl_xor(kn, s, 64));
+    kn = __builtin_amdgcn_readfirstlane(kn);
+    unsigned int key[kLanePer];
+    int jv[kLanePer], rank[kLanePer];
+#pragma unroll
+    for (int k = 0; k < kLanePer; ++k) {
+        key[k] = 0xffffffffu;
+        jv[k] = 0;
+        rank[k] = 0;
+        const int idx = sub + kLaneGroup * k;
+        if (k < kn && idx < m) {
+            const int2 e = a.laneKey[L0 + (size_t)idx * 64];
+            key[k] = (unsigned int)e.x;
+            jv[k] = e.y;   // squared distance bits until the ids are looked up
+            skey[grp][idx] = key[k];
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    for (int f = 0; f < m; ++f) {
+        const unsigned int kf = skey[grp][f];
+#pragma unroll
+        for (int k = 0; k < kLanePer; ++k)
+            if (k < kn) rank[k] += kf < key[k] ? 1 : 0;
+    }
+    if (lane) {
+        // node ids from the caller-order positions in the keys
+        int id[kLanePer];
+#pragma unroll
+        for (int k = 0; k < kLanePer; ++k)
+            id[k] = (k < kn && sub + kLaneGroup * k < m) ? a.devOfCaller[key[k] & 0x07ffffffu] : 0;
+        if (m <= kSelfColCap) {
+#pragma unroll
+            for (int k = 0; k < kLanePer; ++k)
+                if (k < kn && sub + kLaneGroup * k < m) a.contacts[sc_slot(i, rank[k])] = id[k];
+        } else {
+            // the list in rank order with its priorities, for the slot rule
+#pragma unroll
+            for (int k = 0; k < kLanePer; ++k)
+                if (k < kn && sub + kLaneGroup * k < m)
+                    a.laneKey[L0 + (size_t)rank[k] * 64] = make_int2(id[k], __float_as_int(sqrtf(__int_as_float(jv[k]))));
+        }
+    }
+    if (sub == 0 && (lane || n == 0)) {
+        const int cnt = m < kSelfColCap ? m : kSelfColCap;
         a.counts[i] = cnt;
-        a.overflow[i] = n - cnt;
+        a.overflow[i] = m - cnt;
+        if (m > kSelfColCap) a.accCount[i] = kScRule;
     }
-    const bool any = __ballot(n > 0 && cnt > 0) != 0ull;
+    const bool any = __ballot(m > 0) != 0ull;
     if ((threadIdx.x & 63) == 0 && any && B.objOn) a.scCounters[3 * b] = 1;
+}
+
+// The slot rule for the flagged nodes (more than 32 candidates), one thread
+// per node over the list a ranking kernel left in rank order.
+__global__ void __launch_bounds__(256) sc_slot_rule_kernel(int nPad, SelfColArgs a, const float4* __restrict__ P)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nPad) return;
+    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
+    const BodyDev& B = a.bodies[b];
+    const int c = (B.scOn && B.scFine && i < B.nodeEnd) ? a.accCount[i] : 0;
+    if (c != kScRule && c != kScRuleHi) return;
+    const size_t L0 = sc_lane_at(a.finePos[i], c == kScRuleHi ? kLaneList / 2 : 0);
+    const int n = kSelfColCap + a.overflow[i];
+    // priority |p_i - p_j| from the positions the scan tested (the same operations)
+    const float4 pi = P[i];
+    sc_fill_slots(i, n, a, [&](int r) {
+        const int j = a.laneKey[L0 + (size_t)r * 64].x;
+        const float4 pj = P[j];
+        const float ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+        return make_int2(j, __float_as_int(sqrtf(ex * ex + ey * ey + ez * ez)));
+    });
 }
 
 // Statistics of one body's last build (forward.cu:11837-11849 reports them):
@@ -1363,7 +1224,6 @@ __device__ __forceinline__ void sc_node_push(int i, const SelfColArgs& a, const 
                                              const float3* __restrict__ rest, float3& dpOut, float3& crossOut)
 {
     const int cnt = a.counts[i];
-    constexpr int CAP = kSelfColCap;
     const float4 pi4 = P[i];
     const float wi = pi4.w * B.invMassScale;
     const float dc = B.scDc;
@@ -1575,14 +1435,15 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
     launch_fill_int(s, a.tableTotal, a.cellStart, -1);
     sc_range_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, ks);
     sc_stage_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
+    const bool laneScan = tu.scGroupLanes != 0;
     if (a.fineTotal > 0) {
-        sc_fine_keys_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P);
-        const bool fAlt = launch_radix_sort(s, na.nPad, a.fineBits, a.fkeys, a.fidx, a.fkeysAlt, a.fidxAlt, hist, false);
-        const unsigned int* fks = fAlt ? a.fkeysAlt : a.fkeys;
-        const int* fvs = fAlt ? a.fidxAlt : a.fidx;
-        launch_fill_int(s, a.fineTotal, a.fcellStart, -1);
-        sc_fine_range_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, fks);
-        sc_fine_stage_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P, rest, fks, fvs, ks);
+        // counting sort of the predicted-position grid: cell counts (and each
+        // node's rank in its cell), their prefix sum, then the fine-order copies
+        if (mark) mark(markCtx, "selfcol_build_fine_grid");
+        launch_fill_int(s, a.fineTotal, a.fcount, 0);
+        sc_fine_keys_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P, laneScan ? 1 : 0);
+        launch_exclusive_scan(s, a.fineTotal, a.fcount, a.fstart, a.fscan);
+        sc_fine_scatter_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P, rest);
     }
     const int wpb = (tu.scBuildWaves == 1 || tu.scBuildWaves == 2) ? tu.scBuildWaves : 4;
     const int npw = (tu.scBuildNodes == 4 || tu.scBuildNodes == 8 || tu.scBuildNodes == 16 || tu.scBuildNodes == 64)
@@ -1602,23 +1463,10 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
         else K<32><<<grid, 64 * wpb, 0, s>>>(na.nPad, A, X, P, rest, xcd);                               \
     } while (0)
 #define XPBD_SC_LAUNCH(K) XPBD_SC_LAUNCH_A(K, a)
-    // fine scans first (thread, then wave for the nodes with more than 32
-    // candidates): the coarse kernel walks the nodes they hand over
-    if (a.fineTotal > 0) {
-        if (mark) mark(markCtx, "selfcol_build_fine_group");
-        // G lanes per node, 64 / G nodes per wave, 4 waves per block; with
-        // G = 0 the wave scan takes every node
-        const int g = tu.scGroupLanes;
-        if (g == 8 || g == 16 || g == 32) {
-            const int gridG = xcd_grid(nbs(na.nPad, 4 * (64 / g)));
-            if (g == 16) sc_fine_group_kernel<16><<<gridG, 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
-            else if (g == 32) sc_fine_group_kernel<32><<<gridG, 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
-            else sc_fine_group_kernel<8><<<gridG, 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
-        }
-        if (mark) mark(markCtx, "selfcol_build_fine_wave");
-        SelfColArgs aw = a;
-        aw.fineWaveAll = (g == 8 || g == 16 || g == 32) ? 0 : 1;
-        XPBD_SC_LAUNCH_A(sc_fine_kernel, aw);
+    // the lane fine scan first: the coarse kernel walks the nodes it hands over
+    if (a.fineTotal > 0 && laneScan) {
+        if (mark) mark(markCtx, "selfcol_build_fine_lane");
+        sc_fine_lane_kernel<<<xcd_grid(nbs(na.nPad, 256)), 256, 0, s>>>(a, a.fineE, a.fineC, a.fstart, a.laneKey);
     }
     if (mark) mark(markCtx, "selfcol_build_coarse");
     SelfColArgs ac = a;
@@ -1627,7 +1475,14 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
 #undef XPBD_SC_LAUNCH
 #undef XPBD_SC_LAUNCH_A
     if (mark) mark(markCtx, "selfcol_build_slots");
-    if (a.fineTotal > 0) sc_slots_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a);
+    if (a.fineTotal > 0) {
+        if (laneScan)
+            sc_lane_rank_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(a, a.fineC, a.fstart, a.laneKey);
+        if (mark) mark(markCtx, "selfcol_build_rank_long");
+        sc_slots_kernel<<<nbs(na.nPad, 256 / kLaneGroup), 256, 0, s>>>(na.nPad, a);
+        if (mark) mark(markCtx, "selfcol_build_slot_rule");
+        sc_slot_rule_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, P);
+    }
     sc_fill_accum_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a.accum);
     *sortedIdxOut = vs;
     return inAlt;

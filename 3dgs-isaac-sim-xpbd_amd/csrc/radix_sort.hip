@@ -156,4 +156,122 @@ bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0
     return kin == k1;
 }
 
+// ---------------------------------------------------------------------------
+// Exclusive prefix sum of int counts (reduce, scan of the block sums, apply).
+// Each block takes kScanItems consecutive values, kScanPer per thread.
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 16;
+constexpr int kScanItems = kScanThreads * kScanPer;
+
+size_t exclusive_scan_tmp_size(int n) { return (size_t)(n + kScanItems - 1) / kScanItems + 1; }
+
+// Block sum of 256 thread values (every thread gets it).
+__device__ __forceinline__ int scan_block_total(int v, int* wsum)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const int t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    return t;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_reduce_kernel(int n, const int* __restrict__ in,
+                                                                  int* __restrict__ bsum)
+{
+    __shared__ int wsum[kScanThreads / 64];
+    const int base = blockIdx.x * kScanItems;
+    int v = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        const int i = base + k * kScanThreads + (int)threadIdx.x;   // coalesced
+        if (i < n) v += in[i];
+    }
+    const int t = scan_block_total(v, wsum);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = t;
+}
+
+// One block: exclusive scan of the nb block sums in place, total in bsum[nb].
+__global__ void __launch_bounds__(kScanThreads) scan_top_kernel(int nb, int* __restrict__ bsum)
+{
+    __shared__ int wsum[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int carry = 0;
+    for (int base = 0; base < nb; base += kScanThreads) {
+        const int i = base + (int)threadIdx.x;
+        const int v = i < nb ? bsum[i] : 0;
+        int x = v;
+#pragma unroll
+        for (int m = 1; m < 64; m <<= 1) {
+            const int o = __shfl_up(x, m, 64);
+            if (lane >= m) x += o;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        int before = carry;
+        for (int w = 0; w < wave; ++w) before += wsum[w];
+        const int chunk = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+        if (i < nb) bsum[i] = before + x - v;
+        carry += chunk;
+    }
+    if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+// Thread k of a block owns values base + 16 k .. base + 16 k + 15 (staged
+// through LDS so that the global reads and writes stay coalesced).
+__global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(int n, const int* __restrict__ in,
+                                                                 int* __restrict__ out, const int* __restrict__ bsum,
+                                                                 int nb)
+{
+    __shared__ int vals[kScanItems + kScanItems / 32];   // padded: thread rows start on distinct banks
+    __shared__ int wsum[kScanThreads / 64];
+    const int base = blockIdx.x * kScanItems;
+    auto slot = [](int k) { return k + (k >> 5); };
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        const int j = k * kScanThreads + (int)threadIdx.x;
+        vals[slot(j)] = (base + j < n) ? in[base + j] : 0;
+    }
+    __syncthreads();
+    int own[kScanPer];
+    int run = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        own[k] = run;
+        run += vals[slot((int)threadIdx.x * kScanPer + k)];
+    }
+    // exclusive scan of the thread totals over the block
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = run;
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) {
+        const int o = __shfl_up(x, m, 64);
+        if (lane >= m) x += o;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    int before = bsum[blockIdx.x] + x - run;
+    for (int w = 0; w < wave; ++w) before += wsum[w];
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) vals[slot((int)threadIdx.x * kScanPer + k)] = before + own[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        const int j = k * kScanThreads + (int)threadIdx.x;
+        if (base + j < n) out[base + j] = vals[slot(j)];
+    }
+    if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = bsum[nb];
+}
+
+void launch_exclusive_scan(hipStream_t s, int n, const int* in, int* out, int* tmp)
+{
+    if (n <= 0) return;
+    const int nb = (n + kScanItems - 1) / kScanItems;
+    scan_reduce_kernel<<<nb, kScanThreads, 0, s>>>(n, in, tmp);
+    scan_top_kernel<<<1, kScanThreads, 0, s>>>(nb, tmp);
+    scan_apply_kernel<<<nb, kScanThreads, 0, s>>>(n, in, out, tmp, nb);
+}
+
 }  // namespace xpbd

@@ -66,7 +66,8 @@ struct BodyDev {
     // [scTableBase, scTableBase + scTableMask + 1) of the shared cell arrays
     int scOn, scWithin, scTableBase, scTableMask;
     float scDc, scSearch, scExcl2, scFastRel, scCellOverride;
-    // candidate search through the predicted-position grid (extension; same lists)
+    // candidate search through the predicted-position grid (extension; same
+    // lists): cells of edge >= search
     int scFine, scFineBase, scFineMask;
     float scFineInv;
 };

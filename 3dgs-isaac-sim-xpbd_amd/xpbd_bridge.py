@@ -123,6 +123,7 @@ ABI_OPTIONAL = {
     "xpbd_ext_angle_pairs": (_INT, [_PTR, _PTR, _INT]),
     "xpbd_ext_set_self_collision_fine": (_INT, [_INT]),
     "xpbd_ext_set_group_lanes": (_INT, [_INT]),
+    "xpbd_ext_self_collision_paths": (_INT, [_PTR]),
     "xpbd_ext_autotune": (_INT, [_INT, _INT, ctypes.c_char_p, _INT]),
     "xpbd_ext_volume_members": (_INT, [_PTR, _INT]),
     "xpbd_ext_frame_outputs": (_INT, [_PTR, _PTR, _PTR, _INT, _FLT, _PTR, _FLT, ctypes.POINTER(ctypes.c_double), _INT]),
@@ -388,8 +389,8 @@ class XPBD:
         self._check(fn(int(bool(enabled))), "xpbd_ext_set_self_collision_fine")
 
     def ext_set_group_lanes(self, lanes):
-        """Lanes per node of the grouped fine candidate scan (8, 16, 32), 0 for
-        the one-wave-per-node scan, -1 (default) by mean body size; same lists."""
+        """Fine candidate scan: 1 one lane per node in the grid's order, 0 one
+        wave per node, -1 (default) the lane scan; same lists."""
         fn = self.lib.xpbd_ext_set_group_lanes
         fn.argtypes = [ctypes.c_int]
         fn.restype = ctypes.c_int
@@ -455,6 +456,14 @@ class XPBD:
         info = np.zeros(3, np.float32)
         self._check(self.lib.xpbd_ext_self_collision_grid_info(_ptr(info)), "xpbd_ext_self_collision_grid_info")
         return tuple(float(v) for v in info)
+
+    def ext_self_collision_paths(self):
+        """Node counts per candidate-search path of the last build: lane scan
+        (lists of at most 32), lane scan with the slot rule, coarse walk,
+        outside the grid."""
+        out = np.zeros(4, np.int32)
+        self._check(self.lib.xpbd_ext_self_collision_paths(_ptr(out)), "xpbd_ext_self_collision_paths")
+        return tuple(int(v) for v in out)
 
     def ext_set_shape_clusters(self, cluster_of=None):
         """Shape matching over node clusters (cluster id per node, -1 = none); None = 1-ring."""

@@ -400,7 +400,7 @@ def main():
                          "the ones measured best at the compressed frames of the gripper script, which the "
                          "light warmup state does not represent)")
     ap.add_argument("--group-lanes", type=int, default=None,
-                    help="full: lanes per node of the grouped fine candidate scan (8, 16, 32; 0 = wave scan; "
+                    help="full: fine candidate scan (1 = one lane per node, 0 = one wave per node; "
                          "default: the library's automatic choice)")
     ap.add_argument("--eager", action="store_true",
                     help="full/distance: run the substeps as ordinary launches (the event profile) instead of the "

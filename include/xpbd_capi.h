@@ -150,10 +150,10 @@ int xpbd_ext_profile(int substeps, char* json, int len);
  * xpbd_batch_autotune: the state is restored and results do not depend on
  * the launch shapes it picks; flags bit 0 also sizes the hash table. */
 int xpbd_ext_autotune(int substeps, int flags, char* json, int len);
-/* Lanes per node of the grouped fine candidate scan of the instance (8, 16 or
- * 32; 0 sends every node to the one-wave-per-node scan; -1, the default, picks
- * 8 when the self-colliding bodies average at most 300k nodes, else 0).
- * Candidate lists do not depend on it. */
+/* Fine candidate scan of the instance: 1 one lane per node in the
+ * predicted-position grid's order (the nodes it hands over take the
+ * one-wave-per-node scan); 0 sends every node to the one-wave-per-node scan;
+ * -1, the default, is 1.  Candidate lists do not depend on it. */
 int xpbd_ext_set_group_lanes(int lanes);
 
 /* Volume clusters given explicitly instead of by the k-ring rule: node i leads
@@ -211,6 +211,13 @@ int xpbd_ext_set_node_order(int mode);
 /* Grid of the last self-collision build (after the first step): info[0] cell
  * edge, info[1] contact distance d_c, info[2] search radius. */
 int xpbd_ext_self_collision_grid_info(float* info);
+
+/* Candidate-search paths of the last self-collision build (after the first
+ * step), node counts: out[0] lists of at most 32 from the lane fine scan,
+ * out[1] longer lane-scan lists (slot rule), out[2] the coarse walk (the
+ * reference's walk; every node of bodies without the fine grid), out[3]
+ * nodes outside the grid (inactive or non-finite). */
+int xpbd_ext_self_collision_paths(int* out);
 
 /* Shape matching over node clusters (clusterOf[n]: cluster id, -1 = none;
  * e.g. k-means clusters of ~64 nodes) instead of the graph 1-ring, with the
@@ -393,9 +400,8 @@ int xpbd_batch_copy_object(xpbd_batch* dst, int dstObj, xpbd_batch* src, int src
  * cluster shape waves per block, self-collision build waves per block (1, 2,
  * 4), nodes per wave (4, 8, 16, 32, 64), XCD mapping (0 plain block order,
  * 1 one contiguous node run per XCD, 2-256 runs of that many blocks dealt
- * round-robin over the XCDs) and lanes per node of the grouped fine scan (8,
- * 16, 32; 0 sends every node to the one-wave-per-node scan; -1, the default,
- * picks 8 when the self-colliding bodies average at most 300k nodes, else 0).
+ * round-robin over the XCDs) and the fine candidate scan (1 one lane per
+ * node, 0 one wave per node, -1 the default: 1).
  * flags bit 0 also picks the self-collision hash-table size
  * (1, 4 or 16 x nextpow2(2N) cells per object), which can reorder candidate
  * lists that overflow their 32 slots. */

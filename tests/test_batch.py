@@ -157,9 +157,9 @@ def test_autotune_keeps_results(product_path):
     for k in range(len(objs)):
         for f in range(frames):
             assert np.array_equal(got[k][f], ref[k][f]), f"object {k} frame {f}"
-    # every tuned shape also gives the same results (the grouped fine scan
-    # with 8, 16 and 32 lanes per node against the default wave scan)
-    for alt_tuning in ([1, 1, 1, 1, 16, 0, 8], [1, 1, 1, 1, 16, 0, 16], [1, 1, 1, 1, 16, 0, 32]):
+    # every tuned shape also gives the same results (the one-wave-per-node
+    # fine scan and the lane scan, other block shapes)
+    for alt_tuning in ([1, 1, 1, 1, 16, 0, 0], [1, 1, 1, 1, 16, 0, 1]):
         alt, _ = batch_run(objs, frames, tuning=alt_tuning)
         for k in range(len(objs)):
             assert np.array_equal(alt[k][-1], ref[k][-1]), f"tuning {alt_tuning} object {k}"

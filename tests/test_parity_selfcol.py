@@ -167,18 +167,17 @@ def test_fine_scan_lists_bit_exact(product_path, oracle_path, cell, table):
         np.testing.assert_array_equal(lf, lc)
         assert gpu.self_collision_stats() == cpu.self_collision_stats()
         over += cpu.self_collision_stats()[2]
-        # the same state through the grouped fine scan (lists of up to 32
-        # written by the group, up to 64 through the slot kernel, longer ones
-        # handed to the wave scan)
-        for lanes in (8, 16, 32):
-            gpu.ext_set_state(pos, vel)
-            gpu.ext_set_group_lanes(lanes)
-            gpu.step()
-            _, _, cgr, lgr = gpu.ext_self_collision_lists()
-            np.testing.assert_array_equal(cgr, cf, err_msg=f"group lanes {lanes}")
-            np.testing.assert_array_equal(lgr, lf, err_msg=f"group lanes {lanes}")
-            assert gpu.self_collision_stats() == cpu.self_collision_stats()
+        # the same state through the one-wave-per-node fine scan for every
+        # node (the default is the lane scan, which hands crowded nodes and
+        # lists past 64 to the wave scan)
+        gpu.ext_set_state(pos, vel)
         gpu.ext_set_group_lanes(0)
+        gpu.step()
+        _, _, cgr, lgr = gpu.ext_self_collision_lists()
+        np.testing.assert_array_equal(cgr, cf, err_msg="wave fine scan")
+        np.testing.assert_array_equal(lgr, lf, err_msg="wave fine scan")
+        assert gpu.self_collision_stats() == cpu.self_collision_stats()
+        gpu.ext_set_group_lanes(-1)
         # the same state through the coarse walk
         gpu.ext_set_state(pos, vel)
         gpu.ext_set_self_collision_fine(False)
