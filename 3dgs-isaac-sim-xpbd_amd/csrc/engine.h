@@ -462,6 +462,7 @@ private:
     DevBuf<unsigned int> scMax2_, scKeys_, scKeysAlt_, scHist_;
     DevBuf<int> scCounters_, scUseActive_, scExact_, scIdx_, scIdxAlt_, cellStart_, cellEnd_, contacts_, counts_;
     DevBuf<int> scOver_, scStats_;   // per-node overflow; statistics read-back
+    DevBuf<int> scSortGate_;          // 0: the last build skipped the coarse sort (2: sorted since, on demand)
     DevBuf<float> scBodyDisp_, scInvCell_;
     DevBuf<unsigned char> scActive_;
     DevBuf<float3> scAccum_;

@@ -61,6 +61,7 @@ SelfColArgs Engine::scArgs() const
     a.scCounters = scCounters_.p;
     a.scUseActive = scUseActive_.p;
     a.scExact = scExact_.p;
+    a.scSortGate = scSortGate_.p;
     a.devOfCaller = devOfCaller_.p;
     a.scBodyDisp = scBodyDisp_.p;
     a.scInvCell = scInvCell_.p;
@@ -268,7 +269,8 @@ bool Engine::ensureSelfCol(std::string& err)
     contacts_.alloc(n * kSelfColCap); counts_.alloc(n); scOver_.alloc(n);
     scActive_.alloc(n); scAccum_.alloc(n);
     scSortP_.alloc(n); scSortX_.alloc(n); scSortR_.alloc(n);
-    if (!scMax2_.p || !scCounters_.p || !scUseActive_.p || !scExact_.p || !scBodyDisp_.p || !scInvCell_.p || !scKeys_.p ||
+    scSortGate_.alloc(1);
+    if (!scSortGate_.p || !scMax2_.p || !scCounters_.p || !scUseActive_.p || !scExact_.p || !scBodyDisp_.p || !scInvCell_.p || !scKeys_.p ||
         !scKeysAlt_.p || !scIdx_.p || !scIdxAlt_.p || !scHist_.p || !cellStart_.p || !cellEnd_.p || !contacts_.p ||
         !counts_.p || !scOver_.p || !scActive_.p || !scAccum_.p || !scSortP_.p ||
         !scSortX_.p || !scSortR_.p) {
@@ -307,6 +309,7 @@ bool Engine::ensureSelfCol(std::string& err)
     XPBD_HIP(hipMemsetAsync(scActive_.p, 0, n, stream_));
     XPBD_HIP(hipMemsetAsync(scCounters_.p, 0, sizeof(int) * 3 * B, stream_));
     XPBD_HIP(hipMemsetAsync(scUseActive_.p, 0, sizeof(int) * B, stream_));
+    XPBD_HIP(hipMemsetAsync(scSortGate_.p, 0, sizeof(int), stream_));
     XPBD_HIP(hipMemsetAsync(scAccum_.p, 0, sizeof(float3) * n, stream_));
     XPBD_HIP(hipStreamSynchronize(stream_));
     scSortedIdx_ = nullptr;
@@ -391,6 +394,17 @@ bool Engine::selfCollisionLists(int b, unsigned int* keys, int* sorted, int* cou
     // tile * 64 * 32 + 64 k + l (the body starts on a tile boundary)
     const size_t tiles = ((size_t)n + kTile - 1) / kTile;
     std::vector<int> hs(nPad_), hc(n), hl(tiles * kTile * kSelfColCap);
+    // a build whose nodes all took the fine scans left its keys unsorted:
+    // sort them now (once; the sort lands where the build's would have)
+    int gate = 1;
+    XPBD_HIP(hipMemcpyAsync(&gate, scSortGate_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    if (gate == 0) {
+        launch_radix_sort(stream_, nPad_, scSortBits_, scKeys_.p, scIdx_.p, scKeysAlt_.p, scIdxAlt_.p, scHist_.p, false);
+        const int sorted2 = 2;
+        XPBD_HIP(hipMemcpyAsync(scSortGate_.p, &sorted2, sizeof(int), hipMemcpyHostToDevice, stream_));
+        XPBD_HIP(hipStreamSynchronize(stream_));
+    }
     XPBD_HIP(hipMemcpyAsync(hks.data(), scSortedKeys_, sizeof(unsigned int) * nPad_, hipMemcpyDeviceToHost, stream_));
     XPBD_HIP(hipMemcpyAsync(hs.data(), scSortedIdx_, sizeof(int) * nPad_, hipMemcpyDeviceToHost, stream_));
     XPBD_HIP(hipMemcpyAsync(hc.data(), counts_.p + nb, sizeof(int) * n, hipMemcpyDeviceToHost, stream_));

@@ -142,6 +142,8 @@ struct SelfColArgs {
     int* scUseActive;          // [B]
     int* scExact;              // [B] the cell edge is below search + 2 max move (capped or fixed):
                                //     candidates may lie outside the 27 cells around x
+    int* scSortGate;           // [1] 1 when some node takes the coarse walk this build: the coarse
+                               //     sort, cell ranges, sorted copies and walk run only then
     const int* devOfCaller;    // [nPad] device node of each caller-order index
     float* scBodyDisp;         // [B]
     float* scInvCell;          // [B]
@@ -203,8 +205,10 @@ void launch_contact_push(hipStream_t s, const NodeArgs& na, const SelfColArgs& a
 void launch_selfcol_stats(hipStream_t s, int nodeBegin, int nodeEnd, const SelfColArgs& a, int* out);
 int radix_sort_tiles(int n);
 size_t radix_sort_hist_size(int n);   // entries of the sort's histogram buffer
+// gate (device int, may be null): when it reads 0 the sort's kernels return
+// at once and the buffers keep their contents
 bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0, unsigned int* k1, int* v1,
-                       unsigned int* hist, bool iota);
+                       unsigned int* hist, bool iota, const int* gate = nullptr);
 // Exclusive prefix sum of in[0, n) into out[0, n] (out[n] = total); tmp holds
 // exclusive_scan_tmp_size(n) ints.
 size_t exclusive_scan_tmp_size(int n);

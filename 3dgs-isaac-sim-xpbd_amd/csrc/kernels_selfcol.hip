@@ -222,6 +222,15 @@ __global__ void sc_prepare_kernel(int B, SelfColArgs a)
     for (int k = 0; k < D.objK; ++k) maxDisp = fmaxf(maxDisp, a.objDisp[D.objBase + k]);
     a.scUseActive[b] = (D.scOn && D.objActiveAllowed && isfinite(maxDisp)) ? 1 : 0;
     a.scBodyDisp[b] = maxDisp;
+    if (b == 0) {
+        // the coarse sort is needed from the start when a body takes the
+        // coarse walk for all its nodes; the fine scans raise the gate for
+        // the nodes they hand over
+        int gate = a.fineTotal > 0 ? 0 : 1;
+        for (int k = 0; k < B; ++k)
+            if (a.bodies[k].scOn && !a.bodies[k].scFine) gate = 1;
+        *a.scSortGate = gate;
+    }
 }
 
 // Active mask (forward.cu:6998-7030) and max |p - x|^2 over active nodes (7058-7068).
@@ -339,8 +348,17 @@ __global__ void __launch_bounds__(256) sc_keys_kernel(int nPad, SelfColArgs a, c
     a.idx[pos] = i;
 }
 
+// Empty cells of the coarse tables (-1 start), when the coarse walk runs.
+__global__ void __launch_bounds__(256) sc_clear_cells_kernel(SelfColArgs a)
+{
+    if (*a.scSortGate == 0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.tableTotal) a.cellStart[i] = -1;
+}
+
 __global__ void __launch_bounds__(256) sc_range_kernel(int nPad, SelfColArgs a, const unsigned int* __restrict__ keysSorted)
 {
+    if (*a.scSortGate == 0) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nPad) return;
     const unsigned int k = keysSorted[i];
@@ -358,6 +376,7 @@ __global__ void __launch_bounds__(256) sc_stage_kernel(int nPad, SelfColArgs a, 
                                                       const float4* __restrict__ P, const float3* __restrict__ rest,
                                                       const int* __restrict__ sortedIdx)
 {
+    if (*a.scSortGate == 0) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nPad) return;
     const int j = sortedIdx[t];
@@ -448,8 +467,11 @@ __global__ void __launch_bounds__(256) sc_fine_keys_kernel(int nPad, SelfColArgs
     }
     a.fkeys[i] = key;
     if (key < (unsigned int)a.fineTotal) a.frank[i] = atomicAdd(&a.fcount[key], 1);
-    if (B.scOn && B.scFine && i < B.nodeEnd)
-        a.accCount[i] = key < (unsigned int)a.fineTotal ? ((laneScan && !a.scExact[b]) ? kScLane : kScNeedCoarse) : 0;
+    if (B.scOn && B.scFine && i < B.nodeEnd) {
+        const int path = key < (unsigned int)a.fineTotal ? ((laneScan && !a.scExact[b]) ? kScLane : kScNeedCoarse) : 0;
+        a.accCount[i] = path;
+        if (path == kScNeedCoarse) *a.scSortGate = 1;
+    }
 }
 
 // Fine-order copies for the fine scans, written at each node's place in the
@@ -709,6 +731,7 @@ __global__ void __launch_bounds__(256) sc_fine_lane_kernel(SelfColArgs a, const 
     }
     if (n > kLaneList || far) {
         a.accCount[i] = kScNeedCoarse;
+        *a.scSortGate = 1;
         n = -1;
     }
     a.laneN[t] = n;
@@ -731,6 +754,7 @@ __global__ void __launch_bounds__(256) sc_build_kernel(int nPad, SelfColArgs a, 
                                                       const float4* __restrict__ P, const float3* __restrict__ rest,
                                                       int xcd)
 {
+    if (*a.scSortGate == 0) return;   // no node takes the coarse walk
     const int wave = (sc_block(xcd) * (int)blockDim.x + (int)threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     const int i0 = wave * NPW;
@@ -1429,12 +1453,6 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
     sc_active_disp_kernel<<<nbs(na.nPad, kDispThreads), kDispThreads, 0, s>>>(na.nPad, a, X, P);
     sc_reach_kernel<<<nbs(numBodies, 64), 64, 0, s>>>(numBodies, a);
     sc_keys_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X);
-    const bool inAlt = launch_radix_sort(s, na.nPad, sortBits, a.keys, a.idx, keysAlt, idxAlt, hist, false);
-    const unsigned int* ks = inAlt ? keysAlt : a.keys;
-    const int* vs = inAlt ? idxAlt : a.idx;
-    launch_fill_int(s, a.tableTotal, a.cellStart, -1);
-    sc_range_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, ks);
-    sc_stage_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
     const bool laneScan = tu.scGroupLanes != 0;
     if (a.fineTotal > 0) {
         // counting sort of the predicted-position grid: cell counts (and each
@@ -1468,7 +1486,18 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
         if (mark) mark(markCtx, "selfcol_build_fine_lane");
         sc_fine_lane_kernel<<<xcd_grid(nbs(na.nPad, 256)), 256, 0, s>>>(a, a.fineE, a.fineC, a.fstart, a.laneKey);
     }
+    // the coarse walk and what it reads (stable sort of the frame-start keys,
+    // cell ranges, sorted copies), only when a node needs it: the kernels
+    // return at once when the gate reads 0 (the keys stay in place for an
+    // export, which then sorts them)
     if (mark) mark(markCtx, "selfcol_build_coarse");
+    const bool inAlt = launch_radix_sort(s, na.nPad, sortBits, a.keys, a.idx, keysAlt, idxAlt, hist, false,
+                                         a.scSortGate);
+    const unsigned int* ks = inAlt ? keysAlt : a.keys;
+    const int* vs = inAlt ? idxAlt : a.idx;
+    if (a.tableTotal > 0) sc_clear_cells_kernel<<<nbs(a.tableTotal, 256), 256, 0, s>>>(a);
+    sc_range_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, ks);
+    sc_stage_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, X, P, rest, vs);
     SelfColArgs ac = a;
     ac.sortedIdx = vs;
     XPBD_SC_LAUNCH_A(sc_build_kernel, ac);

@@ -25,8 +25,10 @@ constexpr int kSortRounds = 8;
 constexpr int kTileKeys = kSortThreads * kSortRounds;   // 2048 keys per tile
 
 __global__ void __launch_bounds__(kSortThreads) radix_hist_kernel(int n, const unsigned int* __restrict__ keys, int shift,
-                                                                 int tiles, unsigned int* __restrict__ hist)
+                                                                 int tiles, unsigned int* __restrict__ hist,
+                                                                 const int* __restrict__ gate)
 {
+    if (gate && *gate == 0) return;
     __shared__ unsigned int h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -44,8 +46,10 @@ __global__ void __launch_bounds__(kSortThreads) radix_hist_kernel(int n, const u
 // scatter adds the digits' exclusive offsets, so the global order is digit,
 // then tile, then position (stable).
 __global__ void __launch_bounds__(kSortThreads) radix_scan_kernel(int tiles, unsigned int* __restrict__ hist,
-                                                                 unsigned int* __restrict__ dsum)
+                                                                 unsigned int* __restrict__ dsum,
+                                                                 const int* __restrict__ gate)
 {
+    if (gate && *gate == 0) return;
     __shared__ unsigned int wsum[kSortThreads / 64];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     unsigned int* h = hist + (size_t)blockIdx.x * tiles;
@@ -75,8 +79,9 @@ __global__ void __launch_bounds__(kSortThreads) radix_scan_kernel(int tiles, uns
 __global__ void __launch_bounds__(kSortThreads) radix_scatter_kernel(
     int n, const unsigned int* __restrict__ keysIn, const int* __restrict__ valsIn, unsigned int* __restrict__ keysOut,
     int* __restrict__ valsOut, int shift, int tiles, const unsigned int* __restrict__ offsets,
-    const unsigned int* __restrict__ dsum, int iota)
+    const unsigned int* __restrict__ dsum, int iota, const int* __restrict__ gate)
 {
+    if (gate && *gate == 0) return;
     __shared__ unsigned int run[256];          // keys of each digit already placed by this tile
     __shared__ unsigned int wcnt[4][256];      // per-wave digit counts of the current round
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -135,7 +140,7 @@ size_t radix_sort_hist_size(int n) { return 256 * (size_t)radix_sort_tiles(n) + 
 // in (k0, v0) when the number of passes is even, else in (k1, v1).  Returns
 // true when the result is in (k1, v1).
 bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0, unsigned int* k1, int* v1,
-                       unsigned int* hist, bool iota)
+                       unsigned int* hist, bool iota, const int* gate)
 {
     if (n <= 0) return false;
     const int tiles = radix_sort_tiles(n);
@@ -146,10 +151,10 @@ bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0
     int* vout = v1;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
-        radix_hist_kernel<<<tiles, kSortThreads, 0, s>>>(n, kin, shift, tiles, hist);
-        radix_scan_kernel<<<256, kSortThreads, 0, s>>>(tiles, hist, hist + 256 * (size_t)tiles);
+        radix_hist_kernel<<<tiles, kSortThreads, 0, s>>>(n, kin, shift, tiles, hist, gate);
+        radix_scan_kernel<<<256, kSortThreads, 0, s>>>(tiles, hist, hist + 256 * (size_t)tiles, gate);
         radix_scatter_kernel<<<tiles, kSortThreads, 0, s>>>(n, kin, vin, kout, vout, shift, tiles, hist,
-                                                            hist + 256 * (size_t)tiles, (iota && p == 0) ? 1 : 0);
+                                                            hist + 256 * (size_t)tiles, (iota && p == 0) ? 1 : 0, gate);
         unsigned int* tk = kin; kin = kout; kout = tk;
         int* tv = vin; vin = vout; vout = tv;
     }
