@@ -38,6 +38,7 @@ struct Global {
     float scCell = 0.0f;
     int scTable = 0;
     int nodeOrder = 1;   // device node order of the next engine (1 = Morton)
+    int deviceBuilders = -1;   // where k-ring volume member lists are built (-1 by size)
     bool scFine = true;  // candidate search of one-object bodies on the predicted-position grid
     std::vector<int> shapeClusters;     // cluster of every node, empty = 1-ring
     int n = 0;
@@ -235,6 +236,7 @@ int xpbd_step(void)
     if (!g.engine) {
         g.engine.reset(new Engine());
         g.engine->setNodeOrder(g.nodeOrder);
+        g.engine->setDeviceBuilders(g.deviceBuilders);
         if (!g.engine->build({g.input.get()}, err)) {
             g.engine.reset();
             return fail("xpbd_step: " + err);
@@ -553,6 +555,16 @@ int xpbd_ext_self_collision_paths(int* out)
     if (!out) return fail("xpbd_ext_self_collision_paths: null output");
     std::string err;
     if (!g.engine->selfCollisionPaths(0, out, err)) return fail("xpbd_ext_self_collision_paths: " + err);
+    return 0;
+}
+
+int xpbd_ext_set_device_builders(int mode)
+{
+    if (mode != -1 && mode != 0 && mode != 1)
+        return fail("xpbd_ext_set_device_builders: mode must be -1 (by size), 0 (host) or 1 (GPU)");
+    Global& g = G();
+    g.deviceBuilders = mode;
+    if (g.engine) g.engine->setDeviceBuilders(mode);
     return 0;
 }
 

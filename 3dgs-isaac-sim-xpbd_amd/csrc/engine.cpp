@@ -3,9 +3,11 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
-#include <cstdio>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace xpbd {
@@ -195,6 +197,16 @@ Engine::~Engine()
 
 bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err)
 {
+    const auto tBuild = std::chrono::steady_clock::now();
+    struct BuildTimer {
+        std::chrono::steady_clock::time_point t0;
+        ~BuildTimer()
+        {
+            if (getenv("XPBD_BUILD_TIMING"))
+                fprintf(stderr, "[build] engine build %.1f ms\n",
+                        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+    } buildTimer{tBuild};
     XPBD_ALLOC_SCOPE;
     const int B = (int)inputs.size();
     if (B <= 0) { err = "engine: no bodies"; return false; }
@@ -230,6 +242,8 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     colDirty_ = true;
     colStatsValid_ = false;
     volTopo_.assign(B, VolumeTopo());
+    volDev_.clear();
+    for (int b = 0; b < B; ++b) volDev_.emplace_back(new VolDevLists());
     volDeviceValid_ = false;
     volActive_ = 0;
     detRest_.free(); Vrest_.free(); restLen_.free(); matType_.free(); lamVol_.free();
@@ -1033,10 +1047,17 @@ bool Engine::step(int substeps, std::string& err)
     XPBD_ALLOC_SCOPE;
     for (int b = 1; b < numBodies(); ++b)
         if (set_[b].iters != set_[0].iters) { err = "engine: every body of one batch needs the same iteration count"; return false; }
+    const bool timing = getenv("XPBD_BUILD_TIMING") != nullptr;
     for (int s = 0; s < substeps; ++s) {
+        const auto t0 = std::chrono::steady_clock::now();
         if (!prepare(err)) return false;
+        const auto t1 = std::chrono::steady_clock::now();
         if (!graphExec_ || planSignature() != graphSig_) {
             if (!rebuildGraph(err)) return false;
+            if (timing)
+                fprintf(stderr, "[build] prepare %.1f ms, graph capture %.1f ms\n",
+                        std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count());
         }
         XPBD_HIP(hipGraphLaunch(graphExec_, stream_));
         afterSubstep();

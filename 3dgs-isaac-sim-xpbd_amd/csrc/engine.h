@@ -10,6 +10,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -80,8 +81,13 @@ struct VolumeTopo {
     int minMembers = 4;       // rest-precompute gate (forward.cu:7745)
     int k = -1, cap = -1, r = -1;
     bool explicitClusters = false;
-    long long version = 0;    // changes with every finish(): colourings follow it
+    // memIdx and revIdx of a topology built on the GPU stay in the engine's
+    // device copy (Engine::volumeHostLists fetches them); isLeader, memOff
+    // and revOff are always here
+    bool listsOnDevice = false;
+    long long version = 0;    // changes with every finish() or device build: colourings follow it
     void finish(int n);       // builds the reverse CSR from isLeader / members
+    void touch();             // a new version
 };
 
 // Gauss-Seidel colourings of one body in caller ids.
@@ -141,6 +147,14 @@ struct DevBuf {
     ~DevBuf() { free(); }
 };
 
+// The same topology as build_volume_reference, built on the GPU: every
+// node's k-hop ball and its farthest-point sample (balls past the device
+// limit on the host), the greedy leader cover on the host (r >= 2) over the
+// device balls, the reverse lists by a stable device sort.  memIdx and revIdx
+// are left in the device buffers (t.listsOnDevice); identical lists.
+bool device_volume_build(const BodyInput& in, int k, int cap, int leaderR, hipStream_t s, VolumeTopo& t,
+                         DevBuf<int>& memIdx, DevBuf<int>& revIdx, std::string& err);
+
 class Engine {
 public:
     Engine();
@@ -153,6 +167,12 @@ public:
     // (Z-order) curve of the rest positions so that graph neighbours share
     // cache lines.  Every host-facing array stays in the caller's order.
     void setNodeOrder(int mode) { nodeOrderMode_ = mode; }
+    // Where the k-ring volume clusters' member lists are built: 0 on the
+    // host, 1 on the GPU (the same lists), -1 (default) on the GPU for bodies
+    // of at least kDeviceBuildMin nodes.  A change rebuilds the k-ring
+    // topologies at the next step.
+    void setDeviceBuilders(int mode);
+    static constexpr int kDeviceBuildMin = 16384;
     bool permuted() const { return permuted_; }
 
     // Allocate and upload: x = rest, v = 0, invMass = 1 for every body.
@@ -328,6 +348,7 @@ private:
     // caller's node ord_[b][d]; inv_[b] is the inverse
     std::vector<std::vector<int>> ord_, inv_;
     int nodeOrderMode_ = 1;
+    int deviceBuilders_ = -1;
     bool permuted_ = false;
     DevBuf<int> callerPos_;      // [nPad] caller-order global index of each device node
     DevBuf<int> devOfCaller_;    // [nPad] its inverse: device node of each caller-order index
@@ -403,6 +424,12 @@ private:
 
     // volume constraint
     std::vector<VolumeTopo> volTopo_;
+    struct VolDevLists {
+        DevBuf<int> memIdx, revIdx;   // caller order, of topologies built on the GPU
+    };
+    std::vector<std::unique_ptr<VolDevLists>> volDev_;   // per body
+    // Host copies of a device-built topology's member and reverse lists.
+    bool volumeHostLists(int b, std::string& err);
     bool volDeviceValid_ = false;
     int volActive_ = 0;
     double volAvgMembers_ = 0.0;

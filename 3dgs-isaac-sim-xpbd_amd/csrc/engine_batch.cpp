@@ -49,6 +49,8 @@ bool Engine::copyBodyFrom(int b, Engine& src, int sb, std::string& err)
         shapeClusterOf_[b] = src.shapeClusterOf_[sb];
         shcDevValid_ = false;
     }
+    // device-built lists are compared and handed over through host copies
+    if (!src.volumeHostLists(sb, err) || !volumeHostLists(b, err)) return false;
     const VolumeTopo& sv = src.volTopo_[sb];
     VolumeTopo& dv = volTopo_[b];
     if (sv.version != 0 &&

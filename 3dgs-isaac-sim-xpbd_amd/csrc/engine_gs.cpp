@@ -195,6 +195,7 @@ bool Engine::ensureVolumeGS(std::string& err)
         if (!in[b]) continue;
         VolumeGSTopo& t = vgs_[b];
         if (!t.built || t.version != volTopo_[b].version) {
+            if (!volumeHostLists(b, err)) return false;
             build_volume_gs(volTopo_[b], bodyCount_[b], t);
             stale = true;
         }
@@ -235,8 +236,10 @@ bool Engine::gsInfo(int b, int info[6], std::string& err)
     const bool volOn = set_[b].useVolume;
     if (volOn) {
         if (!ensureVolume(false, err)) return false;
-        if (!vgs_[b].built || vgs_[b].version != volTopo_[b].version)
+        if (!vgs_[b].built || vgs_[b].version != volTopo_[b].version) {
+            if (!volumeHostLists(b, err)) return false;
             build_volume_gs(volTopo_[b], bodyCount_[b], vgs_[b]);
+        }
     }
     const DistanceGSTopo& d = dgs_[b];
     const VolumeGSTopo& v = vgs_[b];
@@ -268,8 +271,10 @@ bool Engine::volumeGSOrder(int b, int* order, int* colorOff, std::string& err)
     XPBD_ALLOC_SCOPE;
     if (!set_[b].useVolume) return true;
     if (!ensureVolume(false, err)) return false;
-    if (!vgs_[b].built || vgs_[b].version != volTopo_[b].version)
+    if (!vgs_[b].built || vgs_[b].version != volTopo_[b].version) {
+        if (!volumeHostLists(b, err)) return false;
         build_volume_gs(volTopo_[b], bodyCount_[b], vgs_[b]);
+    }
     const VolumeGSTopo& t = vgs_[b];
     if (order) std::copy(t.order.begin(), t.order.end(), order);
     if (colorOff) std::copy(t.colorOff.begin(), t.colorOff.end(), colorOff);

@@ -3,7 +3,10 @@
 // padded node space, rest precompute on the device, the active-leader list,
 // and J statistics on request.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "engine.h"
@@ -67,6 +70,15 @@ bool Engine::setVolumeClusters(int b, const int* off, const int* members, int mi
     return true;
 }
 
+void Engine::setDeviceBuilders(int mode)
+{
+    if (mode == deviceBuilders_) return;
+    deviceBuilders_ = mode;
+    for (VolumeTopo& t : volTopo_)
+        if (!t.explicitClusters) t.memOff.clear();
+    volDeviceValid_ = false;
+}
+
 // Make the device topology current.  needAll: build clusters for every body
 // (statistics), otherwise only for bodies with the volume constraint on.
 bool Engine::ensureVolume(bool needAll, std::string& err)
@@ -79,41 +91,59 @@ bool Engine::ensureVolume(bool needAll, std::string& err)
         VolumeTopo& t = volTopo_[b];
         if (t.explicitClusters) continue;
         if (!t.memOff.empty() && t.k == s.volRingK && t.cap == s.volMaxMembers && t.r == s.volLeaderMinHop) continue;
-        build_volume_reference(*inputs_[b], s.volRingK, s.volMaxMembers, s.volLeaderMinHop, t);
+        const BodyInput& in = *inputs_[b];
+        const auto tb0 = std::chrono::steady_clock::now();
+        VolDevLists& dl = *volDev_[b];
+        if (deviceBuilders_ == 1 || (deviceBuilders_ < 0 && in.n >= kDeviceBuildMin)) {
+            if (!device_volume_build(in, s.volRingK, s.volMaxMembers, s.volLeaderMinHop, stream_, t, dl.memIdx,
+                                     dl.revIdx, err))
+                return false;
+        } else {
+            build_volume_reference(in, s.volRingK, s.volMaxMembers, s.volLeaderMinHop, t);
+            dl.memIdx.free();
+            dl.revIdx.free();
+        }
         t.k = s.volRingK; t.cap = s.volMaxMembers; t.r = s.volLeaderMinHop;
         changed = true;
+        if (getenv("XPBD_BUILD_TIMING"))
+            fprintf(stderr, "[build] volume topology body %d: %.1f ms\n", b,
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tb0).count());
     }
     if (!changed) return true;
+    const auto tc0 = std::chrono::steady_clock::now();
 
-    // concatenate into the padded node space (bodies without clusters get empty rows)
-    std::vector<int> hOff(nPad_ + 1, 0), hRevOff(nPad_ + 1, 0), hIdx, hRev;
+    // concatenate into the padded node space (bodies without clusters get
+    // empty rows).  Device node d is the caller's node ord[d]; cluster member
+    // lists and the clusters containing a node keep the caller-order
+    // topology's order.  Offsets first, then the rows: on the host for
+    // host-built topologies, by a device permutation for device-built ones.
+    std::vector<int> hOff(nPad_ + 1, 0), hRevOff(nPad_ + 1, 0);
+    long long tot = 0, rtot = 0;
     for (int b = 0; b < B; ++b) {
         const VolumeTopo& t = volTopo_[b];
         const int nb = bodyBegin_[b], n = bodyCount_[b];
         const int padEnd = (b + 1 < B) ? bodyBegin_[b + 1] : nPad_;
-        // device node d is the caller's node ord[d]; cluster member lists and
-        // the clusters containing a node keep the caller-order topology's order
         const std::vector<int>& ord = ord_[b];
-        const std::vector<int>& inv = inv_[b];
         for (int d = 0; d < n; ++d) {
             const int i = ord[d];
-            hOff[nb + d] = (int)hIdx.size();
-            hRevOff[nb + d] = (int)hRev.size();
+            hOff[nb + d] = (int)tot;
+            hRevOff[nb + d] = (int)rtot;
             if (t.memOff.empty()) continue;
-            for (int k = t.memOff[i]; k < t.memOff[i + 1]; ++k) hIdx.push_back(nb + inv[t.memIdx[k]]);
-            for (int k = t.revOff[i]; k < t.revOff[i + 1]; ++k) hRev.push_back(nb + inv[t.revIdx[k]]);
+            tot += t.memOff[i + 1] - t.memOff[i];
+            rtot += t.revOff[i + 1] - t.revOff[i];
+            if (tot >= 0x7fffffffLL || rtot >= 0x7fffffffLL) { err = "volume: cluster lists too large"; return false; }
         }
         for (int i = nb + n; i <= padEnd && i <= nPad_; ++i) {
-            hOff[i] = (int)hIdx.size();
-            hRevOff[i] = (int)hRev.size();
+            hOff[i] = (int)tot;
+            hRevOff[i] = (int)rtot;
         }
     }
-    hOff[nPad_] = (int)hIdx.size();
-    hRevOff[nPad_] = (int)hRev.size();
+    hOff[nPad_] = (int)tot;
+    hRevOff[nPad_] = (int)rtot;
     volOff_.alloc(nPad_ + 1);
     revOff_.alloc(nPad_ + 1);
-    volIdx_.alloc(std::max<size_t>(hIdx.size(), 1));
-    revIdx_.alloc(std::max<size_t>(hRev.size(), 1));
+    volIdx_.alloc(std::max<size_t>((size_t)tot, 1));
+    revIdx_.alloc(std::max<size_t>((size_t)rtot, 1));
     if (!detRest_.p) {
         detRest_.alloc(nPad_); Vrest_.alloc(nPad_); restLen_.alloc(nPad_); matType_.alloc(nPad_);
         lamVol_.alloc(nPad_); bbC_.alloc(nPad_); bbS_.alloc(nPad_); bbS2_.alloc(nPad_); volJ_.alloc(nPad_);
@@ -125,10 +155,46 @@ bool Engine::ensureVolume(bool needAll, std::string& err)
     }
     XPBD_HIP(hipMemcpyAsync(volOff_.p, hOff.data(), sizeof(int) * (nPad_ + 1), hipMemcpyHostToDevice, stream_));
     XPBD_HIP(hipMemcpyAsync(revOff_.p, hRevOff.data(), sizeof(int) * (nPad_ + 1), hipMemcpyHostToDevice, stream_));
-    if (!hIdx.empty())
-        XPBD_HIP(hipMemcpyAsync(volIdx_.p, hIdx.data(), sizeof(int) * hIdx.size(), hipMemcpyHostToDevice, stream_));
-    if (!hRev.empty())
-        XPBD_HIP(hipMemcpyAsync(revIdx_.p, hRev.data(), sizeof(int) * hRev.size(), hipMemcpyHostToDevice, stream_));
+    std::vector<int> seg, rseg;
+    for (int b = 0; b < B; ++b) {
+        const VolumeTopo& t = volTopo_[b];
+        const int nb = bodyBegin_[b], n = bodyCount_[b];
+        if (t.memOff.empty() || n == 0) continue;
+        const std::vector<int>& ord = ord_[b];
+        const std::vector<int>& inv = inv_[b];
+        const int m0 = hOff[nb], m1 = hOff[nb + n], r0 = hRevOff[nb], r1 = hRevOff[nb + n];
+        if (t.listsOnDevice) {
+            const VolDevLists& dl = *volDev_[b];
+            DevBuf<int> dOrd, dInv, dMemOff, dRevOff;
+            dOrd.alloc(n); dInv.alloc(n); dMemOff.alloc(n + 1); dRevOff.alloc(n + 1);
+            if (!dOrd.p || !dInv.p || !dMemOff.p || !dRevOff.p) { err = "volume: hipMalloc failed"; return false; }
+            XPBD_HIP(hipMemcpyAsync(dOrd.p, ord.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream_));
+            XPBD_HIP(hipMemcpyAsync(dInv.p, inv.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream_));
+            XPBD_HIP(hipMemcpyAsync(dMemOff.p, t.memOff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, stream_));
+            XPBD_HIP(hipMemcpyAsync(dRevOff.p, t.revOff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, stream_));
+            if (m1 > m0) launch_permute_rows(stream_, n, nb, dOrd.p, dInv.p, dMemOff.p, dl.memIdx.p, volOff_.p, volIdx_.p);
+            if (r1 > r0) launch_permute_rows(stream_, n, nb, dOrd.p, dInv.p, dRevOff.p, dl.revIdx.p, revOff_.p, revIdx_.p);
+            XPBD_HIP(hipGetLastError());
+            XPBD_HIP(hipStreamSynchronize(stream_));   // before the temporaries go
+            continue;
+        }
+        seg.resize(m1 - m0);
+        rseg.resize(r1 - r0);
+        size_t w = 0, rw = 0;
+        for (int d = 0; d < n; ++d) {
+            const int i = ord[d];
+            for (int k = t.memOff[i]; k < t.memOff[i + 1]; ++k) seg[w++] = nb + inv[t.memIdx[k]];
+            for (int k = t.revOff[i]; k < t.revOff[i + 1]; ++k) rseg[rw++] = nb + inv[t.revIdx[k]];
+        }
+        if (!seg.empty())
+            XPBD_HIP(hipMemcpyAsync(volIdx_.p + m0, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice, stream_));
+        if (!rseg.empty())
+            XPBD_HIP(hipMemcpyAsync(revIdx_.p + r0, rseg.data(), sizeof(int) * rseg.size(), hipMemcpyHostToDevice, stream_));
+        XPBD_HIP(hipStreamSynchronize(stream_));   // seg and rseg are reused
+    }
+    if (getenv("XPBD_BUILD_TIMING"))
+        fprintf(stderr, "[build] volume concatenation: %.1f ms (%lld members, %lld reverse)\n",
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc0).count(), tot, rtot);
     XPBD_HIP(hipMemsetAsync(lamVol_.p, 0, sizeof(float) * nPad_, stream_));
     XPBD_HIP(hipMemsetAsync(bbC_.p, 0, sizeof(float4) * nPad_, stream_));
     // parameters of every body must be on the device before the precompute
@@ -212,10 +278,31 @@ bool Engine::volumeTopology(int b, int* leaders, int* memberCount, int* revCount
     return true;
 }
 
+bool Engine::volumeHostLists(int b, std::string& err)
+{
+    VolumeTopo& t = volTopo_[b];
+    if (!t.listsOnDevice) return true;
+    const int n = bodyCount_[b];
+    VolDevLists& dl = *volDev_[b];
+    const size_t m = (size_t)t.memOff[n], r = (size_t)t.revOff[n];
+    if (t.memIdx.size() != m) {
+        t.memIdx.resize(m);
+        if (m) XPBD_HIP(hipMemcpyAsync(t.memIdx.data(), dl.memIdx.p, sizeof(int) * m, hipMemcpyDeviceToHost, stream_));
+    }
+    t.revIdx.resize(r);
+    if (r) XPBD_HIP(hipMemcpyAsync(t.revIdx.data(), dl.revIdx.p, sizeof(int) * r, hipMemcpyDeviceToHost, stream_));
+    XPBD_HIP(hipStreamSynchronize(stream_));
+    t.listsOnDevice = false;
+    dl.memIdx.free();
+    dl.revIdx.free();
+    return true;
+}
+
 bool Engine::volumeMembers(int b, int* members, int len, int& total, std::string& err)
 {
     XPBD_ALLOC_SCOPE;
     if (!ensureVolume(true, err)) return false;
+    if (!volumeHostLists(b, err)) return false;
     const VolumeTopo& t = volTopo_[b];
     const int n = bodyCount_[b];
     total = t.memOff.empty() ? 0 : t.memOff[n];

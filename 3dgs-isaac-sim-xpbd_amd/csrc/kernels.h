@@ -209,6 +209,32 @@ size_t radix_sort_hist_size(int n);   // entries of the sort's histogram buffer
 // at once and the buffers keep their contents
 bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0, unsigned int* k1, int* v1,
                        unsigned int* hist, bool iota, const int* gate = nullptr);
+// Volume-cluster member lists on the device (one body, caller order): the
+// k-hop ball of every node, farthest-point sampled to cap.  Balls of more
+// than kVbMaxBall nodes are left to the host (count -1).
+constexpr int kVbMaxBall = 1536;
+constexpr int kVbGrid = 4096;   // waves of the persistent ball kernel
+struct VbArgs {
+    int n, k, cap;
+    const int* rowOff;   // [n + 1] symmetric CSR
+    const int* nbr;
+    const float* pos;    // [3n] rest positions
+    int* count;          // [n] members per node (-1: host)
+    const int* off;      // [n] first member of each device-built node (fill pass)
+    int* out;            // members
+};
+void launch_volume_balls(hipStream_t s, const VbArgs& a, bool fill);
+// Reverse lists of a member CSR (caller order): cnt[n] = clusters containing
+// each node; (keys, vals)[L + m] = the (node, cluster) pairs, own cluster
+// first, for a stable sort by node.  leaderPos: rank of each leader among the
+// L leaders, -1 otherwise (null: every node leads, L = n).
+void launch_volume_reverse(hipStream_t s, int n, const int* leaderPos, int L, const int* memOff, const int* memIdx,
+                           int m, int* cnt, unsigned int* keys, int* vals);
+// dst rows nb + d (offsets dstOff, padded device order) = caller rows ord[d]
+// of (srcOff, srcIdx) with entries mapped to nb + inv[x].
+void launch_permute_rows(hipStream_t s, int n, int nb, const int* ord, const int* inv, const int* srcOff,
+                         const int* srcIdx, const int* dstOff, int* dst);
+
 // Exclusive prefix sum of in[0, n) into out[0, n] (out[n] = total); tmp holds
 // exclusive_scan_tmp_size(n) ints.
 size_t exclusive_scan_tmp_size(int n);

@@ -13,12 +13,25 @@
 // deterministic and identical to the reference's for the same graph.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
 #include <thread>
 #include <vector>
 
 #include "engine.h"
 
 namespace xpbd {
+
+#define XPBD_HIP(call)                                                                    \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            err = std::string(#call) + ": " + hipGetErrorString(e_);                      \
+            return false;                                                                 \
+        }                                                                                 \
+    } while (0)
 
 namespace {
 
@@ -101,8 +114,8 @@ void ball_members(const BodyInput& in, int i, int k, int cap, BallScratch& w, st
 
 void VolumeTopo::finish(int n)
 {
-    static std::atomic<long long> versions{0};   // engines may build on several threads
-    version = ++versions;
+    touch();
+    listsOnDevice = false;
     revOff.assign(n + 1, 0);
     std::vector<int> cnt(n, 0);
     for (int i = 0; i < n; ++i) {
@@ -116,6 +129,202 @@ void VolumeTopo::finish(int n)
         if (isLeader[i]) revIdx[cur[i]++] = i;
     for (int i = 0; i < n; ++i)
         for (int k = memOff[i]; k < memOff[i + 1]; ++k) revIdx[cur[memIdx[k]]++] = i;
+}
+
+void VolumeTopo::touch()
+{
+    static std::atomic<long long> versions{0};   // engines may build on several threads
+    version = ++versions;
+}
+
+namespace {
+
+// Leader selection of the greedy cover (leaderR >= 2) over every node's ball
+// (row i of ballOff/ballIdx): the rules of build_volume_reference below.
+void greedy_cover(const BodyInput& in, int leaderR, const std::vector<int>& ballOff, const int* ballIdx,
+                  std::vector<char>& isLeader)
+{
+    const int n = in.n;
+    const int target = std::max(1, std::min(6, leaderR));
+    std::vector<int> cover(n, 0), stampR(n, -1), frontier, next;
+    isLeader.assign(n, 0);
+    auto lead = [&](int i) {
+        isLeader[i] = 1;
+        ++cover[i];
+        for (int q = ballOff[i]; q < ballOff[i + 1]; ++q) ++cover[ballIdx[q]];
+    };
+    for (int seed = 0; seed < n; ++seed) {
+        if (cover[seed] >= target || stampR[seed] == -2) continue;
+        lead(seed);
+        frontier.assign(1, seed);
+        stampR[seed] = -2;
+        for (int d = 0; d < leaderR - 1 && !frontier.empty(); ++d) {
+            next.clear();
+            for (int f : frontier)
+                for (int e = in.rowOff[f]; e < in.rowOff[f + 1]; ++e) {
+                    const int j = in.nbr[e];
+                    if (j < 0 || j >= n || stampR[j] == -2) continue;
+                    stampR[j] = -2;
+                    next.push_back(j);
+                }
+            frontier.swap(next);
+        }
+    }
+    bool progressed = true;
+    while (progressed) {
+        progressed = false;
+        for (int i = 0; i < n; ++i) {
+            if (cover[i] >= target || isLeader[i]) continue;
+            lead(i);
+            progressed = true;
+        }
+    }
+}
+
+template <typename T>
+void swap_bufs(DevBuf<T>& a, DevBuf<T>& b)
+{
+    std::swap(a.p, b.p);
+    std::swap(a.n, b.n);
+}
+
+}  // namespace
+
+bool device_volume_build(const BodyInput& in, int k, int cap, int leaderR, hipStream_t s, VolumeTopo& t,
+                         DevBuf<int>& memIdxD, DevBuf<int>& revIdxD, std::string& err)
+{
+    const int n = in.n;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto ms = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); };
+    const bool timing = getenv("XPBD_BUILD_TIMING") != nullptr;
+    t.isLeader.clear();
+    t.memOff.assign(n + 1, 0);
+    t.memIdx.clear();
+    t.revOff.assign(n + 1, 0);
+    t.revIdx.clear();
+    t.minMembers = 4;
+    t.listsOnDevice = true;
+    t.touch();
+    memIdxD.free();
+    revIdxD.free();
+    if (n <= 0) return true;
+    // 1. every node's ball: counts, offsets, lists
+    DevBuf<int> rowOff, nbr, count, ballOff, ballIdx;
+    DevBuf<float> pos;
+    const size_t e = in.nbr.size();
+    rowOff.alloc(n + 1); nbr.alloc(std::max<size_t>(e, 1)); pos.alloc(3 * (size_t)n); count.alloc(n);
+    ballOff.alloc(n + 1);
+    if (!rowOff.p || !nbr.p || !pos.p || !count.p || !ballOff.p) { err = "volume build: hipMalloc failed"; return false; }
+    XPBD_HIP(hipMemcpyAsync(rowOff.p, in.rowOff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    if (e) XPBD_HIP(hipMemcpyAsync(nbr.p, in.nbr.data(), sizeof(int) * e, hipMemcpyHostToDevice, s));
+    XPBD_HIP(hipMemcpyAsync(pos.p, in.pos.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+    VbArgs a{};
+    a.n = n; a.k = k; a.cap = cap;
+    a.rowOff = rowOff.p; a.nbr = nbr.p; a.pos = pos.p; a.count = count.p; a.off = ballOff.p; a.out = nullptr;
+    launch_volume_balls(s, a, false);
+    XPBD_HIP(hipGetLastError());
+    std::vector<int> cnt(n);
+    XPBD_HIP(hipMemcpyAsync(cnt.data(), count.p, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    XPBD_HIP(hipStreamSynchronize(s));
+    if (timing) fprintf(stderr, "[build] device ball counts: %.1f ms\n", ms());
+    // balls too large for the device: the host builds them
+    std::vector<std::pair<int, std::vector<int>>> host;
+    {
+        BallScratch w;
+        for (int i = 0; i < n; ++i)
+            if (cnt[i] < 0) {
+                if (w.stamp.empty()) w.stamp.assign(n, -1);
+                host.emplace_back(i, std::vector<int>());
+                ball_members(in, i, k, cap, w, host.back().second);
+                cnt[i] = (int)host.back().second.size();
+            }
+    }
+    std::vector<int> bOff(n + 1, 0);
+    for (int i = 0; i < n; ++i) bOff[i + 1] = bOff[i] + cnt[i];
+    if (bOff[n] < 0) { err = "volume build: member lists too large"; return false; }
+    const int total = bOff[n];
+    ballIdx.alloc(std::max(total, 1));
+    if (!ballIdx.p) { err = "volume build: hipMalloc failed"; return false; }
+    XPBD_HIP(hipMemcpyAsync(ballOff.p, bOff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    if (total > 0) {
+        a.out = ballIdx.p;
+        launch_volume_balls(s, a, true);   // rows of host-built balls keep count -1: skipped
+        XPBD_HIP(hipGetLastError());
+        for (const auto& h : host)
+            if (!h.second.empty())
+                XPBD_HIP(hipMemcpyAsync(ballIdx.p + bOff[h.first], h.second.data(), sizeof(int) * h.second.size(),
+                                        hipMemcpyHostToDevice, s));
+    }
+    XPBD_HIP(hipStreamSynchronize(s));
+    if (timing) fprintf(stderr, "[build] device balls: %.1f ms (%d entries, %zu on the host)\n", ms(), total, host.size());
+    // 2. leaders: every node (r <= 1), else the greedy cover over the balls
+    DevBuf<int> memOffD, leaderPos;
+    int L = n;
+    if (leaderR <= 1) {
+        t.isLeader.assign(n, 1);
+        t.memOff = bOff;
+        swap_bufs(memIdxD, ballIdx);
+        swap_bufs(memOffD, ballOff);
+    } else {
+        // every node's ball, unset until the copy lands (no zero fill of a
+        // buffer this size)
+        std::unique_ptr<int[]> hbBuf(new int[std::max(total, 1)]);
+        int* hb = hbBuf.get();
+        if (total > 0)
+            XPBD_HIP(hipMemcpyAsync(hb, ballIdx.p, sizeof(int) * total, hipMemcpyDeviceToHost, s));
+        XPBD_HIP(hipStreamSynchronize(s));
+        greedy_cover(in, leaderR, bOff, hb, t.isLeader);
+        std::vector<int> lp(n, -1);
+        L = 0;
+        for (int i = 0; i < n; ++i) {
+            t.memOff[i + 1] = t.memOff[i] + (t.isLeader[i] ? cnt[i] : 0);
+            if (t.isLeader[i]) lp[i] = L++;
+        }
+        t.memIdx.resize(t.memOff[n]);
+        for (int i = 0; i < n; ++i)
+            if (t.isLeader[i]) std::copy(hb + bOff[i], hb + bOff[i + 1], t.memIdx.begin() + t.memOff[i]);
+        memIdxD.alloc(std::max<size_t>(t.memIdx.size(), 1));
+        memOffD.alloc(n + 1);
+        leaderPos.alloc(n);
+        if (!memIdxD.p || !memOffD.p || !leaderPos.p) { err = "volume build: hipMalloc failed"; return false; }
+        if (!t.memIdx.empty())
+            XPBD_HIP(hipMemcpyAsync(memIdxD.p, t.memIdx.data(), sizeof(int) * t.memIdx.size(), hipMemcpyHostToDevice, s));
+        XPBD_HIP(hipMemcpyAsync(memOffD.p, t.memOff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+        XPBD_HIP(hipMemcpyAsync(leaderPos.p, lp.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        if (timing) fprintf(stderr, "[build] greedy cover: %.1f ms (%d leaders)\n", ms(), L);
+    }
+    ballIdx.free();
+    ballOff.free();
+    rowOff.free(); nbr.free(); pos.free();
+    // 3. reverse lists: (node, cluster) pairs sorted stably by node
+    const int m = t.memOff[n];
+    const long long pairs = (long long)L + m;
+    if (pairs >= 0x7fffffffLL) { err = "volume build: reverse lists too large"; return false; }
+    DevBuf<unsigned int> keys, keysAlt, hist;
+    DevBuf<int> vals, valsAlt, revCnt, revOffD, scanTmp;
+    keys.alloc(std::max<long long>(pairs, 1)); keysAlt.alloc(std::max<long long>(pairs, 1));
+    vals.alloc(std::max<long long>(pairs, 1)); valsAlt.alloc(std::max<long long>(pairs, 1));
+    hist.alloc(radix_sort_hist_size((int)std::max<long long>(pairs, 1)));
+    revCnt.alloc(n); revOffD.alloc(n + 1); scanTmp.alloc(exclusive_scan_tmp_size(n));
+    if (!keys.p || !keysAlt.p || !vals.p || !valsAlt.p || !hist.p || !revCnt.p || !revOffD.p || !scanTmp.p) {
+        err = "volume build: hipMalloc failed";
+        return false;
+    }
+    launch_volume_reverse(s, n, leaderR <= 1 ? nullptr : leaderPos.p, L, memOffD.p, memIdxD.p, m, revCnt.p, keys.p,
+                          vals.p);
+    XPBD_HIP(hipGetLastError());
+    launch_exclusive_scan(s, n, revCnt.p, revOffD.p, scanTmp.p);
+    int bits = 0;
+    while ((1LL << bits) < (long long)n) ++bits;
+    const bool inAlt = launch_radix_sort(s, (int)pairs, bits, keys.p, vals.p, keysAlt.p, valsAlt.p, hist.p, false);
+    XPBD_HIP(hipGetLastError());
+    if (inAlt) swap_bufs(vals, valsAlt);
+    swap_bufs(revIdxD, vals);
+    XPBD_HIP(hipMemcpyAsync(t.revOff.data(), revOffD.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, s));
+    XPBD_HIP(hipStreamSynchronize(s));
+    if (t.revOff[n] != pairs) { err = "volume build: reverse lists do not add up"; return false; }
+    if (timing) fprintf(stderr, "[build] device reverse lists: %.1f ms (%lld pairs)\n", ms(), pairs);
+    return true;
 }
 
 void build_volume_reference(const BodyInput& in, int k, int cap, int leaderR, VolumeTopo& t)
@@ -186,7 +395,11 @@ void build_volume_reference(const BodyInput& in, int k, int cap, int leaderR, Vo
     t.memIdx.resize(t.memOff[n]);
     for (int i = 0; i < n; ++i) std::copy(members[i].begin(), members[i].end(), t.memIdx.begin() + t.memOff[i]);
     t.minMembers = 4;
+    const auto t0 = std::chrono::steady_clock::now();
     t.finish(n);
+    if (getenv("XPBD_BUILD_TIMING"))
+        fprintf(stderr, "[build] reverse lists: %.1f ms\n",
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
 }
 
 }  // namespace xpbd

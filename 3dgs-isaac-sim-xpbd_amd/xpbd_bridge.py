@@ -124,6 +124,7 @@ ABI_OPTIONAL = {
     "xpbd_ext_set_self_collision_fine": (_INT, [_INT]),
     "xpbd_ext_set_group_lanes": (_INT, [_INT]),
     "xpbd_ext_self_collision_paths": (_INT, [_PTR]),
+    "xpbd_ext_set_device_builders": (_INT, [_INT]),
     "xpbd_ext_autotune": (_INT, [_INT, _INT, ctypes.c_char_p, _INT]),
     "xpbd_ext_volume_members": (_INT, [_PTR, _INT]),
     "xpbd_ext_frame_outputs": (_INT, [_PTR, _PTR, _PTR, _INT, _FLT, _PTR, _FLT, ctypes.POINTER(ctypes.c_double), _INT]),
@@ -464,6 +465,10 @@ class XPBD:
         out = np.zeros(4, np.int32)
         self._check(self.lib.xpbd_ext_self_collision_paths(_ptr(out)), "xpbd_ext_self_collision_paths")
         return tuple(int(v) for v in out)
+
+    def ext_set_device_builders(self, mode):
+        """Where k-ring volume member lists are built: 0 host, 1 GPU, -1 by size."""
+        self._check(self.lib.xpbd_ext_set_device_builders(int(mode)), "xpbd_ext_set_device_builders")
 
     def ext_set_shape_clusters(self, cluster_of=None):
         """Shape matching over node clusters (cluster id per node, -1 = none); None = 1-ring."""

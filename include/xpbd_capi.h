@@ -219,6 +219,15 @@ int xpbd_ext_self_collision_grid_info(float* info);
  * nodes outside the grid (inactive or non-finite). */
 int xpbd_ext_self_collision_paths(int* out);
 
+/* Where the volume clusters of the k-ring rule (xpbd_set_volume) get their
+ * member lists: 0 the host, 1 the GPU (each node's k-hop ball and its
+ * farthest-point sample on the device; balls of more than 1536 nodes and the
+ * greedy leader cover stay on the host), -1 (default) the GPU for bodies of
+ * at least 16384 Gaussians.  The lists are identical either way.  A change
+ * takes effect at the next step.  No counterpart in the reference, whose
+ * rebuildVolumeClusters (forward.cu:9814-10240) builds them on the host. */
+int xpbd_ext_set_device_builders(int mode);
+
 /* Shape matching over node clusters (clusterOf[n]: cluster id, -1 = none;
  * e.g. k-means clusters of ~64 nodes) instead of the graph 1-ring, with the
  * same compliance / blend projection (forward.cu:4972-4988).  Clusters of fewer
