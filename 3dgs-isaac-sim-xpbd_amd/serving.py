@@ -100,15 +100,16 @@ class Request:
 
 
 def hbm_bytes(n, table):
-    """Device memory of one object: ~1.7 KB per Gaussian (state, sliced
-    distance entries, volume and shape clusters, self-collision buffers,
-    candidate lists and the sorted-candidate pool, shapes) plus 8 B per cell of
-    its two hash tables (the coarse one and the nextpow2(2n) predicted-position
-    one)."""
+    """Device memory of one object: ~2.5 KB per Gaussian (state, sliced
+    distance entries, volume and shape clusters, self-collision buffers and
+    candidate lists, the lane scan's 160 unsorted (key, distance) entries of
+    8 B, the predicted-position grid's two records, shapes) plus 8 B per cell
+    of its two hash tables (the coarse one and the nextpow2(2n)
+    predicted-position one)."""
     fine = 1024
     while fine < 2 * n:
         fine <<= 1
-    return 1700 * n + 8 * (table + fine)
+    return 2500 * n + 8 * (table + fine)
 
 
 class _Segment:
