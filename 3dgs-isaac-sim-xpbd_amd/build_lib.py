@@ -65,9 +65,28 @@ def build_oracle():
     return ORACLE_LIB
 
 
+ASAN_DIR = os.path.join(ORACLE_DIR, "_asan")
+ASAN_BIN = os.path.join(ASAN_DIR, "oracle_sanitized")
+
+
+def build_oracle_asan():
+    """The oracle and oracle/sanitize/driver.cpp in one host executable with
+    AddressSanitizer and UndefinedBehaviorSanitizer (no GPU code involved)."""
+    os.makedirs(ASAN_DIR, exist_ok=True)
+    srcs = sorted(os.path.join(ORACLE_DIR, f) for f in os.listdir(ORACLE_DIR) if f.endswith(".cpp"))
+    srcs.append(os.path.join(ORACLE_DIR, "sanitize", "driver.cpp"))
+    tmp = ASAN_BIN + ".tmp"
+    _run(["g++", "-O1", "-g", "-std=c++17", "-ffp-contract=off", "-fno-omit-frame-pointer",
+          "-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-I" + INCLUDE, "-o", tmp] + srcs)
+    os.replace(tmp, ASAN_BIN)
+    return ASAN_BIN
+
+
 if __name__ == "__main__":
     what = sys.argv[1:] or ["product", "oracle"]
     if "oracle" in what:
         print(build_oracle())
+    if "oracle-asan" in what:
+        print(build_oracle_asan())
     if "product" in what:
         print(build_product())
