@@ -352,7 +352,7 @@ bool Engine::selfCollisionPaths(int b, int out[4], std::string& err)
     for (int i = 0; i < n; ++i) {
         const int c = acc[i];
         if (c == kScAccLane) ++out[0];
-        else if (c == kScAccRule || c == kScAccRuleHi) ++out[1];
+        else if (c == kScAccRule) ++out[1];
         else if (c == kScAccCoarse) ++out[2];
         else ++out[3];
     }

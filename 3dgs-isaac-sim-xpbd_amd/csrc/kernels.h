@@ -114,7 +114,6 @@ constexpr int kSelfColCap = 32;   // forward.cu:223 SELFCOL_MAX_CONTACTS
 constexpr int kScAccCoarse = -2;   // the coarse walk (the reference's own)
 constexpr int kScAccLane = -5;     // the lane fine scan
 constexpr int kScAccRule = -6;     // the lane fine scan, more than 32 candidates (slot rule)
-constexpr int kScAccRuleHi = -7;   // the same (ranked list stored in the upper half of the list)
 constexpr int kScAccLong = -8;     // the lane fine scan, list ranked by the group kernel (transient)
 constexpr int kLaneList = 160;     // candidates the lane fine scan keeps per node (more: the coarse walk)
 

@@ -189,7 +189,6 @@ constexpr int kScNeedCoarse = kScAccCoarse;  // the node takes the coarse walk
 constexpr int kScLane = kScAccLane;          // the lane fine scan lists its candidates
 constexpr int kScRule = kScAccRule;          // more than 32 candidates: the slot-rule kernel fills its slots
                                              // from the ranked list at list entry 0
-constexpr int kScRuleHi = kScAccRuleHi;      // the same, ranked list at list entry kLaneList / 2
 constexpr int kScLong = kScAccLong;          // a list too long to rank in one lane: the group ranking kernel
 __device__ __forceinline__ unsigned int sc_hash(int cx, int cy, int cz, unsigned int mask)
 {
@@ -981,16 +980,14 @@ __device__ __forceinline__ void sc_lane_first_offsets(int4 ci, unsigned int tmas
     }
 }
 
-// Ranking of the lane fine scan's lists, one lane per list in the grid's own
-// order (the list length and entries of the 64 lanes are contiguous reads).
-// Each lane sorts its keys (all distinct; padding ~0 sorts last) with a
-// bitonic network in registers: N keys, N the wave's power of two at or above
-// its longest list (up to kRankNet), min/max steps only.  With at most 32
-// candidates the sorted keys are the slots; with more, the node ids in key
-// order go to list entries kLaneList / 2 + rank and the node is flagged for
-// the slot-rule kernel.  Longer lists are flagged for the group ranking kernel.
-constexpr int kRankNet = 64;
-static_assert(kRankNet <= kLaneList / 2, "ranked copies sit in the upper half of a list");
+// Ranking of the lane fine scan's lists of at most 32 candidates, one lane
+// per list in the grid's own order (the list length and entries of the 64
+// lanes are contiguous reads).  Each lane sorts its keys (all distinct;
+// padding ~0 sorts last) with a bitonic network in registers: N keys, N the
+// wave's power of two at or above its longest such list, min/max steps only;
+// the sorted keys are the slots.  Longer lists are flagged for the group
+// ranking kernel.
+constexpr int kRankNet = kSelfColCap;
 
 template <int N>
 __device__ __forceinline__ void sc_bitonic(unsigned int* k)
@@ -1011,24 +1008,17 @@ __device__ __forceinline__ void sc_bitonic(unsigned int* k)
             }
 }
 
-// Load, sort and hand out one lane's list with an N-key network.
+// Load, sort and hand out one lane's list (m <= 32) with an N-key network.
 template <int N>
-__device__ __forceinline__ void sc_rank_list(SelfColArgs& a, int i, int m, size_t L0, const int2* __restrict__ LK,
-                                             int2* __restrict__ LKw)
+__device__ __forceinline__ void sc_rank_list(SelfColArgs& a, int i, int m, size_t L0, const int2* __restrict__ LK)
 {
     unsigned int k[N];
 #pragma unroll
     for (int x = 0; x < N; ++x) k[x] = x < m ? (unsigned int)LK[L0 + (size_t)x * 64].x : 0xffffffffu;
     sc_bitonic<N>(k);
-    if (m <= kSelfColCap) {
 #pragma unroll
-        for (int x = 0; x < N && x < kSelfColCap; ++x)
-            if (x < m) a.contacts[sc_slot(i, x)] = a.devOfCaller[k[x] & 0x07ffffffu];
-    } else {
-#pragma unroll
-        for (int x = 0; x < N; ++x)
-            if (x < m) LKw[L0 + (size_t)(kLaneList / 2 + x) * 64] = make_int2(a.devOfCaller[k[x] & 0x07ffffffu], 0);
-    }
+    for (int x = 0; x < N; ++x)
+        if (x < m) a.contacts[sc_slot(i, x)] = a.devOfCaller[k[x] & 0x07ffffffu];
 }
 
 __global__ void __launch_bounds__(256) sc_lane_rank_kernel(SelfColArgs a, const int4* __restrict__ FC,
@@ -1054,57 +1044,50 @@ __global__ void __launch_bounds__(256) sc_lane_rank_kernel(SelfColArgs a, const 
     for (int s = 32; s >= 1; s >>= 1) need = max(need, __shfl_xor(need, s, 64));
     need = __builtin_amdgcn_readfirstlane(need);
     if (here && m > 0) {
-        if (need <= 8) sc_rank_list<8>(a, i, m, L0, LK, LK);
-        else if (need <= 16) sc_rank_list<16>(a, i, m, L0, LK, LK);
-        else if (need <= 32) sc_rank_list<32>(a, i, m, L0, LK, LK);
-        else sc_rank_list<64>(a, i, m, L0, LK, LK);
+        if (need <= 8) sc_rank_list<8>(a, i, m, L0, LK);
+        else if (need <= 16) sc_rank_list<16>(a, i, m, L0, LK);
+        else sc_rank_list<32>(a, i, m, L0, LK);
     }
     if (here) {
-        a.counts[i] = m < kSelfColCap ? m : kSelfColCap;
-        a.overflow[i] = m > kSelfColCap ? m - kSelfColCap : 0;
-        if (m > kSelfColCap) a.accCount[i] = kScRuleHi;
+        a.counts[i] = m;
+        a.overflow[i] = 0;
     }
     // the contact push of object shape matching needs to know whether the
-    // body has any candidate
-    const unsigned long long any = __ballot(here && m > 0);
-    if ((threadIdx.x & 63) == 0 && any) {
-        const int lb = __builtin_amdgcn_readlane(b, __builtin_ctzll(any));
-        if (a.bodies[lb].objOn) a.scCounters[3 * lb] = 1;
-    }
+    // body has any candidate (a wave of fine positions may span two bodies)
+    if (listed && m > 0 && a.bodies[b].objOn) a.scCounters[3 * b] = 1;
 }
 
-// Slots of the nodes whose lane-scan lists are too long for one lane to rank
-// (more than kRankRow), 16 lanes per node (node order).  The node's (key,
-// squared distance) list is spread over the group's lanes; its keys (all
-// distinct) are ranked in LDS; the list goes back to its entries in rank
-// order as (id, priority = |p_i - p_j|) and the node is flagged for the
-// slot-rule kernel.  Nodes outside the grid get empty lists.
+// Ranking of the lane-scan lists of more than 32 candidates, 16 lanes per
+// list, lists in the grid's own order (the four lists of a wave are
+// neighbouring fine positions, so each step of their reads and writes shares
+// cache lines).  The node's (key, squared distance) list is spread over the
+// group's lanes; its keys (all distinct) are ranked by counting in LDS; the
+// list goes back to its entries in rank order as (id, priority = |p_i -
+// p_j|, from the squared distance the scan tested) and the node is flagged
+// for the slot-rule kernel.
 constexpr int kLaneGroup = 16;
 constexpr int kLanePer = kLaneList / kLaneGroup;
 
-__global__ void __launch_bounds__(256) sc_slots_kernel(int nPad, SelfColArgs a)
+__global__ void __launch_bounds__(256) sc_slots_kernel(SelfColArgs a, const int4* __restrict__ FC,
+                                                      const int* __restrict__ FS)
 {
     __shared__ unsigned int skey[256 / kLaneGroup][kLaneList];
     const int sub = threadIdx.x % kLaneGroup, grp = threadIdx.x / kLaneGroup;
-    const int i = blockIdx.x * (256 / kLaneGroup) + grp;   // nPad is a multiple of 64: whole blocks
-    if (i >= nPad) return;
-    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
-    const BodyDev& B = a.bodies[b];
-    if (!B.scOn || !B.scFine) return;
-    const int n = i < B.nodeEnd ? a.accCount[i] : -1;
-    const bool lane = n == kScLong;
-    int m = 0;
-    size_t L0 = 0;
-    if (lane) {
-        const int t = a.finePos[i];
-        m = a.laneN[t];
-        L0 = sc_lane_at(t, 0);
+    const int t = blockIdx.x * (256 / kLaneGroup) + grp;
+    int i = 0, m = 0;
+    bool lane = false;
+    if (t < FS[a.fineTotal]) {
+        i = FC[t].w;
+        lane = a.accCount[i] == kScLong;
+        if (lane) m = a.laneN[t];
     }
+    const size_t L0 = sc_lane_at(t, 0);
     // keys per lane in use: the wave's largest
     int kn = (m + kLaneGroup - 1) / kLaneGroup;
 #pragma unroll
     for (int s = kLaneGroup; s < 64; s <<= 1) kn = max(kn, __shfl_xor(kn, s, 64));
     kn = __builtin_amdgcn_readfirstlane(kn);
+    if (kn == 0) return;   // wave-uniform
     unsigned int key[kLanePer];
     int jv[kLanePer], rank[kLanePer];
 #pragma unroll
@@ -1116,7 +1099,7 @@ __global__ void __launch_bounds__(256) sc_slots_kernel(int nPad, SelfColArgs a)
         if (k < kn && idx < m) {
             const int2 e = a.laneKey[L0 + (size_t)idx * 64];
             key[k] = (unsigned int)e.x;
-            jv[k] = e.y;   // squared distance bits until the ids are looked up
+            jv[k] = e.y;   // squared distance bits
             skey[grp][idx] = key[k];
         }
     }
@@ -1130,53 +1113,32 @@ __global__ void __launch_bounds__(256) sc_slots_kernel(int nPad, SelfColArgs a)
             if (k < kn) rank[k] += kf < key[k] ? 1 : 0;
     }
     if (lane) {
-        // node ids from the caller-order positions in the keys
-        int id[kLanePer];
+        // the list in rank order with its priorities, for the slot rule
 #pragma unroll
         for (int k = 0; k < kLanePer; ++k)
-            id[k] = (k < kn && sub + kLaneGroup * k < m) ? a.devOfCaller[key[k] & 0x07ffffffu] : 0;
-        if (m <= kSelfColCap) {
-#pragma unroll
-            for (int k = 0; k < kLanePer; ++k)
-                if (k < kn && sub + kLaneGroup * k < m) a.contacts[sc_slot(i, rank[k])] = id[k];
-        } else {
-            // the list in rank order with its priorities, for the slot rule
-#pragma unroll
-            for (int k = 0; k < kLanePer; ++k)
-                if (k < kn && sub + kLaneGroup * k < m)
-                    a.laneKey[L0 + (size_t)rank[k] * 64] = make_int2(id[k], __float_as_int(sqrtf(__int_as_float(jv[k]))));
+            if (k < kn && sub + kLaneGroup * k < m)
+                a.laneKey[L0 + (size_t)rank[k] * 64] =
+                    make_int2(a.devOfCaller[key[k] & 0x07ffffffu], __float_as_int(sqrtf(__int_as_float(jv[k]))));
+        if (sub == 0) {
+            a.counts[i] = kSelfColCap;
+            a.overflow[i] = m - kSelfColCap;
+            a.accCount[i] = kScRule;
         }
     }
-    if (sub == 0 && (lane || n == 0)) {
-        const int cnt = m < kSelfColCap ? m : kSelfColCap;
-        a.counts[i] = cnt;
-        a.overflow[i] = m - cnt;
-        if (m > kSelfColCap) a.accCount[i] = kScRule;
-    }
-    const bool any = __ballot(m > 0) != 0ull;
-    if ((threadIdx.x & 63) == 0 && any && B.objOn) a.scCounters[3 * b] = 1;
 }
 
 // The slot rule for the flagged nodes (more than 32 candidates), one thread
-// per node over the list a ranking kernel left in rank order.
-__global__ void __launch_bounds__(256) sc_slot_rule_kernel(int nPad, SelfColArgs a, const float4* __restrict__ P)
+// per node in the grid's own order (the lists of a wave's nodes share cache
+// lines) over the (id, priority) list the group ranking left in rank order.
+__global__ void __launch_bounds__(256) sc_slot_rule_kernel(SelfColArgs a, const int4* __restrict__ FC,
+                                                          const int* __restrict__ FS)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nPad) return;
-    const int b = __builtin_amdgcn_readfirstlane(a.tileBody[i / kTile]);
-    const BodyDev& B = a.bodies[b];
-    const int c = (B.scOn && B.scFine && i < B.nodeEnd) ? a.accCount[i] : 0;
-    if (c != kScRule && c != kScRuleHi) return;
-    const size_t L0 = sc_lane_at(a.finePos[i], c == kScRuleHi ? kLaneList / 2 : 0);
-    const int n = kSelfColCap + a.overflow[i];
-    // priority |p_i - p_j| from the positions the scan tested (the same operations)
-    const float4 pi = P[i];
-    sc_fill_slots(i, n, a, [&](int r) {
-        const int j = a.laneKey[L0 + (size_t)r * 64].x;
-        const float4 pj = P[j];
-        const float ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
-        return make_int2(j, __float_as_int(sqrtf(ex * ex + ey * ey + ez * ez)));
-    });
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= FS[a.fineTotal]) return;
+    const int i = FC[t].w;
+    if (a.accCount[i] != kScRule) return;
+    const size_t L0 = sc_lane_at(t, 0);
+    sc_fill_slots(i, kSelfColCap + a.overflow[i], a, [&](int r) { return a.laneKey[L0 + (size_t)r * 64]; });
 }
 
 // Statistics of one body's last build (forward.cu:11837-11849 reports them):
@@ -1211,10 +1173,18 @@ void launch_selfcol_stats(hipStream_t s, int nodeBegin, int nodeEnd, const SelfC
     if (n > 0) sc_stats_kernel<<<(n + 255) / 256, 256, 0, s>>>(nodeBegin, nodeEnd, a, out);
 }
 
-__global__ void sc_fill_accum_kernel(int nPad, float3* accum)
+// Clears the contact-push accumulators; nodes of predicted-position-grid
+// bodies that have no grid key (inactive, non-finite) get empty lists.
+__global__ void __launch_bounds__(256) sc_finish_kernel(int nPad, SelfColArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nPad) accum[i] = make_float3(0.0f, 0.0f, 0.0f);
+    if (i >= nPad) return;
+    a.accum[i] = make_float3(0.0f, 0.0f, 0.0f);
+    const BodyDev& B = a.bodies[a.tileBody[i / kTile]];
+    if (B.scOn && B.scFine && i < B.nodeEnd && a.accCount[i] == 0) {
+        a.counts[i] = 0;
+        a.overflow[i] = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1508,11 +1478,11 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
         if (laneScan)
             sc_lane_rank_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(a, a.fineC, a.fstart, a.laneKey);
         if (mark) mark(markCtx, "selfcol_build_rank_long");
-        sc_slots_kernel<<<nbs(na.nPad, 256 / kLaneGroup), 256, 0, s>>>(na.nPad, a);
+        sc_slots_kernel<<<nbs(na.nPad, 256 / kLaneGroup), 256, 0, s>>>(a, a.fineC, a.fstart);
         if (mark) mark(markCtx, "selfcol_build_slot_rule");
-        sc_slot_rule_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, P);
+        sc_slot_rule_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(a, a.fineC, a.fstart);
     }
-    sc_fill_accum_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a.accum);
+    sc_finish_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a);
     *sortedIdxOut = vs;
     return inAlt;
 }
