@@ -167,10 +167,11 @@ public:
     // (Z-order) curve of the rest positions so that graph neighbours share
     // cache lines.  Every host-facing array stays in the caller's order.
     void setNodeOrder(int mode) { nodeOrderMode_ = mode; }
-    // Where the k-ring volume clusters' member lists are built: 0 on the
-    // host, 1 on the GPU (the same lists), -1 (default) on the GPU for bodies
-    // of at least kDeviceBuildMin nodes.  A change rebuilds the k-ring
-    // topologies at the next step.
+    // Where the k-ring volume clusters' member lists and the Gauss-Seidel
+    // colourings are built: 0 on the host, 1 on the GPU (the same lists and
+    // colours), -1 (default) on the GPU for bodies of at least
+    // kDeviceBuildMin nodes.  A change rebuilds the k-ring topologies at the
+    // next step.
     void setDeviceBuilders(int mode);
     static constexpr int kDeviceBuildMin = 16384;
     bool permuted() const { return permuted_; }
@@ -430,6 +431,10 @@ private:
     std::vector<std::unique_ptr<VolDevLists>> volDev_;   // per body
     // Host copies of a device-built topology's member and reverse lists.
     bool volumeHostLists(int b, std::string& err);
+    // Gauss-Seidel colourings of body b, on the GPU or the host (the same
+    // colours; see setDeviceBuilders).
+    bool buildDistanceGS(int b, std::string& err);
+    bool buildVolumeGS(int b, std::string& err);
     bool volDeviceValid_ = false;
     int volActive_ = 0;
     double volAvgMembers_ = 0.0;

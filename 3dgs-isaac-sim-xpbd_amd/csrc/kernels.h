@@ -234,6 +234,24 @@ void launch_volume_reverse(hipStream_t s, int n, const int* leaderPos, int L, co
 void launch_permute_rows(hipStream_t s, int n, int nb, const int* ord, const int* inv, const int* srcOff,
                          const int* srcIdx, const int* dstOff, int* dst);
 
+// Gauss-Seidel colourings on the device (the reference's greedy colourings,
+// computed in rounds; colours up to 64 kGsWords - 1, more: flags[1] set).
+constexpr int kGsWords = 4;
+void launch_gs_entry_keys(hipStream_t s, int n, const int* rowOff, const int* nbr, unsigned int* ka, unsigned int* kb,
+                          int* val);
+void launch_gs_gather_key(hipStream_t s, int m, const unsigned int* ka, const int* val, unsigned int* key);
+void launch_gs_heads(hipStream_t s, int n, int m, const unsigned int* sa, const unsigned int* kb, const int* val,
+                     int* head);
+void launch_gs_edges(hipStream_t s, int n, int m, const unsigned int* sa, const unsigned int* kb, const int* val,
+                     const int* head, const int* rank, const float* nrest, const float* nstiff, int* ea, int* eb,
+                     float* erest, float* estiff, int* oneWay);
+void launch_gs_incidence(hipStream_t s, int me, const int* ea, const int* eb, int* deg, int* cursor, int* inc,
+                         bool countOnly);
+void launch_gs_edge_round(hipStream_t s, int me, const int* ea, const int* eb, const int* incOff, const int* inc,
+                          int* color, int* flags);
+void launch_gs_cluster_round(hipStream_t s, int L, const int* leaders, const int* memOff, const int* memIdx,
+                             const int* revOff, const int* revIdx, int* color, int* flags);
+
 // Exclusive prefix sum of in[0, n) into out[0, n] (out[n] = total); tmp holds
 // exclusive_scan_tmp_size(n) ints.
 size_t exclusive_scan_tmp_size(int n);

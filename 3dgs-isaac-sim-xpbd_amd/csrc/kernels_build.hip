@@ -312,6 +312,227 @@ void launch_permute_rows(hipStream_t s, int n, int nb, const int* ord, const int
     if (n > 0) vb_permute_rows_kernel<<<vb_blocks(64LL * n, 256), 256, 0, s>>>(n, nb, ord, inv, srcOff, srcIdx, dstOff, dst);
 }
 
+// ---------------------------------------------------------------------------
+// Gauss-Seidel colourings (buildDistanceGSEdges forward.cu:10407-10517,
+// buildVolumeGSColors 9745-9812).  The reference colours greedily: items in
+// index order, each the smallest colour its earlier neighbours do not hold.
+// The same colouring comes out of rounds in which every item whose earlier
+// neighbours all hold colours takes the smallest free one: an item decides
+// from the same set of colours as in the sequential pass, and the first
+// uncoloured item is always ready, so every round makes progress.  A colour
+// read while its item is being coloured in the same round is either the
+// final value or still -1 (the reader then waits a round).
+
+// Directed CSR entries of one body as (min, max) pairs; invalid entries
+// (out of range, self) get n, n and sort last.  Entry e keeps value e, so a
+// stable sort by (min, max) keeps the CSR order among duplicates.
+__global__ void __launch_bounds__(256) gs_entry_keys_kernel(int n, const int* __restrict__ rowOff,
+                                                           const int* __restrict__ nbr, unsigned int* __restrict__ ka,
+                                                           unsigned int* __restrict__ kb, int* __restrict__ val)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int e = rowOff[i]; e < rowOff[i + 1]; ++e) {
+        const int j = nbr[e];
+        const bool ok = j >= 0 && j < n && j != i;
+        ka[e] = ok ? (unsigned int)min(i, j) : (unsigned int)n;
+        kb[e] = ok ? (unsigned int)max(i, j) : (unsigned int)n;
+        val[e] = e;
+    }
+}
+
+// key2[k] = ka[val[k]]: the major key of the second (stable) pass.
+__global__ void __launch_bounds__(256) gs_gather_key_kernel(int m, const unsigned int* __restrict__ ka,
+                                                           const int* __restrict__ val, unsigned int* __restrict__ key)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) key[k] = ka[val[k]];
+}
+
+// head[k] = 1 where a run of equal (min, max) pairs starts (valid pairs only).
+__global__ void __launch_bounds__(256) gs_heads_kernel(int n, int m, const unsigned int* __restrict__ sa,
+                                                      const unsigned int* __restrict__ kb,
+                                                      const int* __restrict__ val, int* __restrict__ head)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const unsigned int a = sa[k], b = kb[val[k]];
+    head[k] = a < (unsigned int)n && (k == 0 || sa[k - 1] != a || kb[val[k - 1]] != b) ? 1 : 0;
+}
+
+// One thread per run: the undirected edge with its rest length and stiffness
+// summed in the run's order (CSR order) and averaged.
+__global__ void __launch_bounds__(256) gs_edges_kernel(int n, int m, const unsigned int* __restrict__ sa,
+                                                      const unsigned int* __restrict__ kb, const int* __restrict__ val,
+                                                      const int* __restrict__ head, const int* __restrict__ rank,
+                                                      const float* __restrict__ nrest, const float* __restrict__ nstiff,
+                                                      int* __restrict__ ea, int* __restrict__ eb,
+                                                      float* __restrict__ erest, float* __restrict__ estiff,
+                                                      int* __restrict__ oneWay)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m || !head[k]) return;
+    const unsigned int a = sa[k], b = kb[val[k]];
+    float rest = nrest[val[k]], stiff = nstiff[val[k]];
+    int dup = 1;
+    for (int q = k + 1; q < m && sa[q] == a && kb[val[q]] == b; ++q) {
+        rest += nrest[val[q]];
+        stiff += nstiff[val[q]];
+        ++dup;
+    }
+    const int u = rank[k];
+    ea[u] = (int)a;
+    eb[u] = (int)b;
+    erest[u] = rest / float(dup);
+    estiff[u] = stiff / float(dup);
+    if (dup == 1) atomicAdd(oneWay, 1);
+}
+
+// Incidence lists: each edge under both of its nodes (order inside a list free).
+__global__ void __launch_bounds__(256) gs_degree_kernel(int me, const int* __restrict__ ea, const int* __restrict__ eb,
+                                                       int* __restrict__ deg)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= me) return;
+    atomicAdd(&deg[ea[u]], 1);
+    atomicAdd(&deg[eb[u]], 1);
+}
+
+__global__ void __launch_bounds__(256) gs_incidence_kernel(int me, const int* __restrict__ ea,
+                                                          const int* __restrict__ eb, int* __restrict__ cursor,
+                                                          int* __restrict__ inc)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= me) return;
+    inc[atomicAdd(&cursor[ea[u]], 1)] = u;
+    inc[atomicAdd(&cursor[eb[u]], 1)] = u;
+}
+
+// Colours held by the earlier items listed in rows `rows` of (off, idx)
+// (items < u); false when one of them has none yet.
+__device__ __forceinline__ bool gs_collect(int u, int r0, int r1, const int* __restrict__ idx,
+                                           const int* __restrict__ color, unsigned long long* used)
+{
+    for (int k = r0; k < r1; ++k) {
+        const int v = idx[k];
+        if (v >= u) continue;
+        const int c = __atomic_load_n(&color[v], __ATOMIC_RELAXED);
+        if (c < 0) return false;
+        const unsigned long long bit = 1ull << (c & 63);
+#pragma unroll
+        for (int w = 0; w < kGsWords; ++w) used[w] |= (c >> 6) == w ? bit : 0ull;   // static indices: no scratch
+    }
+    return true;
+}
+
+__device__ __forceinline__ int gs_first_free(const unsigned long long* used)
+{
+#pragma unroll
+    for (int w = 0; w < kGsWords; ++w)
+        if (~used[w]) return 64 * w + __builtin_ctzll(~used[w]);
+    return -1;
+}
+
+// One round over the edges: an edge whose earlier incident edges (at either
+// node) hold colours takes the smallest colour none of them holds.
+__global__ void __launch_bounds__(256) gs_edge_round_kernel(int me, const int* __restrict__ ea,
+                                                           const int* __restrict__ eb, const int* __restrict__ incOff,
+                                                           const int* __restrict__ inc, int* __restrict__ color,
+                                                           int* __restrict__ flags)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= me || color[u] >= 0) return;
+    unsigned long long used[kGsWords] = {};
+    const int a = ea[u], b = eb[u];
+    if (!gs_collect(u, incOff[a], incOff[a + 1], inc, color, used) ||
+        !gs_collect(u, incOff[b], incOff[b + 1], inc, color, used)) {
+        flags[0] = 1;   // another round
+        return;
+    }
+    const int c = gs_first_free(used);
+    if (c < 0) {
+        flags[1] = 1;   // more colours than the rounds track: the host colours instead
+        __atomic_store_n(&color[u], 64 * kGsWords, __ATOMIC_RELAXED);
+        return;
+    }
+    __atomic_store_n(&color[u], c, __ATOMIC_RELAXED);
+}
+
+// One round over the leaders (nodes with a cluster, in index order): a
+// leader whose earlier conflicting leaders (clusters holding the leader or
+// one of its members) hold colours takes the smallest colour none holds.
+__global__ void __launch_bounds__(256) gs_cluster_round_kernel(int L, const int* __restrict__ leaders,
+                                                              const int* __restrict__ memOff,
+                                                              const int* __restrict__ memIdx,
+                                                              const int* __restrict__ revOff,
+                                                              const int* __restrict__ revIdx, int* __restrict__ color,
+                                                              int* __restrict__ flags)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= L) return;
+    const int l = leaders[q];
+    if (color[l] >= 0) return;
+    unsigned long long used[kGsWords] = {};
+    bool ready = gs_collect(l, revOff[l], revOff[l + 1], revIdx, color, used);
+    for (int k = memOff[l]; ready && k < memOff[l + 1]; ++k) {
+        const int j = memIdx[k];
+        ready = gs_collect(l, revOff[j], revOff[j + 1], revIdx, color, used);
+    }
+    if (!ready) {
+        flags[0] = 1;
+        return;
+    }
+    const int c = gs_first_free(used);
+    if (c < 0) {
+        flags[1] = 1;
+        __atomic_store_n(&color[l], 64 * kGsWords, __ATOMIC_RELAXED);
+        return;
+    }
+    __atomic_store_n(&color[l], c, __ATOMIC_RELAXED);
+}
+
+void launch_gs_entry_keys(hipStream_t s, int n, const int* rowOff, const int* nbr, unsigned int* ka, unsigned int* kb,
+                          int* val)
+{
+    if (n > 0) gs_entry_keys_kernel<<<vb_blocks(n, 256), 256, 0, s>>>(n, rowOff, nbr, ka, kb, val);
+}
+void launch_gs_gather_key(hipStream_t s, int m, const unsigned int* ka, const int* val, unsigned int* key)
+{
+    if (m > 0) gs_gather_key_kernel<<<vb_blocks(m, 256), 256, 0, s>>>(m, ka, val, key);
+}
+void launch_gs_heads(hipStream_t s, int n, int m, const unsigned int* sa, const unsigned int* kb, const int* val,
+                     int* head)
+{
+    if (m > 0) gs_heads_kernel<<<vb_blocks(m, 256), 256, 0, s>>>(n, m, sa, kb, val, head);
+}
+void launch_gs_edges(hipStream_t s, int n, int m, const unsigned int* sa, const unsigned int* kb, const int* val,
+                     const int* head, const int* rank, const float* nrest, const float* nstiff, int* ea, int* eb,
+                     float* erest, float* estiff, int* oneWay)
+{
+    if (m > 0)
+        gs_edges_kernel<<<vb_blocks(m, 256), 256, 0, s>>>(n, m, sa, kb, val, head, rank, nrest, nstiff, ea, eb, erest,
+                                                           estiff, oneWay);
+}
+void launch_gs_incidence(hipStream_t s, int me, const int* ea, const int* eb, int* deg, int* cursor, int* inc,
+                         bool countOnly)
+{
+    if (me <= 0) return;
+    if (countOnly) gs_degree_kernel<<<vb_blocks(me, 256), 256, 0, s>>>(me, ea, eb, deg);
+    else gs_incidence_kernel<<<vb_blocks(me, 256), 256, 0, s>>>(me, ea, eb, cursor, inc);
+}
+void launch_gs_edge_round(hipStream_t s, int me, const int* ea, const int* eb, const int* incOff, const int* inc,
+                          int* color, int* flags)
+{
+    if (me > 0) gs_edge_round_kernel<<<vb_blocks(me, 256), 256, 0, s>>>(me, ea, eb, incOff, inc, color, flags);
+}
+void launch_gs_cluster_round(hipStream_t s, int L, const int* leaders, const int* memOff, const int* memIdx,
+                             const int* revOff, const int* revIdx, int* color, int* flags)
+{
+    if (L > 0)
+        gs_cluster_round_kernel<<<vb_blocks(L, 256), 256, 0, s>>>(L, leaders, memOff, memIdx, revOff, revIdx, color,
+                                                                   flags);
+}
+
 void launch_volume_balls(hipStream_t s, const VbArgs& a, bool fill)
 {
     if (a.n <= 0) return;

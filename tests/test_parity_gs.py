@@ -73,3 +73,56 @@ def test_gs_parity(scene, product_path, oracle_path, dist_gs, vol_gs, volume, te
     assert worst <= TOL_REL
     gpu.destroy()
     cpu.destroy()
+
+
+def _with_duplicates(scene, seed=5):
+    """The scene with 5% of its edges listed again (other rest length and
+    stiffness, some reversed): duplicate pairs are summed in CSR order."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    e = scene["edges"]
+    pick = rng.choice(len(e), size=len(e) // 20, replace=False)
+    extra = e[pick].copy()
+    flip = rng.random(len(pick)) < 0.5
+    extra[flip] = extra[flip][:, ::-1]
+    out = dict(scene)
+    out["edges"] = np.ascontiguousarray(np.concatenate([e, extra]).astype(np.int32))
+    out["rest"] = np.ascontiguousarray(np.concatenate([scene["rest"], scene["rest"][pick] * 1.01]).astype(np.float32))
+    out["stiff"] = np.ascontiguousarray(np.concatenate([scene["stiff"], np.full(len(pick), 0.5, np.float32)]))
+    return out
+
+
+@pytest.mark.parametrize("tetra", [False, True])
+def test_gs_device_colourings_equal(product_path, oracle_path, tetra):
+    """Both Gauss-Seidel colourings built on the GPU equal the host's (and the
+    oracle's) edge for edge and leader for leader, duplicate edges included,
+    and the solver then steps bit for bit the same."""
+    scene = _with_duplicates(scenes.make_ellipsoid_scene(40000, 10, seed=29))
+    got = {}
+    try:
+        for mode in (0, 1):
+            sim = XPBD(product_path)
+            sim.ext_set_device_builders(mode)
+            configure(sim, scene, True, True, True, tetra)
+            for _ in range(10):
+                sim.step()
+            got[mode] = (sim.positions(), sim.ext_distance_gs_edges(), sim.ext_volume_gs_order(), sim.ext_gs_info())
+            sim.destroy()
+    finally:
+        XPBD(product_path).ext_set_device_builders(-1)
+    (ph, dh, vh, ih), (pd, dd, vd, idv) = got[0], got[1]
+    assert ih == idv
+    for name, a, b in zip(("ab", "rest", "stiff", "colorOff"), dd, dh):
+        np.testing.assert_array_equal(a, b, err_msg=f"distance {name}")
+    for name, a, b in zip(("order", "colorOff"), vd, vh):
+        np.testing.assert_array_equal(a, b, err_msg=f"volume {name}")
+    assert np.array_equal(pd, ph), f"positions after 10 steps: max |dx| {max_abs(pd, ph)}"
+    cpu = XPBD(oracle_path)
+    configure(cpu, scene, True, True, True, tetra)
+    cpu.step()
+    dc, vc = cpu.ext_distance_gs_edges(), cpu.ext_volume_gs_order()
+    cpu.destroy()
+    for name, a, b in zip(("ab", "rest", "stiff", "colorOff"), dd, dc):
+        np.testing.assert_array_equal(a, b, err_msg=f"oracle distance {name}")
+    for name, a, b in zip(("order", "colorOff"), vd, vc):
+        np.testing.assert_array_equal(a, b, err_msg=f"oracle volume {name}")
+    print(f"device GS colourings (tetra={tetra}): {idv}; equal to the host's and the oracle's, positions bit-identical")
