@@ -220,12 +220,16 @@ int xpbd_ext_self_collision_grid_info(float* info);
 int xpbd_ext_self_collision_paths(int* out);
 
 /* Where the volume clusters of the k-ring rule (xpbd_set_volume) get their
- * member lists: 0 the host, 1 the GPU (each node's k-hop ball and its
- * farthest-point sample on the device; balls of more than 1536 nodes and the
- * greedy leader cover stay on the host), -1 (default) the GPU for bodies of
- * at least 16384 Gaussians.  The lists are identical either way.  A change
- * takes effect at the next step.  No counterpart in the reference, whose
- * rebuildVolumeClusters (forward.cu:9814-10240) builds them on the host. */
+ * member lists and the Gauss-Seidel passes (xpbd_ext_set_gauss_seidel) their
+ * edges and colourings: 0 the host, 1 the GPU (each node's k-hop ball and
+ * its farthest-point sample, the reverse lists, the deduplicated distance
+ * edges and both greedy colourings on the device; balls of more than 1536
+ * nodes and the greedy leader cover stay on the host), -1 (default) the GPU
+ * for bodies of at least 16384 Gaussians.  Lists, edges and colours are
+ * identical either way.  A change takes effect at the next step.  No
+ * counterpart in the reference, whose rebuildVolumeClusters,
+ * buildVolumeGSColors and buildDistanceGSEdges (forward.cu:9745-10517) run
+ * on the host. */
 int xpbd_ext_set_device_builders(int mode);
 
 /* Shape matching over node clusters (clusterOf[n]: cluster id, -1 = none;
