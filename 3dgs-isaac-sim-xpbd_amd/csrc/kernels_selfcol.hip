@@ -1057,16 +1057,16 @@ __global__ void __launch_bounds__(256) sc_lane_rank_kernel(SelfColArgs a, const 
     if (listed && m > 0 && a.bodies[b].objOn) a.scCounters[3 * b] = 1;
 }
 
-// Ranking of the lane-scan lists of more than 32 candidates, 16 lanes per
-// list, lists in the grid's own order (the four lists of a wave are
+// Ranking of the lane-scan lists of more than 32 candidates, 32 lanes per
+// list, lists in the grid's own order (the two lists of a wave are
 // neighbouring fine positions, so each step of their reads and writes shares
 // cache lines).  The node's (key, squared distance) list is spread over the
 // group's lanes; its keys (all distinct) are ranked by counting in LDS; the
 // list goes back to its entries in rank order as (id, priority = |p_i -
 // p_j|, from the squared distance the scan tested) and the node is flagged
 // for the slot-rule kernel.
-constexpr int kLaneGroup = 16;
-constexpr int kLanePer = kLaneList / kLaneGroup;
+constexpr int kLaneGroup = 32;
+constexpr int kLanePer = (kLaneList + kLaneGroup - 1) / kLaneGroup;
 
 __global__ void __launch_bounds__(256) sc_slots_kernel(SelfColArgs a, const int4* __restrict__ FC,
                                                       const int* __restrict__ FS)
