@@ -1016,9 +1016,19 @@ __device__ __forceinline__ void sc_rank_list(SelfColArgs& a, int i, int m, size_
 #pragma unroll
     for (int x = 0; x < N; ++x) k[x] = x < m ? (unsigned int)LK[L0 + (size_t)x * 64].x : 0xffffffffu;
     sc_bitonic<N>(k);
+    // device ids eight at a time: the lookups of a chunk are in flight together
+    // (the slots and the lookup table are different arrays, but the compiler
+    // cannot tell, so it would keep each lookup behind the previous store)
+    constexpr int C = N < 8 ? N : 8;
 #pragma unroll
-    for (int x = 0; x < N; ++x)
-        if (x < m) a.contacts[sc_slot(i, x)] = a.devOfCaller[k[x] & 0x07ffffffu];
+    for (int x0 = 0; x0 < N; x0 += C) {
+        int id[C];
+#pragma unroll
+        for (int u = 0; u < C; ++u) id[u] = x0 + u < m ? a.devOfCaller[k[x0 + u] & 0x07ffffffu] : 0;
+#pragma unroll
+        for (int u = 0; u < C; ++u)
+            if (x0 + u < m) a.contacts[sc_slot(i, x0 + u)] = id[u];
+    }
 }
 
 __global__ void __launch_bounds__(256) sc_lane_rank_kernel(SelfColArgs a, const int4* __restrict__ FC,
@@ -1113,12 +1123,16 @@ __global__ void __launch_bounds__(256) sc_slots_kernel(SelfColArgs a, const int4
             if (k < kn) rank[k] += kf < key[k] ? 1 : 0;
     }
     if (lane) {
-        // the list in rank order with its priorities, for the slot rule
+        // the list in rank order with its priorities, for the slot rule (the
+        // id lookups first, all in flight together)
+        int id[kLanePer];
+#pragma unroll
+        for (int k = 0; k < kLanePer; ++k)
+            id[k] = (k < kn && sub + kLaneGroup * k < m) ? a.devOfCaller[key[k] & 0x07ffffffu] : 0;
 #pragma unroll
         for (int k = 0; k < kLanePer; ++k)
             if (k < kn && sub + kLaneGroup * k < m)
-                a.laneKey[L0 + (size_t)rank[k] * 64] =
-                    make_int2(a.devOfCaller[key[k] & 0x07ffffffu], __float_as_int(sqrtf(__int_as_float(jv[k]))));
+                a.laneKey[L0 + (size_t)rank[k] * 64] = make_int2(id[k], __float_as_int(sqrtf(__int_as_float(jv[k]))));
         if (sub == 0) {
             a.counts[i] = kSelfColCap;
             a.overflow[i] = m - kSelfColCap;
