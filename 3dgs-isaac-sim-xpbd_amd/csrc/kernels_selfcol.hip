@@ -1239,7 +1239,8 @@ __device__ __forceinline__ void sc_node_push(int i, const SelfColArgs& a, const 
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, wFast = 0.0f;
     int act = 0, nFast = 0;
     const int* list = &a.contacts[sc_slot(i, 0)];   // slot k at list[k * kTile]
-    const float3 xi = X[i], qi = rest[i];
+    // x_i and the rest position only on the rare paths that need them
+    // (cross-object pairs, coincident positions)
     if (wi > 0.0f) {
         const float dc2 = dc * dc;
         // contacts in groups of kScSolveGroup: the group's codes and partner
@@ -1263,7 +1264,7 @@ __device__ __forceinline__ void sc_node_push(int i, const SelfColArgs& a, const 
                 if (code < 0) {
                     const int j = -code - 1;
                     float nx, ny, nz;
-                    if (!sc_start_normal(xi, X[j], qi, rest[j], nx, ny, nz)) continue;
+                    if (!sc_start_normal(X[i], X[j], rest[i], rest[j], nx, ny, nz)) continue;
                     const float proj = (pi4.x - pj.x) * nx + (pi4.y - pj.y) * ny + (pi4.z - pj.z) * nz;
                     if (!(proj < dc)) continue;
                     const float pen = dc - proj;
@@ -1280,7 +1281,7 @@ __device__ __forceinline__ void sc_node_push(int i, const SelfColArgs& a, const 
                 const float wj = pj.w * B.invMassScale;
                 const float wr = wi / (wi + wj);
                 if (d2 < 1e-24f) {
-                    const float3 qj = rest[j];
+                    const float3 qi = rest[i], qj = rest[j];
                     const float rx = qi.x - qj.x, ry = qi.y - qj.y, rz = qi.z - qj.z;
                     const float r2 = rx * rx + ry * ry + rz * rz;
                     if (!(r2 > 1e-24f)) continue;
@@ -1307,7 +1308,7 @@ __device__ __forceinline__ void sc_node_push(int i, const SelfColArgs& a, const 
                 if (code >= 0) continue;
                 const int j = -code - 1;
                 float nx, ny, nz;
-                if (!sc_start_normal(xi, X[j], qi, rest[j], nx, ny, nz)) continue;
+                if (!sc_start_normal(X[i], X[j], rest[i], rest[j], nx, ny, nz)) continue;
                 const float4 pj = P[j];
                 const float ddx = pi4.x - pj.x, ddy = pi4.y - pj.y, ddz = pi4.z - pj.z;
                 if (!(ddx * nx + ddy * ny + ddz * nz < dc)) continue;
@@ -1339,8 +1340,12 @@ __global__ void __launch_bounds__(256) sc_contact_kernel(int nPad, SelfColArgs a
         float3 d, c;
         sc_node_push(i, a, B, Pin, X, rest, d, c);
         p = make_float3(p.x + d.x, p.y + d.y, p.z + d.z);
-        const float3 acc = a.accum[i];
-        a.accum[i] = make_float3(acc.x + c.x, acc.y + c.y, acc.z + c.z);
+        // the cross-object part accumulates for the contact push; adding a
+        // zero would leave the sum as it is
+        if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) {
+            const float3 acc = a.accum[i];
+            a.accum[i] = make_float3(acc.x + c.x, acc.y + c.y, acc.z + c.z);
+        }
     }
     if (epi && epilogue_node(epi, i, p, p4.w, B, ea)) return;
     Pout[i] = make_float4(p.x, p.y, p.z, p4.w);
