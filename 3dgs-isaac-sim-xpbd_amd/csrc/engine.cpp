@@ -279,6 +279,12 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     angDevValid_ = false;
     shcK_ = 0;
 
+    auto phase = [&](const char* what) {
+        if (getenv("XPBD_BUILD_TIMING"))
+            fprintf(stderr, "[build]   %s at %.1f ms\n", what,
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tBuild).count());
+    };
+    phase("settings");
     // device node order of every body (caller order or Morton order)
     ord_.assign(B, std::vector<int>());
     inv_.assign(B, std::vector<int>());
@@ -307,6 +313,7 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
         }
     }
 
+    phase("node order");
     // host staging of the padded node space (device order); every row keeps
     // the caller's entry order, neighbours are renumbered
     std::vector<float3> hx(nPad_, make_float3(0, 0, 0));
@@ -337,6 +344,7 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
     }
     hrow[nPad_] = (int)eo;
 
+    phase("renumbered rows");
     x_.alloc(nPad_); v_.alloc(nPad_); rest_.alloc(nPad_);
     p0_.alloc(nPad_); p1_.alloc(nPad_);
     invMass_.alloc(nPad_); invMassSaved_.alloc(nPad_); objW_.alloc(nPad_);
@@ -375,6 +383,7 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
         XPBD_HIP(hipMemcpyAsync(nbr_.p, hnbr.data(), sizeof(int) * m, hipMemcpyHostToDevice, stream_));
         XPBD_HIP(hipMemcpyAsync(nstiff_.p, hstiff.data(), sizeof(float) * m, hipMemcpyHostToDevice, stream_));
     }
+    phase("uploads");
     // sliced-ELL copy of the directed entries for the distance pass: slice t
     // = node tile t, width = its longest row, entry k of lane l at
     // sliceOff[t] + 64 k + l; rows keep their order, padding has nbr -1
@@ -419,6 +428,7 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
                                 stream_));
         XPBD_HIP(hipStreamSynchronize(stream_));   // host staging goes out of scope
     }
+    phase("sliced entries");
     XPBD_HIP(hipMemcpyAsync(tileBody_.p, htile.data(), sizeof(int) * htile.size(), hipMemcpyHostToDevice, stream_));
     launch_fill_float(stream_, nPad_, invMass_.p, 1.0f);
     launch_fill_float(stream_, nPad_, objW_.p, 1.0f);
