@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 
 namespace xpbd {
 
@@ -125,6 +126,44 @@ static uint64_t spread_bits21(uint64_t v)
     return v;
 }
 
+// fn(begin, end) over [0, n) on up to 16 host threads (one thread below
+// 65536 items); every item is handled once, so the result does not depend
+// on the split.
+template <typename F>
+static void host_parallel(long long n, F fn)
+{
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 65536 || hw == 1) {
+        fn(0LL, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const long long chunk = (n + hw - 1) / hw;
+    for (unsigned t = 0; t < hw; ++t) {
+        const long long b = (long long)t * chunk, e = std::min(n, b + chunk);
+        if (b < e) pool.emplace_back(fn, b, e);
+    }
+    for (std::thread& th : pool) th.join();
+}
+
+// fn(k) for k in [0, count), tasks spread over up to 16 host threads.
+template <typename F>
+static void host_tasks(int count, F fn)
+{
+    const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (count <= 1 || hw == 1) {
+        for (int k = 0; k < count; ++k) fn(k);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < std::min(hw, count); ++t)
+        pool.emplace_back([&]() {
+            for (int k = next++; k < count; k = next++) fn(k);
+        });
+    for (std::thread& th : pool) th.join();
+}
+
 static void morton_order(const BodyInput& in, std::vector<int>& ord)
 {
     const int n = in.n;
@@ -141,17 +180,30 @@ static void morton_order(const BodyInput& in, std::vector<int>& ord)
     for (int a = 0; a < 3; ++a) ext = std::max(ext, hi[a] - lo[a]);
     const double scale = ext > 0.0 ? 2097151.0 / ext : 0.0;
     std::vector<std::pair<uint64_t, int>> key(n);
-    for (int i = 0; i < n; ++i) {
-        uint64_t c = 0;
-        for (int a = 0; a < 3; ++a) {
-            double q = (in.pos[3 * i + a] - lo[a]) * scale;
-            if (!(q >= 0.0)) q = 0.0;
-            if (q > 2097151.0) q = 2097151.0;
-            c |= spread_bits21((uint64_t)q) << a;
+    host_parallel(n, [&](long long i0, long long i1) {
+        for (long long i = i0; i < i1; ++i) {
+            uint64_t c = 0;
+            for (int a = 0; a < 3; ++a) {
+                double q = (in.pos[3 * i + a] - lo[a]) * scale;
+                if (!(q >= 0.0)) q = 0.0;
+                if (q > 2097151.0) q = 2097151.0;
+                c |= spread_bits21((uint64_t)q) << a;
+            }
+            key[i] = {c, (int)i};
         }
-        key[i] = {c, i};
+    });
+    // (key, index) pairs are distinct: sorted runs merged give std::sort's order
+    const int runs = n >= 65536 ? 16 : 1;
+    std::vector<long long> cut(runs + 1);
+    for (int r = 0; r <= runs; ++r) cut[r] = (long long)n * r / runs;
+    host_tasks(runs, [&](int r) { std::sort(key.begin() + cut[r], key.begin() + cut[r + 1]); });
+    for (int width = 1; width < runs; width *= 2) {
+        const int pairs = (runs + 2 * width - 1) / (2 * width);
+        host_tasks(pairs, [&](int q) {
+            const int a = q * 2 * width, mid = std::min(runs, a + width), e = std::min(runs, a + 2 * width);
+            if (mid < e) std::inplace_merge(key.begin() + cut[a], key.begin() + cut[mid], key.begin() + cut[e]);
+        });
     }
-    std::sort(key.begin(), key.end());
     for (int i = 0; i < n; ++i) ord[i] = key[i].second;
 }
 
@@ -296,12 +348,13 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
             morton_order(in, o);
             // longest rows first inside windows of 8 slices: the slices of the
             // sliced-ELL distance entries then pad far less (SELL-C-sigma)
-            for (int w0 = 0; w0 < in.n; w0 += kSortWindow) {
-                const int w1 = std::min(in.n, w0 + kSortWindow);
+            const int windows = (in.n + kSortWindow - 1) / kSortWindow;
+            host_tasks(windows, [&](int w) {
+                const int w0 = w * kSortWindow, w1 = std::min(in.n, w0 + kSortWindow);
                 std::stable_sort(o.begin() + w0, o.begin() + w1, [&](int x, int y) {
                     return in.rowOff[x + 1] - in.rowOff[x] > in.rowOff[y + 1] - in.rowOff[y];
                 });
-            }
+            });
         } else {
             o.resize(in.n);
             for (int i = 0; i < in.n; ++i) o[i] = i;
@@ -329,17 +382,25 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
         const std::vector<int>& iv = inv_[b];
         const int padEnd = (b + 1 < B) ? bodyBegin_[b + 1] : nPad_;
         for (int t = nb / kTile; t < padEnd / kTile; ++t) htile[t] = b;
+        // row starts first, then the rows on several threads
         for (int d = 0; d < in.n; ++d) {
             const int c = o[d];
-            hx[nb + d] = make_float3(in.pos[3 * c], in.pos[3 * c + 1], in.pos[3 * c + 2]);
-            hcaller[nb + d] = nb + c;
             hrow[nb + d] = (int)eo;
-            for (int k = in.rowOff[c]; k < in.rowOff[c + 1]; ++k, ++eo) {
-                hnbr[eo] = nb + iv[in.nbr[k]];
-                hrest[eo] = in.nrest[k];
-                hstiff[eo] = in.nstiff[k];
-            }
+            eo += in.rowOff[c + 1] - in.rowOff[c];
         }
+        host_parallel(in.n, [&](long long d0, long long d1) {
+            for (long long d = d0; d < d1; ++d) {
+                const int c = o[d];
+                hx[nb + d] = make_float3(in.pos[3 * c], in.pos[3 * c + 1], in.pos[3 * c + 2]);
+                hcaller[nb + d] = nb + c;
+                long long w = hrow[nb + d];
+                for (int k = in.rowOff[c]; k < in.rowOff[c + 1]; ++k, ++w) {
+                    hnbr[w] = nb + iv[in.nbr[k]];
+                    hrest[w] = in.nrest[k];
+                    hstiff[w] = in.nstiff[k];
+                }
+            }
+        });
         for (int i = nb + in.n; i <= padEnd && i <= nPad_; ++i) hrow[i] = (int)eo;
     }
     hrow[nPad_] = (int)eo;
@@ -401,14 +462,16 @@ bool Engine::build(const std::vector<const BodyInput*>& inputs, std::string& err
         }
         std::vector<int> hSellNbr(std::max<long long>(total, 1), -1);
         std::vector<float> hSellRest(std::max<long long>(total, 1), 0.0f);
-        for (int i = 0; i < nPad_; ++i) {
-            const int t = i / kTile, l = i % kTile;
-            for (int e = hrow[i]; e < hrow[i + 1]; ++e) {
-                const size_t slot = (size_t)hOff[t] + (size_t)(e - hrow[i]) * kTile + l;
-                hSellNbr[slot] = hnbr[e];
-                hSellRest[slot] = hrest[e];
+        host_parallel(nPad_, [&](long long i0, long long i1) {
+            for (long long i = i0; i < i1; ++i) {
+                const int t = (int)(i / kTile), l = (int)(i % kTile);
+                for (int e = hrow[i]; e < hrow[i + 1]; ++e) {
+                    const size_t slot = (size_t)hOff[t] + (size_t)(e - hrow[i]) * kTile + l;
+                    hSellNbr[slot] = hnbr[e];
+                    hSellRest[slot] = hrest[e];
+                }
             }
-        }
+        });
         sellEntries_ = total;
         sliceOff_.alloc(std::max(tiles, 1)); sliceW_.alloc(std::max(tiles, 1));
         // per-edge alpha tilde only for bodies whose edges differ in stiffness
