@@ -421,9 +421,7 @@ __device__ __forceinline__ bool sc_stencil_shares_bucket(const float3& x, float 
 // nodes that have a coarse key (active, finite x) and a finite p.  Each
 // node's rank inside its cell comes from the cell's counter (the order
 // inside a cell does not matter).  laneScan: the grid nodes take the lane
-// fine scan, else the coarse walk; so do nodes whose candidates may lie
-// outside the 27 coarse cells around x (cell edge capped or fixed below search
-// + 2 max move).
+// fine scan, else the coarse walk.
 __global__ void __launch_bounds__(256) sc_fine_keys_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
                                                           const float4* __restrict__ P, int laneScan)
 {
@@ -467,7 +465,7 @@ __global__ void __launch_bounds__(256) sc_fine_keys_kernel(int nPad, SelfColArgs
     a.fkeys[i] = key;
     if (key < (unsigned int)a.fineTotal) a.frank[i] = atomicAdd(&a.fcount[key], 1);
     if (B.scOn && B.scFine && i < B.nodeEnd) {
-        const int path = key < (unsigned int)a.fineTotal ? ((laneScan && !a.scExact[b]) ? kScLane : kScNeedCoarse) : 0;
+        const int path = key < (unsigned int)a.fineTotal ? (laneScan ? kScLane : kScNeedCoarse) : 0;
         a.accCount[i] = path;
         if (path == kScNeedCoarse) *a.scSortGate = 1;
     }
@@ -634,18 +632,20 @@ __device__ __forceinline__ size_t sc_lane_at(int t, int k)
 // overlap and their loads share cache lines; each lane keeps kLaneU entries
 // (32 B: p, rest, caller-order position, packed coarse cell) in flight.  An
 // accepted candidate is keyed at once: its coarse bucket's first stencil
-// offset << 27 | its caller-order position.  A candidate within search lies
-// within search + 2 max move of x_i in x, so its coarse cell differs from
-// x_i's by at most one per axis and the low bits of the packed cell give the
-// offset exactly (when two stencil cells share a bucket, the ranking kernel
-// first moves each key to the first offset holding its bucket); nodes
-// for which that does not hold (cell edge capped or fixed below that bound)
-// take the coarse walk, as do nodes with a candidate two coarse cells away
-// (possible within rounding of the bound).  Nodes whose row
-// stretches wrap around the table or meet in the key space walk their 27
-// cells one by one instead, counting each entry only in its own cell.  The
-// (key, squared distance) pairs go unsorted to the lane's list; lists past
-// kLaneList go to the coarse walk.
+// offset << 27 | its caller-order position.  The coarse walk meets exactly
+// the candidates within search whose coarse bucket is one of x_i's 27
+// stencil buckets.  A candidate within search lies within search + 2 max
+// move of x_i in x, so with the reference's cell edge its coarse cell
+// differs from x_i's by at most one per axis and the low bits of the packed
+// cell give the offset (when two stencil cells share a bucket, the ranking
+// kernel first moves each key to the first offset holding its bucket); with
+// a capped or fixed edge the full cells give it.  A candidate more than one
+// cell away (capped or fixed edge, or within rounding of the bound) counts
+// only if its bucket is a stencil bucket, keyed at the first offset holding
+// it.  Nodes whose row stretches wrap around the table or meet in the key
+// space walk their 27 cells one by one instead, counting each entry only in
+// its own cell.  The (key, squared distance) pairs go unsorted to the lane's
+// list; lists past kLaneList go to the coarse walk.
 constexpr int kLaneU = 4;   // entries in flight per lane
 
 // Scan key of an accepted candidate of node i: the first stencil offset of
@@ -660,6 +660,31 @@ __device__ __forceinline__ unsigned int sc_lane_key(unsigned int pi, unsigned in
     const unsigned int ux = (pj - pi + 1u) & 2047u, uy = ((pj >> 11) - (pi >> 11) + 1u) & 2047u,
                        uz = ((pj >> 22) - (pi >> 22) + 1u) & 1023u;
     return (ux <= 2u && uy <= 2u && uz <= 2u) ? (((ux + 3u * uy + 9u * uz) << 27) | cp) : 0xffffffffu;
+}
+
+// Scan key from the full coarse cells of x_i (ci) and the candidate (cj):
+// the stencil offset of their difference when it is at most one cell per
+// axis, else ~0.
+__device__ __forceinline__ unsigned int sc_cell_key(int4 ci, int4 cj, unsigned int cp)
+{
+    const int dx = cj.x - ci.x + 1, dy = cj.y - ci.y + 1, dz = cj.z - ci.z + 1;
+    return ((unsigned int)dx <= 2u && (unsigned int)dy <= 2u && (unsigned int)dz <= 2u)
+               ? (((unsigned int)(dx + 3 * dy + 9 * dz) << 27) | cp)
+               : 0xffffffffu;
+}
+
+// Scan key of a candidate whose coarse cell cj is more than one cell from
+// x_i's cell ci on some axis (possible when the cell edge is capped or fixed
+// below search + 2 max move): the coarse walk meets it only if its bucket is
+// one of the 27 stencil buckets, at the first stencil offset holding that
+// bucket; ~0 when it is none of them (not a candidate).
+__device__ __noinline__ unsigned int sc_far_key(int4 ci, int4 cj, unsigned int mask, unsigned int cp)
+{
+    const unsigned int hj = sc_hash(cj.x, cj.y, cj.z, mask);
+    for (int o = 0; o < 27; ++o)
+        if (sc_hash(ci.x + o % 3 - 1, ci.y + (o / 3) % 3 - 1, ci.z + o / 9 - 1, mask) == hj)
+            return ((unsigned int)o << 27) | cp;
+    return 0xffffffffu;
 }
 
 __global__ void __launch_bounds__(256) sc_fine_lane_kernel(SelfColArgs a, const float4* __restrict__ E,
@@ -681,10 +706,10 @@ __global__ void __launch_bounds__(256) sc_fine_lane_kernel(SelfColArgs a, const 
     // rows whose stretches wrap or meet in the key space: walk the 27 cells
     // one by one, each entry counted only in its own cell
     const bool percell = (a.scActive[i] & 4) != 0;
+    const bool exactCells = a.scExact[b] != 0;
     const unsigned int pi = __float_as_uint(m1.w);
     const size_t L0 = sc_lane_at(t, 0);   // entry k at L0 + 64 k
     int n = 0;
-    bool far = false;   // a candidate two coarse cells away (rounding): the coarse walk decides
 #pragma unroll 1
     for (int w = 0; w < (percell ? 27 : 9); ++w) {
         int lo, h, cx = 0, cy = 0, cz = 0;
@@ -720,15 +745,21 @@ __global__ void __launch_bounds__(256) sc_fine_lane_kernel(SelfColArgs a, const 
                 if (percell && !((int)(sc_cell(A[u].x, g) == cx) & (int)(sc_cell(A[u].y, g) == cy) &
                                  (int)(sc_cell(A[u].z, g) == cz)))
                     continue;
-                const unsigned int key =
-                    sc_lane_key(pi, __float_as_uint(C[u].w), (unsigned int)__float_as_int(C[u].z));
-                far |= key == 0xffffffffu;
+                const unsigned int cp = (unsigned int)__float_as_int(C[u].z);
+                // with a capped or fixed cell the coarse cells may lie far
+                // apart: the full cells decide (the packed low bits could wrap)
+                unsigned int key = exactCells ? sc_cell_key(FC[t], FC[e], cp) : sc_lane_key(pi, __float_as_uint(C[u].w), cp);
+                if (key == 0xffffffffu) {
+                    // more than one coarse cell apart: decided by its bucket
+                    key = sc_far_key(FC[t], FC[e], (unsigned int)B.scTableMask, cp);
+                    if (key == 0xffffffffu) continue;
+                }
                 if (n < kLaneList) LK[L0 + (size_t)n * 64] = make_int2((int)key, __float_as_int(d2));
                 ++n;
             }
         }
     }
-    if (n > kLaneList || far) {
+    if (n > kLaneList) {
         a.accCount[i] = kScNeedCoarse;
         *a.scSortGate = 1;
         n = -1;
