@@ -678,7 +678,7 @@ __device__ __forceinline__ unsigned int sc_cell_key(int4 ci, int4 cj, unsigned i
 // below search + 2 max move): the coarse walk meets it only if its bucket is
 // one of the 27 stencil buckets, at the first stencil offset holding that
 // bucket; ~0 when it is none of them (not a candidate).
-__device__ __noinline__ unsigned int sc_far_key(int4 ci, int4 cj, unsigned int mask, unsigned int cp)
+__device__ __forceinline__ unsigned int sc_far_key(int4 ci, int4 cj, unsigned int mask, unsigned int cp)
 {
     const unsigned int hj = sc_hash(cj.x, cj.y, cj.z, mask);
     for (int o = 0; o < 27; ++o)
@@ -687,6 +687,12 @@ __device__ __noinline__ unsigned int sc_far_key(int4 ci, int4 cj, unsigned int m
     return 0xffffffffu;
 }
 
+// EXACT: the nodes of bodies whose coarse cell edge is capped or fixed below
+// search + 2 max move (full-cell keys, far candidates by bucket); otherwise
+// the other nodes (packed-cell keys; a candidate two cells away, possible
+// only within rounding, hands the node to the coarse walk).  Two
+// instantiations keep the common one's register count low.
+template <bool EXACT>
 __global__ void __launch_bounds__(256) sc_fine_lane_kernel(SelfColArgs a, const float4* __restrict__ E,
                                                           const int4* __restrict__ FC, const int* __restrict__ FS,
                                                           int2* __restrict__ LK)
@@ -705,11 +711,12 @@ __global__ void __launch_bounds__(256) sc_fine_lane_kernel(SelfColArgs a, const 
     const int gx = sc_cell(m0.x, g), gy = sc_cell(m0.y, g), gz = sc_cell(m0.z, g);
     // rows whose stretches wrap or meet in the key space: walk the 27 cells
     // one by one, each entry counted only in its own cell
+    if ((a.scExact[b] != 0) != EXACT) return;   // the other instantiation's node
     const bool percell = (a.scActive[i] & 4) != 0;
-    const bool exactCells = a.scExact[b] != 0;
     const unsigned int pi = __float_as_uint(m1.w);
     const size_t L0 = sc_lane_at(t, 0);   // entry k at L0 + 64 k
     int n = 0;
+    bool far = false;   // (not EXACT) a candidate two coarse cells away: the coarse walk decides
 #pragma unroll 1
     for (int w = 0; w < (percell ? 27 : 9); ++w) {
         int lo, h, cx = 0, cy = 0, cz = 0;
@@ -746,20 +753,26 @@ __global__ void __launch_bounds__(256) sc_fine_lane_kernel(SelfColArgs a, const 
                                  (int)(sc_cell(A[u].z, g) == cz)))
                     continue;
                 const unsigned int cp = (unsigned int)__float_as_int(C[u].z);
-                // with a capped or fixed cell the coarse cells may lie far
-                // apart: the full cells decide (the packed low bits could wrap)
-                unsigned int key = exactCells ? sc_cell_key(FC[t], FC[e], cp) : sc_lane_key(pi, __float_as_uint(C[u].w), cp);
-                if (key == 0xffffffffu) {
-                    // more than one coarse cell apart: decided by its bucket
-                    key = sc_far_key(FC[t], FC[e], (unsigned int)B.scTableMask, cp);
-                    if (key == 0xffffffffu) continue;
+                unsigned int key;
+                if (EXACT) {
+                    // the coarse cells may lie far apart: the full cells
+                    // decide (the packed low bits could wrap), and a
+                    // candidate more than one cell away by its bucket
+                    key = sc_cell_key(FC[t], FC[e], cp);
+                    if (key == 0xffffffffu) {
+                        key = sc_far_key(FC[t], FC[e], (unsigned int)B.scTableMask, cp);
+                        if (key == 0xffffffffu) continue;
+                    }
+                } else {
+                    key = sc_lane_key(pi, __float_as_uint(C[u].w), cp);
+                    far |= key == 0xffffffffu;
                 }
                 if (n < kLaneList) LK[L0 + (size_t)n * 64] = make_int2((int)key, __float_as_int(d2));
                 ++n;
             }
         }
     }
-    if (n > kLaneList) {
+    if (n > kLaneList || far) {
         a.accCount[i] = kScNeedCoarse;
         *a.scSortGate = 1;
         n = -1;
@@ -1504,7 +1517,8 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
     // the lane fine scan first: the coarse kernel walks the nodes it hands over
     if (a.fineTotal > 0 && laneScan) {
         if (mark) mark(markCtx, "selfcol_build_fine_lane");
-        sc_fine_lane_kernel<<<xcd_grid(nbs(na.nPad, 256)), 256, 0, s>>>(a, a.fineE, a.fineC, a.fstart, a.laneKey);
+        sc_fine_lane_kernel<false><<<xcd_grid(nbs(na.nPad, 256)), 256, 0, s>>>(a, a.fineE, a.fineC, a.fstart, a.laneKey);
+        sc_fine_lane_kernel<true><<<xcd_grid(nbs(na.nPad, 256)), 256, 0, s>>>(a, a.fineE, a.fineC, a.fstart, a.laneKey);
     }
     // the coarse walk and what it reads (stable sort of the frame-start keys,
     // cell ranges, sorted copies), only when a node needs it: the kernels
