@@ -294,8 +294,8 @@ bool Engine::ensureSelfCol(std::string& err)
     fcount_.alloc(fineTotal_); fstart_.alloc((size_t)fineTotal_ + 1);
     fscan_.alloc(exclusive_scan_tmp_size(fineTotal_));
     fineE_.alloc(2 * n); fineC_.alloc(n); coarseRank_.alloc(n);
-    // the lane fine scan's unsorted lists: kLaneList per fine position,
-    // slot-major per wave of fine positions
+    // the lane fine scan's unsorted lists: one contiguous list of kLaneList
+    // entries per fine position
     laneN_.alloc(n); laneKey_.alloc(n * kLaneList);
     // per node: what its candidate search still needs
     accCount_.alloc(n);

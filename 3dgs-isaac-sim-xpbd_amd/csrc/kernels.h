@@ -176,14 +176,15 @@ struct SelfColArgs {
     int* fscan;                // block sums of the prefix sum
     int* finePos;              // [nPad] fine position of each node, -1 when not in the grid
     float4* fineE;             // [2 nPad] fine order, two float4 per entry: (p.x, p.y, p.z, rest.x),
-                               //   (rest.y, rest.z, caller-order position bits: the coarse sort's order
-                               //   among equal keys, coarse cell of x packed: low 11/11/10 bits of x/y/z);
+                               //   (rest.y, rest.z, caller-order position | code << 27, node id |
+                               //   (code >> 5) << 27), code = low 3 bits of each coarse cell coordinate
+                               //   of x (x, y, z from bit 0); both fields below 2^27;
                                //   one 32 B entry per candidate test
     int4* fineC;               // [nPad] fine order: coarse cell of x, node id
-    // lists of the lane fine scan, slot-major per wave of 64 fine positions
-    // (entry k of position t at ((t / 64) kLaneList + k) 64 + t % 64)
+    // lists of the lane fine scan, one contiguous list per fine position
+    // (entry k of position t at t kLaneList + k)
     int* laneN;                // [nPad] fine order: candidates of the lane scan, -1 handed over
-    int2* laneKey;             // [nPad * kLaneList] (scan key, squared distance bits) of each candidate,
+    int2* laneKey;             // [nPad * kLaneList] (scan key, node id) of each candidate,
                                //   unsorted; (node id, priority bits) in rank order for lists that take
                                //   the slot rule
     int* accCount;             // [nPad] per node: what its candidate search needs (kScLane, kScNeedCoarse,

@@ -465,16 +465,26 @@ __global__ void __launch_bounds__(256) sc_fine_keys_kernel(int nPad, SelfColArgs
     a.fkeys[i] = key;
     if (key < (unsigned int)a.fineTotal) a.frank[i] = atomicAdd(&a.fcount[key], 1);
     if (B.scOn && B.scFine && i < B.nodeEnd) {
-        const int path = key < (unsigned int)a.fineTotal ? (laneScan ? kScLane : kScNeedCoarse) : 0;
+        int path = key < (unsigned int)a.fineTotal ? (laneScan ? kScLane : kScNeedCoarse) : 0;
+        // the lane scan reads coarse-cell differences from 3 bits per axis,
+        // exact while float rounding keeps neighbours' cells within two of
+        // each other (cell indices below 2^22); farther out the coarse walk
+        if (path == kScLane) {
+            const float3 x = X[i];
+            const float inv = a.scInvCell[b];
+            const int cx = sc_cell(x.x, inv), cy = sc_cell(x.y, inv), cz = sc_cell(x.z, inv);
+            if (max(abs(cx), max(abs(cy), abs(cz))) >= (1 << 22)) path = kScNeedCoarse;
+        }
         a.accCount[i] = path;
         if (path == kScNeedCoarse) *a.scSortGate = 1;
     }
 }
 
 // Fine-order copies for the fine scans, written at each node's place in the
-// counting sort (cell start + rank): (p, rest, caller-order position, coarse
-// cell of x packed) and (coarse cell of x, node id); finePos maps node to
-// place.
+// counting sort (cell start + rank): (p, rest, caller-order position, node
+// id), the low 3 bits of each coarse cell coordinate of x in the spare high
+// bits of the last two words (both below 2^27), and (coarse cell of x, node
+// id); finePos maps node to place.
 __global__ void __launch_bounds__(256) sc_fine_scatter_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X,
                                                              const float4* __restrict__ P,
                                                              const float3* __restrict__ rest)
@@ -493,9 +503,9 @@ __global__ void __launch_bounds__(256) sc_fine_scatter_kernel(int nPad, SelfColA
     const float inv = a.scInvCell[b];
     const int cx = sc_cell(x.x, inv), cy = sc_cell(x.y, inv), cz = sc_cell(x.z, inv);
     a.fineE[2 * (size_t)t] = make_float4(p.x, p.y, p.z, r.x);
-    const unsigned int packed = ((unsigned int)cx & 2047u) | (((unsigned int)cy & 2047u) << 11) |
-                                (((unsigned int)cz & 1023u) << 22);
-    a.fineE[2 * (size_t)t + 1] = make_float4(r.y, r.z, __int_as_float(a.callerPos[i]), __uint_as_float(packed));
+    const unsigned int code = ((unsigned int)cx & 7u) | (((unsigned int)cy & 7u) << 3) | (((unsigned int)cz & 7u) << 6);
+    a.fineE[2 * (size_t)t + 1] = make_float4(r.y, r.z, __uint_as_float((unsigned int)a.callerPos[i] | (code << 27)),
+                                             __uint_as_float((unsigned int)i | ((code >> 5) << 27)));
     a.fineC[t] = make_int4(cx, cy, cz, i);
     a.finePos[i] = t;
 }
@@ -615,11 +625,13 @@ __device__ __forceinline__ int sc_block(int xcd)
     return blk;
 }
 
-// Entry k of the lane fine scan's list of fine position t (slot-major per
-// wave of 64 positions: a step of the 64 lanes is one 256 B row).
+// Entry k of the lane fine scan's list of fine position t: each list is
+// contiguous (a lane appends to its own lines, and the ranking kernels read a
+// list as one stretch).
+constexpr int kLaneStride = 1;
 __device__ __forceinline__ size_t sc_lane_at(int t, int k)
 {
-    return ((size_t)(t >> 6) * kLaneList + (size_t)k) * 64 + (size_t)(t & 63);
+    return (size_t)t * kLaneList + (size_t)k * kLaneStride;
 }
 
 // Fine scan, one lane per node, in the predicted-position grid's own order,
@@ -644,22 +656,40 @@ __device__ __forceinline__ size_t sc_lane_at(int t, int k)
 // only if its bucket is a stencil bucket, keyed at the first offset holding
 // it.  Nodes whose row stretches wrap around the table or meet in the key
 // space walk their 27 cells one by one instead, counting each entry only in
-// its own cell.  The (key, squared distance) pairs go unsorted to the lane's
-// list; lists past kLaneList go to the coarse walk.
+// its own cell.  The (key, node id) pairs go unsorted to the lane's list;
+// lists past kLaneList go to the coarse walk.
 constexpr int kLaneU = 4;   // entries in flight per lane
 
 // Scan key of an accepted candidate of node i: the first stencil offset of
 // its coarse bucket << 27 | its caller-order position cp.  The candidate's
 // coarse cell differs from x_i's by at most one per axis (the node takes
 // this scan only when the cell edge covers search + 2 max move), so the low
-// bits of the packed cells (pi, pj) give the offset.  ~0 when the cells differ by more (possible only
-// within rounding of that bound): the node then takes the coarse walk,
-// which decides such a candidate by its bucket.
-__device__ __forceinline__ unsigned int sc_lane_key(unsigned int pi, unsigned int pj, unsigned int cp)
+// 3 bits per axis of the cells (codes ci, cj) give the offset.  ~0 when the
+// cells differ by more (possible only within rounding of that bound, by two
+// at most below 2^22 cells): the node then takes the coarse walk, which
+// decides such a candidate by its bucket.
+__device__ __forceinline__ unsigned int sc_lane_key(unsigned int ci, unsigned int cj, unsigned int cp)
 {
-    const unsigned int ux = (pj - pi + 1u) & 2047u, uy = ((pj >> 11) - (pi >> 11) + 1u) & 2047u,
-                       uz = ((pj >> 22) - (pi >> 22) + 1u) & 1023u;
+    const unsigned int ux = (cj - ci + 1u) & 7u, uy = ((cj >> 3) - (ci >> 3) + 1u) & 7u,
+                       uz = ((cj >> 6) - (ci >> 6) + 1u) & 7u;
     return (ux <= 2u && uy <= 2u && uz <= 2u) ? (((ux + 3u * uy + 9u * uz) << 27) | cp) : 0xffffffffu;
+}
+
+// The fine record's second half: rest.y, rest.z, caller position | low code
+// bits << 27, node id | high code bits << 27.
+__device__ __forceinline__ unsigned int sc_rec_code(float4 r)
+{
+    return (__float_as_uint(r.z) >> 27) | ((__float_as_uint(r.w) >> 27) << 5);
+}
+__device__ __forceinline__ unsigned int sc_rec_caller(float4 r) { return __float_as_uint(r.z) & 0x07ffffffu; }
+__device__ __forceinline__ int sc_rec_node(float4 r) { return (int)(__float_as_uint(r.w) & 0x07ffffffu); }
+
+// |p_i - p_j|^2 as the scan tests it (products and sums unfused, so the
+// ranking of long lists recomputes the same value)
+__device__ __forceinline__ float sc_sq_dist(float4 pi, float4 pj)
+{
+    const float ex = pi.x - pj.x, ey = pi.y - pj.y, ez = pi.z - pj.z;
+    return ex * ex + ey * ey + ez * ez;
 }
 
 // Scan key from the full coarse cells of x_i (ci) and the candidate (cj):
@@ -713,8 +743,8 @@ __global__ void __launch_bounds__(256) sc_fine_lane_kernel(SelfColArgs a, const 
     // one by one, each entry counted only in its own cell
     if ((a.scExact[b] != 0) != EXACT) return;   // the other instantiation's node
     const bool percell = (a.scActive[i] & 4) != 0;
-    const unsigned int pi = __float_as_uint(m1.w);
-    const size_t L0 = sc_lane_at(t, 0);   // entry k at L0 + 64 k
+    const unsigned int ci = sc_rec_code(m1);
+    const size_t L0 = sc_lane_at(t, 0);   // entry k at L0 + k
     int n = 0;
     bool far = false;   // (not EXACT) a candidate two coarse cells away: the coarse walk decides
 #pragma unroll 1
@@ -744,15 +774,14 @@ __global__ void __launch_bounds__(256) sc_fine_lane_kernel(SelfColArgs a, const 
             for (int u = 0; u < kLaneU; ++u) {
                 const int e = e0 + u;
                 if (e >= h) break;
-                const float ex = m0.x - A[u].x, ey = m0.y - A[u].y, ez = m0.z - A[u].z;
-                const float d2 = ex * ex + ey * ey + ez * ez;
+                const float d2 = sc_sq_dist(m0, A[u]);
                 const float qx = m0.w - A[u].w, qy = m1.x - C[u].x, qz = m1.y - C[u].y;
                 const float q2 = qx * qx + qy * qy + qz * qz;
                 if (!((int)(d2 < search2) & (int)(q2 >= excl2) & (int)(e != t))) continue;
                 if (percell && !((int)(sc_cell(A[u].x, g) == cx) & (int)(sc_cell(A[u].y, g) == cy) &
                                  (int)(sc_cell(A[u].z, g) == cz)))
                     continue;
-                const unsigned int cp = (unsigned int)__float_as_int(C[u].z);
+                const unsigned int cp = sc_rec_caller(C[u]);
                 unsigned int key;
                 if (EXACT) {
                     // the coarse cells may lie far apart: the full cells
@@ -764,10 +793,10 @@ __global__ void __launch_bounds__(256) sc_fine_lane_kernel(SelfColArgs a, const 
                         if (key == 0xffffffffu) continue;
                     }
                 } else {
-                    key = sc_lane_key(pi, __float_as_uint(C[u].w), cp);
+                    key = sc_lane_key(ci, sc_rec_code(C[u]), cp);
                     far |= key == 0xffffffffu;
                 }
-                if (n < kLaneList) LK[L0 + (size_t)n * 64] = make_int2((int)key, __float_as_int(d2));
+                if (n < kLaneList) LK[L0 + (size_t)n * kLaneStride] = make_int2((int)key, sc_rec_node(C[u]));
                 ++n;
             }
         }
@@ -1014,27 +1043,29 @@ __device__ __forceinline__ void sc_lane_first_offsets(int4 ci, unsigned int tmas
         first[o / 6] |= f << (5 * (o % 6));
     }
     for (int k = 0; k < m; ++k) {
-        const int2 en = LK[L0 + (size_t)k * 64];
+        const int2 en = LK[L0 + (size_t)k * kLaneStride];
         const unsigned int o = (unsigned int)en.x >> 27;
         const unsigned int wsel = o / 6u;
         const unsigned int word = wsel == 0u ? first[0] : wsel == 1u ? first[1] : wsel == 2u ? first[2]
                                 : wsel == 3u ? first[3] : first[4];
         const unsigned int f = (word >> (5u * (o % 6u))) & 31u;
-        if (f != o) LK[L0 + (size_t)k * 64] = make_int2((int)((f << 27) | ((unsigned int)en.x & 0x07ffffffu)), en.y);
+        if (f != o) LK[L0 + (size_t)k * kLaneStride] = make_int2((int)((f << 27) | ((unsigned int)en.x & 0x07ffffffu)), en.y);
     }
 }
 
 // Ranking of the lane fine scan's lists of at most 32 candidates, one lane
-// per list in the grid's own order (the list length and entries of the 64
-// lanes are contiguous reads).  Each lane sorts its keys (all distinct;
-// padding ~0 sorts last) with a bitonic network in registers: N keys, N the
-// wave's power of two at or above its longest such list, min/max steps only;
-// the sorted keys are the slots.  Longer lists are flagged for the group
+// per list in node order (a wave's 64 nodes are one slot tile, so each slot
+// is written as one contiguous 256 B row; the lists are read from the nodes'
+// fine positions).  Each lane sorts its (key, node id) pairs as
+// 64-bit values key << 32 | id (keys all distinct; padding ~0 sorts last)
+// with a bitonic network in registers: N pairs, N the wave's power of two at
+// or above its longest such list, compare-exchange steps only; the ids in
+// sorted order are the slots.  Longer lists are flagged for the group
 // ranking kernel.
 constexpr int kRankNet = kSelfColCap;
 
 template <int N>
-__device__ __forceinline__ void sc_bitonic(unsigned int* k)
+__device__ __forceinline__ void sc_bitonic(unsigned long long* k)
 {
 #pragma unroll
     for (int size = 2; size <= N; size <<= 1)
@@ -1044,7 +1075,8 @@ __device__ __forceinline__ void sc_bitonic(unsigned int* k)
             for (int x = 0; x < N; ++x) {
                 const int y = x ^ stride;
                 if (y > x) {
-                    const unsigned int lo = min(k[x], k[y]), hi = max(k[x], k[y]);
+                    const bool sw = k[y] < k[x];
+                    const unsigned long long lo = sw ? k[y] : k[x], hi = sw ? k[x] : k[y];
                     const bool up = (x & size) == 0;
                     k[x] = up ? lo : hi;
                     k[y] = up ? hi : lo;
@@ -1056,36 +1088,34 @@ __device__ __forceinline__ void sc_bitonic(unsigned int* k)
 template <int N>
 __device__ __forceinline__ void sc_rank_list(SelfColArgs& a, int i, int m, size_t L0, const int2* __restrict__ LK)
 {
-    unsigned int k[N];
+    // rows past the list (N <= 32 < kLaneList, in the buffer) are read and
+    // replaced by padding
+    unsigned long long k[N];
 #pragma unroll
-    for (int x = 0; x < N; ++x) k[x] = x < m ? (unsigned int)LK[L0 + (size_t)x * 64].x : 0xffffffffu;
-    sc_bitonic<N>(k);
-    // device ids eight at a time: the lookups of a chunk are in flight together
-    // (the slots and the lookup table are different arrays, but the compiler
-    // cannot tell, so it would keep each lookup behind the previous store)
-    constexpr int C = N < 8 ? N : 8;
-#pragma unroll
-    for (int x0 = 0; x0 < N; x0 += C) {
-        int id[C];
-#pragma unroll
-        for (int u = 0; u < C; ++u) id[u] = x0 + u < m ? a.devOfCaller[k[x0 + u] & 0x07ffffffu] : 0;
-#pragma unroll
-        for (int u = 0; u < C; ++u)
-            if (x0 + u < m) a.contacts[sc_slot(i, x0 + u)] = id[u];
+    for (int x = 0; x < N; ++x) {
+        const int2 e = LK[L0 + (size_t)x * kLaneStride];
+        k[x] = x < m ? ((unsigned long long)(unsigned int)e.x << 32) | (unsigned int)e.y : ~0ull;
     }
+    sc_bitonic<N>(k);
+#pragma unroll
+    for (int x = 0; x < N; ++x)
+        if (x < m) a.contacts[sc_slot(i, x)] = (int)(unsigned int)k[x];
 }
 
-__global__ void __launch_bounds__(256) sc_lane_rank_kernel(SelfColArgs a, const int4* __restrict__ FC,
-                                                          const int* __restrict__ FS, int2* __restrict__ LK)
+__global__ void __launch_bounds__(256) sc_lane_rank_kernel(int nPad, SelfColArgs a, const int4* __restrict__ FC,
+                                                          int2* __restrict__ LK)
 {
-    const int t = blockIdx.x * blockDim.x + (int)threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + (int)threadIdx.x;
     bool listed = false;
-    int m = 0, i = 0, b = 0;
-    if (t < FS[a.fineTotal]) {
-        i = FC[t].w;
+    int m = 0, t = 0, b = 0;
+    if (i < nPad) {
         b = a.tileBody[i / kTile];
-        listed = a.accCount[i] == kScLane;
-        if (listed) m = a.laneN[t];
+        const BodyDev& B = a.bodies[b];
+        if (B.scOn && B.scFine && i < B.nodeEnd && a.accCount[i] == kScLane) {
+            listed = true;
+            t = a.finePos[i];   // in the grid: the fine keys gave the node the lane scan
+            m = a.laneN[t];
+        }
     }
     const size_t L0 = sc_lane_at(t, 0);
     if (listed && (a.scActive[i] & 2))
@@ -1107,23 +1137,23 @@ __global__ void __launch_bounds__(256) sc_lane_rank_kernel(SelfColArgs a, const 
         a.overflow[i] = 0;
     }
     // the contact push of object shape matching needs to know whether the
-    // body has any candidate (a wave of fine positions may span two bodies)
+    // body has any candidate
     if (listed && m > 0 && a.bodies[b].objOn) a.scCounters[3 * b] = 1;
 }
 
 // Ranking of the lane-scan lists of more than 32 candidates, 32 lanes per
 // list, lists in the grid's own order (the two lists of a wave are
 // neighbouring fine positions, so each step of their reads and writes shares
-// cache lines).  The node's (key, squared distance) list is spread over the
-// group's lanes; its keys (all distinct) are ranked by counting in LDS; the
-// list goes back to its entries in rank order as (id, priority = |p_i -
-// p_j|, from the squared distance the scan tested) and the node is flagged
+// cache lines; the nodes with long lists crowd the same grid regions).  The node's (key, node id) list is spread over the group's
+// lanes; its keys (all distinct) are ranked by counting in LDS; the list goes
+// back to its entries in rank order as (id, priority = |p_i - p_j|, from the
+// squared distance recomputed as the scan tested it) and the node is flagged
 // for the slot-rule kernel.
 constexpr int kLaneGroup = 32;
 constexpr int kLanePer = (kLaneList + kLaneGroup - 1) / kLaneGroup;
 
 __global__ void __launch_bounds__(256) sc_slots_kernel(SelfColArgs a, const int4* __restrict__ FC,
-                                                      const int* __restrict__ FS)
+                                                      const int* __restrict__ FS, const float4* __restrict__ P)
 {
     __shared__ unsigned int skey[256 / kLaneGroup][kLaneList];
     const int sub = threadIdx.x % kLaneGroup, grp = threadIdx.x / kLaneGroup;
@@ -1143,17 +1173,17 @@ __global__ void __launch_bounds__(256) sc_slots_kernel(SelfColArgs a, const int4
     kn = __builtin_amdgcn_readfirstlane(kn);
     if (kn == 0) return;   // wave-uniform
     unsigned int key[kLanePer];
-    int jv[kLanePer], rank[kLanePer];
+    int id[kLanePer], rank[kLanePer];
 #pragma unroll
     for (int k = 0; k < kLanePer; ++k) {
         key[k] = 0xffffffffu;
-        jv[k] = 0;
+        id[k] = 0;
         rank[k] = 0;
         const int idx = sub + kLaneGroup * k;
         if (k < kn && idx < m) {
-            const int2 e = a.laneKey[L0 + (size_t)idx * 64];
+            const int2 e = a.laneKey[L0 + (size_t)idx * kLaneStride];
             key[k] = (unsigned int)e.x;
-            jv[k] = e.y;   // squared distance bits
+            id[k] = e.y;
             skey[grp][idx] = key[k];
         }
     }
@@ -1168,15 +1198,16 @@ __global__ void __launch_bounds__(256) sc_slots_kernel(SelfColArgs a, const int4
     }
     if (lane) {
         // the list in rank order with its priorities, for the slot rule (the
-        // id lookups first, all in flight together)
-        int id[kLanePer];
+        // candidates' positions first, all in flight together)
+        const float4 pi = a.fineE[2 * (size_t)t];
+        float4 pj[kLanePer];
 #pragma unroll
         for (int k = 0; k < kLanePer; ++k)
-            id[k] = (k < kn && sub + kLaneGroup * k < m) ? a.devOfCaller[key[k] & 0x07ffffffu] : 0;
+            if (k < kn && sub + kLaneGroup * k < m) pj[k] = P[id[k]];
 #pragma unroll
         for (int k = 0; k < kLanePer; ++k)
             if (k < kn && sub + kLaneGroup * k < m)
-                a.laneKey[L0 + (size_t)rank[k] * 64] = make_int2(id[k], __float_as_int(sqrtf(__int_as_float(jv[k]))));
+                a.laneKey[L0 + (size_t)rank[k] * kLaneStride] = make_int2(id[k], __float_as_int(sqrtf(sc_sq_dist(pi, pj[k]))));
         if (sub == 0) {
             a.counts[i] = kSelfColCap;
             a.overflow[i] = m - kSelfColCap;
@@ -1196,7 +1227,7 @@ __global__ void __launch_bounds__(256) sc_slot_rule_kernel(SelfColArgs a, const 
     const int i = FC[t].w;
     if (a.accCount[i] != kScRule) return;
     const size_t L0 = sc_lane_at(t, 0);
-    sc_fill_slots(i, kSelfColCap + a.overflow[i], a, [&](int r) { return a.laneKey[L0 + (size_t)r * 64]; });
+    sc_fill_slots(i, kSelfColCap + a.overflow[i], a, [&](int r) { return a.laneKey[L0 + (size_t)r * kLaneStride]; });
 }
 
 // Statistics of one body's last build (forward.cu:11837-11849 reports them):
@@ -1540,9 +1571,9 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
     if (mark) mark(markCtx, "selfcol_build_slots");
     if (a.fineTotal > 0) {
         if (laneScan)
-            sc_lane_rank_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(a, a.fineC, a.fstart, a.laneKey);
+            sc_lane_rank_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, a.fineC, a.laneKey);
         if (mark) mark(markCtx, "selfcol_build_rank_long");
-        sc_slots_kernel<<<nbs(na.nPad, 256 / kLaneGroup), 256, 0, s>>>(a, a.fineC, a.fstart);
+        sc_slots_kernel<<<nbs(na.nPad, 256 / kLaneGroup), 256, 0, s>>>(a, a.fineC, a.fstart, P);
         if (mark) mark(markCtx, "selfcol_build_slot_rule");
         sc_slot_rule_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(a, a.fineC, a.fstart);
     }
