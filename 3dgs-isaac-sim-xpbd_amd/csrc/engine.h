@@ -479,7 +479,7 @@ private:
     int fineTotal_ = 0;
     bool scFineOn_ = true;
     DevBuf<unsigned int> fkeys_;
-    DevBuf<int> frank_, fcount_, fstart_, fscan_, finePos_, laneN_;
+    DevBuf<int> frank_, fcount_, fstart_, fscan_, finePos_, laneN_, scLong_;
     DevBuf<int> coarseRank_, accCount_;
     DevBuf<float4> fineE_;
     DevBuf<int4> fineC_;

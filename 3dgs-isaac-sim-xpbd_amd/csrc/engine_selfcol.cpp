@@ -96,6 +96,7 @@ SelfColArgs Engine::scArgs() const
     a.fineE = fineE_.p;
     a.fineC = fineC_.p;
     a.laneN = laneN_.p;
+    a.scLong = scLong_.p;
     a.laneKey = laneKey_.p;
     a.accCount = fine ? accCount_.p : nullptr;
     return a;
@@ -296,11 +297,11 @@ bool Engine::ensureSelfCol(std::string& err)
     fineE_.alloc(2 * n); fineC_.alloc(n); coarseRank_.alloc(n);
     // the lane fine scan's unsorted lists: one contiguous list of kLaneList
     // entries per fine position
-    laneN_.alloc(n); laneKey_.alloc(n * kLaneList);
+    laneN_.alloc(n); laneKey_.alloc(n * kLaneList); scLong_.alloc(n + 1);
     // per node: what its candidate search still needs
     accCount_.alloc(n);
     if (!fkeys_.p || !frank_.p || !finePos_.p || !fcount_.p || !fstart_.p || !fscan_.p || !fineE_.p ||
-        !fineC_.p || !coarseRank_.p || !laneN_.p || !laneKey_.p || !accCount_.p) {
+        !fineC_.p || !coarseRank_.p || !laneN_.p || !laneKey_.p || !scLong_.p || !accCount_.p) {
         err = "self-collision: hipMalloc failed";
         return false;
     }

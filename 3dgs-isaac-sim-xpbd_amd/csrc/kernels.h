@@ -184,6 +184,8 @@ struct SelfColArgs {
     // lists of the lane fine scan, one contiguous list per fine position
     // (entry k of position t at t kLaneList + k)
     int* laneN;                // [nPad] fine order: candidates of the lane scan, -1 handed over
+    int* scLong;               // [1 + nPad] this build's long-list queue: count, then the nodes whose
+                               //   lane-scan lists exceed 32 (any order; null without the fine grid)
     int2* laneKey;             // [nPad * kLaneList] (scan key, node id) of each candidate,
                                //   unsorted; (node id, priority bits) in rank order for lists that take
                                //   the slot rule

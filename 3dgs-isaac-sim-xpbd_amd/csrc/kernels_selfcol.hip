@@ -229,6 +229,7 @@ __global__ void sc_prepare_kernel(int B, SelfColArgs a)
         for (int k = 0; k < B; ++k)
             if (a.bodies[k].scOn && !a.bodies[k].scFine) gate = 1;
         *a.scSortGate = gate;
+        if (a.scLong) a.scLong[0] = 0;   // the long-list queue of this build
     }
 }
 
@@ -1121,7 +1122,19 @@ __global__ void __launch_bounds__(256) sc_lane_rank_kernel(int nPad, SelfColArgs
     if (listed && (a.scActive[i] & 2))
         sc_lane_first_offsets(FC[t], (unsigned int)a.bodies[b].scTableMask, m, L0, LK);
     const bool here = listed && m <= kRankNet;
-    if (listed && !here) a.accCount[i] = kScLong;
+    // longer lists go to the queue of the group ranking and the slot rule
+    // (one atomic per wave)
+    const bool lng = listed && !here;
+    if (lng) a.accCount[i] = kScLong;
+    const unsigned long long lb = __ballot(lng);
+    if (lb != 0ull) {
+        const int lane = (int)(threadIdx.x & 63);
+        const int lead = __builtin_ctzll(lb);
+        int base = 0;
+        if (lane == lead) base = atomicAdd(&a.scLong[0], __popcll(lb));
+        base = __shfl(base, lead, 64);
+        if (lng) a.scLong[1 + base + __popcll(lb & ((1ull << lane) - 1ull))] = i;
+    }
     // the wave's network size
     int need = here ? m : 0;
 #pragma unroll
@@ -1142,92 +1155,99 @@ __global__ void __launch_bounds__(256) sc_lane_rank_kernel(int nPad, SelfColArgs
 }
 
 // Ranking of the lane-scan lists of more than 32 candidates, 32 lanes per
-// list, lists in the grid's own order (the two lists of a wave are
-// neighbouring fine positions, so each step of their reads and writes shares
-// cache lines; the nodes with long lists crowd the same grid regions).  The node's (key, node id) list is spread over the group's
-// lanes; its keys (all distinct) are ranked by counting in LDS; the list goes
-// back to its entries in rank order as (id, priority = |p_i - p_j|, from the
-// squared distance recomputed as the scan tested it) and the node is flagged
-// for the slot-rule kernel.
+// list, the lists taken from the build's long-list queue by a fixed grid
+// (each list is one contiguous stretch).  The node's (key, node id) list is
+// spread over the group's lanes; its keys (all distinct) are ranked by
+// counting in LDS; the list goes back to its entries in rank order as (id,
+// priority = |p_i - p_j|, from the squared distance recomputed as the scan
+// tested it) and the node is flagged for the slot-rule kernel.
 constexpr int kLaneGroup = 32;
 constexpr int kLanePer = (kLaneList + kLaneGroup - 1) / kLaneGroup;
+constexpr int kLaneGroups = 256 / kLaneGroup;
 
-__global__ void __launch_bounds__(256) sc_slots_kernel(SelfColArgs a, const int4* __restrict__ FC,
-                                                      const int* __restrict__ FS, const float4* __restrict__ P)
+__global__ void __launch_bounds__(256) sc_slots_kernel(SelfColArgs a, const float4* __restrict__ P)
 {
-    __shared__ unsigned int skey[256 / kLaneGroup][kLaneList];
+    __shared__ unsigned int skey[kLaneGroups][kLaneList];
     const int sub = threadIdx.x % kLaneGroup, grp = threadIdx.x / kLaneGroup;
-    const int t = blockIdx.x * (256 / kLaneGroup) + grp;
-    int i = 0, m = 0;
-    bool lane = false;
-    if (t < FS[a.fineTotal]) {
-        i = FC[t].w;
-        lane = a.accCount[i] == kScLong;
-        if (lane) m = a.laneN[t];
-    }
-    const size_t L0 = sc_lane_at(t, 0);
-    // keys per lane in use: the wave's largest
-    int kn = (m + kLaneGroup - 1) / kLaneGroup;
-#pragma unroll
-    for (int s = kLaneGroup; s < 64; s <<= 1) kn = max(kn, __shfl_xor(kn, s, 64));
-    kn = __builtin_amdgcn_readfirstlane(kn);
-    if (kn == 0) return;   // wave-uniform
-    unsigned int key[kLanePer];
-    int id[kLanePer], rank[kLanePer];
-#pragma unroll
-    for (int k = 0; k < kLanePer; ++k) {
-        key[k] = 0xffffffffu;
-        id[k] = 0;
-        rank[k] = 0;
-        const int idx = sub + kLaneGroup * k;
-        if (k < kn && idx < m) {
-            const int2 e = a.laneKey[L0 + (size_t)idx * kLaneStride];
-            key[k] = (unsigned int)e.x;
-            id[k] = e.y;
-            skey[grp][idx] = key[k];
+    const int count = a.scLong[0];
+    // the loop bound is the same for the whole block
+    for (int q0 = (int)blockIdx.x * kLaneGroups; q0 < count; q0 += (int)gridDim.x * kLaneGroups) {
+        const int q = q0 + grp;
+        const bool lane = q < count;
+        int i = 0, t = 0, m = 0;
+        if (lane) {
+            i = a.scLong[1 + q];
+            t = a.finePos[i];
+            m = a.laneN[t];
         }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    for (int f = 0; f < m; ++f) {
-        const unsigned int kf = skey[grp][f];
+        const size_t L0 = sc_lane_at(t, 0);
+        // keys per lane in use: the wave's largest
+        int kn = (m + kLaneGroup - 1) / kLaneGroup;
 #pragma unroll
-        for (int k = 0; k < kLanePer; ++k)
-            if (k < kn) rank[k] += kf < key[k] ? 1 : 0;
-    }
-    if (lane) {
-        // the list in rank order with its priorities, for the slot rule (the
-        // candidates' positions first, all in flight together)
-        const float4 pi = a.fineE[2 * (size_t)t];
-        float4 pj[kLanePer];
+        for (int s = kLaneGroup; s < 64; s <<= 1) kn = max(kn, __shfl_xor(kn, s, 64));
+        kn = __builtin_amdgcn_readfirstlane(kn);
+        if (kn == 0) continue;   // wave-uniform: this wave's groups are past the queue
+        unsigned int key[kLanePer];
+        int id[kLanePer], rank[kLanePer];
 #pragma unroll
-        for (int k = 0; k < kLanePer; ++k)
-            if (k < kn && sub + kLaneGroup * k < m) pj[k] = P[id[k]];
-#pragma unroll
-        for (int k = 0; k < kLanePer; ++k)
-            if (k < kn && sub + kLaneGroup * k < m)
-                a.laneKey[L0 + (size_t)rank[k] * kLaneStride] = make_int2(id[k], __float_as_int(sqrtf(sc_sq_dist(pi, pj[k]))));
-        if (sub == 0) {
-            a.counts[i] = kSelfColCap;
-            a.overflow[i] = m - kSelfColCap;
-            a.accCount[i] = kScRule;
+        for (int k = 0; k < kLanePer; ++k) {
+            key[k] = 0xffffffffu;
+            id[k] = 0;
+            rank[k] = 0;
+            const int idx = sub + kLaneGroup * k;
+            if (k < kn && idx < m) {
+                const int2 e = a.laneKey[L0 + (size_t)idx * kLaneStride];
+                key[k] = (unsigned int)e.x;
+                id[k] = e.y;
+                skey[grp][idx] = key[k];
+            }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (int f = 0; f < m; ++f) {
+            const unsigned int kf = skey[grp][f];
+#pragma unroll
+            for (int k = 0; k < kLanePer; ++k)
+                if (k < kn) rank[k] += kf < key[k] ? 1 : 0;
+        }
+        if (lane) {
+            // the list in rank order with its priorities, for the slot rule (the
+            // candidates' positions first, all in flight together)
+            const float4 pi = a.fineE[2 * (size_t)t];
+            float4 pj[kLanePer];
+#pragma unroll
+            for (int k = 0; k < kLanePer; ++k)
+                if (k < kn && sub + kLaneGroup * k < m) pj[k] = P[id[k]];
+#pragma unroll
+            for (int k = 0; k < kLanePer; ++k)
+                if (k < kn && sub + kLaneGroup * k < m)
+                    a.laneKey[L0 + (size_t)rank[k] * kLaneStride] = make_int2(id[k], __float_as_int(sqrtf(sc_sq_dist(pi, pj[k]))));
+            if (sub == 0) {
+                a.counts[i] = kSelfColCap;
+                a.overflow[i] = m - kSelfColCap;
+                a.accCount[i] = kScRule;
+            }
+        }
+        // the group's keys in LDS are read before the next list overwrites them
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
 }
 
-// The slot rule for the flagged nodes (more than 32 candidates), one thread
-// per node in the grid's own order (the lists of a wave's nodes share cache
-// lines) over the (id, priority) list the group ranking left in rank order.
-__global__ void __launch_bounds__(256) sc_slot_rule_kernel(SelfColArgs a, const int4* __restrict__ FC,
-                                                          const int* __restrict__ FS)
+// The slot rule for the nodes of the long-list queue (more than 32
+// candidates), one thread per node, over the (id, priority) list the group
+// ranking left in rank order.
+__global__ void __launch_bounds__(256) sc_slot_rule_kernel(SelfColArgs a)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= FS[a.fineTotal]) return;
-    const int i = FC[t].w;
-    if (a.accCount[i] != kScRule) return;
-    const size_t L0 = sc_lane_at(t, 0);
-    sc_fill_slots(i, kSelfColCap + a.overflow[i], a, [&](int r) { return a.laneKey[L0 + (size_t)r * kLaneStride]; });
+    const int count = a.scLong[0];
+    for (int q = (int)(blockIdx.x * blockDim.x + threadIdx.x); q < count; q += (int)(gridDim.x * blockDim.x)) {
+        const int i = a.scLong[1 + q];
+        const size_t L0 = sc_lane_at(a.finePos[i], 0);
+        sc_fill_slots(i, kSelfColCap + a.overflow[i], a,
+                      [&](int r) { return a.laneKey[L0 + (size_t)r * kLaneStride]; });
+    }
 }
 
 // Statistics of one body's last build (forward.cu:11837-11849 reports them):
@@ -1573,9 +1593,10 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
         if (laneScan)
             sc_lane_rank_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, a.fineC, a.laneKey);
         if (mark) mark(markCtx, "selfcol_build_rank_long");
-        sc_slots_kernel<<<nbs(na.nPad, 256 / kLaneGroup), 256, 0, s>>>(a, a.fineC, a.fstart, P);
+        // fixed grids over the long-list queue (at most one entry per node)
+        sc_slots_kernel<<<nbs(na.nPad, kLaneGroups) < 2048 ? nbs(na.nPad, kLaneGroups) : 2048, 256, 0, s>>>(a, P);
         if (mark) mark(markCtx, "selfcol_build_slot_rule");
-        sc_slot_rule_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(a, a.fineC, a.fstart);
+        sc_slot_rule_kernel<<<nbs(na.nPad, 256) < 1024 ? nbs(na.nPad, 256) : 1024, 256, 0, s>>>(a);
     }
     sc_finish_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a);
     *sortedIdxOut = vs;
