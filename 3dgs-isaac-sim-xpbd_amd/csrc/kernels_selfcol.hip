@@ -731,6 +731,9 @@ __global__ void __launch_bounds__(256) sc_fine_lane_kernel(SelfColArgs a, const 
     const int t = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
     if (t >= FS[a.fineTotal]) return;   // fine positions in use
     const int i = FC[t].w;
+    // the capped-cell instantiation skips the (usual) other nodes before
+    // loading their records
+    if (EXACT && a.scExact[a.tileBody[i / kTile]] == 0) return;
     const float4 m0 = E[2 * (size_t)t], m1 = E[2 * (size_t)t + 1];
     if (a.accCount[i] != kScLane) return;   // flagged for the coarse walk
     const int b = a.tileBody[i / kTile];
