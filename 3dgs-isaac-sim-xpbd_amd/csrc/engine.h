@@ -491,7 +491,7 @@ private:
     std::vector<char> scBuilt_;             // a build has filled the body's lists since the arrays were made
     const int* scSortedIdx_ = nullptr;
     const unsigned int* scSortedKeys_ = nullptr;
-    DevBuf<unsigned int> scMax2_, scKeys_, scKeysAlt_, scHist_;
+    DevBuf<unsigned int> scMax2_, scKeysDev_, scKeys_, scKeysAlt_, scHist_;
     DevBuf<int> scCounters_, scUseActive_, scExact_, scIdx_, scIdxAlt_, cellStart_, cellEnd_, contacts_, counts_;
     DevBuf<int> scOver_, scStats_;   // per-node overflow; statistics read-back
     DevBuf<int> scSortGate_;          // 0: the last build skipped the coarse sort (2: sorted since, on demand)

@@ -66,6 +66,7 @@ SelfColArgs Engine::scArgs() const
     a.scBodyDisp = scBodyDisp_.p;
     a.scInvCell = scInvCell_.p;
     a.scActive = scActive_.p;
+    a.keysDev = scKeysDev_.p;
     a.keys = scKeys_.p;
     a.idx = scIdx_.p;
     a.sentinel = (unsigned int)scTableTotal_;
@@ -264,14 +265,14 @@ bool Engine::ensureSelfCol(std::string& err)
     const size_t n = nPad_;
     scMax2_.alloc(B); scCounters_.alloc(3 * (size_t)B); scUseActive_.alloc(B); scExact_.alloc(B); scBodyDisp_.alloc(B);
     scInvCell_.alloc(B);
-    scKeys_.alloc(n); scKeysAlt_.alloc(n); scIdx_.alloc(n); scIdxAlt_.alloc(n);
+    scKeysDev_.alloc(n); scKeys_.alloc(n); scKeysAlt_.alloc(n); scIdx_.alloc(n); scIdxAlt_.alloc(n);
     scHist_.alloc(radix_sort_hist_size(nPad_));
     cellStart_.alloc(scTableTotal_); cellEnd_.alloc(scTableTotal_);
     contacts_.alloc(n * kSelfColCap); counts_.alloc(n); scOver_.alloc(n);
     scActive_.alloc(n); scAccum_.alloc(n);
     scSortP_.alloc(n); scSortX_.alloc(n); scSortR_.alloc(n);
     scSortGate_.alloc(1);
-    if (!scSortGate_.p || !scMax2_.p || !scCounters_.p || !scUseActive_.p || !scExact_.p || !scBodyDisp_.p || !scInvCell_.p || !scKeys_.p ||
+    if (!scSortGate_.p || !scMax2_.p || !scCounters_.p || !scUseActive_.p || !scExact_.p || !scBodyDisp_.p || !scInvCell_.p || !scKeysDev_.p || !scKeys_.p ||
         !scKeysAlt_.p || !scIdx_.p || !scIdxAlt_.p || !scHist_.p || !cellStart_.p || !cellEnd_.p || !contacts_.p ||
         !counts_.p || !scOver_.p || !scActive_.p || !scAccum_.p || !scSortP_.p ||
         !scSortX_.p || !scSortR_.p) {
@@ -401,6 +402,7 @@ bool Engine::selfCollisionLists(int b, unsigned int* keys, int* sorted, int* cou
     XPBD_HIP(hipMemcpyAsync(&gate, scSortGate_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
     XPBD_HIP(hipStreamSynchronize(stream_));
     if (gate == 0) {
+        launch_sc_caller_order(stream_, nPad_, scKeysDev_.p, devOfCaller_.p, scKeys_.p, scIdx_.p, nullptr);
         launch_radix_sort(stream_, nPad_, scSortBits_, scKeys_.p, scIdx_.p, scKeysAlt_.p, scIdxAlt_.p, scHist_.p, false);
         const int sorted2 = 2;
         XPBD_HIP(hipMemcpyAsync(scSortGate_.p, &sorted2, sizeof(int), hipMemcpyHostToDevice, stream_));

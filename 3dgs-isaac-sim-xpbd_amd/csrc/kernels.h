@@ -147,8 +147,9 @@ struct SelfColArgs {
     float* scBodyDisp;         // [B]
     float* scInvCell;          // [B]
     unsigned char* scActive;   // [nPad]
-    unsigned int* keys;        // [nPad]
-    int* idx;                  // [nPad]
+    unsigned int* keysDev;     // [nPad] coarse key of each node (node order)
+    unsigned int* keys;        // [nPad] the coarse sort's input: keys in caller order,
+    int* idx;                  // [nPad]   with their nodes (written when the sort runs)
     unsigned int sentinel;     // = tableTotal
     int tableTotal;
     int* cellStart;            // [tableTotal]
@@ -211,6 +212,10 @@ size_t radix_sort_hist_size(int n);   // entries of the sort's histogram buffer
 // at once and the buffers keep their contents
 bool launch_radix_sort(hipStream_t s, int n, int bits, unsigned int* k0, int* v0, unsigned int* k1, int* v1,
                        unsigned int* hist, bool iota, const int* gate = nullptr);
+// The coarse sort's input from the node-order keys: keys[pos] =
+// keysDev[devOfCaller[pos]], idx[pos] = devOfCaller[pos] (gate as above).
+void launch_sc_caller_order(hipStream_t s, int nPad, const unsigned int* keysDev, const int* devOfCaller,
+                            unsigned int* keys, int* idx, const int* gate);
 // Volume-cluster member lists on the device (one body, caller order): the
 // k-hop ball of every node, farthest-point sampled to cap.  Balls of more
 // than kVbMaxBall nodes are left to the host (count -1).

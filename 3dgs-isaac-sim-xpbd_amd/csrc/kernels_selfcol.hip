@@ -326,8 +326,9 @@ __global__ void sc_reach_kernel(int B, SelfColArgs a)
 }
 
 // Keys of the frame-start positions (forward.cu:7034-7044) in the body's table,
-// stored at the node's caller-order position with the device index as value,
-// so that the stable sort breaks ties by the caller's node order.
+// in node order; the coarse sort reads them in caller order with the node
+// index as value, so that the stable sort breaks ties by the caller's node
+// order.
 __global__ void __launch_bounds__(256) sc_keys_kernel(int nPad, SelfColArgs a, const float3* __restrict__ X)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -343,9 +344,21 @@ __global__ void __launch_bounds__(256) sc_keys_kernel(int nPad, SelfColArgs a, c
                   sc_hash(sc_cell(p.x, inv), sc_cell(p.y, inv), sc_cell(p.z, inv), (unsigned int)B.scTableMask);
         }
     }
-    const int pos = a.callerPos[i];
-    a.keys[pos] = key;
-    a.idx[pos] = i;
+    a.keysDev[i] = key;
+}
+
+// The coarse sort's input in caller order, only when the sort runs.
+__global__ void __launch_bounds__(256) sc_caller_order_kernel(int nPad, const unsigned int* __restrict__ keysDev,
+                                                             const int* __restrict__ devOfCaller,
+                                                             unsigned int* __restrict__ keys, int* __restrict__ idx,
+                                                             const int* __restrict__ gate)
+{
+    if (gate && *gate == 0) return;
+    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= nPad) return;
+    const int i = devOfCaller[pos];
+    keys[pos] = keysDev[i];
+    idx[pos] = i;
 }
 
 // Empty cells of the coarse tables (-1 start), when the coarse walk runs.
@@ -1531,6 +1544,12 @@ void launch_object_shape(hipStream_t s, const NodeArgs& na, const SelfColArgs& a
     obj_match_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, P, rest);
 }
 
+void launch_sc_caller_order(hipStream_t s, int nPad, const unsigned int* keysDev, const int* devOfCaller,
+                            unsigned int* keys, int* idx, const int* gate)
+{
+    sc_caller_order_kernel<<<nbs(nPad, 256), 256, 0, s>>>(nPad, keysDev, devOfCaller, keys, idx, gate);
+}
+
 bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, const SelfColArgs& a, const float3* X,
                           const float4* P, const float3* rest, int sortBits, unsigned int* keysAlt, int* idxAlt,
                           unsigned int* hist, const int** sortedIdxOut, const Tuning& tu,
@@ -1579,6 +1598,8 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
     // return at once when the gate reads 0 (the keys stay in place for an
     // export, which then sorts them)
     if (mark) mark(markCtx, "selfcol_build_coarse");
+    sc_caller_order_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a.keysDev, a.devOfCaller, a.keys, a.idx,
+                                                             a.scSortGate);
     const bool inAlt = launch_radix_sort(s, na.nPad, sortBits, a.keys, a.idx, keysAlt, idxAlt, hist, false,
                                          a.scSortGate);
     const unsigned int* ks = inAlt ? keysAlt : a.keys;
