@@ -1618,7 +1618,7 @@ bool launch_selfcol_build(hipStream_t s, const NodeArgs& na, int numBodies, cons
             sc_lane_rank_kernel<<<nbs(na.nPad, 256), 256, 0, s>>>(na.nPad, a, a.fineC, a.laneKey);
         if (mark) mark(markCtx, "selfcol_build_rank_long");
         // fixed grids over the long-list queue (at most one entry per node)
-        sc_slots_kernel<<<nbs(na.nPad, kLaneGroups) < 2048 ? nbs(na.nPad, kLaneGroups) : 2048, 256, 0, s>>>(a, P);
+        sc_slots_kernel<<<nbs(na.nPad, kLaneGroups) < 16384 ? nbs(na.nPad, kLaneGroups) : 16384, 256, 0, s>>>(a, P);
         if (mark) mark(markCtx, "selfcol_build_slot_rule");
         sc_slot_rule_kernel<<<nbs(na.nPad, 256) < 1024 ? nbs(na.nPad, 256) : 1024, 256, 0, s>>>(a);
     }
